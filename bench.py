@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Throughput of the SIMOX MD hot path on one MI355X.
+
+One step = one REGULAR NVE step of the reference's loop (main.f90:41-61):
+VV_NVE1 -> CELL_SORT -> CTIP_SOLVER -> FORCE -> PME -> VV_NVE2, with the cell
+sort and the charge re-equilibration every step, on device-resident state.
+
+Default workload: 1M-atom Al/O slab (64x64x60 Al FCC cells = 983,040 Al +
+65,536 O, PME grid 256x256x512, Rcut 10 A, 900 K), synthetic seeded inputs.
+Prints one JSON line (metric atom-steps/s) with the dominant kernel's roofline
+(HIP events over the timed region) and a CPU baseline (the FP64 oracle
+restatement, single core, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = {0: "k_spread", 1: "k_fft_x_r2c", 2: "k_fft_strided<fwd/inv y>",
+           3: "k_fft_strided<z, fused influence>", 4: "k_fft_x_c2r", 5: "k_gather_force",
+           6: "k_dudq", 7: "k_force", 8: "k_density", 9: "k_ctip_matrix"}
+WORKLOADS = {
+    "c1": "C1: Al slab 6x6x8 cells (1,152) + 64 O, PME 32x32x96, Rcut 8 A, 300 K",
+    "c2": "C2: Al slab 24x24x32 cells (73,728) + 4,096 O, PME 96x96x192, Rcut 10 A, 600 K",
+    "c3": "C3: Al slab 64x64x60 cells (983,040) + 65,536 O, PME 256x256x512, Rcut 10 A, 900 K",
+}
+
+
+def algorithmic_bytes(which, s, mode, nb_entries):
+    """Unique HBM bytes one launch of kernel `which` must move (each array read or
+    written once).  r = bytes of the grid/spectrum real type."""
+    n = s.n
+    Kx, Ky, Kz = s.fft
+    ng = Kx * Ky * Kz
+    nh = (Kx // 2 + 1) * Ky * Kz
+    r = 4 if mode == "mixed" else 8
+    c = 2 * r
+    if which == 0:   # spread: x, q (+p, ql) in; fixed-point grid accumulated once
+        return n * (24 + 8) + ng * 8
+    if which == 1:   # R2C: fixed-point grid read + zeroed, half spectrum written
+        return ng * 16 + nh * c
+    if which == 2:   # y pass: spectrum read + written
+        return nh * 2 * c
+    if which == 3:   # z pass: spectrum read + written, influence table read
+        return nh * 2 * c + nh * r
+    if which == 4:   # C2R: spectrum read, real grid written
+        return nh * c + ng * r
+    if which == 5:   # force gather: grid read, x q in, frec out
+        return ng * r + n * (32 + 24)
+    if which == 6:   # dUdq: grid read, x id q b p in, g out, ELL (col + value) rows
+        return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
+    if which == 7:   # pair force: x q id dF in, f out
+        return n * (24 + 8 + 4 + r + 24)
+    if which == 8:   # density: x id in, rho dF out
+        return n * (24 + 4 + 8 + r)
+    if which == 9:   # CTIP matrix: x id in, ELL + b out
+        return n * (24 + 4 + 8 + 4) + nb_entries * (4 + r)
+    return 0
+
+
+def cpu_baseline(seconds=15.0):
+    """The FP64 oracle (single core) on a bounded sample: full NVE steps (sort,
+    nonlinear-CG CTIP, FORCE, PME, Verlet) of the C1 slab (1,216 atoms)."""
+    from oracle import oracle as om
+    from simox_amd.systems import config
+    s = config("c1")
+    o = om.Oracle(s.n, s.box, s.fft, s.rcut)
+    o.cell_sort(s.x)
+    q = s.q.copy()
+    _, psi = o.pme_self(s.ids, q)
+    ref = o.force(s.ids, s.x, q)
+    _, f = o.pme(s.ids, s.x, q, psi, ref["f"])
+    x, v = s.x, s.v
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        r = o.step(s.ids, x, v, f, q, psi, s.dt)
+        x, v, f, q = r["x"], r["v"], r["f"], r["q"]
+        k += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": s.n * k / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle FP64 restatement (1 core): %d full NVE steps (sort, nonlinear-CG "
+                      "CTIP from q=0 then warm, FORCE, PME, Verlet) of C1 (1,216 atoms)" % k}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="faithful", choices=["faithful", "fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU hot path "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        return 2
+
+    from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox
+    from simox_amd.systems import config
+    mode_id = {"faithful": MODE_FAITHFUL, "fp64": MODE_FP64, "mixed": MODE_MIXED}[args.mode]
+    print("bench: building %s inputs" % args.config, file=sys.stderr, flush=True)
+    s = config(args.config)
+    g = Simox.from_system(s, mode=mode_id)
+    # initial forces at x(0) with q = 0 (the first step re-equilibrates charges)
+    g.force()
+    g.pme()
+    print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        g.step(1, s.dt, 1, 1, stats=False)
+    g.timing(True)
+    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    st = g.step(args.steps, s.dt, 1, 1, stats=True)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    times = {w: g.kernel_time(w) for w in KERNELS}
+    g.timing(False)
+    cnt, _ = g.neighbors(stride=0)
+    nb_entries = int(cnt.sum())
+    # dominant kernel: largest total time in the timed region
+    dom = max(times, key=lambda w: times[w][0])
+    ms, launches = times[dom]
+    per_launch_s = ms / 1e3 / max(launches, 1)
+    bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries)
+    achieved = bytes_launch / per_launch_s / 1e9
+    value = s.n * args.steps / elapsed
+    out = {
+        "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64 mixed" if args.mode == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.config], "mode": args.mode, "n_atoms": s.n,
+                   "pme_grid": list(s.fft), "ctip_every": 1, "sort_every": 1,
+                   "ns_per_day": args.steps / elapsed * 1.0 * 86400 * 1e-6,
+                   "neighbor_entries": nb_entries, "E_total_last": st["Etotal"]},
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bytes_launch,
+                     "avg_launch_ms": per_launch_s * 1e3, "launches": launches},
+        "kernels_ms": {KERNELS[w]: round(times[w][0], 3) for w in KERNELS},
+        "kernels_launches": {KERNELS[w]: times[w][1] for w in KERNELS},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

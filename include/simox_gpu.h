@@ -1,0 +1,166 @@
+/*
+ * simox_gpu.h - C-ABI of the MI355X (gfx950) implementation of the SIMOX
+ * per-step MD hot path: cell list, EAM density/embedding/pair forces,
+ * real-space Ewald + CTIP short-range terms, smooth-PME reciprocal forces and
+ * potential, CTIP charge equilibration, velocity Verlet.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (SIMOX_OK = 0) and is synchronous at return unless stated otherwise.  On a
+ * nonzero status simox_last_error() returns the text the reference would have
+ * printed before its STOP.  One host thread and one HIP stream per context.
+ *
+ * Host arrays use the reference's conventions: atoms in the caller's order,
+ * species ids 1..5 (Al, Ni, Fe, O, H; datafmt.f90:29-30), 3-vectors stored
+ * atom-major (x[3*i+d], the layout of TYPE PTCL's xx/xv/ff components,
+ * datafmt.f90:49-52), positions in [-L/2, L/2), velocities in Angstrom per
+ * TFM fs, forces in eV/Angstrom, charges in e.
+ *
+ * Scope: Al, Ni, Fe homo-metal pairs, metal-O and O-O pairs (eam.f90:3-35,
+ * 171-328).  Ni/Fe cross pairs (METAL_HETERO, AL_NI) and H return
+ * SIMOX_ERR_UNKNOWN_PAIR.
+ */
+#ifndef SIMOX_GPU_H
+#define SIMOX_GPU_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes; texts follow the reference's STOP messages ---------- */
+#define SIMOX_OK 0
+#define SIMOX_ERR_CELL_OVERFLOW 1 /* "=== Cell particle limit crashed !!! ===" cellsort.f90:75 */
+#define SIMOX_ERR_NB_OVERFLOW 2   /* "=== Memeory error for EAM density ===" force.f90:79,
+                                     "=== neighbor particle limit crashed !!! === " ctip.f90:153 */
+#define SIMOX_ERR_UNKNOWN_PAIR 3  /* "=== Unknown particle pair ===" force.f90:73 */
+#define SIMOX_ERR_BRENT 4         /* "# **** Brent solver limit breached *** " ctip.f90:302 */
+#define SIMOX_ERR_ARG 5           /* invalid argument / geometry (main.f90:191-193) */
+#define SIMOX_ERR_HIP 6           /* HIP runtime error */
+#define SIMOX_ERR_NOT_CONVERGED 7 /* linear charge solve hit its iteration limit */
+
+/* ---- numerical modes ----------------------------------------------------- */
+/* FP64, reproducing the reference's arithmetic including its quirks: FP32-
+   rounded PME spreading weights (force.f90:406, ctip.f90:359) and the
+   nonlinear CG + Brent charge solve with its 1e-3 stop (ctip.f90:167-304). */
+#define SIMOX_MODE_FAITHFUL 0
+/* FP64 kernels with a converged charge solve: projected linear CG with an
+   active-set outer loop on the same charge energy, to a relative residual tol. */
+#define SIMOX_MODE_FP64 1
+/* FP32 pair / PME / sparse arithmetic with FP32 partial sums; FP64 positions,
+   velocities, Verlet update, CG residuals, dot products, neutrality multiplier
+   and energy totals.  Charge solve = iterative refinement: FP32 inner CG, FP64
+   outer residual. */
+#define SIMOX_MODE_MIXED 2
+
+/* ---- potential parameters: TYPE METL / OXID / POTN (datafmt.f90:54-74) --- */
+typedef struct {
+  double r_e, f_e, rho_e, rho_s, alpha, beta, A, B, kappa, lambda;
+  double Fn[4], Fm[4], Fp[4], eta, Fe;
+} simox_metal;
+
+typedef struct {
+  double r_e[5], alph[5], beta[5], A[5], B[5], kapp[5], lamb[5];
+  double F[5][4]; /* F0..F4 */
+  double re[5], rmax[5], fe, gamm, nu;
+} simox_oxid;
+
+typedef struct {
+  double xm, qmin, qmax, chi, J, xi, Z, fafb[5][4];
+} simox_potn;
+
+typedef struct {
+  simox_metal eam[3];
+  simox_oxid ox;
+  simox_potn prm[5];
+} simox_params;
+
+/* POTENTIAL_SETUP (datafmt.f90:142-262): fills the published Zhou-Johnson-
+   Wadley EAM, oxygen EAM and Streitz-Mintmire CTIP parameters. */
+void simox_potential_setup(simox_params* p);
+/* Recomputes the Slater Coulomb-integral coefficients fafb from xi
+   (datafmt.f90:201-220); call after changing any xi. */
+void simox_fafb(simox_params* p);
+
+typedef struct simox_ctx simox_ctx;
+
+/* Context = one system: periodic box, PME grid, cutoff, atom count, mode.
+   Replaces READ_INPUT's geometry set-up (main.f90:137-201): Ncell =
+   INT(box/rcut), Lcell = box/Ncell, Ewald alpha = 3.8/rcut.  params may be
+   NULL (published parameters).  Allocates all device buffers; nothing is
+   allocated by later calls except when a neighbour table must grow. */
+int simox_create(int n, const double box[3], const int fft[3], double rcut, int mode,
+                 const simox_params* params, simox_ctx** out);
+int simox_destroy(simox_ctx* ctx);
+const char* simox_last_error(const simox_ctx* ctx);
+int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
+int simox_set_mode(simox_ctx* ctx, int mode);
+/* Tolerance / iteration cap of the linear charge solve (modes FP64, MIXED). */
+int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
+
+/* Host <-> device state of TYPE PTCL q(:) (datafmt.f90:49-52), caller order.
+   Any pointer may be NULL to leave that component untouched.  set_state with
+   id != NULL also re-sorts the atoms into cells (CELL_SORT). */
+int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double* v,
+                    const double* f, const double* q);
+int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q);
+
+/* CELL_SORT (cellsort.f90:53-80): bins atoms into Ncell cells; within a cell
+   atoms keep ascending caller index.  Replaces cell(:)%Npt/link. */
+int simox_cell_sort(simox_ctx* ctx);
+/* Cell occupancy (Npt per cell, 0-based cell id = ix + iy*Nx + iz*Nx*Ny) and
+   members in link order (caller indices, 0-based); link may be NULL. */
+int simox_get_cells(simox_ctx* ctx, int* npt, int* link, int link_stride);
+
+/* FORCE (force.f90:3-361): overwrites forces with EAM + real-space Ewald +
+   CTIP short-range forces.  energies[4] = Epair, Edir, Ectip, Egroup (eV),
+   already multiplied by kc as sys%Edir/Ectip are (force.f90:250-252). */
+int simox_force(simox_ctx* ctx, double energies[4]);
+/* Per-atom EAM density rho_i (caller order) from the last simox_force. */
+int simox_get_rho(simox_ctx* ctx, double* rho);
+
+/* PME_SELF (force.f90:467-517): Eself of the current charges (penalty 2w,
+   Q4) and the influence function; the k=0 term is zero. */
+int simox_pme_self(simox_ctx* ctx, double* eself);
+/* PME (force.f90:364-464): adds reciprocal-space forces (mean removed) to
+   the forces; *erec = E_rec (eV). */
+int simox_pme(simox_ctx* ctx, double* erec);
+/* PHI_MESH (ctip.f90:336-396) for the current charges, scaled by
+   c_pme = kc*Ng/(pi V) as CTIP_SOLVER scales it (ctip.f90:59,170). */
+int simox_phi(simox_ctx* ctx, double* phi);
+
+/* CTIP_SOLVER (ctip.f90:3-212): re-equilibrates the charges in place and
+   sets *eself as CTIP_SOLVER does (penalty w, Q4).  *iters = CG iterations,
+   *n_eval = gradient (PME + sparse matvec) evaluations.  Mode FAITHFUL runs
+   the reference's nonlinear CG + Brent; FP64/MIXED run the converged solve. */
+int simox_ctip_solve(simox_ctx* ctx, double* eself, int* iters, int* n_eval);
+/* dUdq (ctip.f90:307-333) at the current charges: g = -dU/dq (caller order). */
+int simox_dudq(simox_ctx* ctx, double* g);
+/* Neighbour rows of the CTIP matrix (full shell, caller indices, per-atom
+   order of the build's enumeration): counts[n]; cols[n*stride] (may be NULL). */
+int simox_get_neighbors(simox_ctx* ctx, int* counts, int* cols, int stride);
+
+/* VV_NVE1 (vverlet.f90:74-96) and VV_NVE2 (vverlet.f90:132-157); dt in TFM
+   units; *mv2 = sum m v^2 as sys%mv2. */
+int simox_vv1(simox_ctx* ctx, double dt);
+int simox_vv2(simox_ctx* ctx, double dt, double* mv2);
+
+/* One or more REGULAR NVE steps, device-resident (main.f90:41-61):
+   VV_NVE1 -> [CELL_SORT] -> [CTIP_SOLVER] -> FORCE -> PME -> VV_NVE2.
+   sort_every / ctip_every: run the sort / the charge solve when
+   step % every == 0 (counting from 0 for this call; 0 = never).
+   stats (may be NULL): 8 doubles of the last step =
+   Eself, Edir, Erec, Ectip, Epair, Egroup, mv2, E_total (util.f90:12-15). */
+int simox_step(simox_ctx* ctx, int nsteps, double dt, int sort_every, int ctip_every,
+               double* stats);
+
+/* Per-kernel timing (HIP events recorded on the context stream around each
+   launch while enabled): which = 0 spread, 1 FFT x R2C, 2 FFT y (both
+   directions), 3 FFT z + influence multiply, 4 FFT x C2R, 5 PME force gather,
+   6 dUdq (potential gather + sparse matvec), 7 pair force, 8 EAM density +
+   embedding, 9 CTIP matrix build.  Returns accumulated ms and launch count
+   since the last simox_timing_enable. */
+int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launches);
+int simox_timing_enable(simox_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

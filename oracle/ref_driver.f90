@@ -1,0 +1,125 @@
+! Driver for the reference build (oracle/_ref): links the SIMOX routines that
+! compile from their own sources without the absent FFTE library and runs them
+! on one input record.  TEST INFRASTRUCTURE ONLY: it writes golden vectors for
+! tests/golden and is never part of the product.
+!
+! The reference's own main program (main.f90) is not used: its RNG is unseeded
+! and its config file cannot express the inputs the tests need (main.f90:28,
+! 122-180).  This driver sets Nset/sys exactly as READ_INPUT does
+! (main.f90:137-201) and then calls, in order:
+!   CELL_HIERARCHY, CELL_SORT            cellsort.f90:3-80
+!   POTENTIAL_SETUP                      datafmt.f90:142-262
+!   PME_SELF                             force.f90:467-517
+!   FORCE                                force.f90:3-361
+!   VV_NVE1 / VV_NVE2 (on copies)        vverlet.f90:74-157
+!   dUdq (on matrices read from input)   ctip.f90:307-333
+! PME, PHI_MESH, CTIP_SOLVER and BRENT_SOLVER call FFTE's ZFFT3D and are not
+! linked (link-time optimisation drops them because nothing here calls them).
+!
+! Input  (stream file, native endian):
+!   int32 n, fft(3), nnb; real64 box(3), rcut, dt
+!   int32 id(n); real64 x(3,n), v(3,n), f(3,n), q(n)
+!   int32 nlist(n), list(n,500); real64 asym(n,500), sym(n,500), phi(n)
+!   (nnb = 0 skips the dUdq block)
+! Output (stream file):
+!   int32 ncell(3), npt(ncell_all), link(500,ncell_all), pair(13,ncell_all)
+!   real64 f(3,n), energies(4) = Epair, Edir, Ectip, Egroup, eself, psi(fft)
+!   real64 x1(3,n), v1(3,n) after VV_NVE1, v2(3,n), mv2 after VV_NVE2 (from v1)
+!   int32 npt_new; int32 rdf(5,5,201)
+!   real64 g(n) (if nnb > 0)
+program ref_driver
+use datafmt
+implicit none
+type(amnt) :: nset
+type(stat) :: sys
+type(metl) :: eam(nmetal)
+type(potn) :: param(natom)
+type(oxid) :: oxygn
+type(corr) :: pair
+type(ptcl), allocatable :: q(:), q1(:)
+type(sort), allocatable :: cell(:)
+real(dp), allocatable :: psi(:,:,:), xyz(:,:), vel(:,:), frc(:,:), chg(:)
+real(dp), allocatable :: xjz(:,:), asym(:,:), sym(:,:), phi(:), qmx(:,:), g(:)
+integer(gi), allocatable :: ids(:), list(:,:), nlist(:)
+integer(gi) :: n, fft(3), nnb, i, m
+real(dp) :: box(3), rcut, dt
+character(len=512) :: fin, fout
+
+call get_command_argument(1, fin)
+call get_command_argument(2, fout)
+open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+read(10) n, fft, nnb
+read(10) box, rcut, dt
+allocate(ids(n), xyz(3,n), vel(3,n), frc(3,n), chg(n))
+read(10) ids, xyz, vel, frc, chg
+
+nset%npt_all = n
+nset%box(:) = box(:)
+nset%v = box(1)*box(2)*box(3)
+nset%fft(:) = fft(:)
+sys%box(:) = box(:)
+sys%rcut = rcut
+sys%rcut2 = rcut**2
+sys%a = 3.8d0/rcut
+nset%ncell(:) = int(box(:)/rcut)
+nset%lcell(:) = box(:)/dble(nset%ncell(:))
+nset%ncell_all = nset%ncell(1)*nset%ncell(2)*nset%ncell(3)
+pair%dr = rcut/dble(nrdf)
+pair%v = nset%v
+pair%rdf(:,:,:) = 0
+
+allocate(q(n), q1(n), cell(nset%ncell_all))
+allocate(psi(fft(1),fft(2),fft(3)))
+psi(:,:,:) = 0.0d0
+do i = 1, n
+   q(i)%id = ids(i)
+   q(i)%xx(:) = xyz(:,i)
+   q(i)%xv(:) = vel(:,i)
+   q(i)%ff(:) = frc(:,i)
+   q(i)%q = chg(i)
+end do
+
+call cell_hierarchy(nset, cell)
+call cell_sort(nset, cell, q)
+call potential_setup(eam, param, oxygn)
+call pme_self(nset, sys, param, q, psi)
+
+! integrator on copies, using the input forces
+q1(:) = q(:)
+call vv_nve1(nset, param, q1, dt)
+
+open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+write(11) nset%ncell
+write(11) (cell(m)%npt, m = 1, nset%ncell_all)
+write(11) (cell(m)%link(:), m = 1, nset%ncell_all)
+write(11) (cell(m)%pair(:), m = 1, nset%ncell_all)
+
+call force(nset, eam, oxygn, sys, param, pair, q, cell)
+write(11) (q(i)%ff(:), i = 1, n)
+write(11) sys%epair, sys%edir, sys%ectip, sys%egroup, sys%eself
+write(11) psi
+write(11) (q1(i)%xx(:), i = 1, n)
+write(11) (q1(i)%xv(:), i = 1, n)
+call vv_nve2(nset, param, sys, q1, dt)
+write(11) (q1(i)%xv(:), i = 1, n)
+write(11) sys%mv2
+write(11) pair%npt_new
+write(11) pair%rdf
+
+if (nnb > 0) then
+   allocate(xjz(n,3), asym(n,nnb_max), sym(n,nnb_max), phi(n), qmx(n,2), g(n))
+   allocate(list(n,nnb_max), nlist(n))
+   read(10) nlist, list, asym, sym, phi
+   do i = 1, n
+      xjz(i,1) = param(ids(i))%chi
+      xjz(i,2) = param(ids(i))%j - kc*2.d0*sys%a/sqrtpi
+      xjz(i,3) = param(ids(i))%z
+      qmx(i,1) = param(ids(i))%qmin
+      qmx(i,2) = param(ids(i))%qmax
+   end do
+   call dudq(n, xjz, asym, sym, list, nlist, chg, qmx, phi, g)
+   write(11) g
+end if
+close(10)
+close(11)
+end program ref_driver

@@ -1,0 +1,273 @@
+"""ctypes binding of libsimox_gpu.so (include/simox_gpu.h).
+
+Simox mirrors the reference's external subroutines for the hot path; each
+method cites the routine it replaces.  Arrays follow the reference's
+conventions (caller atom order, species 1..5, 3-vectors as (n, 3) float64).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsimox_gpu.so")
+
+MODE_FAITHFUL, MODE_FP64, MODE_MIXED = 0, 1, 2
+
+# names of the entry points declared in include/simox_gpu.h
+EXPORTS = [
+    "simox_potential_setup", "simox_fafb", "simox_create", "simox_destroy", "simox_last_error",
+    "simox_get_geometry", "simox_set_mode", "simox_set_charge_tol", "simox_set_state",
+    "simox_get_state", "simox_cell_sort", "simox_get_cells", "simox_force", "simox_get_rho",
+    "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
+    "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
+    "simox_timing_enable",
+]
+
+# simox_kernel_timing categories
+T_SPREAD, T_FFT, T_GATHER, T_SPMV, T_FORCE, T_DENSITY = range(6)
+
+
+class SimoxError(RuntimeError):
+    """Raised where the reference would STOP; carries the reference's text."""
+
+    def __init__(self, code, text):
+        super().__init__(text)
+        self.code = code
+
+
+class Metal(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("r_e", "f_e", "rho_e", "rho_s", "alpha", "beta", "A", "B", "kappa", "lambda_")] + [
+        ("Fn", C.c_double * 4), ("Fm", C.c_double * 4), ("Fp", C.c_double * 4),
+        ("eta", C.c_double), ("Fe", C.c_double)]
+
+
+class Oxid(C.Structure):
+    _fields_ = [(n, C.c_double * 5) for n in ("r_e", "alph", "beta", "A", "B", "kapp", "lamb")] + [
+        ("F", (C.c_double * 4) * 5), ("re", C.c_double * 5), ("rmax", C.c_double * 5),
+        ("fe", C.c_double), ("gamm", C.c_double), ("nu", C.c_double)]
+
+
+class Potn(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("xm", "qmin", "qmax", "chi", "J", "xi", "Z")] + [
+        ("fafb", (C.c_double * 4) * 5)]
+
+
+class Params(C.Structure):
+    _fields_ = [("eam", Metal * 3), ("ox", Oxid), ("prm", Potn * 5)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libsimox_gpu.so is not built: run `python -m simox_amd.build`")
+    lib = C.CDLL(path)
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    P = C.POINTER
+    lib.simox_last_error.restype = C.c_char_p
+    lib.simox_last_error.argtypes = [vp]
+    lib.simox_create.argtypes = [i, P(d), P(i), d, i, vp, P(vp)]
+    lib.simox_destroy.argtypes = [vp]
+    lib.simox_get_geometry.argtypes = [vp, P(i), P(d), P(d)]
+    lib.simox_set_mode.argtypes = [vp, i]
+    lib.simox_set_charge_tol.argtypes = [vp, d, i]
+    lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
+    lib.simox_cell_sort.argtypes = [vp]
+    lib.simox_get_cells.argtypes = [vp, vp, vp, i]
+    lib.simox_force.argtypes = [vp, P(d)]
+    lib.simox_get_rho.argtypes = [vp, vp]
+    lib.simox_pme_self.argtypes = [vp, P(d)]
+    lib.simox_pme.argtypes = [vp, P(d)]
+    lib.simox_phi.argtypes = [vp, vp]
+    lib.simox_ctip_solve.argtypes = [vp, P(d), P(i), P(i)]
+    lib.simox_dudq.argtypes = [vp, vp]
+    lib.simox_get_neighbors.argtypes = [vp, vp, vp, i]
+    lib.simox_vv1.argtypes = [vp, d]
+    lib.simox_vv2.argtypes = [vp, d, P(d)]
+    lib.simox_step.argtypes = [vp, i, d, i, i, vp]
+    lib.simox_kernel_timing.argtypes = [vp, i, P(d), P(C.c_longlong)]
+    lib.simox_timing_enable.argtypes = [vp, i]
+    lib.simox_potential_setup.argtypes = [vp]
+    lib.simox_fafb.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def potential_setup():
+    """POTENTIAL_SETUP (datafmt.f90:142-262) -> Params."""
+    p = Params()
+    load_library().simox_potential_setup(C.byref(p))
+    return p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Simox:
+    """One system on the GPU: box, PME grid, cutoff, mode (include/simox_gpu.h)."""
+
+    def __init__(self, n, box, fft, rcut, mode=MODE_FAITHFUL, params=None):
+        self.lib = load_library()
+        self.n = int(n)
+        self.mode = mode
+        box = (C.c_double * 3)(*[float(b) for b in box])
+        fftc = (C.c_int * 3)(*[int(k) for k in fft])
+        h = C.c_void_p()
+        rc = self.lib.simox_create(self.n, box, fftc, float(rcut), int(mode),
+                                   C.byref(params) if params is not None else None, C.byref(h))
+        self.h = h
+        self._check(rc)
+
+    @classmethod
+    def from_system(cls, s, mode=MODE_FAITHFUL, params=None, with_state=True):
+        obj = cls(s.n, s.box, s.fft, s.rcut, mode=mode, params=params)
+        if with_state:
+            obj.set_state(s.ids, s.x, s.v, None, s.q)
+        return obj
+
+    def _check(self, rc):
+        if rc != 0:
+            txt = self.lib.simox_last_error(self.h).decode() if self.h else "error"
+            raise SimoxError(rc, txt)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.simox_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geometry(self):
+        nc = (C.c_int * 3)()
+        lc = (C.c_double * 3)()
+        a = C.c_double()
+        self._check(self.lib.simox_get_geometry(self.h, nc, lc, C.byref(a)))
+        return tuple(nc), tuple(lc), a.value
+
+    def set_charge_tol(self, tol, maxit=1000):
+        self._check(self.lib.simox_set_charge_tol(self.h, float(tol), int(maxit)))
+
+    # TYPE PTCL q(:) in / out
+    def set_state(self, ids=None, x=None, v=None, f=None, q=None):
+        n = self.n
+        idc = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        xa, va, fa = _f64(x, (n, 3)), _f64(v, (n, 3)), _f64(f, (n, 3))
+        qa = _f64(q, (n,))
+        self._check(self.lib.simox_set_state(self.h, _ptr(idc), _ptr(xa), _ptr(va), _ptr(fa),
+                                             _ptr(qa)))
+
+    def get_state(self):
+        n = self.n
+        x, v, f, q = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        self._check(self.lib.simox_get_state(self.h, _ptr(x), _ptr(v), _ptr(f), _ptr(q)))
+        return x, v, f, q
+
+    # CELL_SORT (cellsort.f90:53-80)
+    def cell_sort(self):
+        self._check(self.lib.simox_cell_sort(self.h))
+
+    def get_cells(self, link_stride=500):
+        nc, _, _ = self.geometry()
+        ncall = int(np.prod(nc))
+        npt = np.zeros(ncall, dtype=np.int32)
+        link = np.full((ncall, link_stride), -1, dtype=np.int32)
+        self._check(self.lib.simox_get_cells(self.h, _ptr(npt), _ptr(link), link_stride))
+        return npt, link
+
+    # FORCE (force.f90:3-361)
+    def force(self):
+        e = (C.c_double * 4)()
+        self._check(self.lib.simox_force(self.h, e))
+        return dict(Epair=e[0], Edir=e[1], Ectip=e[2], Egroup=e[3])
+
+    def rho(self):
+        r = np.zeros(self.n)
+        self._check(self.lib.simox_get_rho(self.h, _ptr(r)))
+        return r
+
+    # PME_SELF (force.f90:467-517)
+    def pme_self(self):
+        e = C.c_double()
+        self._check(self.lib.simox_pme_self(self.h, C.byref(e)))
+        return e.value
+
+    # PME (force.f90:364-464)
+    def pme(self):
+        e = C.c_double()
+        self._check(self.lib.simox_pme(self.h, C.byref(e)))
+        return e.value
+
+    # PHI_MESH x c_pme (ctip.f90:336-396, 170)
+    def phi(self):
+        p = np.zeros(self.n)
+        self._check(self.lib.simox_phi(self.h, _ptr(p)))
+        return p
+
+    # CTIP_SOLVER (ctip.f90:3-212)
+    def ctip_solve(self):
+        e, it, ne = C.c_double(), C.c_int(), C.c_int()
+        self._check(self.lib.simox_ctip_solve(self.h, C.byref(e), C.byref(it), C.byref(ne)))
+        return dict(Eself=e.value, iters=it.value, n_eval=ne.value)
+
+    # dUdq (ctip.f90:307-333)
+    def dudq(self):
+        g = np.zeros(self.n)
+        self._check(self.lib.simox_dudq(self.h, _ptr(g)))
+        return g
+
+    def neighbors(self, stride=500):
+        """Counts and (stride > 0) column lists of the CTIP matrix rows."""
+        cnt = np.zeros(self.n, dtype=np.int32)
+        cols = np.full((self.n, stride), -1, dtype=np.int32) if stride > 0 else None
+        self._check(self.lib.simox_get_neighbors(self.h, _ptr(cnt), _ptr(cols), max(stride, 0)))
+        return cnt, cols
+
+    # VV_NVE1 / VV_NVE2 (vverlet.f90:74-157)
+    def vv1(self, dt):
+        self._check(self.lib.simox_vv1(self.h, float(dt)))
+
+    def vv2(self, dt):
+        m = C.c_double()
+        self._check(self.lib.simox_vv2(self.h, float(dt), C.byref(m)))
+        return m.value
+
+    # REGULAR NVE loop body (main.f90:41-61)
+    def step(self, nsteps, dt, sort_every=1, ctip_every=1, stats=True):
+        st = np.zeros(8) if stats else None
+        self._check(self.lib.simox_step(self.h, int(nsteps), float(dt), int(sort_every),
+                                        int(ctip_every), _ptr(st)))
+        if st is None:
+            return None
+        keys = ["Eself", "Edir", "Erec", "Ectip", "Epair", "Egroup", "mv2", "Etotal"]
+        return dict(zip(keys, st.tolist()))
+
+    def timing(self, on=True):
+        self._check(self.lib.simox_timing_enable(self.h, 1 if on else 0))
+
+    def kernel_time(self, which):
+        ms = C.c_double()
+        cnt = C.c_longlong()
+        self._check(self.lib.simox_kernel_timing(self.h, int(which), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,345 @@
+// Short-range kernels: cell binning, EAM density + embedding, pair forces,
+// CTIP sparse matrix build.  One thread per atom in cell-sorted order; each
+// thread walks the full 27-cell shell of its own cell, so every pair is
+// evaluated from both sides and no write conflicts arise (deterministic sums
+// without atomics).  With Ncell = 2 along an axis the shell visits the
+// neighbour cell twice, which reproduces the reference's duplicated pairs
+// (main.f90:191 guard, quirk Q8).
+#pragma once
+#include "common.hpp"
+
+namespace sx {
+
+// ---------------------------------------------------------------------------
+// Radial forms (eam.f90).  T = double for the FP64 paths, float for MIXED.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T fexp(T v) { return exp(v); }
+template <>
+__device__ inline float fexp<float>(float v) { return __expf(v); }
+
+template <class T>
+__device__ inline void dens_eval(const DensCoef& c, T r, T& f, T& dfr) {
+  const T re = T(c.re), l = T(c.l), b = T(c.b), fe = T(c.fe);
+  T r_re = r / re, x = r_re - l;
+  T x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+  T l20 = x16 * x4, l19 = x16 * x2 * x;
+  T eb = fexp<T>(-b * (r_re - T(1)));
+  T den = T(1) + l20;
+  f = fe * (eb / den);
+  dfr = fe * (eb * (-b * den - T(20) * l19) / re / (den * den)) / r;
+}
+
+template <class T>
+__device__ inline void phi_eval(const PhiCoef& c, T r, T& phi, T& df) {
+  const T re = T(c.re);
+  T r_re = r / re, r1 = r_re - T(1);
+  T xk = r_re - T(c.k), xl = r_re - T(c.l);
+  T k2 = xk * xk, k4 = k2 * k2, k8 = k4 * k4, k16 = k8 * k8;
+  T l2 = xl * xl, l4 = l2 * l2, l8 = l4 * l4, l16 = l8 * l8;
+  T k20 = k16 * k4, k19 = k16 * k2 * xk, l20 = l16 * l4, l19 = l16 * l2 * xl;
+  T ea = fexp<T>(-T(c.a) * r1), eb = fexp<T>(-T(c.b) * r1);
+  T dk = T(1) + k20, dl = T(1) + l20;
+  T dphidr = T(c.A) * ea * (-T(c.a) * dk - T(20) * k19) / re / (dk * dk) -
+             T(c.B) * eb * (-T(c.b) * dl - T(20) * l19) / re / (dl * dl);
+  phi = T(c.A) * ea / dk - T(c.B) * eb / dl;
+  df = -dphidr / r;
+}
+
+// CTIP short-range Slater terms of FORCE (force.f90:101-140).  Same-species
+// fifj lacks the 1/r factor (quirk Q2) and is reproduced as written.
+template <class T>
+__device__ inline void ctip_force_terms(const PairCoef& pc, T r, T r2, T qi, T qj, T& df,
+                                        T& e) {
+  const T xi_i = T(pc.xi_i), xi_j = T(pc.xi_j), Zi = T(pc.Z_i), Zj = T(pc.Z_j);
+  T jfi, ifj, djfidr, difjdr, fifj, dfifjdr;
+  T ir = T(1) / r;
+  if (pc.same) {
+    T xir = T(2) * xi_i * r, ei = fexp<T>(-xir);
+    jfi = -ei * (xi_i + ir);
+    ifj = jfi;
+    djfidr = ei * (T(2) * (xi_i * xi_i) + (T(1) + xir) / r2);
+    difjdr = djfidr;
+    const T c1 = T(pc.c[1]), c2 = T(pc.c[2]), c3 = T(pc.c[3]);
+    fifj = -ei * (T(1) + c1 * r + c2 * r2 + c3 * r2 * r);
+    dfifjdr = ei * ((xir + T(1)) * (T(1) / r2 + c1 * ir + c2 + c3 * r) -
+                    (c1 * ir + c2 * T(2) + c3 * T(3) * r));
+  } else {
+    T xir_i = T(2) * xi_i * r, ei = fexp<T>(-xir_i);
+    T xir_j = T(2) * xi_j * r, ej = fexp<T>(-xir_j);
+    jfi = -ei * (xi_i + ir);
+    ifj = -ej * (xi_j + ir);
+    djfidr = ei * (T(2) * (xi_i * xi_i) + (T(1) + xir_i) / r2);
+    difjdr = ej * (T(2) * (xi_j * xi_j) + (T(1) + xir_j) / r2);
+    const T c0 = T(pc.c[0]), c1 = T(pc.c[1]), c2 = T(pc.c[2]), c3 = T(pc.c[3]);
+    fifj = -c0 * ei - c1 * ej - c2 * ei / r - c3 * ej / r;
+    dfifjdr = c0 * ei * T(2) * xi_i + c1 * ej * T(2) * xi_j + c2 * ei * (T(1) + xir_i) / r2 +
+              c3 * ej * (T(1) + xir_j) / r2;
+  }
+  df = T(KC) * ((qi * Zj + qj * Zi - qi * qj) * dfifjdr - qi * Zj * djfidr - qj * Zi * difjdr) / r;
+  e = qi * Zj * (jfi - fifj) + qj * Zi * (ifj - fifj) + qi * qj * fifj;
+}
+
+// CTIP solver matrix entries for row i (ctip.f90:81-110): sym_ij and asym_ij.
+// Same-species [fi|fj] is the solver's own variant (quirk Q3).
+template <class T>
+__device__ inline void ctip_matrix_terms(const PairCoef& pc, T r, T alpha, T& sym, T& asym) {
+  const T xi_i = T(pc.xi_i), xi_j = T(pc.xi_j);
+  T ej = fexp<T>(T(-2) * xi_j * r), ej_r = ej / r;
+  T afb_i, fifj;
+  if (pc.same) {
+    afb_i = -xi_j * ej - ej_r;
+    T xir = xi_i * r;
+    fifj = -ej * (T(1) + xir * (T(pc.c[1]) + xir * (T(pc.c[2]) + xir * T(pc.c[3])))) / r;
+  } else {
+    T ei = fexp<T>(T(-2) * xi_i * r), ei_r = ei / r;
+    afb_i = -xi_i * ei - ei_r;
+    fifj = -ei * (T(pc.c[0]) + T(pc.c[2]) / r) - ej * (T(pc.c[1]) + T(pc.c[3]) / r);
+  }
+  asym = T(KC) * (afb_i - fifj);
+  sym = T(KC) * (fifj + erfc(alpha * r) / r);
+}
+
+// ---------------------------------------------------------------------------
+// Embedding function F(rho) and F'(rho) (force.f90:256-348)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T poly3(const double* c, T t) {
+  T s = T(0), tp = T(1);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) { s += T(c[j]) * tp; tp *= t; }
+  return s;
+}
+template <class T>
+__device__ inline T dpoly3(const double* c, T t, T scale) {
+  T s = T(0), tp = T(1);
+#pragma unroll
+  for (int j = 1; j < 4; ++j) { s += T(c[j]) * T(j) * tp / scale; tp *= t; }
+  return s;
+}
+
+template <class T>
+__device__ inline void embed(const EmbedCoef& e, T rho, T& F, T& dF) {
+  F = T(0); dF = T(0);
+  if (e.metal) {
+    if (rho < T(e.rho_n)) {
+      T t = rho / T(e.rho_n) - T(1);
+      F = poly3<T>(e.Fn, t); dF = dpoly3<T>(e.Fn, t, T(e.rho_n));
+    } else if (rho < T(e.rho_e)) {
+      T t = rho / T(e.rho_e) - T(1);
+      F = poly3<T>(e.Fm, t); dF = dpoly3<T>(e.Fm, t, T(e.rho_e));
+    } else if (rho < T(e.rho_0)) {
+      T t = rho / T(e.rho_e) - T(1);
+      F = poly3<T>(e.Fp, t); dF = dpoly3<T>(e.Fp, t, T(e.rho_e));
+    } else if (rho > T(1.0e-6)) {
+      T t = pow(rho / T(e.rho_s), T(e.eta));
+      T lt = log(t);
+      F = T(e.Fe) * (T(1) - lt) * t;
+      dF = -T(e.Fe) * T(e.eta) * lt * t / rho;
+    }
+  } else {
+    int k = 4;
+    for (int s = 0; s < 4; ++s)
+      if (rho < T(e.rmax[s])) { k = s; break; }
+    T t = rho / T(e.re[k]) - T(1);
+    F = poly3<T>(e.F[k], t); dF = dpoly3<T>(e.F[k], t, T(e.re[k]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Full-shell neighbour walk over the 27 cells around atom i's cell.
+// ---------------------------------------------------------------------------
+template <class Fn>
+__device__ inline void for_each_neighbor(const Geom& g, const int* __restrict__ cell_start, int i,
+                                         int ci, Fn&& fn) {
+  const int cx = ci % g.nc[0];
+  const int t = ci / g.nc[0];
+  const int cy = t % g.nc[1];
+  const int cz = t / g.nc[1];
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int zz = cell_wrap(cz + dz, g.nc[2]);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int yy = cell_wrap(cy + dy, g.nc[1]);
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = cell_wrap(cx + dx, g.nc[0]);
+        const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
+        const int j0 = cell_start[c], j1 = cell_start[c + 1];
+        for (int j = j0; j < j1; ++j) {
+          if (j == i) continue;
+          fn(j);
+        }
+      }
+    }
+  }
+}
+
+#define SX_ERR_SET(ptr, code) atomicCAS((ptr), 0, (code))
+
+// K2 + K3: EAM density rho_i = sum_j f_j(r) and the embedding F, F' per atom.
+// egroup_part[blockIdx.x] = sum of F over the block.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_density(Geom g, const Tables* __restrict__ tb,
+                                                   const double* __restrict__ x,
+                                                   const int* __restrict__ id,
+                                                   const int* __restrict__ cell_of,
+                                                   const int* __restrict__ cell_start,
+                                                   double* __restrict__ rho_out,
+                                                   T* __restrict__ dF_out,
+                                                   double* __restrict__ egroup_part,
+                                                   int* __restrict__ err) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double eg[1] = {0.0};
+  if (i < g.n) {
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    const int si = id[i];
+    T rho = T(0);
+    int nnb = 0;
+    bool bad = si < 1 || si > NSPEC;
+    for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
+      double rx[3], r2;
+      min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
+      if (r2 < g.rcut2) {
+        const int sj = id[j];
+        ++nnb;
+        if (bad || sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid) { bad = true; return; }
+        T r = T(sqrt(r2));
+        T f, dfr;
+        dens_eval<T>(tb->pc[pidx(si, sj)].dj, r, f, dfr);
+        rho += f;
+      }
+    });
+    if (bad) SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR);
+    if (nnb > NNB_MAX) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
+    T F = T(0), dF = T(0);
+    if (!bad) embed<T>(tb->ec[si - 1], rho, F, dF);
+    rho_out[i] = double(rho);
+    dF_out[i] = dF;
+    eg[0] = double(F);
+  }
+  block_sum<1>(eg, lds);
+  if (threadIdx.x == 0) egroup_part[blockIdx.x] = eg[0];
+}
+
+// K4: EAM pair + embedding forces, real-space Ewald, CTIP short-range terms.
+// Overwrites f (force.f90:28); e_part[3*blockIdx.x + {0,1,2}] = half-pair sums of
+// phi, q_i q_j erfc(ar)/r and the CTIP energy (FORCE's Epair, Edir/kc, Ectip/kc).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restrict__ tb,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ q,
+                                                 const int* __restrict__ id,
+                                                 const int* __restrict__ cell_of,
+                                                 const int* __restrict__ cell_start,
+                                                 const T* __restrict__ dF,
+                                                 double* __restrict__ f,
+                                                 double* __restrict__ e_part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double es[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    const int si = id[i];
+    const T qi = T(q[i]);
+    const T dFi = dF[i];
+    const T a = T(g.alpha), a2 = a * a;
+    T fx = T(0), fy = T(0), fz = T(0);
+    T ep = T(0), ed = T(0), ec = T(0);
+    if (si >= 1 && si <= NSPEC) {
+      for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
+        double rxd[3], r2d;
+        min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rxd, r2d);
+        if (r2d < g.rcut2) {
+          const int sj = id[j];
+          if (sj < 1 || sj > NSPEC) return;
+          const PairCoef& pc = tb->pc[pidx(si, sj)];
+          if (!pc.valid) return;
+          const T r2 = T(r2d);
+          const T r = T(sqrt(r2d));
+          const T qj = T(q[j]);
+          T fi_, dfi, fj_, dfj, phi, dfp;
+          dens_eval<T>(pc.di, r, fi_, dfi);
+          dens_eval<T>(pc.dj, r, fj_, dfj);
+          phi_eval<T>(pc.ph, r, phi, dfp);
+          // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356)
+          T s = dfp - dFi * dfj - dF[j] * dfi;
+          // real-space Ewald (force.f90:94-96)
+          T ar = a * r;
+          T tmp = erfc(ar);
+          s += T(KC) * qi * qj * (T(2) * fexp<T>(-a2 * r2) * a * r / T(SQRTPI) + tmp) / r2 / r;
+          ed += qi * qj * tmp / r;
+          // CTIP Slater terms (force.f90:101-140)
+          T dfc, e;
+          ctip_force_terms<T>(pc, r, r2, qi, qj, dfc, e);
+          s += dfc;
+          ec += e;
+          ep += phi;
+          fx += T(rxd[0]) * s;
+          fy += T(rxd[1]) * s;
+          fz += T(rxd[2]) * s;
+        }
+      });
+    }
+    f[3 * i] = double(fx);
+    f[3 * i + 1] = double(fy);
+    f[3 * i + 2] = double(fz);
+    es[0] = 0.5 * double(ep);
+    es[1] = 0.5 * double(ed);
+    es[2] = 0.5 * double(ec);
+  }
+  block_sum<3>(es, lds);
+  if (threadIdx.x == 0) {
+    e_part[3 * blockIdx.x] = es[0];
+    e_part[3 * blockIdx.x + 1] = es[1];
+    e_part[3 * blockIdx.x + 2] = es[2];
+  }
+}
+
+// K5: CTIP sparse matrix, ELL layout stored column-major (entry k of row i at
+// k*n + i, coalesced across a wave).  Only sym is stored; the asym part enters
+// the solve only through b_i = sum_j Z_j asym_ij, which is constant for a
+// whole solve (the reference recomputes it per gradient, ctip.f90:327).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __restrict__ tb,
+                                                       const double* __restrict__ x,
+                                                       const int* __restrict__ id,
+                                                       const int* __restrict__ cell_of,
+                                                       const int* __restrict__ cell_start,
+                                                       int* __restrict__ cnt,
+                                                       int* __restrict__ col,
+                                                       T* __restrict__ val,
+                                                       double* __restrict__ b, int cap,
+                                                       int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+  const int si = id[i];
+  const T alpha = T(g.alpha);
+  int k = 0;
+  double bi = 0.0;
+  const size_t n = size_t(g.n);
+  if (si < 1 || si > NSPEC) { SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR); cnt[i] = 0; b[i] = 0; return; }
+  for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
+    double rx[3], r2;
+    min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
+    if (r2 < g.rcut2) {
+      const int sj = id[j];
+      if (sj < 1 || sj > NSPEC) { SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR); return; }
+      const PairCoef& pc = tb->pc[pidx(si, sj)];
+      T r = T(sqrt(r2));
+      T sym, asym;
+      ctip_matrix_terms<T>(pc, r, alpha, sym, asym);
+      bi += double(T(pc.Z_j) * asym);
+      if (k < cap) {
+        col[size_t(k) * n + i] = j;
+        val[size_t(k) * n + i] = sym;
+      }
+      ++k;
+    }
+  });
+  if (k > cap) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
+  cnt[i] = k < cap ? k : cap;
+  b[i] = bi;
+}
+
+}  // namespace sx

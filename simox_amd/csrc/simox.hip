@@ -1,0 +1,1164 @@
+// C-ABI and host orchestration of the SIMOX hot path on MI355X (gfx950).
+// Kernels live in the headers included below; this file owns the device
+// state (one context = one system), the launch sequences, the charge-solver
+// control logic and the error mapping to the reference's STOP texts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cells.hpp"
+#include "common.hpp"
+#include "pair.hpp"
+#include "pme.hpp"
+#include "solve.hpp"
+
+using namespace sx;
+
+namespace {
+
+const char* status_text(int code, int where_ctip) {
+  switch (code) {
+    case SIMOX_ERR_CELL_OVERFLOW: return "=== Cell particle limit crashed !!! ===";
+    case SIMOX_ERR_NB_OVERFLOW:
+      return where_ctip ? "=== neighbor particle limit crashed !!! === "
+                        : "=== Memeory error for EAM density ===";
+    case SIMOX_ERR_UNKNOWN_PAIR: return "=== Unknown particle pair ===";
+    case SIMOX_ERR_BRENT: return "# **** Brent solver limit breached *** ";
+    default: return "unknown error";
+  }
+}
+
+int grid_blocks(long long n, int b = BLOCK) { return int((n + b - 1) / b); }
+
+bool make_plan(int L, FftPlan* pl) {
+  pl->L = L;
+  pl->nst = 0;
+  int m = L;
+  while (m % 4 == 0 && m > 1) { pl->rad[pl->nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { pl->rad[pl->nst++] = 2; m /= 2; }
+  while (m % 3 == 0 && m > 1) { pl->rad[pl->nst++] = 3; m /= 3; }
+  while (m % 5 == 0 && m > 1) { pl->rad[pl->nst++] = 5; m /= 5; }
+  return m == 1 && pl->nst <= 16;
+}
+
+struct TimedLaunch { int which; hipEvent_t a, b; };
+
+}  // namespace
+
+struct simox_ctx {
+  int n = 0;
+  int mode = SIMOX_MODE_FAITHFUL;
+  Geom g{};
+  double V = 0.0, c_pme = 0.0, rcut = 0.0;
+  simox_params prm{};
+  Tables htab{};
+  Tables* dtab = nullptr;
+  hipStream_t st = nullptr;
+  // atom state in cell-sorted order
+  double *x = nullptr, *v = nullptr, *f = nullptr, *q = nullptr;
+  int *id = nullptr, *perm = nullptr, *cell = nullptr;
+  double *x2 = nullptr, *v2 = nullptr, *f2 = nullptr, *q2 = nullptr;
+  int *id2 = nullptr, *perm2 = nullptr, *cell2 = nullptr;
+  int *count = nullptr, *start = nullptr, *fill = nullptr, *slot = nullptr, *order = nullptr;
+  double *rho = nullptr, *dF64 = nullptr, *frec = nullptr;
+  float* dF32 = nullptr;
+  // PME
+  long long* gfix = nullptr;
+  double* greal64 = nullptr;
+  float* greal32 = nullptr;
+  double2* spec64 = nullptr;
+  float2* spec32 = nullptr;
+  double* G64 = nullptr;
+  float* G32 = nullptr;
+  double2* tw64[3] = {nullptr, nullptr, nullptr};
+  float2* tw32[3] = {nullptr, nullptr, nullptr};
+  FftPlan pl[3]{};
+  int M64[3] = {0, 0, 0}, M32[3] = {0, 0, 0}, stride[3] = {0, 0, 0};
+  // CTIP
+  int cap = NNB_MAX;
+  int* nb_cnt = nullptr;
+  int* nb_col = nullptr;
+  double* nb_val64 = nullptr;
+  float* nb_val32 = nullptr;
+  double* bvec = nullptr;
+  double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
+  bool matrix_valid = false;
+  // reductions / scalars
+  double* part = nullptr;
+  size_t part_cap = 0;
+  double* dscal = nullptr;
+  double* hscal = nullptr;
+  int* derr = nullptr;
+  int* herr = nullptr;
+  // host staging
+  double* hstage = nullptr;
+  size_t hstage_cap = 0;
+  double* dstage = nullptr;
+  // solver settings
+  double tol = 1e-10;
+  int maxit = 1000;
+  // energies of the last FORCE/PME/CTIP (eV)
+  double Epair = 0, Edir = 0, Ectip = 0, Egroup = 0, Erec = 0, Eself = 0, mv2 = 0;
+  bool have_state = false;
+  std::string err;
+  // timing
+  bool timing = false;
+  std::vector<TimedLaunch> tl;
+  double tms[16] = {0};
+  long long tcnt[16] = {0};
+};
+
+namespace {
+
+#define HIPCHK(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ctx->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                 __FILE__ + ":" + std::to_string(__LINE__);                   \
+      return SIMOX_ERR_HIP;                                                   \
+    }                                                                         \
+  } while (0)
+
+#define CHK(x)               \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_) return rc_;     \
+  } while (0)
+
+template <class T>
+int dalloc(simox_ctx* ctx, T** p, size_t count) {
+  HIPCHK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+  return 0;
+}
+
+void tstart(simox_ctx* ctx, int which, hipEvent_t* a) {
+  if (!ctx->timing) return;
+  (void)hipEventCreate(a);
+  (void)hipEventRecord(*a, ctx->st);
+  (void)which;
+}
+void tstop(simox_ctx* ctx, int which, hipEvent_t a) {
+  if (!ctx->timing) return;
+  hipEvent_t b;
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(b, ctx->st);
+  ctx->tl.push_back({which, a, b});
+}
+void tresolve(simox_ctx* ctx) {
+  for (auto& t : ctx->tl) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(t.b);
+    (void)hipEventElapsedTime(&ms, t.a, t.b);
+    ctx->tms[t.which] += ms;
+    ctx->tcnt[t.which] += 1;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  ctx->tl.clear();
+}
+
+bool use32(const simox_ctx* ctx) { return ctx->mode == SIMOX_MODE_MIXED; }
+
+void build_tables(simox_ctx* ctx) {
+  const simox_params& p = ctx->prm;
+  Tables& t = ctx->htab;
+  memset(&t, 0, sizeof t);
+  const double kc_const = KC * 2.0 * ctx->g.alpha / SQRTPI;  // ctip.f90:31
+  for (int s = 1; s <= NSPEC; ++s) {
+    SpecCoef& sc = t.sc[s - 1];
+    const simox_potn& pp = p.prm[s - 1];
+    sc.chi = pp.chi; sc.J = pp.J; sc.Jp = pp.J - kc_const; sc.Z = pp.Z;
+    sc.qmin = pp.qmin; sc.qmax = pp.qmax; sc.xm = pp.xm; sc.xi = pp.xi;
+    EmbedCoef& ec = t.ec[s - 1];
+    if (s < 4) {
+      const simox_metal& m = p.eam[s - 1];
+      ec.metal = 1;
+      ec.rho_e = m.rho_e; ec.rho_s = m.rho_s; ec.rho_n = 0.85 * m.rho_e; ec.rho_0 = 1.15 * m.rho_e;
+      ec.eta = m.eta; ec.Fe = m.Fe;
+      for (int k = 0; k < 4; ++k) { ec.Fn[k] = m.Fn[k]; ec.Fm[k] = m.Fm[k]; ec.Fp[k] = m.Fp[k]; }
+    } else {
+      ec.metal = 0;
+      for (int k = 0; k < 5; ++k) {
+        ec.re[k] = p.ox.re[k]; ec.rmax[k] = p.ox.rmax[k];
+        for (int j = 0; j < 4; ++j) ec.F[k][j] = p.ox.F[k][j];
+      }
+    }
+  }
+  auto metal_dens = [&](int s) {
+    const simox_metal& m = p.eam[s - 1];
+    return DensCoef{m.r_e, m.f_e, m.beta, m.lambda};
+  };
+  auto oxy_dens = [&](int partner) {  // r_e of the partner row (Q15)
+    return DensCoef{p.ox.r_e[partner - 1], p.ox.fe, p.ox.gamm, p.ox.nu};
+  };
+  auto oxy_phi = [&](int row) {
+    const int r = row - 1;
+    return PhiCoef{p.ox.r_e[r], p.ox.alph[r], p.ox.beta[r], p.ox.A[r], p.ox.B[r], p.ox.kapp[r],
+                   p.ox.lamb[r]};
+  };
+  for (int si = 1; si <= NSPEC; ++si)
+    for (int sj = 1; sj <= NSPEC; ++sj) {
+      PairCoef& pc = t.pc[pidx(si, sj)];
+      pc.valid = 1;
+      pc.same = si == sj;
+      pc.xi_i = p.prm[si - 1].xi; pc.xi_j = p.prm[sj - 1].xi;
+      pc.Z_i = p.prm[si - 1].Z; pc.Z_j = p.prm[sj - 1].Z;
+      for (int k = 0; k < 4; ++k) pc.c[k] = p.prm[si - 1].fafb[sj - 1][k];
+      if (si < 4 && sj < 4) {
+        if (si != sj) { pc.valid = 0; continue; }  // METAL_HETERO / AL_NI: out of scope
+        const simox_metal& m = p.eam[si - 1];
+        pc.di = metal_dens(si);
+        pc.dj = pc.di;
+        pc.ph = PhiCoef{m.r_e, m.alpha, m.beta, m.A, m.B, m.kappa, m.lambda};
+      } else if (si < 4 && sj == 4) {  // METAL_OXYGEN
+        pc.di = metal_dens(si);
+        pc.dj = oxy_dens(si);
+        pc.ph = oxy_phi(si);
+      } else if (si == 4 && sj < 4) {  // OXYGEN_METAL
+        pc.di = oxy_dens(sj);
+        pc.dj = metal_dens(sj);
+        pc.ph = oxy_phi(sj);
+      } else {  // OXYGEN_ONLY
+        pc.di = oxy_dens(4);
+        pc.dj = pc.di;
+        pc.ph = oxy_phi(4);
+      }
+    }
+}
+
+int ensure_part(simox_ctx* ctx, size_t count) {
+  if (count <= ctx->part_cap) return 0;
+  if (ctx->part) HIPCHK(hipFree(ctx->part));
+  ctx->part_cap = count;
+  return dalloc(ctx, &ctx->part, count);
+}
+
+int ensure_stage(simox_ctx* ctx, size_t doubles) {
+  if (doubles <= ctx->hstage_cap) return 0;
+  if (ctx->hstage) HIPCHK(hipHostFree(ctx->hstage));
+  if (ctx->dstage) HIPCHK(hipFree(ctx->dstage));
+  HIPCHK(hipHostMalloc((void**)&ctx->hstage, doubles * sizeof(double), hipHostMallocDefault));
+  ctx->hstage_cap = doubles;
+  return dalloc(ctx, &ctx->dstage, doubles);
+}
+
+// Reduce nv interleaved partial values into dscal[off .. off+nv).
+int reduce(simox_ctx* ctx, int nparts, int nv, int maxmask, int off) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nparts, nv,
+                     maxmask, ctx->dscal + off);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Copy dscal[off .. off+nv) to the host (synchronises the stream).
+int fetch(simox_ctx* ctx, int off, int nv, double* out) {
+  HIPCHK(hipMemcpyAsync(ctx->hscal + off, ctx->dscal + off, nv * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  for (int k = 0; k < nv; ++k) out[k] = ctx->hscal[off + k];
+  return 0;
+}
+
+int check_err(simox_ctx* ctx, int where_ctip) {
+  HIPCHK(hipMemcpyAsync(ctx->herr, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const int e = *ctx->herr;
+  if (e) {
+    ctx->err = status_text(e, where_ctip);
+    HIPCHK(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return e;
+  }
+  return 0;
+}
+
+__global__ void k_fill_int(int n, int v, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_iota(int n, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+// dst[p*w + k] = src[perm[p]*w + k]  (caller order -> sorted order)
+__global__ void k_from_caller(int n, int w, const int* __restrict__ perm,
+                              const double* __restrict__ src, double* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int c = perm[p];
+  for (int k = 0; k < w; ++k) dst[size_t(p) * w + k] = src[size_t(c) * w + k];
+}
+// dst[perm[p]*w + k] = src[p*w + k]  (sorted order -> caller order)
+__global__ void k_to_caller(int n, int w, const int* __restrict__ perm,
+                            const double* __restrict__ src, double* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int c = perm[p];
+  for (int k = 0; k < w; ++k) dst[size_t(c) * w + k] = src[size_t(p) * w + k];
+}
+__global__ void k_to_caller_int(int n, const int* __restrict__ perm, const int* __restrict__ src,
+                                int* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  dst[perm[p]] = src[p];
+}
+__global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+__global__ void k_int_to_double(int n, const int* __restrict__ a, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// K1: cell sort
+// ---------------------------------------------------------------------------
+int do_cell_sort(simox_ctx* ctx) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(ctx->g.ncall)), dim3(BLOCK), 0, ctx->st,
+                     ctx->g.ncall, 0, ctx->count);
+  hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, ctx->cell,
+                     ctx->count);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, ctx->st, ctx->g.ncall, ctx->count,
+                     ctx->start, ctx->fill, ctx->derr);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->cell, ctx->start,
+                     ctx->fill, ctx->slot);
+  hipLaunchKernelGGL(k_cell_order, dim3(ctx->g.ncall), dim3(BLOCK), 0, ctx->st, ctx->start,
+                     ctx->slot, ctx->perm, ctx->order);
+  hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order, ctx->cell,
+                     ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->x2, ctx->v2,
+                     ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2);
+  HIPCHK(hipGetLastError());
+  std::swap(ctx->x, ctx->x2); std::swap(ctx->v, ctx->v2); std::swap(ctx->f, ctx->f2);
+  std::swap(ctx->q, ctx->q2); std::swap(ctx->id, ctx->id2); std::swap(ctx->perm, ctx->perm2);
+  std::swap(ctx->cell, ctx->cell2);
+  ctx->matrix_valid = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// K2-K4: FORCE
+// ---------------------------------------------------------------------------
+int do_force(simox_ctx* ctx, bool fetch_energies) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  CHK(ensure_part(ctx, size_t(nb) * 4));
+  hipEvent_t ev{};
+  tstart(ctx, 8, &ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->rho, ctx->dF32, ctx->part,
+                       ctx->derr);
+  else
+    hipLaunchKernelGGL(k_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->rho, ctx->dF64, ctx->part,
+                       ctx->derr);
+  tstop(ctx, 8, ev);
+  CHK(reduce(ctx, nb, 1, 0, 0));
+  tstart(ctx, 7, &ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->q, ctx->id, ctx->cell, ctx->start, ctx->dF32, ctx->f,
+                       ctx->part + nb);
+  else
+    hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->q, ctx->id, ctx->cell, ctx->start, ctx->dF64, ctx->f,
+                       ctx->part + nb);
+  tstop(ctx, 7, ev);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part + nb, nb, 3, 0,
+                     ctx->dscal + 1);
+  HIPCHK(hipGetLastError());
+  if (fetch_energies) {
+    double e[4];
+    CHK(fetch(ctx, 0, 4, e));
+    ctx->Egroup = e[0];
+    ctx->Epair = e[1];
+    ctx->Edir = e[2] * KC;
+    ctx->Ectip = e[3] * KC;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// K6-K8: one PME convolution: spread(q or q + step p) -> ... -> real grid
+// ---------------------------------------------------------------------------
+template <class C>
+int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const int* M,
+              const decltype(C{}.x)* G, decltype(C{}.x)* greal) {
+  const Geom& g = ctx->g;
+  const int H = g.H, Ky = g.K[1], Kz = g.K[2];
+  const size_t csz = sizeof(C);
+  hipEvent_t ev{};
+  {
+    const int nl = Ky * Kz;
+    const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
+    tstart(ctx, 1, &ev);
+    hipLaunchKernelGGL(k_fft_x_r2c<C>, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[0], tw[0], ctx->gfix, spec, M[0], ctx->stride[0]);
+    tstop(ctx, 1, ev);
+  }
+  {
+    const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
+    tstart(ctx, 2, &ev);
+    hipLaunchKernelGGL((k_fft_strided<C, false>), dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK),
+                       sh, ctx->st, g, ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1],
+                       -1, (const decltype(C{}.x)*)nullptr, (double*)nullptr);
+    tstop(ctx, 2, ev);
+  }
+  {
+    const size_t sh = 2 * size_t(M[2]) * ctx->stride[2] * csz;
+    tstart(ctx, 3, &ev);
+    hipLaunchKernelGGL((k_fft_strided<C, true>), dim3((H + M[2] - 1) / M[2], Ky), dim3(BLOCK), sh,
+                       ctx->st, g, ctx->pl[2], tw[2], spec, H * Ky, H, M[2], ctx->stride[2], 0,
+                       G, ctx->part);
+    tstop(ctx, 3, ev);
+  }
+  {
+    const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
+    tstart(ctx, 2, &ev);
+    hipLaunchKernelGGL((k_fft_strided<C, false>), dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK),
+                       sh, ctx->st, g, ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1],
+                       +1, (const decltype(C{}.x)*)nullptr, (double*)nullptr);
+    tstop(ctx, 2, ev);
+  }
+  {
+    const int nl = Ky * Kz;
+    const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
+    tstart(ctx, 4, &ev);
+    hipLaunchKernelGGL(k_fft_x_c2r<C>, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[0], tw[0], spec, greal, M[0], ctx->stride[0]);
+    tstop(ctx, 4, ev);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int zpass_blocks(const simox_ctx* ctx) {
+  const int* M = use32(ctx) ? ctx->M32 : ctx->M64;
+  return ((ctx->g.H + M[2] - 1) / M[2]) * ctx->g.K[1];
+}
+
+// Spread (charges q, or q + step*p written to ctx->ql) and convolve.
+int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step) {
+  const int nb = grid_blocks(ctx->n);
+  CHK(ensure_part(ctx, size_t(zpass_blocks(ctx)) + 16));
+  const bool f32w = ctx->mode == SIMOX_MODE_FAITHFUL;
+  hipEvent_t ev{};
+  tstart(ctx, 0, &ev);
+  if (f32w)
+    hipLaunchKernelGGL(k_spread<true>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, q, p,
+                       step, ctx->ql, (unsigned long long*)ctx->gfix);
+  else
+    hipLaunchKernelGGL(k_spread<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, q, p,
+                       step, ctx->ql, (unsigned long long*)ctx->gfix);
+  tstop(ctx, 0, ev);
+  HIPCHK(hipGetLastError());
+  if (use32(ctx))
+    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->M32, ctx->G32, ctx->greal32);
+  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->M64, ctx->G64, ctx->greal64);
+}
+
+// PME (force.f90:364-464): reciprocal forces added to ctx->f, E_rec.
+int do_pme(simox_ctx* ctx, bool fetch_energy) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  CHK(pme_eval(ctx, ctx->q, nullptr, 0.0));
+  const int zb = zpass_blocks(ctx);
+  CHK(reduce(ctx, zb, 1, 0, 4));  // energy sum before the part buffer is reused
+  CHK(ensure_part(ctx, size_t(nb) * 3));
+  const double Ng = double(ctx->g.K[0]) * double(ctx->g.K[1]) * double(ctx->g.K[2]);
+  const double cst = KC * Ng / ctx->V / PI;  // force.f90:436
+  hipEvent_t ev{};
+  tstart(ctx, 5, &ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_gather_force<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                       ctx->q, ctx->greal32, cst, ctx->frec, ctx->part);
+  else
+    hipLaunchKernelGGL(k_gather_force<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                       ctx->q, ctx->greal64, cst, ctx->frec, ctx->part);
+  tstop(ctx, 5, ev);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 3, 0, 5));
+  hipLaunchKernelGGL(k_pme_apply, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->frec, ctx->dscal + 5,
+                     ctx->f);
+  HIPCHK(hipGetLastError());
+  if (fetch_energy) {
+    double s;
+    CHK(fetch(ctx, 4, 1, &s));
+    ctx->Erec = KC * Ng * s / ctx->V / PI / 2.0;  // force.f90:435,461
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CTIP
+// ---------------------------------------------------------------------------
+int build_matrix(simox_ctx* ctx) {
+  const int nb = grid_blocks(ctx->n);
+  hipEvent_t ev{};
+  tstart(ctx, 9, &ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_ctip_matrix<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt, ctx->nb_col,
+                       ctx->nb_val32, ctx->bvec, ctx->cap, ctx->derr);
+  else
+    hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt,
+                       ctx->nb_col, ctx->nb_val64, ctx->bvec, ctx->cap, ctx->derr);
+  tstop(ctx, 9, ev);
+  HIPCHK(hipGetLastError());
+  CHK(check_err(ctx, 1));
+  ctx->matrix_valid = true;
+  return 0;
+}
+
+// g = -dU/dq at (q + step*p) or at q; returns {sum g, -sum g.p} in dscal[0..1].
+int grad_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool want_phi) {
+  const int nb = grid_blocks(ctx->n);
+  const double* qeval = q;
+  if (p) qeval = ctx->ql;
+  CHK(pme_eval(ctx, q, p, step));
+  CHK(ensure_part(ctx, size_t(nb) * 2));
+  hipEvent_t ev{};
+  tstart(ctx, 6, &ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL((k_dudq<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->greal32, ctx->c_pme, ctx->nb_cnt,
+                       ctx->nb_col, ctx->nb_val32, ctx->bvec, p, ctx->gv,
+                       want_phi ? ctx->phi : nullptr, ctx->part);
+  else
+    hipLaunchKernelGGL((k_dudq<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->greal64, ctx->c_pme, ctx->nb_cnt,
+                       ctx->nb_col, ctx->nb_val64, ctx->bvec, p, ctx->gv,
+                       want_phi ? ctx->phi : nullptr, ctx->part);
+  tstop(ctx, 6, ev);
+  HIPCHK(hipGetLastError());
+  return reduce(ctx, nb, 2, 0, 0);
+}
+
+int self_energy(simox_ctx* ctx, const double* q, int use_jp, double wpen, double* out3) {
+  const int nb = grid_blocks(ctx->n);
+  CHK(ensure_part(ctx, size_t(nb) * 3));
+  hipLaunchKernelGGL(k_self_energy, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id,
+                     q, use_jp, wpen, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 3, 0, 8));
+  return fetch(ctx, 8, 3, out3);
+}
+
+// BRENT_SOLVER (ctip.f90:216-304): root of f(a) = -g(q + a p).p on [-1, 1].
+int brent(simox_ctx* ctx, double* alpha, int* n_eval) {
+  const double E_crit = 1.0e-8, E_end = 1.0e-11;
+  double s2[2];
+  double a = -1.0, b = 1.0, c, d = 0.0, e = 0.0, fa, fb, fc, xm, x, y, z;
+  CHK(grad_eval(ctx, ctx->qc, ctx->pv, a, false));
+  CHK(fetch(ctx, 0, 2, s2));
+  fa = s2[1];
+  CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false));
+  CHK(fetch(ctx, 0, 2, s2));
+  fb = s2[1];
+  *n_eval += 2;
+  c = b;
+  fc = fb;
+  for (int it = 1; it <= 100; ++it) {
+    if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) { c = a; fc = fa; d = b - a; e = d; }
+    if (std::fabs(fc) < std::fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+    xm = 0.5 * (c - b);
+    if (std::fabs(xm) < E_crit || std::fabs(fb) < E_end) { *alpha = b; return 0; }
+    if (std::fabs(e) >= E_crit && std::fabs(fa) > std::fabs(fb)) {
+      const double s = fb / fa;
+      if (a == c) { x = 2.0 * xm * s; y = 1.0 - s; }
+      else {
+        y = fa / fc; z = fb / fc;
+        x = s * (2.0 * xm * y * (y - z) - (b - a) * (z - 1.0));
+        y = (y - 1.0) * (z - 1.0) * (s - 1.0);
+      }
+      if (x > 0.0) y = -y;
+      x = std::fabs(x);
+      const double m1 = 3.0 * xm * y - std::fabs(E_crit * y), m2 = std::fabs(e * y);
+      if (2.0 * x < std::min(m1, m2)) { e = d; d = x / y; }
+      else { d = xm; e = d; }
+    } else { d = xm; e = d; }
+    a = b; fa = fb;
+    if (std::fabs(d) > E_crit) b = b + d;
+    else b = b + std::copysign(E_crit, xm);
+    CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false));
+    CHK(fetch(ctx, 0, 2, s2));
+    fb = s2[1];
+    *n_eval += 1;
+  }
+  ctx->err = status_text(SIMOX_ERR_BRENT, 1);
+  return SIMOX_ERR_BRENT;
+}
+
+// CTIP_SOLVER with the reference's nonlinear CG (ctip.f90:167-190).
+int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qc);
+  HIPCHK(hipMemsetAsync(ctx->pv, 0, sizeof(double) * n, ctx->st));
+  double Error = 1.0, g_sum = 1.0, g_sum_old, alpha = 0.0;
+  int k = 0;
+  *n_eval = 0;
+  while (Error > 1.0e-3) {
+    double s2[2], pmax;
+    CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false));
+    CHK(fetch(ctx, 0, 2, s2));
+    *n_eval += 1;
+    g_sum_old = g_sum;
+    g_sum = s2[0];
+    CHK(ensure_part(ctx, size_t(nb)));
+    hipLaunchKernelGGL(k_cg_dir1, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->gv, ctx->pv, g_sum,
+                       g_sum_old, ctx->dv, ctx->part);
+    CHK(reduce(ctx, nb, 1, 0, 2));
+    hipLaunchKernelGGL(k_cg_dir2, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dv, ctx->dscal + 2,
+                       ctx->pv, ctx->part);
+    CHK(reduce(ctx, nb, 1, 1, 3));
+    CHK(fetch(ctx, 3, 1, &pmax));
+    CHK(brent(ctx, &alpha, n_eval));
+    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, alpha, ctx->pv, ctx->qc);
+    HIPCHK(hipGetLastError());
+    Error = pmax * std::fabs(alpha);
+    ++k;
+    if (k >= ctx->maxit && Error > 1.0e-3) {
+      // the reference loops without a cap (ctip.f90:169); bound it here
+      ctx->err = "CTIP nonlinear CG did not reach max|alpha p| < 1e-3 within the iteration cap";
+      return SIMOX_ERR_NOT_CONVERGED;
+    }
+  }
+  *iters = k;
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->qc, ctx->q);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
+  CHK(build_matrix(ctx));
+  int it = 0, ne = 0;
+  CHK(ctip_faithful(ctx, &it, &ne));
+  if (iters) *iters = it;
+  if (n_eval) *n_eval = ne;
+  double e3[3];
+  CHK(self_energy(ctx, ctx->q, 1, WPEN, e3));  // ctip.f90:194-209
+  ctx->Eself = e3[0] * KC * ctx->g.alpha / SQRTPI + (e3[1] + e3[2]);
+  return 0;
+}
+
+int do_vv1(simox_ctx* ctx, double dt) {
+  hipLaunchKernelGGL(k_vv1, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab,
+                     ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt, ctx->x, ctx->v,
+                     ctx->f);
+  HIPCHK(hipGetLastError());
+  ctx->matrix_valid = false;
+  return 0;
+}
+
+int do_vv2(simox_ctx* ctx, double dt, bool fetch_mv2) {
+  const int nb = grid_blocks(ctx->n);
+  CHK(ensure_part(ctx, size_t(nb)));
+  hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id, dt,
+                     ctx->v, ctx->f, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 1, 0, 12));
+  if (fetch_mv2) CHK(fetch(ctx, 12, 1, &ctx->mv2));
+  return 0;
+}
+
+int upload_caller(simox_ctx* ctx, const double* h, int w, double* dsorted) {
+  const size_t cnt = size_t(ctx->n) * w;
+  CHK(ensure_stage(ctx, cnt));
+  memcpy(ctx->hstage, h, cnt * sizeof(double));
+  HIPCHK(hipMemcpyAsync(ctx->dstage, ctx->hstage, cnt * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->st));
+  hipLaunchKernelGGL(k_from_caller, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, w,
+                     ctx->perm, ctx->dstage, dsorted);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int download_caller(simox_ctx* ctx, const double* dsorted, int w, double* h) {
+  const size_t cnt = size_t(ctx->n) * w;
+  CHK(ensure_stage(ctx, cnt));
+  hipLaunchKernelGGL(k_to_caller, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, w,
+                     ctx->perm, dsorted, ctx->dstage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ctx->hstage, ctx->dstage, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  memcpy(h, ctx->hstage, cnt * sizeof(double));
+  return 0;
+}
+
+bool need_state(simox_ctx* ctx) {
+  if (!ctx->have_state) {
+    ctx->err = "state not set (simox_set_state with species ids first)";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+void simox_fafb(simox_params* p) {
+  // datafmt.f90:201-220: [fa|fb] coefficients from the Slater exponents xi
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) {
+      double* c = p->prm[i].fafb[j];
+      const double a = p->prm[i].xi, b = p->prm[j].xi;
+      if (i == j) {
+        c[0] = 1.0;
+        c[1] = 11.0 * a / 8.0;
+        c[2] = 0.75 * (a * a);
+        c[3] = (a * a * a) / 6.0;
+      } else {
+        const double s = a + b, da = a - b, db = b - a;
+        c[0] = a * (b * b * b * b) / (s * s) / (da * da);
+        c[1] = b * (a * a * a * a) / (s * s) / (db * db);
+        c[2] = (3.0 * (a * a) * (b * b * b * b) - (b * b * b * b * b * b)) / (s * s * s) / (da * da * da);
+        c[3] = (3.0 * (b * b) * (a * a * a * a) - (a * a * a * a * a * a)) / (s * s * s) / (db * db * db);
+      }
+    }
+}
+
+void simox_potential_setup(simox_params* p) {
+  memset(p, 0, sizeof *p);
+  // Zhou-Johnson-Wadley EAM for Al, Ni, Fe (datafmt.f90:155-178)
+  static const double metal[3][22] = {
+      {2.86392, 1.20378, 17.51747, 19.90041, 6.61317, 3.52702, 0.31487, 0.36555, 0.37985,
+       0.75969, -2.80760, -0.30144, 1.25856, -1.24760, -2.83000, 0.0, 0.62225, -2.48824,
+       -2.83000, 0.0, 0.62225, -2.48824},
+      {2.48875, 2.21149, 30.37003, 30.37137, 8.38345, 4.47117, 0.42905, 0.63353, 0.44360,
+       0.82066, -2.69351, -0.07644, 0.24144, -2.37563, -2.70000, 0.0, 0.26539, -0.15286,
+       -2.70000, 0.0, 0.26539, 4.58568},
+      {2.48199, 2.31453, 24.59573, 24.59573, 9.81827, 5.23641, 0.39281, 0.64624, 0.17031,
+       0.34061, -2.53499, -0.05960, 0.19306, -2.28232, -2.54000, 0.0, 0.20027, -0.14877,
+       -2.54000, 0.0, 0.20027, 6.69465}};
+  static const double eta_fe[3][2] = {{0.78591, -2.82453}, {1.01318, -2.70839}, {1.18290, -2.55187}};
+  for (int m = 0; m < 3; ++m) {
+    simox_metal& e = p->eam[m];
+    const double* t = metal[m];
+    e.r_e = t[0]; e.f_e = t[1]; e.rho_e = t[2]; e.rho_s = t[3]; e.alpha = t[4]; e.beta = t[5];
+    e.A = t[6]; e.B = t[7]; e.kappa = t[8]; e.lambda = t[9];
+    for (int k = 0; k < 4; ++k) { e.Fn[k] = t[10 + k]; e.Fm[k] = t[14 + k]; e.Fp[k] = t[18 + k]; }
+    e.eta = eta_fe[m][0]; e.Fe = eta_fe[m][1];
+  }
+  // CTIP: mass, qmin, qmax, chi, J, xi, Z for Al, Ni, Fe, O, H (datafmt.f90:183-197)
+  static const double ctip[5][7] = {{26.981539, 0.0, 3.0, -1.47914, 9.07222, 0.968, 1.07514},
+                                    {58.6934, 0.0, 2.0, -1.70804, 9.10954, 1.087, 1.44450},
+                                    {55.84500, 0.0, 3.0, -1.90587, 8.99819, 1.024, 1.28612},
+                                    {15.9994, -2.0, 0.0, 2.000, 14.99523, 2.144, 0.0},
+                                    {1.00794, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+  for (int s = 0; s < 5; ++s) {
+    simox_potn& q = p->prm[s];
+    q.xm = ctip[s][0]; q.qmin = ctip[s][1]; q.qmax = ctip[s][2]; q.chi = ctip[s][3];
+    q.J = ctip[s][4]; q.xi = ctip[s][5]; q.Z = ctip[s][6];
+  }
+  simox_fafb(p);
+  // oxygen EAM: per row r_e, alph, beta, A, B, kapp, lamb (datafmt.f90:225-239)
+  static const double ox[5][7] = {{2.98520, 8.49741, 4.52114, 0.09738, 0.38121, 0.18967, 0.95234},
+                                  {2.95732, 7.96528, 4.42411, 0.13521, 0.25332, 0.47077, 0.65524},
+                                  {3.07992, 7.52309, 4.13330, 0.17108, 0.39869, 0.22335, 0.34380},
+                                  {3.64857, 5.44072, 3.59746, 0.34900, 0.57438, 0.08007, 0.39310},
+                                  {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+  for (int r = 0; r < 5; ++r) {
+    p->ox.r_e[r] = ox[r][0]; p->ox.alph[r] = ox[r][1]; p->ox.beta[r] = ox[r][2];
+    p->ox.A[r] = ox[r][3]; p->ox.B[r] = ox[r][4]; p->ox.kapp[r] = ox[r][5]; p->ox.lamb[r] = ox[r][6];
+  }
+  // O embedding pieces F0..F4, coefficient index 1..4 (datafmt.f90:241-249)
+  static const double F[5][4] = {{-1.56489, -1.39123, 1.77199, 1.59833},
+                                 {-1.58967, 1.30636, 9.81033, 0.0},
+                                 {-1.54116, 2.02821, 6.56240, 0.0},
+                                 {-1.51798, 2.30979, 7.69582, 0.0},
+                                 {-1.19082, 4.12936, 10.32338, 0.0}};
+  for (int k = 0; k < 5; ++k)
+    for (int j = 0; j < 4; ++j) p->ox.F[k][j] = F[k][j];
+  static const double re[5] = {54.62910, 64.26953, 66.21202, 66.92391, 74.23105};
+  static const double rmax[5] = {54.62910, 65.24078, 66.56797, 70.57748, 99.99999e9};
+  for (int k = 0; k < 5; ++k) { p->ox.re[k] = re[k]; p->ox.rmax[k] = rmax[k]; }
+  p->ox.fe = 1.39478; p->ox.gamm = 2.11725; p->ox.nu = 0.37457;  // datafmt.f90:257-259
+}
+
+int simox_create(int n, const double box[3], const int fft[3], double rcut, int mode,
+                 const simox_params* params, simox_ctx** out) {
+  if (!out) return SIMOX_ERR_ARG;
+  *out = nullptr;
+  simox_ctx* ctx = new simox_ctx();
+  *out = ctx;
+  if (n <= 0 || rcut <= 0.0 || mode < 0 || mode > 2) {
+    ctx->err = "invalid n / rcut / mode";
+    return SIMOX_ERR_ARG;
+  }
+  ctx->n = n;
+  ctx->mode = mode;
+  ctx->rcut = rcut;
+  Geom& g = ctx->g;
+  g.n = n;
+  for (int d = 0; d < 3; ++d) {
+    g.box[d] = box[d];
+    g.hbox[d] = 0.5 * box[d];
+    g.qbox[d] = 0.25 * box[d];
+    g.nc[d] = int(box[d] / rcut);  // main.f90:189
+    g.K[d] = fft[d];
+    if (g.nc[d] < 2) {
+      ctx->err = "==== Error - unitcell is too small or cut-off radius is too small";
+      return SIMOX_ERR_ARG;
+    }
+    g.lcell[d] = box[d] / double(g.nc[d]);  // main.f90:190
+    if (fft[d] < 2 || !make_plan(fft[d], &ctx->pl[d])) {
+      ctx->err = "PME grid sizes must be products of 2, 3 and 5";
+      return SIMOX_ERR_ARG;
+    }
+  }
+  if (fft[0] % 2) {
+    ctx->err = "PME grid size along x must be even";
+    return SIMOX_ERR_ARG;
+  }
+  g.ncall = g.nc[0] * g.nc[1] * g.nc[2];
+  g.rcut2 = rcut * rcut;
+  g.alpha = 3.8 / rcut;
+  g.H = fft[0] / 2 + 1;
+  ctx->V = box[0] * box[1] * box[2];
+  ctx->c_pme = KC * double(fft[0]) * double(fft[1]) * double(fft[2]) / ctx->V / PI;  // ctip.f90:59
+  if (params) ctx->prm = *params;
+  else simox_potential_setup(&ctx->prm);
+  build_tables(ctx);
+
+  HIPCHK(hipStreamCreate(&ctx->st));
+  CHK(dalloc(ctx, &ctx->dtab, 1));
+  HIPCHK(hipMemcpy(ctx->dtab, &ctx->htab, sizeof(Tables), hipMemcpyHostToDevice));
+  const size_t n3 = size_t(n) * 3;
+  CHK(dalloc(ctx, &ctx->x, n3)); CHK(dalloc(ctx, &ctx->v, n3)); CHK(dalloc(ctx, &ctx->f, n3));
+  CHK(dalloc(ctx, &ctx->x2, n3)); CHK(dalloc(ctx, &ctx->v2, n3)); CHK(dalloc(ctx, &ctx->f2, n3));
+  CHK(dalloc(ctx, &ctx->q, n)); CHK(dalloc(ctx, &ctx->q2, n));
+  CHK(dalloc(ctx, &ctx->id, n)); CHK(dalloc(ctx, &ctx->id2, n));
+  CHK(dalloc(ctx, &ctx->perm, n)); CHK(dalloc(ctx, &ctx->perm2, n));
+  CHK(dalloc(ctx, &ctx->cell, n)); CHK(dalloc(ctx, &ctx->cell2, n));
+  CHK(dalloc(ctx, &ctx->slot, n)); CHK(dalloc(ctx, &ctx->order, n));
+  CHK(dalloc(ctx, &ctx->count, g.ncall)); CHK(dalloc(ctx, &ctx->fill, g.ncall));
+  CHK(dalloc(ctx, &ctx->start, g.ncall + 1));
+  CHK(dalloc(ctx, &ctx->rho, n)); CHK(dalloc(ctx, &ctx->dF64, n)); CHK(dalloc(ctx, &ctx->dF32, n));
+  CHK(dalloc(ctx, &ctx->frec, n3));
+  CHK(dalloc(ctx, &ctx->qc, n)); CHK(dalloc(ctx, &ctx->ql, n)); CHK(dalloc(ctx, &ctx->gv, n));
+  CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
+  CHK(dalloc(ctx, &ctx->bvec, n)); CHK(dalloc(ctx, &ctx->nb_cnt, n));
+  CHK(dalloc(ctx, &ctx->nb_col, size_t(ctx->cap) * n));
+  if (mode == SIMOX_MODE_MIXED) CHK(dalloc(ctx, &ctx->nb_val32, size_t(ctx->cap) * n));
+  else CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
+  // PME
+  const size_t ng = size_t(fft[0]) * fft[1] * fft[2];
+  const size_t nh = size_t(g.H) * fft[1] * fft[2];
+  CHK(dalloc(ctx, &ctx->gfix, ng));
+  HIPCHK(hipMemset(ctx->gfix, 0, ng * sizeof(long long)));
+  if (mode == SIMOX_MODE_MIXED) {
+    CHK(dalloc(ctx, &ctx->greal32, ng)); CHK(dalloc(ctx, &ctx->spec32, nh)); CHK(dalloc(ctx, &ctx->G32, nh));
+  } else {
+    CHK(dalloc(ctx, &ctx->greal64, ng)); CHK(dalloc(ctx, &ctx->spec64, nh)); CHK(dalloc(ctx, &ctx->G64, nh));
+  }
+  for (int d = 0; d < 3; ++d) {
+    const int L = fft[d];
+    std::vector<double2> t64(L);
+    std::vector<float2> t32(L);
+    for (int k = 0; k < L; ++k) {
+      const double ang = 2.0 * PI * double(k) / double(L);
+      t64[k] = make_double2(std::cos(ang), -std::sin(ang));
+      t32[k] = make_float2(float(std::cos(ang)), float(-std::sin(ang)));
+    }
+    CHK(dalloc(ctx, &ctx->tw64[d], L)); CHK(dalloc(ctx, &ctx->tw32[d], L));
+    HIPCHK(hipMemcpy(ctx->tw64[d], t64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->tw32[d], t32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+    ctx->stride[d] = d == 0 ? L : L + 1;
+    const size_t lds_cap = 60 * 1024;
+    ctx->M64[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(double2)))));
+    ctx->M32[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(float2)))));
+    const size_t csz = mode == SIMOX_MODE_MIXED ? sizeof(float2) : sizeof(double2);
+    if (2 * size_t(ctx->stride[d]) * csz > lds_cap) {
+      ctx->err = "PME grid dimension too large for the LDS-resident FFT";
+      return SIMOX_ERR_ARG;
+    }
+  }
+  {
+    const double a2 = g.alpha * g.alpha;
+    const int nbk = int((nh + BLOCK - 1) / BLOCK);
+    if (mode == SIMOX_MODE_MIXED) hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G32);
+    else hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G64);
+    HIPCHK(hipGetLastError());
+  }
+  CHK(dalloc(ctx, &ctx->dscal, 64));
+  HIPCHK(hipHostMalloc((void**)&ctx->hscal, 64 * sizeof(double), hipHostMallocDefault));
+  CHK(dalloc(ctx, &ctx->derr, 1));
+  HIPCHK(hipMemset(ctx->derr, 0, sizeof(int)));
+  HIPCHK(hipHostMalloc((void**)&ctx->herr, sizeof(int), hipHostMallocDefault));
+  {
+    // partial-sum buffer sized once for every launch that uses it
+    const size_t zb = size_t(zpass_blocks(ctx));
+    CHK(ensure_part(ctx, std::max(size_t(grid_blocks(n)) * 4, zb) + 16));
+  }
+  hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->order);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_destroy(simox_ctx* ctx) {
+  if (!ctx) return SIMOX_OK;
+  if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+  tresolve(ctx);
+  void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
+                  ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
+                  ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
+                  ctx->dF32, ctx->frec, ctx->gfix, ctx->greal64, ctx->greal32, ctx->spec64,
+                  ctx->spec32, ctx->G64, ctx->G32, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->part, ctx->dscal, ctx->derr, ctx->dstage};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int d = 0; d < 3; ++d) {
+    if (ctx->tw64[d]) (void)hipFree(ctx->tw64[d]);
+    if (ctx->tw32[d]) (void)hipFree(ctx->tw32[d]);
+  }
+  if (ctx->hscal) (void)hipHostFree(ctx->hscal);
+  if (ctx->herr) (void)hipHostFree(ctx->herr);
+  if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+  if (ctx->st) (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+  return SIMOX_OK;
+}
+
+const char* simox_last_error(const simox_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  for (int d = 0; d < 3; ++d) {
+    if (ncell) ncell[d] = ctx->g.nc[d];
+    if (lcell) lcell[d] = ctx->g.lcell[d];
+  }
+  if (alpha) *alpha = ctx->g.alpha;
+  return SIMOX_OK;
+}
+
+int simox_set_mode(simox_ctx* ctx, int mode) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  const bool was32 = ctx->mode == SIMOX_MODE_MIXED, is32 = mode == SIMOX_MODE_MIXED;
+  if (mode < 0 || mode > 2 || was32 != is32) {
+    ctx->err = "mode change between FP64 and MIXED storage needs a new context";
+    return SIMOX_ERR_ARG;
+  }
+  ctx->mode = mode;
+  return SIMOX_OK;
+}
+
+int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit) {
+  if (!ctx || tol <= 0.0 || maxit <= 0) return SIMOX_ERR_ARG;
+  ctx->tol = tol;
+  ctx->maxit = maxit;
+  return SIMOX_OK;
+}
+
+int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double* v,
+                    const double* f, const double* q) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  const int n = ctx->n;
+  if (id) {
+    for (int i = 0; i < n; ++i) {
+      if (id[i] < 1 || id[i] > 5) {
+        ctx->err = "=== Particle type error ===";  // main.f90:241
+        return SIMOX_ERR_ARG;
+      }
+      if (id[i] > NSPEC) {  // H has no pair functions (force.f90:72-73)
+        ctx->err = "=== Unknown particle pair ===";
+        return SIMOX_ERR_UNKNOWN_PAIR;
+      }
+    }
+    if (!x) {
+      ctx->err = "positions are required together with species ids";
+      return SIMOX_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->perm);
+    HIPCHK(hipMemcpyAsync(ctx->id, id, sizeof(int) * n, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->v, 0, sizeof(double) * 3 * n, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->f, 0, sizeof(double) * 3 * n, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->q, 0, sizeof(double) * n, ctx->st));
+    CHK(upload_caller(ctx, x, 3, ctx->x));
+    if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
+    if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
+    if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
+    ctx->have_state = true;
+    CHK(do_cell_sort(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return check_err(ctx, 0);
+  }
+  if (!need_state(ctx)) return SIMOX_ERR_ARG;
+  if (x) CHK(upload_caller(ctx, x, 3, ctx->x));
+  if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
+  if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
+  if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
+  if (x) ctx->matrix_valid = false;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (x) CHK(download_caller(ctx, ctx->x, 3, x));
+  if (v) CHK(download_caller(ctx, ctx->v, 3, v));
+  if (f) CHK(download_caller(ctx, ctx->f, 3, f));
+  if (q) CHK(download_caller(ctx, ctx->q, 1, q));
+  return SIMOX_OK;
+}
+
+int simox_cell_sort(simox_ctx* ctx) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_cell_sort(ctx));
+  return check_err(ctx, 0);
+}
+
+int simox_get_cells(simox_ctx* ctx, int* npt, int* link, int link_stride) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  std::vector<int> start(ctx->g.ncall + 1), perm(ctx->n);
+  HIPCHK(hipMemcpy(start.data(), ctx->start, sizeof(int) * (ctx->g.ncall + 1), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(perm.data(), ctx->perm, sizeof(int) * ctx->n, hipMemcpyDeviceToHost));
+  for (int c = 0; c < ctx->g.ncall; ++c) {
+    const int m = start[c + 1] - start[c];
+    if (npt) npt[c] = m;
+    if (link)
+      for (int k = 0; k < m && k < link_stride; ++k) link[size_t(c) * link_stride + k] = perm[start[c] + k];
+  }
+  return SIMOX_OK;
+}
+
+int simox_force(simox_ctx* ctx, double energies[4]) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_force(ctx, true));
+  CHK(check_err(ctx, 0));
+  if (energies) {
+    energies[0] = ctx->Epair; energies[1] = ctx->Edir; energies[2] = ctx->Ectip; energies[3] = ctx->Egroup;
+  }
+  return SIMOX_OK;
+}
+
+int simox_get_rho(simox_ctx* ctx, double* rho) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  return download_caller(ctx, ctx->rho, 1, rho);
+}
+
+int simox_pme_self(simox_ctx* ctx, double* eself) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  double e3[3];
+  CHK(self_energy(ctx, ctx->q, 0, 2.0 * WPEN, e3));  // force.f90:486-498
+  ctx->Eself = e3[0] * ctx->g.alpha / SQRTPI * KC + (e3[1] + e3[2]);
+  if (eself) *eself = ctx->Eself;
+  return SIMOX_OK;
+}
+
+int simox_pme(simox_ctx* ctx, double* erec) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_pme(ctx, true));
+  if (erec) *erec = ctx->Erec;
+  return SIMOX_OK;
+}
+
+int simox_phi(simox_ctx* ctx, double* phi) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, true));
+  return download_caller(ctx, ctx->phi, 1, phi);
+}
+
+int simox_dudq(simox_ctx* ctx, double* g) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, false));
+  return download_caller(ctx, ctx->gv, 1, g);
+}
+
+int simox_get_neighbors(simox_ctx* ctx, int* counts, int* cols, int stride) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  const int n = ctx->n;
+  std::vector<int> cnt(n), perm(n);
+  HIPCHK(hipMemcpy(cnt.data(), ctx->nb_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(perm.data(), ctx->perm, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<int> col;
+  if (cols) {
+    col.resize(size_t(ctx->cap) * n);
+    HIPCHK(hipMemcpy(col.data(), ctx->nb_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost));
+  }
+  for (int p = 0; p < n; ++p) {
+    const int c = perm[p];
+    if (counts) counts[c] = cnt[p];
+    if (cols)
+      for (int k = 0; k < cnt[p] && k < stride; ++k) cols[size_t(c) * stride + k] = perm[col[size_t(k) * n + p]];
+  }
+  return SIMOX_OK;
+}
+
+int simox_ctip_solve(simox_ctx* ctx, double* eself, int* iters, int* n_eval) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_ctip(ctx, iters, n_eval));
+  if (eself) *eself = ctx->Eself;
+  return SIMOX_OK;
+}
+
+int simox_vv1(simox_ctx* ctx, double dt) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_vv1(ctx, dt));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_vv2(simox_ctx* ctx, double dt, double* mv2) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  CHK(do_vv2(ctx, dt, true));
+  if (mv2) *mv2 = ctx->mv2;
+  return SIMOX_OK;
+}
+
+int simox_step(simox_ctx* ctx, int nsteps, double dt, int sort_every, int ctip_every, double* stats) {
+  if (!ctx || !need_state(ctx) || nsteps < 0) return SIMOX_ERR_ARG;
+  for (int s = 0; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    CHK(do_vv1(ctx, dt));
+    if (sort_every > 0 && s % sort_every == 0) CHK(do_cell_sort(ctx));
+    if (ctip_every > 0 && s % ctip_every == 0) CHK(do_ctip(ctx, nullptr, nullptr));
+    CHK(do_force(ctx, last && stats));
+    CHK(do_pme(ctx, last && stats));
+    CHK(do_vv2(ctx, dt, last && stats));
+  }
+  CHK(check_err(ctx, 0));
+  if (stats) {
+    stats[0] = ctx->Eself; stats[1] = ctx->Edir; stats[2] = ctx->Erec; stats[3] = ctx->Ectip;
+    stats[4] = ctx->Epair; stats[5] = ctx->Egroup; stats[6] = ctx->mv2;
+    stats[7] = ctx->Eself + ctx->Edir + ctx->Erec + ctx->Ectip + ctx->Epair + ctx->Egroup + ctx->mv2 / 2.0;
+  }
+  return SIMOX_OK;
+}
+
+int simox_timing_enable(simox_ctx* ctx, int on) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  tresolve(ctx);
+  ctx->timing = on != 0;
+  for (int k = 0; k < 16; ++k) { ctx->tms[k] = 0; ctx->tcnt[k] = 0; }
+  return SIMOX_OK;
+}
+
+int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launches) {
+  if (!ctx || which < 0 || which > 15) return SIMOX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  tresolve(ctx);
+  if (ms) *ms = ctx->tms[which];
+  if (launches) *launches = ctx->tcnt[which];
+  return SIMOX_OK;
+}
+
+}  // extern "C"

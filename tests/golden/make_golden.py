@@ -1,0 +1,55 @@
+"""Writes the golden vectors of tests/golden from the reference build.
+
+Runs oracle/_ref/ref_driver (the reference's own FFT-free Fortran routines,
+compiled from their sources by oracle/Makefile) on seeded synthetic inputs and
+stores inputs and outputs as .npz fixtures.  Run in the container that holds
+the reference sources:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle.oracle import Oracle, run_reference  # noqa: E402
+from simox_amd.systems import config, tiny  # noqa: E402
+
+
+def one(name, s, seed, with_dudq=True):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=s.n) * 0.3
+    q[s.ids == 4] = -np.abs(q[s.ids == 4])
+    f0 = rng.normal(size=(s.n, 3))
+    dud = None
+    o = Oracle(s.n, s.box, s.fft, s.rcut)
+    o.cell_sort(s.x)
+    if with_dudq:
+        nlist, lst, asym, sym = o.ctip_matrix(s.ids, s.x)
+        phi = rng.normal(size=s.n)
+        dud = (nlist, lst, asym, sym, phi)
+    ref = run_reference(s.ids, s.x, s.v, f0, q, s.box, s.fft, s.rcut, s.dt, dudq_inputs=dud)
+    out = dict(ids=s.ids, x=s.x, v=s.v, f0=f0, q=q, box=s.box, fft=np.array(s.fft),
+               rcut=np.array(s.rcut), dt=np.array(s.dt),
+               ref_npt=ref["npt"], ref_pair=ref["pair"], ref_f=ref["f"],
+               ref_E=np.array([ref["Epair"], ref["Edir"], ref["Ectip"], ref["Egroup"], ref["Eself"]]),
+               ref_x1=ref["x1"], ref_v1=ref["v1"], ref_v2=ref["v2"], ref_mv2=np.array(ref["mv2"]),
+               ref_rdf=ref["rdf"])
+    # links only for occupied slots (compact)
+    out["ref_link"] = np.concatenate([ref["link"][c, :ref["npt"][c]] for c in range(len(ref["npt"]))])
+    if s.n <= 400:
+        out["ref_psi"] = ref["psi"]
+    if with_dudq:
+        out["dudq_phi"] = dud[4]
+        out["ref_g"] = ref["g"]
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    one("tiny_s1", tiny(seed=1), 11)
+    one("tiny_s2", tiny(seed=2, nx=3, ny=3, nz=4, n_o=8, fft=(12, 12, 24), rcut=5.0), 12)
+    one("c1_s1", config("c1", seed=1), 21)
