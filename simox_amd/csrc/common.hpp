@@ -17,10 +17,12 @@ constexpr int NSPEC = 4;                        // Al, Ni, Fe, O handled on the 
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 
-// Fixed-point scale of the PME charge grid (2^44): integer atomics make the
-// spread order-independent, hence deterministic.
-constexpr double GRID_SCALE = 17592186044416.0;
-constexpr double GRID_INV_SCALE = 1.0 / 17592186044416.0;
+// Fixed-point scale of the PME charge grid (2^50, quantum 8.9e-16): integer
+// atomics make the spread order-independent, hence deterministic.  Grid values
+// must stay below 2^13 = 8192 in magnitude (a grid point collects a few atoms'
+// charges times B-spline weights <= 1).
+constexpr double GRID_SCALE = 1125899906842624.0;
+constexpr double GRID_INV_SCALE = 1.0 / 1125899906842624.0;
 
 // ---------------------------------------------------------------------------
 // Per species-pair coefficient table.  Every EAM branch of eam.f90 used here

@@ -87,6 +87,8 @@ struct simox_ctx {
   float* nb_val32 = nullptr;
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
+  double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
+  int last_inner = 0, last_outer = 0;
   bool matrix_valid = false;
   // reductions / scalars
   double* part = nullptr;
@@ -441,12 +443,13 @@ int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const int* M,
 }
 
 int zpass_blocks(const simox_ctx* ctx) {
-  const int* M = use32(ctx) ? ctx->M32 : ctx->M64;
-  return ((ctx->g.H + M[2] - 1) / M[2]) * ctx->g.K[1];
+  const int m = std::min(ctx->M32[2], ctx->M64[2]);
+  return ((ctx->g.H + m - 1) / m) * ctx->g.K[1];
 }
 
-// Spread (charges q, or q + step*p written to ctx->ql) and convolve.
-int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step) {
+// Spread (charges q, or q + step*p written to ctx->ql) and convolve, with the
+// FFT chain in FP32 (f32) or FP64.
+int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool f32) {
   const int nb = grid_blocks(ctx->n);
   CHK(ensure_part(ctx, size_t(zpass_blocks(ctx)) + 16));
   const bool f32w = ctx->mode == SIMOX_MODE_FAITHFUL;
@@ -460,7 +463,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step) {
                        step, ctx->ql, (unsigned long long*)ctx->gfix);
   tstop(ctx, 0, ev);
   HIPCHK(hipGetLastError());
-  if (use32(ctx))
+  if (f32)
     return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->M32, ctx->G32, ctx->greal32);
   return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->M64, ctx->G64, ctx->greal64);
 }
@@ -468,7 +471,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step) {
 // PME (force.f90:364-464): reciprocal forces added to ctx->f, E_rec.
 int do_pme(simox_ctx* ctx, bool fetch_energy) {
   const int n = ctx->n, nb = grid_blocks(n);
-  CHK(pme_eval(ctx, ctx->q, nullptr, 0.0));
+  CHK(pme_eval(ctx, ctx->q, nullptr, 0.0, use32(ctx)));
   const int zb = zpass_blocks(ctx);
   CHK(reduce(ctx, zb, 1, 0, 4));  // energy sum before the part buffer is reused
   CHK(ensure_part(ctx, size_t(nb) * 3));
@@ -503,31 +506,29 @@ int build_matrix(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
   hipEvent_t ev{};
   tstart(ctx, 9, &ev);
-  if (use32(ctx))
-    hipLaunchKernelGGL(k_ctip_matrix<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt, ctx->nb_col,
-                       ctx->nb_val32, ctx->bvec, ctx->cap, ctx->derr);
-  else
-    hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt,
-                       ctx->nb_col, ctx->nb_val64, ctx->bvec, ctx->cap, ctx->derr);
+  hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                     ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt, ctx->nb_col,
+                     ctx->nb_val64, ctx->bvec, ctx->cap, ctx->derr);
   tstop(ctx, 9, ev);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_ell_to_f32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
+                       ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
   HIPCHK(hipGetLastError());
   CHK(check_err(ctx, 1));
   ctx->matrix_valid = true;
   return 0;
 }
 
-// g = -dU/dq at (q + step*p) or at q; returns {sum g, -sum g.p} in dscal[0..1].
-int grad_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool want_phi) {
+// g = -dU/dq at (q + step*p) or at q; dscal[0..2] = {sum g, -sum g.p, sum g^2}.
+int grad_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool want_phi,
+              bool f32) {
   const int nb = grid_blocks(ctx->n);
   const double* qeval = q;
   if (p) qeval = ctx->ql;
-  CHK(pme_eval(ctx, q, p, step));
-  CHK(ensure_part(ctx, size_t(nb) * 2));
+  CHK(pme_eval(ctx, q, p, step, f32));
   hipEvent_t ev{};
   tstart(ctx, 6, &ev);
-  if (use32(ctx))
+  if (f32)
     hipLaunchKernelGGL((k_dudq<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, qeval, ctx->greal32, ctx->c_pme, ctx->nb_cnt,
                        ctx->nb_col, ctx->nb_val32, ctx->bvec, p, ctx->gv,
@@ -539,7 +540,7 @@ int grad_eval(simox_ctx* ctx, const double* q, const double* p, double step, boo
                        want_phi ? ctx->phi : nullptr, ctx->part);
   tstop(ctx, 6, ev);
   HIPCHK(hipGetLastError());
-  return reduce(ctx, nb, 2, 0, 0);
+  return reduce(ctx, nb, 3, 0, 0);
 }
 
 int self_energy(simox_ctx* ctx, const double* q, int use_jp, double wpen, double* out3) {
@@ -557,10 +558,10 @@ int brent(simox_ctx* ctx, double* alpha, int* n_eval) {
   const double E_crit = 1.0e-8, E_end = 1.0e-11;
   double s2[2];
   double a = -1.0, b = 1.0, c, d = 0.0, e = 0.0, fa, fb, fc, xm, x, y, z;
-  CHK(grad_eval(ctx, ctx->qc, ctx->pv, a, false));
+  CHK(grad_eval(ctx, ctx->qc, ctx->pv, a, false, false));
   CHK(fetch(ctx, 0, 2, s2));
   fa = s2[1];
-  CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false));
+  CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false, false));
   CHK(fetch(ctx, 0, 2, s2));
   fb = s2[1];
   *n_eval += 2;
@@ -588,7 +589,7 @@ int brent(simox_ctx* ctx, double* alpha, int* n_eval) {
     a = b; fa = fb;
     if (std::fabs(d) > E_crit) b = b + d;
     else b = b + std::copysign(E_crit, xm);
-    CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false));
+    CHK(grad_eval(ctx, ctx->qc, ctx->pv, b, false, false));
     CHK(fetch(ctx, 0, 2, s2));
     fb = s2[1];
     *n_eval += 1;
@@ -607,7 +608,7 @@ int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
   *n_eval = 0;
   while (Error > 1.0e-3) {
     double s2[2], pmax;
-    CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false));
+    CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false, false));
     CHK(fetch(ctx, 0, 2, s2));
     *n_eval += 1;
     g_sum_old = g_sum;
@@ -637,10 +638,119 @@ int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
   return 0;
 }
 
+// Inner projected PCG on H d = P g (g = ctx->gv, its sum in dscal[0]) at the
+// active set of qc; d -> ctx->dv.  Operator precision: FP32 grid/FFT/ELL
+// values (f32) or FP64.  Residuals, dot products and the neutrality
+// projection are FP64.
+int pcg_inner(simox_ctx* ctx, bool f32, double rel_tol, double abs_tol, int maxit, int* iters) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  double s3[3], rz, rz_new, pap, s2[2];
+  hipLaunchKernelGGL(k_pcg_start, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->qc,
+                     ctx->gv, ctx->dscal + 0, ctx->rv, ctx->dv, ctx->mv, ctx->part);
+  CHK(reduce(ctx, nb, 3, 0, 16));  // {sum minv r, sum minv, sum r^2} at 16..18
+  CHK(fetch(ctx, 16, 3, s3));
+  const double rms0 = std::sqrt(s3[2] / n);
+  const double target = std::max(abs_tol, rel_tol * rms0);
+  hipLaunchKernelGGL(k_pcg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->rv, ctx->mv,
+                     ctx->dscal + 16, ctx->dscal + 17, 1, ctx->zv, ctx->pv, ctx->part);
+  CHK(reduce(ctx, nb, 1, 0, 20));
+  CHK(fetch(ctx, 20, 1, &rz));
+  int it = 0;
+  double rms = rms0;
+  while (rms > target && it < maxit) {
+    CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
+    hipEvent_t ev{};
+    tstart(ctx, 6, &ev);
+    if (f32)
+      hipLaunchKernelGGL((k_hmatvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->greal32, ctx->c_pme,
+                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
+    else
+      hipLaunchKernelGGL((k_hmatvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->greal64, ctx->c_pme,
+                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+    tstop(ctx, 6, ev);
+    HIPCHK(hipGetLastError());
+    CHK(reduce(ctx, nb, 2, 0, 21));  // {p.Ap, sum Ap} at 21..22
+    CHK(fetch(ctx, 21, 1, &pap));
+    if (!(pap > 0.0)) break;  // H is positive definite on sum(d) = 0; guards rounding
+    const double a = rz / pap;
+    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(BLOCK), 0, ctx->st, n, a, ctx->pv, ctx->apv,
+                       ctx->dscal + 22, ctx->mv, ctx->dv, ctx->rv, ctx->part);
+    HIPCHK(hipGetLastError());
+    CHK(reduce(ctx, nb, 2, 0, 24));  // {sum minv r, sum r^2} at 24..25
+    CHK(fetch(ctx, 24, 2, s2));
+    rms = std::sqrt(s2[1] / n);
+    ++it;
+    if (rms <= target) break;
+    hipLaunchKernelGGL(k_pcg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->rv, ctx->mv,
+                       ctx->dscal + 24, ctx->dscal + 17, 0, ctx->zv, ctx->pv, ctx->part);
+    CHK(reduce(ctx, nb, 1, 0, 20));
+    CHK(fetch(ctx, 20, 1, &rz_new));
+    const double b = rz_new / rz;
+    rz = rz_new;
+    hipLaunchKernelGGL(k_pcg_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, b, ctx->zv, ctx->pv);
+    HIPCHK(hipGetLastError());
+  }
+  *iters = it;
+  return 0;
+}
+
+// RMS of the projected gradient g - mean(g) of ctx->gv (sum in dscal[0]).
+int projected_rms(simox_ctx* ctx, double* rms) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  hipLaunchKernelGGL(k_center_sq, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->gv, ctx->dscal + 0,
+                     ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 1, 0, 3));
+  double s;
+  CHK(fetch(ctx, 3, 1, &s));
+  *rms = std::sqrt(s / n);
+  return 0;
+}
+
+// Converged charge solve: FP64 outer residual (the true gradient, k_dudq),
+// inner PCG in FP64 (mode FP64) or with the FP32 operator (mode MIXED), i.e.
+// iterative refinement.  Stops when the RMS of the projected gradient
+// (g - mean g, eV/e) is below ctx->tol.
+int ctip_linear(simox_ctx* ctx, int* iters, int* n_eval) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  const bool f32 = use32(ctx);
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qc);
+  HIPCHK(hipGetLastError());
+  int inner_total = 0, outer = 0, nev = 0;
+  bool ok = false;
+  for (outer = 0; outer < 40; ++outer) {
+    double rms;
+    CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false, false));
+    CHK(projected_rms(ctx, &rms));
+    ++nev;
+    if (rms <= ctx->tol) { ok = true; break; }
+    int it = 0;
+    CHK(pcg_inner(ctx, f32, f32 ? 1e-4 : 0.0, 0.3 * ctx->tol, ctx->maxit, &it));
+    inner_total += it;
+    nev += it;
+    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, 1.0, ctx->dv, ctx->qc);
+    HIPCHK(hipGetLastError());
+  }
+  ctx->last_inner = inner_total;
+  ctx->last_outer = outer;
+  if (iters) *iters = inner_total;
+  if (n_eval) *n_eval = nev;
+  if (!ok) {
+    ctx->err = "charge solve did not reach the requested tolerance";
+    return SIMOX_ERR_NOT_CONVERGED;
+  }
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->qc, ctx->q);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
   CHK(build_matrix(ctx));
   int it = 0, ne = 0;
-  CHK(ctip_faithful(ctx, &it, &ne));
+  if (ctx->mode == SIMOX_MODE_FAITHFUL) CHK(ctip_faithful(ctx, &it, &ne));
+  else CHK(ctip_linear(ctx, &it, &ne));
   if (iters) *iters = it;
   if (n_eval) *n_eval = ne;
   double e3[3];
@@ -678,6 +788,8 @@ int upload_caller(simox_ctx* ctx, const double* h, int w, double* dsorted) {
   hipLaunchKernelGGL(k_from_caller, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, w,
                      ctx->perm, ctx->dstage, dsorted);
   HIPCHK(hipGetLastError());
+  // the pinned and device staging buffers are reused by the next upload
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
 
@@ -850,19 +962,21 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->frec, n3));
   CHK(dalloc(ctx, &ctx->qc, n)); CHK(dalloc(ctx, &ctx->ql, n)); CHK(dalloc(ctx, &ctx->gv, n));
   CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
+  CHK(dalloc(ctx, &ctx->rv, n)); CHK(dalloc(ctx, &ctx->zv, n)); CHK(dalloc(ctx, &ctx->apv, n));
+  CHK(dalloc(ctx, &ctx->mv, n));
   CHK(dalloc(ctx, &ctx->bvec, n)); CHK(dalloc(ctx, &ctx->nb_cnt, n));
   CHK(dalloc(ctx, &ctx->nb_col, size_t(ctx->cap) * n));
+  CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
   if (mode == SIMOX_MODE_MIXED) CHK(dalloc(ctx, &ctx->nb_val32, size_t(ctx->cap) * n));
-  else CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
   // PME
   const size_t ng = size_t(fft[0]) * fft[1] * fft[2];
   const size_t nh = size_t(g.H) * fft[1] * fft[2];
   CHK(dalloc(ctx, &ctx->gfix, ng));
   HIPCHK(hipMemset(ctx->gfix, 0, ng * sizeof(long long)));
+  // FP64 chain always (faithful / FP64 modes, MIXED outer residual); FP32 chain for MIXED
+  CHK(dalloc(ctx, &ctx->greal64, ng)); CHK(dalloc(ctx, &ctx->spec64, nh)); CHK(dalloc(ctx, &ctx->G64, nh));
   if (mode == SIMOX_MODE_MIXED) {
     CHK(dalloc(ctx, &ctx->greal32, ng)); CHK(dalloc(ctx, &ctx->spec32, nh)); CHK(dalloc(ctx, &ctx->G32, nh));
-  } else {
-    CHK(dalloc(ctx, &ctx->greal64, ng)); CHK(dalloc(ctx, &ctx->spec64, nh)); CHK(dalloc(ctx, &ctx->G64, nh));
   }
   for (int d = 0; d < 3; ++d) {
     const int L = fft[d];
@@ -880,8 +994,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     const size_t lds_cap = 60 * 1024;
     ctx->M64[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(double2)))));
     ctx->M32[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(float2)))));
-    const size_t csz = mode == SIMOX_MODE_MIXED ? sizeof(float2) : sizeof(double2);
-    if (2 * size_t(ctx->stride[d]) * csz > lds_cap) {
+    if (2 * size_t(ctx->stride[d]) * sizeof(double2) > lds_cap) {
       ctx->err = "PME grid dimension too large for the LDS-resident FFT";
       return SIMOX_ERR_ARG;
     }
@@ -889,8 +1002,9 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   {
     const double a2 = g.alpha * g.alpha;
     const int nbk = int((nh + BLOCK - 1) / BLOCK);
-    if (mode == SIMOX_MODE_MIXED) hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G32);
-    else hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G64);
+    hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G64);
+    if (mode == SIMOX_MODE_MIXED)
+      hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G32);
     HIPCHK(hipGetLastError());
   }
   CHK(dalloc(ctx, &ctx->dscal, 64));
@@ -919,6 +1033,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->dF32, ctx->frec, ctx->gfix, ctx->greal64, ctx->greal32, ctx->spec64,
                   ctx->spec32, ctx->G64, ctx->G32, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1070,14 +1185,14 @@ int simox_pme(simox_ctx* ctx, double* erec) {
 int simox_phi(simox_ctx* ctx, double* phi) {
   if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
   if (!ctx->matrix_valid) CHK(build_matrix(ctx));
-  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, true));
+  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, true, use32(ctx)));
   return download_caller(ctx, ctx->phi, 1, phi);
 }
 
 int simox_dudq(simox_ctx* ctx, double* g) {
   if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
   if (!ctx->matrix_valid) CHK(build_matrix(ctx));
-  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, false));
+  CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, false, false));
   return download_caller(ctx, ctx->gv, 1, g);
 }
 

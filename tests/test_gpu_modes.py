@@ -1,0 +1,115 @@
+"""Converged (FP64) and mixed-precision (MIXED) modes on the GPU.
+
+FP64 mode: exact PME weights, charges solved to an RMS projected-gradient
+tolerance; checked against the exact minimiser of the same charge energy from
+the oracle (dense KKT solve with exact weights).  Bars: charges 1e-9, forces
+1e-10 relative.
+MIXED mode: FP32 pair / PME / sparse arithmetic, FP64 outer residual; checked
+against FP64 mode on the same state.  Bars: relative force error
+e_F = max_i |dF_i| / RMS_i |F_i| <= 1e-5, charges 1e-6, energies 1e-6 relative.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from simox_amd import MODE_FP64, MODE_MIXED, Simox
+from simox_amd.systems import tiny
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_oracle_math import kkt_minimiser  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def e_F(f, ref):
+    rms = np.sqrt((ref ** 2).sum(1).mean())
+    return np.sqrt(((f - ref) ** 2).sum(1)).max() / rms
+
+
+@pytest.fixture(scope="module")
+def small():
+    return tiny(seed=6, nx=3, ny=3, nz=3, n_o=8, fft=(12, 12, 24), rcut=5.0)
+
+
+def test_fp64_charges_are_the_exact_minimiser(small, oracle_mod):
+    s = small
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
+    g.set_charge_tol(1e-11, 2000)
+    g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+    res = g.ctip_solve()
+    q = g.get_state()[3]
+    o = oracle_mod.Oracle(s.n, s.box, s.fft, s.rcut)
+    o.cell_sort(s.x)
+    oracle_mod.set_spread_rounding(False)
+    try:
+        _, psi = o.pme_self(s.ids, np.zeros(s.n))
+        q_kkt = kkt_minimiser(o, s.ids, s.x, psi, q.copy())
+    finally:
+        oracle_mod.set_spread_rounding(True)
+    assert abs(q.sum()) < 1e-10
+    assert np.abs(q - q_kkt).max() <= 1e-9
+    assert res["iters"] > 0
+
+
+def test_fp64_forces_match_oracle_exact_weights(small, oracle_mod):
+    s = small
+    rng = np.random.default_rng(1)
+    q = rng.normal(size=s.n) * 0.3
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
+    g.set_state(s.ids, s.x, s.v, None, q)
+    g.force()
+    erec = g.pme()
+    f = g.get_state()[2]
+    o = oracle_mod.Oracle(s.n, s.box, s.fft, s.rcut)
+    o.cell_sort(s.x)
+    oracle_mod.set_spread_rounding(False)
+    try:
+        _, psi = o.pme_self(s.ids, q)
+        ref = o.force(s.ids, s.x, q)
+        e_o, f_o = o.pme(s.ids, s.x, q, psi, ref["f"])
+    finally:
+        oracle_mod.set_spread_rounding(True)
+    assert np.abs(f - f_o).max() <= 1e-10 * np.abs(f_o).max()
+    assert erec == pytest.approx(e_o, rel=1e-10)
+
+
+@pytest.mark.parametrize("name", ["tiny", "c1"])
+def test_mixed_matches_fp64(name):
+    from simox_amd.systems import config
+    s = tiny(seed=2) if name == "tiny" else config("c1")
+    out = {}
+    for mode in (MODE_FP64, MODE_MIXED):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_charge_tol(1e-10, 2000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        r = g.ctip_solve()
+        e = g.force()
+        erec = g.pme()
+        x, v, f, q = g.get_state()
+        out[mode] = dict(f=f, q=q, e=e, erec=erec, eself=r["Eself"])
+    a, b = out[MODE_MIXED], out[MODE_FP64]
+    assert e_F(a["f"], b["f"]) <= 1e-5
+    assert np.abs(a["q"] - b["q"]).max() <= 1e-6
+    for k in ("Epair", "Egroup", "Edir", "Ectip"):
+        assert a["e"][k] == pytest.approx(b["e"][k], rel=1e-6, abs=1e-6)
+    assert a["erec"] == pytest.approx(b["erec"], rel=1e-6)
+    assert a["eself"] == pytest.approx(b["eself"], rel=1e-8)
+
+
+def test_mixed_nve_energy_tracks_fp64():
+    """20 NVE steps (CTIP every step) in both modes: same total-energy history
+    to 1e-6 relative."""
+    s = tiny(seed=2)
+    hist = {}
+    for mode in (MODE_FP64, MODE_MIXED):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_charge_tol(1e-10, 2000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        g.ctip_solve()
+        g.force()
+        g.pme()
+        hist[mode] = [g.step(1, s.dt, 1, 1)["Etotal"] for _ in range(20)]
+    a, b = np.array(hist[MODE_MIXED]), np.array(hist[MODE_FP64])
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
