@@ -92,41 +92,78 @@ def cpu_baseline(seconds=15.0):
                       "CTIP from q=0 then warm, FORCE, PME, Verlet) of C1 (1,216 atoms)" % k}
 
 
+def run_mode(s, mode_name, args, timed=True):
+    from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox
+    mode_id = {"faithful": MODE_FAITHFUL, "fp64": MODE_FP64, "mixed": MODE_MIXED}[mode_name]
+    g = Simox.from_system(s, mode=mode_id)
+    g.set_charge_tol(args.tol, 5000)
+    # initial forces at x(0) with q = 0 (the first step re-equilibrates charges)
+    g.force()
+    g.pme()
+    print("bench: %s warmup %d steps" % (mode_name, args.warmup), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        g.step(1, s.dt, 1, 1, stats=False)
+    g.timing(True)
+    print("bench: %s timing %d steps" % (mode_name, args.steps), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    st = g.step(args.steps, s.dt, 1, 1, stats=True)
+    elapsed = time.perf_counter() - t0
+    times = {w: g.kernel_time(w) for w in KERNELS}
+    g.timing(False)
+    return g, st, elapsed, times
+
+
+def accuracy_vs_fp64(s, g_mixed, args):
+    """Force / charge / energy error of the mixed path against the FP64 path on
+    the same state (positions and velocities of the mixed run; FP64 charges
+    solved to the same tolerance, warm-started from the mixed charges)."""
+    from simox_amd import MODE_FP64, Simox
+    x, v, f_m, q_m = g_mixed.get_state()
+    e_m = g_mixed.force()
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
+    g.set_charge_tol(args.tol, 5000)
+    g.set_state(s.ids, x, v, None, q_m)
+    g.ctip_solve()
+    e_d = g.force()
+    g.pme()
+    _, _, f_d, q_d = g.get_state()
+    g_mixed.force()
+    g_mixed.pme()
+    f_m = g_mixed.get_state()[2]
+    rms = np.sqrt((f_d ** 2).sum(1).mean())
+    eF = float(np.sqrt(((f_m - f_d) ** 2).sum(1)).max() / rms)
+    eF_frob = float(np.linalg.norm(f_m - f_d) / np.linalg.norm(f_d))
+    e_q = float(np.abs(q_m - q_d).max())
+    e_E = max(abs(e_m[k] - e_d[k]) / abs(e_d[k]) for k in ("Epair", "Egroup", "Edir", "Ectip"))
+    g.close()
+    return {"force_rel_err_max": eF, "force_rel_err_frob": eF_frob, "charge_abs_err_max": e_q,
+            "energy_rel_err_max": float(e_E), "reference": "FP64 GPU path, same state"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="faithful", choices=["faithful", "fp64", "mixed"])
+    ap.add_argument("--mode", default="mixed", choices=["faithful", "fp64", "mixed"])
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64 comparison run (mixed mode only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tol", type=float, default=1e-10,
+                    help="converged charge solve: RMS projected gradient (eV/e)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU hot path "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         return 2
 
-    from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox
     from simox_amd.systems import config
-    mode_id = {"faithful": MODE_FAITHFUL, "fp64": MODE_FP64, "mixed": MODE_MIXED}[args.mode]
     print("bench: building %s inputs" % args.config, file=sys.stderr, flush=True)
     s = config(args.config)
-    g = Simox.from_system(s, mode=mode_id)
-    # initial forces at x(0) with q = 0 (the first step re-equilibrates charges)
-    g.force()
-    g.pme()
-    print("bench: warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        g.step(1, s.dt, 1, 1, stats=False)
-    g.timing(True)
-    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    st = g.step(args.steps, s.dt, 1, 1, stats=True)
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    times = {w: g.kernel_time(w) for w in KERNELS}
-    g.timing(False)
+    g, st, elapsed, times = run_mode(s, args.mode, args)
+    cg_iters, cg_restarts = g.charge_stats()
     cnt, _ = g.neighbors(stride=0)
     nb_entries = int(cnt.sum())
     # dominant kernel: largest total time in the timed region
@@ -144,7 +181,9 @@ def main():
         "config": {"workload": WORKLOADS[args.config], "mode": args.mode, "n_atoms": s.n,
                    "pme_grid": list(s.fft), "ctip_every": 1, "sort_every": 1,
                    "ns_per_day": args.steps / elapsed * 1.0 * 86400 * 1e-6,
-                   "neighbor_entries": nb_entries, "E_total_last": st["Etotal"]},
+                   "neighbor_entries": nb_entries, "E_total_last": st["Etotal"],
+                   "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
+                   "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bytes_launch,
@@ -152,6 +191,19 @@ def main():
         "kernels_ms": {KERNELS[w]: round(times[w][0], 3) for w in KERNELS},
         "kernels_launches": {KERNELS[w]: times[w][1] for w in KERNELS},
     }
+    if args.mode == "mixed" and not args.no_compare:
+        print("bench: accuracy vs fp64", file=sys.stderr, flush=True)
+        out["accuracy"] = accuracy_vs_fp64(s, g, args)
+        g.close()
+        g2, _, el2, _ = run_mode(s, "fp64", args)
+        it2, _ = g2.charge_stats()
+        g2.close()
+        v2 = s.n * args.steps / el2
+        out["fp64_path"] = {"value": v2, "ms_per_step": el2 / args.steps * 1e3,
+                            "charge_iters_last_step": it2}
+        out["speedup_vs_fp64"] = value / v2
+    else:
+        g.close()
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

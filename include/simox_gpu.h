@@ -41,13 +41,14 @@ extern "C" {
    rounded PME spreading weights (force.f90:406, ctip.f90:359) and the
    nonlinear CG + Brent charge solve with its 1e-3 stop (ctip.f90:167-304). */
 #define SIMOX_MODE_FAITHFUL 0
-/* FP64 kernels with a converged charge solve: projected linear CG with an
-   active-set outer loop on the same charge energy, to a relative residual tol. */
+/* FP64 kernels with a converged charge solve: preconditioned nonlinear CG with
+   exact line searches on the same (piecewise-quadratic) charge energy, to an
+   RMS projected-gradient tolerance (simox_set_charge_tol, default 1e-10 eV/e). */
 #define SIMOX_MODE_FP64 1
 /* FP32 pair / PME / sparse arithmetic with FP32 partial sums; FP64 positions,
    velocities, Verlet update, CG residuals, dot products, neutrality multiplier
-   and energy totals.  Charge solve = iterative refinement: FP32 inner CG, FP64
-   outer residual. */
+   and energy totals.  Charge solve = iterative refinement: nonlinear CG with the
+   FP32 operator, restarted from the FP64 true gradient. */
 #define SIMOX_MODE_MIXED 2
 
 /* ---- potential parameters: TYPE METL / OXID / POTN (datafmt.f90:54-74) --- */
@@ -83,7 +84,8 @@ typedef struct simox_ctx simox_ctx;
 
 /* Context = one system: periodic box, PME grid, cutoff, atom count, mode.
    Replaces READ_INPUT's geometry set-up (main.f90:137-201): Ncell =
-   INT(box/rcut), Lcell = box/Ncell, Ewald alpha = 3.8/rcut.  params may be
+   INT(box/rcut), Lcell = box/Ncell, Ewald alpha = 3.8/rcut.  PME grid sizes
+   must be multiples of 4 with prime factors 2, 3, 5 only.  params may be
    NULL (published parameters).  Allocates all device buffers; nothing is
    allocated by later calls except when a neighbour table must grow. */
 int simox_create(int n, const double box[3], const int fft[3], double rcut, int mode,
@@ -92,8 +94,15 @@ int simox_destroy(simox_ctx* ctx);
 const char* simox_last_error(const simox_ctx* ctx);
 int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
 int simox_set_mode(simox_ctx* ctx, int mode);
-/* Tolerance / iteration cap of the linear charge solve (modes FP64, MIXED). */
+/* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged
+   charge solve (modes FP64, MIXED). */
 int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
+/* Preconditioner of the converged charge solve: 0 = Jacobi (default),
+   1 = Coulomb screening (one extra PME-grid pass per iteration; fewer
+   iterations from cold starts, not for warm-started MD steps). */
+int simox_set_charge_precond(simox_ctx* ctx, int on);
+/* Iterations and FP64 restarts of the last converged charge solve. */
+int simox_charge_stats(simox_ctx* ctx, int* iters, int* restarts);
 
 /* Host <-> device state of TYPE PTCL q(:) (datafmt.f90:49-52), caller order.
    Any pointer may be NULL to leave that component untouched.  set_state with

@@ -21,7 +21,7 @@ EXPORTS = [
     "simox_get_state", "simox_cell_sort", "simox_get_cells", "simox_force", "simox_get_rho",
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
-    "simox_timing_enable",
+    "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats",
 ]
 
 # simox_kernel_timing categories
@@ -95,6 +95,8 @@ def load_library(path=LIB_PATH):
     lib.simox_step.argtypes = [vp, i, d, i, i, vp]
     lib.simox_kernel_timing.argtypes = [vp, i, P(d), P(C.c_longlong)]
     lib.simox_timing_enable.argtypes = [vp, i]
+    lib.simox_set_charge_precond.argtypes = [vp, i]
+    lib.simox_charge_stats.argtypes = [vp, P(i), P(i)]
     lib.simox_potential_setup.argtypes = [vp]
     lib.simox_fafb.argtypes = [vp]
     _lib = lib
@@ -168,6 +170,14 @@ class Simox:
 
     def set_charge_tol(self, tol, maxit=1000):
         self._check(self.lib.simox_set_charge_tol(self.h, float(tol), int(maxit)))
+
+    def set_charge_precond(self, on=True):
+        self._check(self.lib.simox_set_charge_precond(self.h, 1 if on else 0))
+
+    def charge_stats(self):
+        it, rs = C.c_int(), C.c_int()
+        self._check(self.lib.simox_charge_stats(self.h, C.byref(it), C.byref(rs)))
+        return it.value, rs.value
 
     # TYPE PTCL q(:) in / out
     def set_state(self, ids=None, x=None, v=None, f=None, q=None):

@@ -175,42 +175,56 @@ __device__ inline void for_each_neighbor(const Geom& g, const int* __restrict__ 
 
 #define SX_ERR_SET(ptr, code) atomicCAS((ptr), 0, (code))
 
-// K2 + K3: EAM density rho_i = sum_j f_j(r) and the embedding F, F' per atom.
+// Separation of sorted atoms i and j (exact reference arithmetic).
+__device__ inline void pair_sep(const Geom& g, const double* __restrict__ x, int i, int j,
+                                double rx[3], double& r2) {
+  min_image(g, x[3 * i], x[3 * i + 1], x[3 * i + 2], x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
+}
+
+// K1b + K2 + K3: neighbour list, EAM density and embedding in one walk of the
+// 27-cell shell.  The list holds every j with r2 < Rcut^2 (the exact test of
+// force.f90:53-54), in ELL layout stored column-major (entry k of atom i at
+// k*n + i: coalesced across a wave); the force and CTIP-matrix kernels
+// iterate it instead of re-testing ~7x more candidates.
 // egroup_part[blockIdx.x] = sum of F over the block.
 template <class T>
-__global__ void __launch_bounds__(BLOCK) k_density(Geom g, const Tables* __restrict__ tb,
-                                                   const double* __restrict__ x,
-                                                   const int* __restrict__ id,
-                                                   const int* __restrict__ cell_of,
-                                                   const int* __restrict__ cell_start,
-                                                   double* __restrict__ rho_out,
-                                                   T* __restrict__ dF_out,
-                                                   double* __restrict__ egroup_part,
-                                                   int* __restrict__ err) {
+__global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* __restrict__ tb,
+                                                         const double* __restrict__ x,
+                                                         const int* __restrict__ id,
+                                                         const int* __restrict__ cell_of,
+                                                         const int* __restrict__ cell_start,
+                                                         int cap, int* __restrict__ cnt,
+                                                         int* __restrict__ col,
+                                                         double* __restrict__ rho_out,
+                                                         T* __restrict__ dF_out,
+                                                         double* __restrict__ egroup_part,
+                                                         int* __restrict__ err) {
   __shared__ double lds[BLOCK / WAVE];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double eg[1] = {0.0};
   if (i < g.n) {
     const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
     const int si = id[i];
+    const size_t n = size_t(g.n);
     T rho = T(0);
-    int nnb = 0;
+    int k = 0;
     bool bad = si < 1 || si > NSPEC;
     for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
       double rx[3], r2;
       min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
       if (r2 < g.rcut2) {
+        if (k < cap) col[size_t(k) * n + i] = j;
+        ++k;
         const int sj = id[j];
-        ++nnb;
         if (bad || sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid) { bad = true; return; }
-        T r = T(sqrt(r2));
         T f, dfr;
-        dens_eval<T>(tb->pc[pidx(si, sj)].dj, r, f, dfr);
+        dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
         rho += f;
       }
     });
     if (bad) SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR);
-    if (nnb > NNB_MAX) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
+    if (k > NNB_MAX || k > cap) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
+    cnt[i] = k < cap ? k : cap;
     T F = T(0), dF = T(0);
     if (!bad) embed<T>(tb->ec[si - 1], rho, F, dF);
     rho_out[i] = double(rho);
@@ -221,16 +235,17 @@ __global__ void __launch_bounds__(BLOCK) k_density(Geom g, const Tables* __restr
   if (threadIdx.x == 0) egroup_part[blockIdx.x] = eg[0];
 }
 
-// K4: EAM pair + embedding forces, real-space Ewald, CTIP short-range terms.
-// Overwrites f (force.f90:28); e_part[3*blockIdx.x + {0,1,2}] = half-pair sums of
-// phi, q_i q_j erfc(ar)/r and the CTIP energy (FORCE's Epair, Edir/kc, Ectip/kc).
+// K4: EAM pair + embedding forces, real-space Ewald, CTIP short-range terms,
+// over the neighbour list.  Overwrites f (force.f90:28);
+// e_part[3*blockIdx.x + {0,1,2}] = half-pair sums of phi, q_i q_j erfc(ar)/r
+// and the CTIP energy (FORCE's Epair, Edir/kc, Ectip/kc).
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restrict__ tb,
                                                  const double* __restrict__ x,
                                                  const double* __restrict__ q,
                                                  const int* __restrict__ id,
-                                                 const int* __restrict__ cell_of,
-                                                 const int* __restrict__ cell_start,
+                                                 const int* __restrict__ cnt,
+                                                 const int* __restrict__ col,
                                                  const T* __restrict__ dF,
                                                  double* __restrict__ f,
                                                  double* __restrict__ e_part) {
@@ -238,47 +253,47 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double es[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
-    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
     const int si = id[i];
     const T qi = T(q[i]);
     const T dFi = dF[i];
     const T a = T(g.alpha), a2 = a * a;
     T fx = T(0), fy = T(0), fz = T(0);
     T ep = T(0), ed = T(0), ec = T(0);
+    const size_t n = size_t(g.n);
+    const int c = cnt[i];
     if (si >= 1 && si <= NSPEC) {
-      for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
+      for (int k = 0; k < c; ++k) {
+        const int j = col[size_t(k) * n + i];
         double rxd[3], r2d;
-        min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rxd, r2d);
-        if (r2d < g.rcut2) {
-          const int sj = id[j];
-          if (sj < 1 || sj > NSPEC) return;
-          const PairCoef& pc = tb->pc[pidx(si, sj)];
-          if (!pc.valid) return;
-          const T r2 = T(r2d);
-          const T r = T(sqrt(r2d));
-          const T qj = T(q[j]);
-          T fi_, dfi, fj_, dfj, phi, dfp;
-          dens_eval<T>(pc.di, r, fi_, dfi);
-          dens_eval<T>(pc.dj, r, fj_, dfj);
-          phi_eval<T>(pc.ph, r, phi, dfp);
-          // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356)
-          T s = dfp - dFi * dfj - dF[j] * dfi;
-          // real-space Ewald (force.f90:94-96)
-          T ar = a * r;
-          T tmp = erfc(ar);
-          s += T(KC) * qi * qj * (T(2) * fexp<T>(-a2 * r2) * a * r / T(SQRTPI) + tmp) / r2 / r;
-          ed += qi * qj * tmp / r;
-          // CTIP Slater terms (force.f90:101-140)
-          T dfc, e;
-          ctip_force_terms<T>(pc, r, r2, qi, qj, dfc, e);
-          s += dfc;
-          ec += e;
-          ep += phi;
-          fx += T(rxd[0]) * s;
-          fy += T(rxd[1]) * s;
-          fz += T(rxd[2]) * s;
-        }
-      });
+        pair_sep(g, x, i, j, rxd, r2d);
+        const int sj = id[j];
+        if (sj < 1 || sj > NSPEC) continue;
+        const PairCoef& pc = tb->pc[pidx(si, sj)];
+        if (!pc.valid) continue;
+        const T r2 = T(r2d);
+        const T r = T(sqrt(r2d));
+        const T qj = T(q[j]);
+        T fi_, dfi, fj_, dfj, phi, dfp;
+        dens_eval<T>(pc.di, r, fi_, dfi);
+        dens_eval<T>(pc.dj, r, fj_, dfj);
+        phi_eval<T>(pc.ph, r, phi, dfp);
+        // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356)
+        T s = dfp - dFi * dfj - dF[j] * dfi;
+        // real-space Ewald (force.f90:94-96)
+        T ar = a * r;
+        T tmp = erfc(ar);
+        s += T(KC) * qi * qj * (T(2) * fexp<T>(-a2 * r2) * a * r / T(SQRTPI) + tmp) / r2 / r;
+        ed += qi * qj * tmp / r;
+        // CTIP Slater terms (force.f90:101-140)
+        T dfc, e;
+        ctip_force_terms<T>(pc, r, r2, qi, qj, dfc, e);
+        s += dfc;
+        ec += e;
+        ep += phi;
+        fx += T(rxd[0]) * s;
+        fy += T(rxd[1]) * s;
+        fz += T(rxd[2]) * s;
+      }
     }
     f[3 * i] = double(fx);
     f[3 * i + 1] = double(fy);
@@ -295,50 +310,39 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
   }
 }
 
-// K5: CTIP sparse matrix, ELL layout stored column-major (entry k of row i at
-// k*n + i, coalesced across a wave).  Only sym is stored; the asym part enters
-// the solve only through b_i = sum_j Z_j asym_ij, which is constant for a
-// whole solve (the reference recomputes it per gradient, ctip.f90:327).
+// K5: CTIP sparse matrix over the neighbour list: val[k*n + i] = sym_ij for
+// the list's entry k.  Only sym is stored; the asym part enters the solve only
+// through b_i = sum_j Z_j asym_ij, which is constant for a whole solve (the
+// reference recomputes it per gradient, ctip.f90:327).
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __restrict__ tb,
                                                        const double* __restrict__ x,
                                                        const int* __restrict__ id,
-                                                       const int* __restrict__ cell_of,
-                                                       const int* __restrict__ cell_start,
-                                                       int* __restrict__ cnt,
-                                                       int* __restrict__ col,
+                                                       const int* __restrict__ cnt,
+                                                       const int* __restrict__ col,
                                                        T* __restrict__ val,
-                                                       double* __restrict__ b, int cap,
-                                                       int* __restrict__ err) {
+                                                       double* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
-  const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
   const int si = id[i];
   const T alpha = T(g.alpha);
-  int k = 0;
-  double bi = 0.0;
   const size_t n = size_t(g.n);
-  if (si < 1 || si > NSPEC) { SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR); cnt[i] = 0; b[i] = 0; return; }
-  for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
-    double rx[3], r2;
-    min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
-    if (r2 < g.rcut2) {
+  const int c = cnt[i];
+  double bi = 0.0;
+  if (si >= 1 && si <= NSPEC) {
+    for (int k = 0; k < c; ++k) {
+      const int j = col[size_t(k) * n + i];
+      double rx[3], r2;
+      pair_sep(g, x, i, j, rx, r2);
       const int sj = id[j];
-      if (sj < 1 || sj > NSPEC) { SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR); return; }
+      if (sj < 1 || sj > NSPEC) continue;
       const PairCoef& pc = tb->pc[pidx(si, sj)];
-      T r = T(sqrt(r2));
       T sym, asym;
-      ctip_matrix_terms<T>(pc, r, alpha, sym, asym);
+      ctip_matrix_terms<T>(pc, T(sqrt(r2)), alpha, sym, asym);
       bi += double(T(pc.Z_j) * asym);
-      if (k < cap) {
-        col[size_t(k) * n + i] = j;
-        val[size_t(k) * n + i] = sym;
-      }
-      ++k;
+      val[size_t(k) * n + i] = sym;
     }
-  });
-  if (k > cap) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
-  cnt[i] = k < cap ? k : cap;
+  }
   b[i] = bi;
 }
 

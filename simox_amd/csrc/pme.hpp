@@ -133,6 +133,79 @@ __device__ inline C* lds_fft(C* b0, C* b1, int M, int L, int stride, const FftPl
   return src;
 }
 
+// Compile-time plans: for the grid sizes of the workloads the line length is a
+// template parameter, so every stage's radix, span and the index arithmetic
+// (divisions by L/R and Ns) are constants.
+__host__ __device__ constexpr int ct_radix(int m) {
+  return (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
+}
+__host__ __device__ constexpr int ct_nstages(int L) {
+  int m = L, n = 0;
+  while (m > 1) { m /= ct_radix(m); ++n; }
+  return n;
+}
+__host__ __device__ constexpr int ct_radix_at(int L, int s) {
+  int m = L;
+  for (int k = 0; k < s; ++k) m /= ct_radix(m);
+  return ct_radix(m);
+}
+__host__ __device__ constexpr int ct_ns_at(int L, int s) {
+  int m = L, ns = 1;
+  for (int k = 0; k < s; ++k) { const int r = ct_radix(m); ns *= r; m /= r; }
+  return ns;
+}
+
+template <class C, int L, int R, int Ns>
+__device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ dst, int M,
+                                    int stride, const C* __restrict__ tw, int sign) {
+  constexpr int nb = L / R;
+  constexpr int tws = L / (Ns * R);
+  const int total = M * nb;
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    const int line = b / nb;
+    const int j = b - line * nb;
+    const int k = j % Ns;
+    const C* s = src + line * stride;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C a = s[j + r * nb];
+      if (r > 0 && k > 0) {
+        C w = tw[r * k * tws];
+        if (sign > 0) w.y = -w.y;
+        a = cmul(a, w);
+      }
+      v[r] = a;
+    }
+    Dft<C, R>::run(v, sign);
+    C* d = dst + line * stride + (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
+  }
+}
+
+template <class C, int L, int S>
+__device__ inline void fft_stages_ct(C* a, C* b, int M, int stride, const C* __restrict__ tw,
+                                     int sign) {
+  if constexpr (S < ct_nstages(L)) {
+    fft_stage_ct<C, L, ct_radix_at(L, S), ct_ns_at(L, S)>(a, b, M, stride, tw, sign);
+    __syncthreads();
+    fft_stages_ct<C, L, S + 1>(b, a, M, stride, tw, sign);
+  }
+}
+
+// L_CT > 0: compile-time length; 0: runtime plan
+template <class C, int L_CT>
+__device__ inline C* lds_fft_any(C* b0, C* b1, int M, int L, int stride, const FftPlan& pl,
+                                 const C* __restrict__ tw, int sign) {
+  if constexpr (L_CT > 0) {
+    fft_stages_ct<C, L_CT, 0>(b0, b1, M, stride, tw, sign);
+    return (ct_nstages(L_CT) % 2 == 0) ? b0 : b1;
+  } else {
+    return lds_fft(b0, b1, M, L, stride, pl, tw, sign);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // B-spline weights of order 4 and their u-derivatives (force.f90:388-400)
 // ---------------------------------------------------------------------------
@@ -168,54 +241,11 @@ __device__ inline int wrapk(int b, int nn, int K) {
   return v < 0 ? v + K : v;
 }
 
-// K6: spread q (or q + step*p, written to ql) onto the fixed-point grid.
-// F32W: FP32-rounded weights as the reference's default-kind CMPLX makes them
-// (force.f90:406, ctip.f90:359; quirk Q1).
-template <bool F32W>
-__global__ void __launch_bounds__(BLOCK) k_spread(Geom g, const double* __restrict__ x,
-                                                  const double* __restrict__ q,
-                                                  const double* __restrict__ p, double step,
-                                                  double* __restrict__ ql,
-                                                  unsigned long long* __restrict__ grid) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
-  double qi = q[i];
-  if (p) {
-    qi = qi + step * p[i];
-    ql[i] = qi;
-  }
-  double u[3], AX[3][4], AXX[3][4];
-  int b[3];
-  grid_coord(g, x + 3 * i, u, b);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) bspline4(u[d], AX[d], AXX[d]);
-  int ix[4], iy[4], iz[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    ix[a] = wrapk(b[0], a, g.K[0]);
-    iy[a] = wrapk(b[1], a, g.K[1]);
-    iz[a] = wrapk(b[2], a, g.K[2]);
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-#pragma unroll
-    for (int bb = 0; bb < 4; ++bb) {
-      const size_t row = size_t(g.K[0]) * (size_t(iy[bb]) + size_t(g.K[1]) * size_t(iz[c]));
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        double w = AX[0][a] * AX[1][bb] * AX[2][c];
-        if (F32W) w = double(float(w));
-        const long long v = llrint(qi * w * GRID_SCALE);
-        atomicAdd(&grid[row + ix[a]], (unsigned long long)v);
-      }
-    }
-}
-
-// K7a: R2C along x.  Reads the fixed-point grid (and zeroes it for the next
-// spread), transforms M lines per block in LDS, writes kx = 0..Kx/2.
-template <class C>
+// K7a: R2C along x.  Reads the real charge grid, transforms M lines per
+// block in LDS, writes kx = 0..Kx/2.
+template <class C, int L_CT>
 __global__ void __launch_bounds__(BLOCK) k_fft_x_r2c(Geom g, FftPlan pl, const C* __restrict__ tw,
-                                                     long long* __restrict__ gfix,
+                                                     const decltype(C{}.x)* __restrict__ qgrid,
                                                      C* __restrict__ spec, int M, int stride) {
   using R = decltype(C{}.x);
   extern __shared__ __align__(16) unsigned char smem[];
@@ -227,16 +257,11 @@ __global__ void __launch_bounds__(BLOCK) k_fft_x_r2c(Geom g, FftPlan pl, const C
     const int t = e / L, xk = e - t * L;
     const int line = line0 + t;
     C v; v.x = R(0); v.y = R(0);
-    if (line < nlines) {
-      const size_t gi = size_t(line) * L + xk;
-      const long long iv = gfix[gi];
-      gfix[gi] = 0;
-      v.x = R(double(iv) * GRID_INV_SCALE);
-    }
+    if (line < nlines) v.x = qgrid[size_t(line) * L + xk];
     b0[t * stride + xk] = v;
   }
   __syncthreads();
-  const C* res = lds_fft(b0, b1, M, L, stride, pl, tw, -1);
+  const C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, -1);
   for (int e = threadIdx.x; e < M * H; e += blockDim.x) {
     const int t = e / H, k = e - t * H;
     const int line = line0 + t;
@@ -249,7 +274,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_x_r2c(Geom g, FftPlan pl, const C
 // outer index by.  FUSE: forward FFT, multiply by the influence table G
 // (same layout as spec, 1/Ng folded in), accumulate sum w_kx G |X|^2 into
 // epart, inverse FFT.
-template <class C, bool FUSE>
+template <class C, bool FUSE, int L_CT>
 __global__ void __launch_bounds__(BLOCK) k_fft_strided(Geom g, FftPlan pl,
                                                        const C* __restrict__ tw,
                                                        C* __restrict__ spec, int es, int os,
@@ -272,7 +297,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_strided(Geom g, FftPlan pl,
     b0[t * stride + l] = v;
   }
   __syncthreads();
-  C* res = lds_fft(b0, b1, M, L, stride, pl, tw, FUSE ? -1 : sign);
+  C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, FUSE ? -1 : sign);
   if (FUSE) {
     double en[1] = {0.0};
     for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
@@ -288,7 +313,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_strided(Geom g, FftPlan pl,
     }
     __syncthreads();
     C* other = (res == b0) ? b1 : b0;
-    res = lds_fft(res, other, M, L, stride, pl, tw, +1);
+    res = lds_fft_any<C, L_CT>(res, other, M, L, stride, pl, tw, +1);
     block_sum<1>(en, lds_red);
     if (threadIdx.x == 0) epart[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = en[0];
   }
@@ -300,7 +325,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_strided(Geom g, FftPlan pl,
 }
 
 // K7c: C2R along x (inverse, unnormalised); rebuilds the Hermitian line.
-template <class C>
+template <class C, int L_CT>
 __global__ void __launch_bounds__(BLOCK) k_fft_x_c2r(Geom g, FftPlan pl, const C* __restrict__ tw,
                                                      const C* __restrict__ spec,
                                                      decltype(C{}.x)* __restrict__ out, int M,
@@ -326,7 +351,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_x_c2r(Geom g, FftPlan pl, const C
     b0[t * stride + k] = v;
   }
   __syncthreads();
-  const C* res = lds_fft(b0, b1, M, L, stride, pl, tw, +1);
+  const C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, +1);
   for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
     const int t = e / L, k = e - t * L;
     const int line = line0 + t;
@@ -452,6 +477,222 @@ __device__ inline double gather_phi(const Geom& g, const double* xi, const Rg* _
         e += AX[0][a] * AX[1][bb] * AX[2][c] * double(grid[gi]);
       }
   return e;
+}
+
+}  // namespace sx
+
+namespace sx {
+
+// ---------------------------------------------------------------------------
+// Brick-tiled spread.  Per set of positions, atoms are binned into bricks of
+// 4x4x4 grid points by their base point INT(u) mod K (grid sizes are multiples
+// of 4) and listed brick by brick in ascending atom order.  Per evaluation a
+// workgroup owns one tile of TX x TY x TZ grid points: it walks the bricks
+// whose atoms can reach the tile (base within [lo-2, hi+1]), accumulates
+// q w into the tile in LDS with 64-bit integer atomics at the fixed-point
+// scale (order-independent, hence deterministic), and writes every grid point
+// of the tile exactly once.  No global atomics and no zeroing pass.
+// ---------------------------------------------------------------------------
+constexpr int BRK = 4;
+constexpr int TX = 16, TY = 16, TZ = 8;
+constexpr int TILE = TX * TY * TZ;
+constexpr int MAXB = 6 * 6 * 4;  // candidate bricks per tile (TX/4+2)(TY/4+2)(TZ/4+2)
+
+__global__ void k_brick_assign(Geom g, const double* __restrict__ x, int* __restrict__ brick,
+                               int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  double u[3];
+  int b[3];
+  grid_coord(g, x + 3 * i, u, b);
+  int bi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int r = b[d] % g.K[d];
+    if (r < 0) r += g.K[d];
+    bi[d] = r / BRK;
+  }
+  const int nbx = g.K[0] / BRK, nby = g.K[1] / BRK;
+  const int id = bi[0] + nbx * (bi[1] + nby * bi[2]);
+  brick[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// slot of atom i in its brick = number of atoms of the same brick with a
+// smaller index (bricks hold a few atoms); writes border[start + slot] = i.
+__global__ void k_brick_place(int n, const int* __restrict__ brick, const int* __restrict__ start,
+                              const int* __restrict__ fillpos, const int* __restrict__ unordered,
+                              int* __restrict__ border) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = brick[i];
+  const int s0 = start[b], s1 = start[b + 1];
+  int rank = 0;
+  for (int k = s0; k < s1; ++k) rank += unordered[k] < i;
+  border[s0 + rank] = i;
+  (void)fillpos;
+}
+
+__global__ void k_brick_scatter(int n, const int* __restrict__ brick, const int* __restrict__ start,
+                                int* __restrict__ fill, int* __restrict__ unordered) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = brick[i];
+  unordered[start[b] + atomicAdd(&fill[b], 1)] = i;
+}
+
+// K6 (tiled): grid[tile] = sum over atoms of q w, written as R (double/float).
+template <bool F32W, class R>
+__global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __restrict__ x,
+                                                       const double* __restrict__ q,
+                                                       const int* __restrict__ bstart,
+                                                       const int* __restrict__ border,
+                                                       R* __restrict__ grid) {
+  __shared__ unsigned long long acc[TILE];
+  __shared__ int bofs[MAXB + 1];
+  __shared__ int bbeg[MAXB];
+  const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int lo[3] = {tx * TX, ty * TY, tz * TZ};
+  const int ext[3] = {min(TX, g.K[0] - lo[0]), min(TY, g.K[1] - lo[1]), min(TZ, g.K[2] - lo[2])};
+  const int nb[3] = {g.K[0] / BRK, g.K[1] / BRK, g.K[2] / BRK};
+  // candidate brick ranges per axis
+  int fb[3], cb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int a = lo[d] - 2, b = lo[d] + ext[d];  // base residues lo-2 .. hi+1
+    int f0 = (a >= 0) ? a / BRK : -((-a + BRK - 1) / BRK);
+    int l0 = b / BRK;
+    int c = l0 - f0 + 1;
+    if (c >= nb[d]) { f0 = 0; c = nb[d]; }
+    fb[d] = f0;
+    cb[d] = c;
+  }
+  const int ncand = cb[0] * cb[1] * cb[2];
+  for (int k = threadIdx.x; k < TILE; k += blockDim.x) acc[k] = 0ull;
+  if (threadIdx.x < ncand) {
+    const int k = threadIdx.x;
+    const int ix = k % cb[0], iy = (k / cb[0]) % cb[1], iz = k / (cb[0] * cb[1]);
+    int bx = (fb[0] + ix) % nb[0]; if (bx < 0) bx += nb[0];
+    int by = (fb[1] + iy) % nb[1]; if (by < 0) by += nb[1];
+    int bz = (fb[2] + iz) % nb[2]; if (bz < 0) bz += nb[2];
+    const int bid = bx + nb[0] * (by + nb[1] * bz);
+    bbeg[k] = bstart[bid];
+    bofs[k + 1] = bstart[bid + 1] - bstart[bid];
+  }
+  if (threadIdx.x == 0) bofs[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 1; k <= ncand; ++k) bofs[k] += bofs[k - 1];
+  __syncthreads();
+  const int total = bofs[ncand];
+  for (int f = threadIdx.x; f < total; f += blockDim.x) {
+    // brick of flattened candidate f (binary search in the prefix array)
+    int l = 0, h = ncand - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (bofs[mid] <= f) l = mid; else h = mid - 1;
+    }
+    const int ai = border[bbeg[l] + (f - bofs[l])];
+    const double qi = q[ai];
+    double u[3], AX[3][4], AXX[3][4];
+    int b[3];
+    grid_coord(g, x + 3 * ai, u, b);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline4(u[d], AX[d], AXX[d]);
+    int o[3][4];
+    bool in[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        int p = wrapk(b[d], a, g.K[d]) - lo[d];
+        if (p < 0) p += g.K[d];  // periodic distance from the tile origin
+        o[d][a] = p;
+        in[d][a] = p < ext[d];
+      }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!in[2][c]) continue;
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb) {
+        if (!in[1][bb]) continue;
+        const int row = TX * (o[1][bb] + TY * o[2][c]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (!in[0][a]) continue;
+          double w = AX[0][a] * AX[1][bb] * AX[2][c];
+          if (F32W) w = double(float(w));
+          const long long v = llrint(qi * w * GRID_SCALE);
+          atomicAdd(&acc[row + o[0][a]], (unsigned long long)v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < TILE; k += blockDim.x) {
+    const int px = k % TX, py = (k / TX) % TY, pz = k / (TX * TY);
+    if (px < ext[0] && py < ext[1] && pz < ext[2]) {
+      const size_t gi = size_t(lo[0] + px) +
+                        size_t(g.K[0]) * (size_t(lo[1] + py) + size_t(g.K[1]) * size_t(lo[2] + pz));
+      grid[gi] = R(double((long long)acc[k]) * GRID_INV_SCALE);
+    }
+  }
+}
+
+// ql = q + step p (the trial charges of a line search)
+__global__ void k_lincomb(int n, const double* __restrict__ q, const double* __restrict__ p,
+                          double step, double* __restrict__ ql) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ql[i] = q[i] + step * p[i];
+}
+
+// generic exclusive scan of int counts: per-1024 block scan + block sums
+__global__ void k_scan_block(int n, const int* __restrict__ in, int* __restrict__ out,
+                             int* __restrict__ bsum) {
+  __shared__ int s[1024];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+__global__ void k_scan_sums(int nb, int* __restrict__ bsum, int* __restrict__ total) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int k = base + threadIdx.x;
+    const int v = k < nb ? bsum[k] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (k < nb) bsum[k] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ void k_scan_add(int n, int* __restrict__ out, const int* __restrict__ bsum,
+                           const int* __restrict__ total) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+  if (i == 0) out[n] = *total;
 }
 
 }  // namespace sx

@@ -68,13 +68,23 @@ struct simox_ctx {
   double *rho = nullptr, *dF64 = nullptr, *frec = nullptr;
   float* dF32 = nullptr;
   // PME
-  long long* gfix = nullptr;
+  int nbrick = 0;
+  int *brick = nullptr, *bcount = nullptr, *bstart = nullptr, *bfill = nullptr;
+  int *bunord = nullptr, *border = nullptr, *bsum = nullptr, *btotal = nullptr;
+  bool bricks_valid = false;
   double* greal64 = nullptr;
   float* greal32 = nullptr;
   double2* spec64 = nullptr;
   float2* spec32 = nullptr;
   double* G64 = nullptr;
   float* G32 = nullptr;
+  double* F64 = nullptr;  // preconditioner filter tables (same layout as G)
+  float* F32 = nullptr;
+  bool prec_on = false;
+  bool prec_ready = false;
+  double prec_j0 = 1.0;
+  double* sv = nullptr;
+  double* cv = nullptr;  // per-atom density scaling of the preconditioner
   double2* tw64[3] = {nullptr, nullptr, nullptr};
   float2* tw32[3] = {nullptr, nullptr, nullptr};
   FftPlan pl[3]{};
@@ -88,8 +98,12 @@ struct simox_ctx {
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
   double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
+  double *qh = nullptr, *qh2 = nullptr;  // charges before the last solve (predictor history)
+  bool hist_valid = false;
   int last_inner = 0, last_outer = 0;
   bool matrix_valid = false;
+  bool nlist_valid = false;
+  double* part_eg = nullptr;  // per-block embedding-energy partials of the last list build
   // reductions / scalars
   double* part = nullptr;
   size_t part_cap = 0;
@@ -310,6 +324,11 @@ __global__ void k_to_caller_int(int n, const int* __restrict__ perm, const int* 
   if (p >= n) return;
   dst[perm[p]] = src[p];
 }
+__global__ void k_gather_f64(int n, const int* __restrict__ order, const double* __restrict__ a,
+                             double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[order[i]];
+}
 __global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -337,40 +356,58 @@ int do_cell_sort(simox_ctx* ctx) {
   hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order, ctx->cell,
                      ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->x2, ctx->v2,
                      ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2);
+  if (ctx->hist_valid) {
+    hipLaunchKernelGGL(k_gather_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order, ctx->qh,
+                       ctx->qh2);
+    std::swap(ctx->qh, ctx->qh2);
+  }
   HIPCHK(hipGetLastError());
   std::swap(ctx->x, ctx->x2); std::swap(ctx->v, ctx->v2); std::swap(ctx->f, ctx->f2);
   std::swap(ctx->q, ctx->q2); std::swap(ctx->id, ctx->id2); std::swap(ctx->perm, ctx->perm2);
   std::swap(ctx->cell, ctx->cell2);
   ctx->matrix_valid = false;
+  ctx->nlist_valid = false;
+  ctx->bricks_valid = false;
   return 0;
 }
 
 // ---------------------------------------------------------------------------
 // K2-K4: FORCE
 // ---------------------------------------------------------------------------
-int do_force(simox_ctx* ctx, bool fetch_energies) {
-  const int n = ctx->n, nb = grid_blocks(n);
-  CHK(ensure_part(ctx, size_t(nb) * 4));
+// Neighbour list + EAM density/embedding (once per set of positions).
+int ensure_nlist(simox_ctx* ctx) {
+  if (ctx->nlist_valid) return 0;
+  const int nb = grid_blocks(ctx->n);
   hipEvent_t ev{};
   tstart(ctx, 8, &ev);
   if (use32(ctx))
-    hipLaunchKernelGGL(k_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->rho, ctx->dF32, ctx->part,
-                       ctx->derr);
+    hipLaunchKernelGGL(k_nlist_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
+                       ctx->nb_col, ctx->rho, ctx->dF32, ctx->part_eg, ctx->derr);
   else
-    hipLaunchKernelGGL(k_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->cell, ctx->start, ctx->rho, ctx->dF64, ctx->part,
-                       ctx->derr);
+    hipLaunchKernelGGL(k_nlist_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
+                       ctx->nb_col, ctx->rho, ctx->dF64, ctx->part_eg, ctx->derr);
   tstop(ctx, 8, ev);
-  CHK(reduce(ctx, nb, 1, 0, 0));
+  HIPCHK(hipGetLastError());
+  ctx->nlist_valid = true;
+  return 0;
+}
+
+int do_force(simox_ctx* ctx, bool fetch_energies) {
+  const int nb = grid_blocks(ctx->n);
+  CHK(ensure_nlist(ctx));
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part_eg, nb, 1, 0,
+                     ctx->dscal + 0);
+  hipEvent_t ev{};
   tstart(ctx, 7, &ev);
   if (use32(ctx))
     hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->q, ctx->id, ctx->cell, ctx->start, ctx->dF32, ctx->f,
+                       ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF32, ctx->f,
                        ctx->part + nb);
   else
     hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->q, ctx->id, ctx->cell, ctx->start, ctx->dF64, ctx->f,
+                       ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF64, ctx->f,
                        ctx->part + nb);
   tstop(ctx, 7, ev);
   HIPCHK(hipGetLastError());
@@ -391,50 +428,88 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
 // ---------------------------------------------------------------------------
 // K6-K8: one PME convolution: spread(q or q + step p) -> ... -> real grid
 // ---------------------------------------------------------------------------
+// Kernel pointers of the three FFT kernels for line length L (compile-time
+// instantiations for the workloads' sizes, runtime plan otherwise).
+template <class C, int L>
+struct FftKernels {
+  static constexpr auto r2c = &k_fft_x_r2c<C, L>;
+  static constexpr auto c2r = &k_fft_x_c2r<C, L>;
+  static constexpr auto str = &k_fft_strided<C, false, L>;
+  static constexpr auto fus = &k_fft_strided<C, true, L>;
+};
+template <class C>
+struct FftSet {
+  decltype(&k_fft_x_r2c<C, 0>) r2c;
+  decltype(&k_fft_x_c2r<C, 0>) c2r;
+  decltype(&k_fft_strided<C, false, 0>) str;
+  decltype(&k_fft_strided<C, true, 0>) fus;
+};
+template <class C, int L>
+FftSet<C> fft_set_of() {
+  return {FftKernels<C, L>::r2c, FftKernels<C, L>::c2r, FftKernels<C, L>::str, FftKernels<C, L>::fus};
+}
+template <class C>
+FftSet<C> fft_set(int L) {
+  switch (L) {
+    case 16: return fft_set_of<C, 16>();
+    case 24: return fft_set_of<C, 24>();
+    case 32: return fft_set_of<C, 32>();
+    case 48: return fft_set_of<C, 48>();
+    case 64: return fft_set_of<C, 64>();
+    case 96: return fft_set_of<C, 96>();
+    case 128: return fft_set_of<C, 128>();
+    case 192: return fft_set_of<C, 192>();
+    case 256: return fft_set_of<C, 256>();
+    case 384: return fft_set_of<C, 384>();
+    case 512: return fft_set_of<C, 512>();
+    default: return fft_set_of<C, 0>();
+  }
+}
+
 template <class C>
 int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const int* M,
               const decltype(C{}.x)* G, decltype(C{}.x)* greal) {
   const Geom& g = ctx->g;
   const int H = g.H, Ky = g.K[1], Kz = g.K[2];
   const size_t csz = sizeof(C);
+  const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
   hipEvent_t ev{};
   {
     const int nl = Ky * Kz;
     const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
     tstart(ctx, 1, &ev);
-    hipLaunchKernelGGL(k_fft_x_r2c<C>, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[0], tw[0], ctx->gfix, spec, M[0], ctx->stride[0]);
+    hipLaunchKernelGGL(fx.r2c, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[0], tw[0], greal, spec, M[0], ctx->stride[0]);
     tstop(ctx, 1, ev);
   }
   {
     const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
     tstart(ctx, 2, &ev);
-    hipLaunchKernelGGL((k_fft_strided<C, false>), dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK),
-                       sh, ctx->st, g, ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1],
-                       -1, (const decltype(C{}.x)*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL(fy.str, dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1], -1,
+                       (const decltype(C{}.x)*)nullptr, (double*)nullptr);
     tstop(ctx, 2, ev);
   }
   {
     const size_t sh = 2 * size_t(M[2]) * ctx->stride[2] * csz;
     tstart(ctx, 3, &ev);
-    hipLaunchKernelGGL((k_fft_strided<C, true>), dim3((H + M[2] - 1) / M[2], Ky), dim3(BLOCK), sh,
-                       ctx->st, g, ctx->pl[2], tw[2], spec, H * Ky, H, M[2], ctx->stride[2], 0,
-                       G, ctx->part);
+    hipLaunchKernelGGL(fz.fus, dim3((H + M[2] - 1) / M[2], Ky), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[2], tw[2], spec, H * Ky, H, M[2], ctx->stride[2], 0, G, ctx->part);
     tstop(ctx, 3, ev);
   }
   {
     const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
     tstart(ctx, 2, &ev);
-    hipLaunchKernelGGL((k_fft_strided<C, false>), dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK),
-                       sh, ctx->st, g, ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1],
-                       +1, (const decltype(C{}.x)*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL(fy.str, dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK), sh, ctx->st, g,
+                       ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1], +1,
+                       (const decltype(C{}.x)*)nullptr, (double*)nullptr);
     tstop(ctx, 2, ev);
   }
   {
     const int nl = Ky * Kz;
     const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
     tstart(ctx, 4, &ev);
-    hipLaunchKernelGGL(k_fft_x_c2r<C>, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
+    hipLaunchKernelGGL(fx.c2r, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
                        ctx->pl[0], tw[0], spec, greal, M[0], ctx->stride[0]);
     tstop(ctx, 4, ev);
   }
@@ -447,25 +522,62 @@ int zpass_blocks(const simox_ctx* ctx) {
   return ((ctx->g.H + m - 1) / m) * ctx->g.K[1];
 }
 
-// Spread (charges q, or q + step*p written to ctx->ql) and convolve, with the
-// FFT chain in FP32 (f32) or FP64.
-int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool f32) {
+// Brick binning of the atoms for the tiled spread (once per set of positions).
+int ensure_bricks(simox_ctx* ctx) {
+  if (ctx->bricks_valid) return 0;
+  const int n = ctx->n, nb = grid_blocks(n), B = ctx->nbrick;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, ctx->bcount);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, ctx->bfill);
+  hipLaunchKernelGGL(k_brick_assign, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, ctx->brick,
+                     ctx->bcount);
+  const int nsb = (B + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(1024), 0, ctx->st, B, ctx->bcount, ctx->bstart,
+                     ctx->bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->st, nsb, ctx->bsum, ctx->btotal);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(1024), 0, ctx->st, B, ctx->bstart, ctx->bsum,
+                     ctx->btotal);
+  hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->brick, ctx->bstart,
+                     ctx->bfill, ctx->bunord);
+  hipLaunchKernelGGL(k_brick_place, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->brick, ctx->bstart,
+                     ctx->bfill, ctx->bunord, ctx->border);
+  HIPCHK(hipGetLastError());
+  ctx->bricks_valid = true;
+  return 0;
+}
+
+// Spread charges q (or ql = q + step p when p != null) and convolve, with the
+// FFT chain in FP32 (f32) or FP64.  FAITHFUL mode rounds the spreading weights
+// to FP32 as the reference does (quirk Q1).
+int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool f32,
+             bool filter = false) {
   const int nb = grid_blocks(ctx->n);
-  CHK(ensure_part(ctx, size_t(zpass_blocks(ctx)) + 16));
+  CHK(ensure_bricks(ctx));
+  const double* src = q;
+  if (p) {
+    hipLaunchKernelGGL(k_lincomb, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, q, p, step, ctx->ql);
+    src = ctx->ql;
+  }
   const bool f32w = ctx->mode == SIMOX_MODE_FAITHFUL;
+  const int ntile = ((ctx->g.K[0] + TX - 1) / TX) * ((ctx->g.K[1] + TY - 1) / TY) *
+                    ((ctx->g.K[2] + TZ - 1) / TZ);
   hipEvent_t ev{};
   tstart(ctx, 0, &ev);
-  if (f32w)
-    hipLaunchKernelGGL(k_spread<true>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, q, p,
-                       step, ctx->ql, (unsigned long long*)ctx->gfix);
+  if (f32)
+    hipLaunchKernelGGL((k_spread_tile<false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal32);
+  else if (f32w)
+    hipLaunchKernelGGL((k_spread_tile<true, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal64);
   else
-    hipLaunchKernelGGL(k_spread<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, q, p,
-                       step, ctx->ql, (unsigned long long*)ctx->gfix);
+    hipLaunchKernelGGL((k_spread_tile<false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal64);
   tstop(ctx, 0, ev);
   HIPCHK(hipGetLastError());
   if (f32)
-    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->M32, ctx->G32, ctx->greal32);
-  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->M64, ctx->G64, ctx->greal64);
+    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->M32, filter ? ctx->F32 : ctx->G32,
+                             ctx->greal32);
+  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->M64, filter ? ctx->F64 : ctx->G64,
+                            ctx->greal64);
 }
 
 // PME (force.f90:364-464): reciprocal forces added to ctx->f, E_rec.
@@ -504,17 +616,17 @@ int do_pme(simox_ctx* ctx, bool fetch_energy) {
 // ---------------------------------------------------------------------------
 int build_matrix(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
+  CHK(ensure_nlist(ctx));
+  CHK(check_err(ctx, 1));
   hipEvent_t ev{};
   tstart(ctx, 9, &ev);
   hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                     ctx->x, ctx->id, ctx->cell, ctx->start, ctx->nb_cnt, ctx->nb_col,
-                     ctx->nb_val64, ctx->bvec, ctx->cap, ctx->derr);
+                     ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->bvec);
   tstop(ctx, 9, ev);
   if (use32(ctx))
     hipLaunchKernelGGL(k_ell_to_f32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
                        ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
   HIPCHK(hipGetLastError());
-  CHK(check_err(ctx, 1));
   ctx->matrix_valid = true;
   return 0;
 }
@@ -638,111 +750,172 @@ int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
   return 0;
 }
 
-// Inner projected PCG on H d = P g (g = ctx->gv, its sum in dscal[0]) at the
-// active set of qc; d -> ctx->dv.  Operator precision: FP32 grid/FFT/ELL
-// values (f32) or FP64.  Residuals, dot products and the neutrality
-// projection are FP64.
-int pcg_inner(simox_ctx* ctx, bool f32, double rel_tol, double abs_tol, int maxit, int* iters) {
+// Coulomb-screening preconditioner set-up for the current positions (once per
+// solve): low-passed atom density (a spread of ones through the FFT chain with
+// the Gaussian table, sigma = 1.5 A), per-atom scaling c_i, filter table F.
+int build_prec(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
-  double s3[3], rz, rz_new, pap, s2[2];
-  hipLaunchKernelGGL(k_pcg_start, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->qc,
-                     ctx->gv, ctx->dscal + 0, ctx->rv, ctx->dv, ctx->mv, ctx->part);
-  CHK(reduce(ctx, nb, 3, 0, 16));  // {sum minv r, sum minv, sum r^2} at 16..18
-  CHK(fetch(ctx, 16, 3, s3));
-  const double rms0 = std::sqrt(s3[2] / n);
-  const double target = std::max(abs_tol, rel_tol * rms0);
-  hipLaunchKernelGGL(k_pcg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->rv, ctx->mv,
-                     ctx->dscal + 16, ctx->dscal + 17, 1, ctx->zv, ctx->pv, ctx->part);
-  CHK(reduce(ctx, nb, 1, 0, 20));
-  CHK(fetch(ctx, 20, 1, &rz));
-  int it = 0;
-  double rms = rms0;
-  while (rms > target && it < maxit) {
-    CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
-    hipEvent_t ev{};
-    tstart(ctx, 6, &ev);
-    if (f32)
-      hipLaunchKernelGGL((k_hmatvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->greal32, ctx->c_pme,
-                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
-    else
-      hipLaunchKernelGGL((k_hmatvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->greal64, ctx->c_pme,
-                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
-    tstop(ctx, 6, ev);
-    HIPCHK(hipGetLastError());
-    CHK(reduce(ctx, nb, 2, 0, 21));  // {p.Ap, sum Ap} at 21..22
-    CHK(fetch(ctx, 21, 1, &pap));
-    if (!(pap > 0.0)) break;  // H is positive definite on sum(d) = 0; guards rounding
-    const double a = rz / pap;
-    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(BLOCK), 0, ctx->st, n, a, ctx->pv, ctx->apv,
-                       ctx->dscal + 22, ctx->mv, ctx->dv, ctx->rv, ctx->part);
-    HIPCHK(hipGetLastError());
-    CHK(reduce(ctx, nb, 2, 0, 24));  // {sum minv r, sum r^2} at 24..25
-    CHK(fetch(ctx, 24, 2, s2));
-    rms = std::sqrt(s2[1] / n);
-    ++it;
-    if (rms <= target) break;
-    hipLaunchKernelGGL(k_pcg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->rv, ctx->mv,
-                       ctx->dscal + 24, ctx->dscal + 17, 0, ctx->zv, ctx->pv, ctx->part);
-    CHK(reduce(ctx, nb, 1, 0, 20));
-    CHK(fetch(ctx, 20, 1, &rz_new));
-    const double b = rz_new / rz;
-    rz = rz_new;
-    hipLaunchKernelGGL(k_pcg_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, b, ctx->zv, ctx->pv);
-    HIPCHK(hipGetLastError());
+  if (!ctx->prec_ready) {
+    std::vector<int> ids(n);
+    HIPCHK(hipMemcpy(ids.data(), ctx->id, sizeof(int) * n, hipMemcpyDeviceToHost));
+    double jsum = 0.0;
+    for (int i = 0; i < n; ++i) jsum += ctx->htab.sc[ids[i] - 1].J;
+    ctx->prec_j0 = jsum / n;
+    ctx->prec_ready = true;
   }
-  *iters = it;
-  return 0;
-}
-
-// RMS of the projected gradient g - mean(g) of ctx->gv (sum in dscal[0]).
-int projected_rms(simox_ctx* ctx, double* rms) {
-  const int n = ctx->n, nb = grid_blocks(n);
-  hipLaunchKernelGGL(k_center_sq, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->gv, ctx->dscal + 0,
-                     ctx->part);
+  const double sigma2 = 1.5 * 1.5;
+  const double ng = double(ctx->g.K[0]) * ctx->g.K[1] * ctx->g.K[2];
+  const double h3 = ctx->V / ng;
+  const size_t nh = size_t(ctx->g.H) * ctx->g.K[1] * ctx->g.K[2];
+  const int nbk = int((nh + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->G64,
+                     sigma2, 0.0, 0.0, 0.0, 0, ctx->F64);
+  hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, 1.0, ctx->sv);
   HIPCHK(hipGetLastError());
-  CHK(reduce(ctx, nb, 1, 0, 3));
-  double s;
-  CHK(fetch(ctx, 3, 1, &s));
-  *rms = std::sqrt(s / n);
+  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, false, true));
+  hipLaunchKernelGGL(k_prec_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                     ctx->greal64, ctx->cv, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 1, 0, 27));
+  hipLaunchKernelGGL(k_prec_c, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 27, ctx->cv);
+  double rsum;
+  CHK(fetch(ctx, 27, 1, &rsum));
+  const double rho_ref = rsum / n;
+  const double lam_scale = KC / PI * (rho_ref / h3);  // (kc/pi) n_ref
+  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->G64,
+                     sigma2, lam_scale, 1.0 / ctx->prec_j0, 1.0 / rho_ref, 1, ctx->F64);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_filter_table<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->G64, sigma2, lam_scale, 1.0 / ctx->prec_j0, 1.0 / rho_ref, 1, ctx->F32);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
-// Converged charge solve: FP64 outer residual (the true gradient, k_dudq),
-// inner PCG in FP64 (mode FP64) or with the FP32 operator (mode MIXED), i.e.
-// iterative refinement.  Stops when the RMS of the projected gradient
-// (g - mean g, eV/e) is below ctx->tol.
-int ctip_linear(simox_ctx* ctx, int* iters, int* n_eval) {
+// z = P M gv (preconditioned, projected on sum = 0); dscal[19..20] = {z.g, z.g_old}
+int apply_prec(simox_ctx* ctx, const double* g_old, bool f32) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  if (!ctx->prec_on) {
+    hipLaunchKernelGGL(k_ncg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->gv, g_old, ctx->mv,
+                       ctx->dscal + 16, ctx->dscal + 17, ctx->zv, ctx->part);
+    HIPCHK(hipGetLastError());
+    return reduce(ctx, nb, 2, 0, 19);
+  }
+  hipLaunchKernelGGL(k_prec_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ctx->qc, ctx->gv, ctx->cv, ctx->sv);
+  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true));
+  if (f32)
+    hipLaunchKernelGGL(k_prec_finish<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, ctx->greal32, ctx->zv, ctx->part);
+  else
+    hipLaunchKernelGGL(k_prec_finish<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, ctx->greal64, ctx->zv, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 1, 0, 26));
+  hipLaunchKernelGGL(k_prec_project, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 26, ctx->gv,
+                     g_old, ctx->zv, ctx->part);
+  HIPCHK(hipGetLastError());
+  return reduce(ctx, nb, 2, 0, 19);
+}
+
+// Converged charge solve (modes FP64 / MIXED): preconditioned nonlinear CG
+// (Polak-Ribiere+) on the piecewise-quadratic charge energy with exact line
+// searches, one H0 matvec per iteration (FP64, or FP32 operator in MIXED),
+// gradient recurrence in FP64.  Restarts recompute the true gradient with the
+// FP64 operator (iterative refinement); the solve ends when the RMS of the
+// projected true gradient (eV/e) is below ctx->tol.
+int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
   const int n = ctx->n, nb = grid_blocks(n);
   const bool f32 = use32(ctx);
-  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qc);
+  if (ctx->hist_valid)
+    hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qh, ctx->qc);
+  else
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qc);
   HIPCHK(hipGetLastError());
-  int inner_total = 0, outer = 0, nev = 0;
+  int total_it = 0, nev = 0, restarts = 0;
   bool ok = false;
-  for (outer = 0; outer < 40; ++outer) {
-    double rms;
+  double s6[6];
+  if (ctx->prec_on) CHK(build_prec(ctx));
+  for (;;) {
+    // true gradient (FP64 operator) and its projected RMS
     CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false, false));
-    CHK(projected_rms(ctx, &rms));
     ++nev;
-    if (rms <= ctx->tol) { ok = true; break; }
+    hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                       ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
+    CHK(reduce(ctx, nb, 3, 0, 16));  // {sum m g, sum m, sum g^2} at 16..18
+    CHK(apply_prec(ctx, nullptr, f32));  // {z.g, -} at 19..20
+    CHK(fetch(ctx, 16, 4, s6));
+    const double rms_true = std::sqrt(s6[2] / n);
+    if (rms_true <= ctx->tol) { ok = true; break; }
+    if (restarts >= 60) break;
+    ++restarts;
+    const double target = f32 ? std::max(0.3 * ctx->tol, 1e-4 * rms_true) : 0.3 * ctx->tol;
+    double rz = s6[3];
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv);
     int it = 0;
-    CHK(pcg_inner(ctx, f32, f32 ? 1e-4 : 0.0, 0.3 * ctx->tol, ctx->maxit, &it));
-    inner_total += it;
+    double rms = rms_true;
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, rz, ctx->dscal + 46);  // ls[6]
+    while (rms > target && it < ctx->maxit) {
+      double* ls = ctx->dscal + 40;
+      // v = H0 p with p.v, p.g, sum p^2 pen''(q)
+      CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
+      hipEvent_t ev{};
+      tstart(ctx, 6, &ev);
+      if (f32)
+        hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->gv, ctx->greal32,
+                           ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
+      else
+        hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->gv, ctx->greal64,
+                           ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+      tstop(ctx, 6, ev);
+      CHK(reduce(ctx, nb, 3, 0, 22));
+      hipLaunchKernelGGL(k_ls_init, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 22, ls);
+      // exact line search, Newton with bracketing, on the device
+      for (int k = 0; k < 8; ++k) {
+        hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                           ctx->qc, ctx->pv, ls, ctx->part);
+        hipLaunchKernelGGL(k_reduce_if, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2, ls,
+                           ctx->dscal + 24);
+        hipLaunchKernelGGL(k_ls_update, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 24, ls);
+      }
+      hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                         ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
+      CHK(reduce(ctx, nb, 1, 0, 0));
+      hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                         ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
+      CHK(reduce(ctx, nb, 3, 0, 16));
+      CHK(apply_prec(ctx, ctx->rv, f32));
+      hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 19, ls);
+      hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv);
+      HIPCHK(hipGetLastError());
+      double st2[2];
+      ++it;
+      // one host round trip per iteration: convergence and descent check
+      HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+      HIPCHK(hipMemcpyAsync(ctx->hscal + 48, ctx->dscal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+      HIPCHK(hipStreamSynchronize(ctx->st));
+      st2[0] = ctx->hscal[18];
+      st2[1] = ctx->hscal[48];
+      rms = std::sqrt(st2[0] / n);
+      if (st2[1] != 0.0) break;  // not a descent direction: restart from the true gradient
+    }
+    total_it += it;
     nev += it;
-    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, 1.0, ctx->dv, ctx->qc);
-    HIPCHK(hipGetLastError());
   }
-  ctx->last_inner = inner_total;
-  ctx->last_outer = outer;
-  if (iters) *iters = inner_total;
+  ctx->last_inner = total_it;
+  ctx->last_outer = restarts;
+  if (iters) *iters = total_it;
   if (n_eval) *n_eval = nev;
   if (!ok) {
     ctx->err = "charge solve did not reach the requested tolerance";
     return SIMOX_ERR_NOT_CONVERGED;
   }
+  // history for the next predictor: qh = charges before this solve
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qh);
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->qc, ctx->q);
   HIPCHK(hipGetLastError());
+  ctx->hist_valid = true;
   return 0;
 }
 
@@ -750,7 +923,7 @@ int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
   CHK(build_matrix(ctx));
   int it = 0, ne = 0;
   if (ctx->mode == SIMOX_MODE_FAITHFUL) CHK(ctip_faithful(ctx, &it, &ne));
-  else CHK(ctip_linear(ctx, &it, &ne));
+  else CHK(ctip_ncg(ctx, &it, &ne));
   if (iters) *iters = it;
   if (n_eval) *n_eval = ne;
   double e3[3];
@@ -765,6 +938,8 @@ int do_vv1(simox_ctx* ctx, double dt) {
                      ctx->f);
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
+  ctx->nlist_valid = false;
+  ctx->bricks_valid = false;
   return 0;
 }
 
@@ -931,8 +1106,8 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
       return SIMOX_ERR_ARG;
     }
   }
-  if (fft[0] % 2) {
-    ctx->err = "PME grid size along x must be even";
+  if (fft[0] % BRK || fft[1] % BRK || fft[2] % BRK) {
+    ctx->err = "PME grid sizes must be multiples of 4";
     return SIMOX_ERR_ARG;
   }
   g.ncall = g.nc[0] * g.nc[1] * g.nc[2];
@@ -964,6 +1139,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
   CHK(dalloc(ctx, &ctx->rv, n)); CHK(dalloc(ctx, &ctx->zv, n)); CHK(dalloc(ctx, &ctx->apv, n));
   CHK(dalloc(ctx, &ctx->mv, n));
+  CHK(dalloc(ctx, &ctx->qh, n)); CHK(dalloc(ctx, &ctx->qh2, n));
   CHK(dalloc(ctx, &ctx->bvec, n)); CHK(dalloc(ctx, &ctx->nb_cnt, n));
   CHK(dalloc(ctx, &ctx->nb_col, size_t(ctx->cap) * n));
   CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
@@ -971,12 +1147,17 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   // PME
   const size_t ng = size_t(fft[0]) * fft[1] * fft[2];
   const size_t nh = size_t(g.H) * fft[1] * fft[2];
-  CHK(dalloc(ctx, &ctx->gfix, ng));
-  HIPCHK(hipMemset(ctx->gfix, 0, ng * sizeof(long long)));
+  ctx->nbrick = (fft[0] / BRK) * (fft[1] / BRK) * (fft[2] / BRK);
+  CHK(dalloc(ctx, &ctx->brick, n)); CHK(dalloc(ctx, &ctx->bunord, n)); CHK(dalloc(ctx, &ctx->border, n));
+  CHK(dalloc(ctx, &ctx->bcount, ctx->nbrick)); CHK(dalloc(ctx, &ctx->bfill, ctx->nbrick));
+  CHK(dalloc(ctx, &ctx->bstart, size_t(ctx->nbrick) + 1));
+  CHK(dalloc(ctx, &ctx->bsum, (ctx->nbrick + 1023) / 1024 + 1)); CHK(dalloc(ctx, &ctx->btotal, 1));
   // FP64 chain always (faithful / FP64 modes, MIXED outer residual); FP32 chain for MIXED
   CHK(dalloc(ctx, &ctx->greal64, ng)); CHK(dalloc(ctx, &ctx->spec64, nh)); CHK(dalloc(ctx, &ctx->G64, nh));
+  CHK(dalloc(ctx, &ctx->F64, nh)); CHK(dalloc(ctx, &ctx->sv, n)); CHK(dalloc(ctx, &ctx->cv, n));
   if (mode == SIMOX_MODE_MIXED) {
     CHK(dalloc(ctx, &ctx->greal32, ng)); CHK(dalloc(ctx, &ctx->spec32, nh)); CHK(dalloc(ctx, &ctx->G32, nh));
+    CHK(dalloc(ctx, &ctx->F32, nh));
   }
   for (int d = 0; d < 3; ++d) {
     const int L = fft[d];
@@ -1008,6 +1189,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     HIPCHK(hipGetLastError());
   }
   CHK(dalloc(ctx, &ctx->dscal, 64));
+  CHK(dalloc(ctx, &ctx->part_eg, grid_blocks(n)));
   HIPCHK(hipHostMalloc((void**)&ctx->hscal, 64 * sizeof(double), hipHostMallocDefault));
   CHK(dalloc(ctx, &ctx->derr, 1));
   HIPCHK(hipMemset(ctx->derr, 0, sizeof(int)));
@@ -1030,11 +1212,12 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->gfix, ctx->greal64, ctx->greal32, ctx->spec64,
-                  ctx->spec32, ctx->G64, ctx->G32, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->frec, ctx->brick, ctx->bunord, ctx->border, ctx->bcount,
+                  ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
+                  ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
-                  ctx->rv, ctx->zv, ctx->apv, ctx->mv,
-                  ctx->part, ctx->dscal, ctx->derr, ctx->dstage};
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qh, ctx->qh2,
+                  ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int d = 0; d < 3; ++d) {
@@ -1069,6 +1252,19 @@ int simox_set_mode(simox_ctx* ctx, int mode) {
     return SIMOX_ERR_ARG;
   }
   ctx->mode = mode;
+  return SIMOX_OK;
+}
+
+int simox_set_charge_precond(simox_ctx* ctx, int on) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  ctx->prec_on = on != 0;
+  return SIMOX_OK;
+}
+
+int simox_charge_stats(simox_ctx* ctx, int* iters, int* restarts) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  if (iters) *iters = ctx->last_inner;
+  if (restarts) *restarts = ctx->last_outer;
   return SIMOX_OK;
 }
 
@@ -1108,6 +1304,8 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
     if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
     if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
     ctx->have_state = true;
+    ctx->hist_valid = false;
+    ctx->prec_ready = false;
     CHK(do_cell_sort(ctx));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return check_err(ctx, 0);
@@ -1116,8 +1314,8 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
   if (x) CHK(upload_caller(ctx, x, 3, ctx->x));
   if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
   if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
-  if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
-  if (x) ctx->matrix_valid = false;
+  if (q) { CHK(upload_caller(ctx, q, 1, ctx->q)); ctx->hist_valid = false; }
+  if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->bricks_valid = false; }
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }

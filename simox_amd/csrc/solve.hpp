@@ -365,3 +365,443 @@ __global__ void k_pcg_dir(int n, double b, const double* __restrict__ z, double*
 }
 
 }  // namespace sx
+
+namespace sx {
+
+// ---------------------------------------------------------------------------
+// Nonlinear preconditioned CG for the converged charge solve.  U(q) = U0(q) +
+// sum_i pen_i(q_i) with U0 quadratic (Hessian H0 = diag(J') + c_pme P + S) and
+// pen(q) = w (min(q-qmin,0)^2 + max(q-qmax,0)^2), C1 and piecewise quadratic.
+// Along a direction p, with v = H0 p (one matvec),
+//   phi'(a) = -g.p + a p.v + sum_i p_i (pen'(q_i + a p_i) - pen'(q_i))
+// is piecewise linear and increasing, so the exact line search and the
+// gradient update g(q + a p) = g - a v - dpen' need no further matvec.
+// ---------------------------------------------------------------------------
+__device__ inline double pen_d1(const SpecCoef& sc, double q) {
+  double lo = q - sc.qmin, hi = q - sc.qmax;
+  if (lo >= 0.0) lo = 0.0;
+  if (hi <= 0.0) hi = 0.0;
+  return 2.0 * WPEN * (lo + hi);
+}
+
+// y = H0 p (no penalty term); part[3k..] = p.y, p.gr, sum p^2 pen''(q)
+template <class Rg, class Tv>
+__global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __restrict__ tb,
+                                                    const double* __restrict__ x,
+                                                    const int* __restrict__ id,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ gr,
+                                                    const Rg* __restrict__ grid, double c_pme,
+                                                    const int* __restrict__ cnt,
+                                                    const int* __restrict__ col,
+                                                    const Tv* __restrict__ val,
+                                                    double* __restrict__ y,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const double phi = c_pme * gather_phi(g, x + 3 * i, grid);
+    const double pi = p[i];
+    const int c = cnt[i];
+    const size_t n = size_t(g.n);
+    Tv s = Tv(0);
+    for (int k = 0; k < c; ++k) s += val[size_t(k) * n + i] * Tv(p[col[size_t(k) * n + i]]);
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double yi = sc.Jp * pi + phi + double(s);
+    y[i] = yi;
+    acc[0] = pi * yi;
+    acc[1] = pi * gr[i];
+    acc[2] = outside(sc, q[i]) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = acc[0];
+    part[3 * blockIdx.x + 1] = acc[1];
+    part[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
+// charge predictor q_guess = 2 q - q_prev (linear extrapolation in time;
+// preserves the total charge)
+__global__ void k_extrapolate(int n, const double* __restrict__ q, const double* __restrict__ qh,
+                              double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 2.0 * q[i] - qh[i];
+}
+
+// Line-search terms at step a: part[2k] = sum p (pen'(q + a p) - pen'(q)),
+// part[2k+1] = sum p^2 pen''(q + a p)
+__global__ void __launch_bounds__(BLOCK) k_ncg_ls(int n, const Tables* __restrict__ tb,
+                                                  const int* __restrict__ id,
+                                                  const double* __restrict__ q,
+                                                  const double* __restrict__ p, double a,
+                                                  double* __restrict__ part) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double pi = p[i], qi = q[i], qa = qi + a * pi;
+    acc[0] = pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
+    acc[1] = outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+// q += a p; g_old = g; g -= a v + (pen'(q_new) - pen'(q_old)); part[k] = sum g
+__global__ void __launch_bounds__(BLOCK) k_ncg_step(int n, const Tables* __restrict__ tb,
+                                                    const int* __restrict__ id, double a,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ q,
+                                                    double* __restrict__ g,
+                                                    double* __restrict__ g_old,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double qi = q[i], qn = qi + a * p[i];
+    const double gi = g[i];
+    g_old[i] = gi;
+    const double gn = gi - a * v[i] - (pen_d1(sc, qn) - pen_d1(sc, qi));
+    q[i] = qn;
+    g[i] = gn;
+    acc[0] = gn;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Centre g (subtract its mean; the constant part is the neutrality multiplier
+// and drops out of every update) and the preconditioner m = 1/(J + pen'').
+// part[3k..] = sum m g, sum m, sum g^2 (of the centred g)
+__global__ void __launch_bounds__(BLOCK) k_ncg_center(int n, const Tables* __restrict__ tb,
+                                                      const int* __restrict__ id,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ gsum,
+                                                      double* __restrict__ g,
+                                                      double* __restrict__ m,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const double gi = g[i] - gsum[0] / double(n);
+    g[i] = gi;
+    const double mi = 1.0 / diag_m(tb->sc[id[i] - 1], q[i]);
+    m[i] = mi;
+    acc[0] = mi * gi; acc[1] = mi; acc[2] = gi * gi;
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = acc[k];
+}
+
+// z = m g - m (sum m g / sum m) (sum z = 0); part[2k..] = z.g, z.g_old
+__global__ void __launch_bounds__(BLOCK) k_ncg_z(int n, const double* __restrict__ g,
+                                                 const double* __restrict__ g_old,
+                                                 const double* __restrict__ m,
+                                                 const double* __restrict__ s_mg,
+                                                 const double* __restrict__ s_m,
+                                                 double* __restrict__ z,
+                                                 double* __restrict__ part) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    const double zi = m[i] * g[i] - m[i] * (s_mg[0] / s_m[0]);
+    z[i] = zi;
+    acc[0] = zi * g[i];
+    acc[1] = g_old ? zi * g_old[i] : 0.0;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+}  // namespace sx
+
+namespace sx {
+
+// ---------------------------------------------------------------------------
+// Coulomb-screening preconditioner for the charge solve.
+//   M^-1 = D^-1/2 (I - W^T F W) D^-1/2,   D = J + 2w [q outside (qmin, qmax)]
+// W spreads atom values onto the PME grid with the B-spline weights, F is a
+// k-space filter.  For a smooth charge mode of wave vector k the Hessian acts
+// as J + lambda(k), lambda(k) = (kc/pi) n0 G(k) (n0 = atoms per volume, G the
+// PME influence function), which grows like 1/k^2 and makes plain Jacobi CG
+// slow for large boxes.  With f = (lambda/J0)/(1 + lambda/J0) and
+// F = f/(n0 h^3), W^T F W ~ f on smooth modes, so M^-1 ~ 1/(J + lambda) there
+// and ~ 1/J on short wavelengths.  Neutrality: z = M g - u (u.g)/(1.u), u = M 1.
+// ---------------------------------------------------------------------------
+// Filter tables on the half spectrum (same layout as G; values / Ng):
+//   stage 0: S(k) = exp(-k^2 sigma^2)                       (low-pass)
+//   stage 1: F(k) = f(k) S(k) / rho_ref,  f = l/(1+l),  l = lam_scale G(k)/J0
+// rho_ref = mean low-passed atom density per grid point at the atoms; with the
+// per-atom scaling c_i = sqrt(rho_ref/rho_i), C W^T F W C ~ f on smooth modes
+// in every region (slab, oxide layer), so 0 < M.
+template <class R>
+__global__ void k_filter_table(Geom g, const double* __restrict__ G64, double sigma2,
+                               double lam_scale, double inv_j0, double inv_rho, int stage,
+                               R* __restrict__ F) {
+  const size_t total = size_t(g.H) * g.K[1] * g.K[2];
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int kx = int(t % g.H);
+  const int ky = int((t / g.H) % g.K[1]);
+  const int kz = int(t / (size_t(g.H) * g.K[1]));
+  const int k3[3] = {kx, ky, kz};
+  double m2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int K = g.K[d];
+    const int m = (k3[d] + K / 2) % K - K / 2;
+    m2 += double(m * m) / (g.box[d] * g.box[d]);
+  }
+  const double ng = double(g.K[0]) * g.K[1] * g.K[2];
+  const double S = exp(-4.0 * PI * PI * m2 * sigma2);
+  double v = S;
+  if (stage == 1) {
+    const double lam = lam_scale * G64[t] * ng * inv_j0;  // G64 holds G/Ng
+    v = lam / (1.0 + lam) * S * inv_rho;
+  }
+  F[t] = R(v / ng);
+}
+
+// part[block] = max of grid values over the block's span
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_grid_max(size_t ng, const R* __restrict__ grid,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  double m = -1e300;
+  for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < ng;
+       t += size_t(gridDim.x) * blockDim.x)
+    m = fmax(m, double(grid[t]));
+  m = wave_max(m);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  if (lane == 0) lds[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = lds[0];
+    for (int q = 1; q < int(blockDim.x / WAVE); ++q) t = fmax(t, lds[q]);
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_fill_f64(int n, double v, double* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// local smoothed atom density at each atom, rho_i = W^T (S * W 1); part = sum rho
+template <class Rg>
+__global__ void __launch_bounds__(BLOCK) k_prec_density(Geom g, const double* __restrict__ x,
+                                                        const Rg* __restrict__ grid,
+                                                        double* __restrict__ rho,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < g.n) {
+    const double r = fmax(gather_phi(g, x + 3 * i, grid), 1e-12);
+    rho[i] = r;
+    acc[0] = r;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// c_i = sqrt(rho_ref / rho_i), rho_ref = mean rho (from its sum)
+__global__ void k_prec_c(int n, const double* __restrict__ rho_sum, double* __restrict__ c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = sqrt(rho_sum[0] / double(n) / c[i]);
+}
+
+// s = c g / sqrt(D)
+__global__ void k_prec_scale(int n, const Tables* __restrict__ tb, const int* __restrict__ id,
+                             const double* __restrict__ q, const double* __restrict__ g,
+                             const double* __restrict__ c, double* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = c[i] * g[i] / sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+}
+
+// z = (g/sqrt(D) - c W^T grid)/sqrt(D); part[k] = sum z
+template <class Rg>
+__global__ void __launch_bounds__(BLOCK) k_prec_finish(Geom g, const Tables* __restrict__ tb,
+                                                       const double* __restrict__ x,
+                                                       const int* __restrict__ id,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ gr,
+                                                       const double* __restrict__ c,
+                                                       const Rg* __restrict__ grid,
+                                                       double* __restrict__ z,
+                                                       double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < g.n) {
+    const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+    const double zi = (gr[i] / sd - c[i] * gather_phi(g, x + 3 * i, grid)) / sd;
+    z[i] = zi;
+    acc[0] = zi;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Neutrality: z -= mean(z) (P z with P the projector on sum = 0; P M P is
+// symmetric positive definite on that subspace).  part[2k..] = z.g, z.g_old
+__global__ void __launch_bounds__(BLOCK) k_prec_project(int n, const double* __restrict__ zsum,
+                                                        const double* __restrict__ gr,
+                                                        const double* __restrict__ g_old,
+                                                        double* __restrict__ z,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    const double zi = z[i] - zsum[0] / double(n);
+    z[i] = zi;
+    acc[0] = zi * gr[i];
+    acc[1] = g_old ? zi * g_old[i] : 0.0;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+}  // namespace sx
+
+namespace sx {
+
+// Device-side scalar logic of one nonlinear-CG iteration (no host round trip).
+// ls layout: [0] g.p  [1] p.H0p  [2] a  [3] lo  [4] hi  [5] done  [6] rz
+//            [7] beta  [8] descent-failure flag
+// red: reduction outputs; red[0] = p.H0p, red[1] = p.g, red[2] = sum p^2 pen''(q)
+__global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ ls) {
+  const double pHp = red[0], gp = red[1], s2 = red[2];
+  ls[0] = gp;
+  ls[1] = pHp;
+  ls[3] = 0.0;
+  ls[4] = -1.0;
+  const bool bad = !(pHp > 0.0) || !(gp > 0.0);
+  ls[8] = bad ? 1.0 : 0.0;
+  ls[5] = bad ? 1.0 : 0.0;
+  ls[2] = bad ? 0.0 : gp / (pHp + s2);
+}
+
+// Newton step on phi'(a) = -gp + a pHp + D1(a), bracketed; red = {D1(a), S2(a)}
+__global__ void k_ls_update(const double* __restrict__ red, double* __restrict__ ls) {
+  if (ls[5] != 0.0) return;
+  const double gp = ls[0], pHp = ls[1], a = ls[2];
+  const double fprime = -gp + a * pHp + red[0];
+  if (fabs(fprime) <= 1e-13 * (gp + fabs(a * pHp))) { ls[5] = 1.0; return; }
+  double lo = ls[3], hi = ls[4];
+  if (fprime < 0.0) lo = a; else hi = a;
+  double an = a - fprime / (pHp + red[1]);
+  if (an <= lo || (hi >= 0.0 && an >= hi)) an = hi >= 0.0 ? 0.5 * (lo + hi) : 2.0 * a;
+  if (fabs(an - a) <= 1e-15 * fabs(a)) ls[5] = 1.0;
+  ls[2] = an;
+  ls[3] = lo;
+  ls[4] = hi;
+}
+
+// beta = max(0, (z.g - z.g_old)/rz); rz = z.g; red = {z.g, z.g_old}
+__global__ void k_beta(const double* __restrict__ red, double* __restrict__ ls) {
+  const double b = (red[0] - red[1]) / ls[6];
+  ls[7] = b > 0.0 ? b : 0.0;
+  ls[6] = red[0];
+}
+
+// line-search terms with the step read from device memory (skipped once done)
+__global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __restrict__ tb,
+                                                      const int* __restrict__ id,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ p,
+                                                      const double* __restrict__ ls,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  if (ls[5] != 0.0) return;
+  const double a = ls[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double pi = p[i], qi = q[i], qa = qi + a * pi;
+    acc[0] = pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
+    acc[1] = outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+__global__ void k_reduce_if(const double* __restrict__ part, int nparts, int nv,
+                            const double* __restrict__ ls, double* __restrict__ out) {
+  __shared__ double lds[1024 / WAVE];
+  if (ls[5] != 0.0) return;
+  for (int v = 0; v < nv; ++v) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(nv) * k + v];
+    s = wave_sum(s);
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    if (lane == 0) lds[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = lds[0];
+      for (int q = 1; q < int(blockDim.x / WAVE); ++q) t += lds[q];
+      out[v] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// q += a p; g_old = g; g -= a v + dpen' with a from device memory; part = sum g
+__global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __restrict__ tb,
+                                                        const int* __restrict__ id,
+                                                        const double* __restrict__ ls,
+                                                        const double* __restrict__ p,
+                                                        const double* __restrict__ v,
+                                                        double* __restrict__ q,
+                                                        double* __restrict__ g,
+                                                        double* __restrict__ g_old,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const double a = ls[8] != 0.0 ? 0.0 : ls[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double qi = q[i], qn = qi + a * p[i];
+    const double gi = g[i];
+    g_old[i] = gi;
+    const double gn = gi - a * v[i] - (pen_d1(sc, qn) - pen_d1(sc, qi));
+    q[i] = qn;
+    g[i] = gn;
+    acc[0] = gn;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ void k_pcg_dir_dev(int n, const double* __restrict__ ls, const double* __restrict__ z,
+                              double* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + ls[7] * p[i];
+}
+
+}  // namespace sx

@@ -36,7 +36,7 @@ def small():
 def test_fp64_charges_are_the_exact_minimiser(small, oracle_mod):
     s = small
     g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
-    g.set_charge_tol(1e-11, 2000)
+    g.set_charge_tol(1e-12, 4000)
     g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
     res = g.ctip_solve()
     q = g.get_state()[3]
@@ -95,7 +95,7 @@ def test_mixed_matches_fp64(name):
     for k in ("Epair", "Egroup", "Edir", "Ectip"):
         assert a["e"][k] == pytest.approx(b["e"][k], rel=1e-6, abs=1e-6)
     assert a["erec"] == pytest.approx(b["erec"], rel=1e-6)
-    assert a["eself"] == pytest.approx(b["eself"], rel=1e-8)
+    assert a["eself"] == pytest.approx(b["eself"], rel=1e-6)
 
 
 def test_mixed_nve_energy_tracks_fp64():
