@@ -92,6 +92,11 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
                  const simox_params* params, simox_ctx** out);
 int simox_destroy(simox_ctx* ctx);
 const char* simox_last_error(const simox_ctx* ctx);
+/* Replaces the potential parameters of an existing context (the caller's
+   POTENTIAL_SETUP output, datafmt.f90:142-262, possibly edited) without
+   touching atoms or cells: the drop-in FORCE / CTIP_SOLVER pass their eam,
+   oxygn, param arguments through here (force.f90:3, ctip.f90:3). */
+int simox_set_params(simox_ctx* ctx, const simox_params* params);
 int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
 int simox_set_mode(simox_ctx* ctx, int mode);
 /* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged

@@ -21,7 +21,7 @@ EXPORTS = [
     "simox_get_state", "simox_cell_sort", "simox_get_cells", "simox_force", "simox_get_rho",
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
-    "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats",
+    "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
 ]
 
 # simox_kernel_timing categories
@@ -78,6 +78,7 @@ def load_library(path=LIB_PATH):
     lib.simox_get_geometry.argtypes = [vp, P(i), P(d), P(d)]
     lib.simox_set_mode.argtypes = [vp, i]
     lib.simox_set_charge_tol.argtypes = [vp, d, i]
+    lib.simox_set_params.argtypes = [vp, vp]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
     lib.simox_cell_sort.argtypes = [vp]
@@ -167,6 +168,10 @@ class Simox:
         a = C.c_double()
         self._check(self.lib.simox_get_geometry(self.h, nc, lc, C.byref(a)))
         return tuple(nc), tuple(lc), a.value
+
+    def set_params(self, params):
+        """Replace the potential parameters (a Params from potential_setup())."""
+        self._check(self.lib.simox_set_params(self.h, C.byref(params)))
 
     def set_charge_tol(self, tol, maxit=1000):
         self._check(self.lib.simox_set_charge_tol(self.h, float(tol), int(maxit)))

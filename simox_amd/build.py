@@ -38,20 +38,42 @@ def build_lib(out=LIB):
 
 
 def build_fortran():
-    """Fortran iso_c_binding module, the drop-in FORCE/PME/... shim and the host
-    driver program (simox_amd/fortran)."""
+    """The iso_c_binding module and the Fortran host program simox_md
+    (simox_amd/fortran/_build, executable simox_amd/fortran/simox_md)."""
     fdir = os.path.join(HERE, "fortran")
     bdir = os.path.join(fdir, "_build")
     os.makedirs(bdir, exist_ok=True)
-    srcs = ["simox_datafmt.f90", "simox_gpu_iface.f90", "simox_dropin.f90", "simox_md.f90"]
     objs = []
-    for s in srcs:
-        o = os.path.join(bdir, s.replace(".f90", ".o"))
-        _run([FC, "-O2", "-c", os.path.join(fdir, s), "-o", o, "-module-dir", bdir], cwd=bdir)
+    for src in ("simox_gpu.f90", "simox_md.f90"):
+        o = os.path.join(bdir, src.replace(".f90", ".o"))
+        _run([FC, "-O2", "-c", os.path.join(fdir, src), "-o", o, "-module-dir", bdir], cwd=bdir)
         objs.append(o)
     exe = os.path.join(fdir, "simox_md")
-    _run([FC, "-O2", "-o", exe, *objs, "-L" + HERE, "-lsimox_gpu",
-          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE], cwd=bdir)
+    _run([FC, "-O2", "-o", exe, *objs, "-L" + HERE, "-lsimox_gpu", "-Wl,-rpath," + HERE,
+          "-Wl,-rpath,$ORIGIN/.."], cwd=bdir)
+    return exe
+
+
+def build_dropin_demo(ref="/root/reference"):
+    """The reference's own program (main.f90 and its other units) with
+    force.f90 / ctip.f90 / cellsort.f90 replaced by simox_dropin.f90: SIMOX
+    running its hot path on the GPU, FFTE no longer needed.  Built only where the reference
+    sources are present; the executable is simox_amd/fortran/_build/simox_dropin_demo."""
+    if not os.path.isdir(ref):
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(fdir, "_build", "dropin")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for src in [os.path.join(ref, f + ".f90") for f in ("datafmt", "eam", "vverlet", "util")] + \
+            [os.path.join(fdir, "simox_gpu.f90"), os.path.join(fdir, "simox_dropin.f90"),
+             os.path.join(ref, "main.f90")]:
+        o = os.path.join(bdir, os.path.basename(src).replace(".f90", ".o"))
+        _run([FC, "-O2", "-c", src, "-o", o, "-module-dir", bdir], cwd=bdir)
+        objs.append(o)
+    exe = os.path.join(fdir, "_build", "simox_dropin_demo")
+    _run([FC, "-O2", "-o", exe, *objs, "-L" + HERE, "-lsimox_gpu", "-Wl,-rpath," + HERE,
+          "-Wl,-rpath,$ORIGIN/../.."], cwd=bdir)
     return exe
 
 
@@ -59,3 +81,4 @@ if __name__ == "__main__":
     print(build_lib())
     if "--all" in sys.argv:
         print(build_fortran())
+        print(build_dropin_demo())

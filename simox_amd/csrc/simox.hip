@@ -1329,6 +1329,18 @@ int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) 
   return SIMOX_OK;
 }
 
+int simox_set_params(simox_ctx* ctx, const simox_params* params) {
+  if (!ctx || !params) return SIMOX_ERR_ARG;
+  ctx->prm = *params;
+  build_tables(ctx);
+  HIPCHK(hipMemcpyAsync(ctx->dtab, &ctx->htab, sizeof(Tables), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->matrix_valid = false;
+  ctx->nlist_valid = false;
+  ctx->prec_ready = false;
+  return SIMOX_OK;
+}
+
 int simox_cell_sort(simox_ctx* ctx) {
   if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
   CHK(do_cell_sort(ctx));

@@ -1,0 +1,154 @@
+"""The Fortran host side on the GPU.
+
+simox_md (simox_amd/fortran/simox_md.f90) reads the reference's input files
+(config.prm, restart.xyz; main.f90:122-245), runs the REGULAR NVE loop of
+main.f90:41-61 through simox_step with TAG_SET's cadence (main.f90:296-334)
+and writes temporal_status.dat in PRINT_STAT's format (util.f90:12-18).  Its
+full-precision status file must equal the Python binding driving the same
+library with the same step flags, bit for bit.
+
+The drop-in build (the reference's main.f90 / util / vverlet / cellsort with
+simox_dropin.f90 in place of force.f90 and ctip.f90) exists only where the
+reference sources were present at build time; when it was built, its
+temporal_status.dat must agree with simox_md's to the file's 5 significant
+digits.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from simox_amd import MODE_FAITHFUL, Simox
+from simox_amd.systems import TFM, tiny
+
+pytestmark = pytest.mark.gpu
+
+FDIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "simox_amd", "fortran")
+SIMOX_MD = os.path.join(FDIR, "simox_md")
+DROPIN = os.path.join(FDIR, "_build", "simox_dropin_demo")
+ELEMENTS = {1: "Al", 2: "Ni", 3: "Fe", 4: "O", 5: "H"}
+NSTEPS, FREQ_NEW, FREQ_SORT = 12, 5, 3
+DT_FS = 1.0
+DT = DT_FS / TFM   # main.f90:142, as simox_md computes it
+
+
+def write_inputs(d, s, f, q, nsteps=NSTEPS, restart=True):
+    dt_fs = DT_FS
+    with open(os.path.join(d, "config.prm"), "w") as fh:
+        fh.write("# SIMOX configuration\n# simulation type\nREGULAR\n"
+                 "# time_max time_snap time_rdf time_stat dt (fs)\n"
+                 f"{nsteps * dt_fs:.17e} {6 * dt_fs:.17e} {100 * nsteps * dt_fs:.17e} {dt_fs:.17e} {dt_fs:.17e}\n"
+                 f"# box\n{s.box[0]:.17e} {s.box[1]:.17e} {s.box[2]:.17e}\n"
+                 f"# fft\n{s.fft[0]} {s.fft[1]} {s.fft[2]}\n"
+                 "# thermostat T alpha\nNO 300.0 0.1\n# restart\n" + ("YES" if restart else "NO") + "\n"
+                 f"# rcut\n{s.rcut:.17e}\n# freq_new freq_sort\n{FREQ_NEW} {FREQ_SORT}\n"
+                 "# z_new N_o2\n0.0 0\n# q_crit\n0.0\n")
+    with open(os.path.join(d, "restart.xyz" if restart else "input.xyz"), "w") as fh:
+        fh.write(f"{s.n}\nframe\n")
+        for i in range(s.n):
+            vals = list(s.x[i]) + list(s.v[i]) + list(f[i]) + [q[i]] if restart else \
+                list(s.x[i]) + [q[i]]
+            fh.write(ELEMENTS[int(s.ids[i])] + " " + " ".join(f"{v:.17e}" for v in vals) + "\n")
+
+
+@pytest.fixture(scope="module")
+def case(tmp_path_factory):
+    """A tiny slab with charges and forces from one faithful solve."""
+    s = tiny(seed=3)
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FAITHFUL)
+    g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+    g.ctip_solve()
+    g.force()
+    g.pme()
+    _, _, f, q = g.get_state()
+    d = str(tmp_path_factory.mktemp("fortran"))
+    write_inputs(d, s, f, q)
+    return s, f, q, d
+
+
+def python_history(s, f, q):
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FAITHFUL)
+    g.set_state(s.ids, s.x, s.v, f, q)
+    g.pme_self()
+    rows = []
+    for nloop in range(1, NSTEPS + 1):
+        st = g.step(1, DT, 1 if nloop % FREQ_SORT == 0 else 0, 1 if nloop % FREQ_NEW == 1 else 0)
+        e_es = st["Eself"] + st["Edir"] + st["Erec"] + st["Ectip"]
+        e_eam = st["Epair"] + st["Egroup"]
+        rows.append([nloop * TFM * DT, st["mv2"] / (3 * s.n) / 8.617343e-5,
+                     e_es + e_eam + st["mv2"] / 2, e_es, e_eam, st["Epair"], st["Egroup"]])
+    return np.array(rows), g.get_state()
+
+
+def load_status(path):
+    """Rows of a status file.  The header is written list-directed
+    (main.f90:37-38), which may wrap it over two lines; data rows are the
+    lines of 7 numbers."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            try:
+                rows.append([float(t) for t in tok])
+            except ValueError:
+                continue
+    assert rows and all(len(r) == 7 for r in rows), path
+    return np.array(rows)
+
+
+def run(exe, d):
+    print("running", exe, "in", d, flush=True)
+    r = subprocess.run([exe], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+
+
+def test_simox_md_matches_python_binding(case):
+    s, f, q, d = case
+    assert os.path.exists(SIMOX_MD), "simox_md not built (python -m simox_amd.build --all)"
+    run(SIMOX_MD, d)
+    full = load_status(os.path.join(d, "temporal_status_full.dat"))
+    ref, (x, v, ff, qq) = python_history(s, f, q)
+    assert full.shape == ref.shape
+    np.testing.assert_allclose(full, ref, rtol=1e-15, atol=0)
+    short = load_status(os.path.join(d, "temporal_status.dat"))
+    np.testing.assert_allclose(short, ref, rtol=6e-4)
+    # restart snapshot at step 6 (PRINT_SNAPSHOT format) parses and has every atom
+    with open(os.path.join(d, "snapshot001.xyz")) as fh:
+        lines = fh.read().splitlines()
+    assert int(lines[0]) == s.n and len(lines) == s.n + 2
+    assert lines[2].split()[0] == ELEMENTS[int(s.ids[0])]
+
+
+def test_reference_program_with_dropin(case, tmp_path):
+    s, f, q, d = case
+    if not os.path.exists(DROPIN):
+        pytest.skip("drop-in demo is built only where the reference sources are present")
+    if not os.path.exists(os.path.join(d, "temporal_status_full.dat")):
+        run(SIMOX_MD, d)
+    dd = str(tmp_path)
+    write_inputs(dd, s, f, q)
+    run(DROPIN, dd)
+    a = load_status(os.path.join(dd, "temporal_status.dat"))
+    b = load_status(os.path.join(d, "temporal_status_full.dat"))
+    assert a.shape == b.shape
+    np.testing.assert_allclose(a, b, rtol=6e-4)
+
+
+def test_simox_md_vinit(case, tmp_path):
+    """restart = NO: VINIT's velocities (vverlet.f90:2-71) at T_given = 300 K,
+    then 12 NVE steps; the first status row is near 300 K and the run is
+    reproducible for a fixed SIMOX_SEED."""
+    s, f, q, _ = case
+    rows = []
+    for k in range(2):
+        d = str(tmp_path / str(k))
+        os.makedirs(d)
+        write_inputs(d, s, f, q, restart=False)
+        run(SIMOX_MD, d)
+        rows.append(load_status(os.path.join(d, "temporal_status_full.dat")))
+    assert rows[0].shape == (NSTEPS, 7)
+    np.testing.assert_array_equal(rows[0], rows[1])
+    assert abs(rows[0][0, 1] - 300.0) < 30.0
+    assert np.all(np.isfinite(rows[0]))
