@@ -50,6 +50,27 @@ template <class C> struct Dft<C, 4> {
     v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
   }
 };
+template <class C> struct Dft<C, 8> {
+  // two 4-point DFTs (even / odd samples) joined by the twiddles W_8^{s k}
+  __device__ static void run(C* v, int s) {
+    using R = decltype(v[0].x);
+    const R h = R(0.70710678118654752440);
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<C, 4>::run(e, s);
+    Dft<C, 4>::run(o, s);
+    C w1, w3;
+    w1.x = h * (o[1].x - R(s) * o[1].y);
+    w1.y = h * (o[1].y + R(s) * o[1].x);
+    const C w2 = cis(o[2], s);
+    w3.x = h * (-o[3].x - R(s) * o[3].y);
+    w3.y = h * (-o[3].y + R(s) * o[3].x);
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);   v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);   v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);   v[7] = csub(e[3], w3);
+  }
+};
 template <class C> struct Dft<C, 3> {
   __device__ static void run(C* v, int s) {
     using R = decltype(v[0].x);
@@ -135,9 +156,10 @@ __device__ inline C* lds_fft(C* b0, C* b1, int M, int L, int stride, const FftPl
 
 // Compile-time plans: for the grid sizes of the workloads the line length is a
 // template parameter, so every stage's radix, span and the index arithmetic
-// (divisions by L/R and Ns) are constants.
+// (divisions by L/R and Ns) are constants.  Radix 8 first (3 stages for 256,
+// 512, 192), then 4, 2, 3, 5.
 __host__ __device__ constexpr int ct_radix(int m) {
-  return (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
+  return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : 5;
 }
 __host__ __device__ constexpr int ct_nstages(int L) {
   int m = L, n = 0;
@@ -161,7 +183,8 @@ __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ d
   constexpr int nb = L / R;
   constexpr int tws = L / (Ns * R);
   const int total = M * nb;
-  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+#pragma unroll 4
+  for (int b = threadIdx.x; b < total; b += BLOCK) {
     const int line = b / nb;
     const int j = b - line * nb;
     const int k = j % Ns;
@@ -241,121 +264,242 @@ __device__ inline int wrapk(int b, int nn, int K) {
   return v < 0 ? v + K : v;
 }
 
-// K7a: R2C along x.  Reads the real charge grid, transforms M lines per
-// block in LDS, writes kx = 0..Kx/2.
-template <class C, int L_CT>
-__global__ void __launch_bounds__(BLOCK) k_fft_x_r2c(Geom g, FftPlan pl, const C* __restrict__ tw,
-                                                     const decltype(C{}.x)* __restrict__ qgrid,
-                                                     C* __restrict__ spec, int M, int stride) {
-  using R = decltype(C{}.x);
-  extern __shared__ __align__(16) unsigned char smem[];
-  C* b0 = reinterpret_cast<C*>(smem);
-  C* b1 = b0 + M * stride;
-  const int L = g.K[0], H = g.H, nlines = g.K[1] * g.K[2];
-  const int line0 = blockIdx.x * M;
-  for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-    const int t = e / L, xk = e - t * L;
-    const int line = line0 + t;
-    C v; v.x = R(0); v.y = R(0);
-    if (line < nlines) v.x = qgrid[size_t(line) * L + xk];
-    b0[t * stride + xk] = v;
-  }
-  __syncthreads();
-  const C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, -1);
-  for (int e = threadIdx.x; e < M * H; e += blockDim.x) {
-    const int t = e / H, k = e - t * H;
-    const int line = line0 + t;
-    if (line < nlines) spec[size_t(line) * H + k] = res[t * stride + k];
+// Half-spectrum layout: spec[(kx * Ky + ky) * Kz + kz], kx = 0 .. Kx/2.
+// The x passes transpose between the real grid (x fastest) and this layout,
+// so the z pass works on contiguous lines and the y pass on columns of
+// consecutive kz (coalesced in both).
+__device__ inline size_t spec_index(const Geom& g, int kx, int ky, int kz) {
+  return (size_t(kx) * g.K[1] + ky) * size_t(g.K[2]) + kz;
+}
+
+// K7a: R2C along x.  Block (bz, y) transforms the M real lines
+// (y, z = bz*M .. bz*M+M-1), two per complex FFT (a + i b), and writes
+// kx = 0..Kx/2 of each line, M consecutive kz per kx.
+//   A[k] = (Z[k] + conj(Z[-k])) / 2,  B[k] = (Z[k] - conj(Z[-k])) / (2i).
+// Element loop e = threadIdx.x, +BLOCK, ... < total in chunks of U: the U
+// loads of a chunk are issued before any of them is consumed, so a thread
+// keeps U global loads in flight instead of waiting on each in turn.
+template <int U, class T, class Ld, class St>
+__device__ inline void staged(int total, Ld ld, St st) {
+  for (int e0 = threadIdx.x; e0 < total; e0 += U * BLOCK) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * BLOCK;
+      if (e < total) v[u] = ld(e);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * BLOCK;
+      if (e < total) st(e, v[u]);
+    }
   }
 }
 
-// K7b/K8: FFT along a strided axis of the half spectrum.  Block (bx, by)
-// transforms the lines at x = bx*M .. bx*M+M-1 (contiguous, coalesced) and
-// outer index by.  FUSE: forward FFT, multiply by the influence table G
-// (same layout as spec, 1/Ng folded in), accumulate sum w_kx G |X|^2 into
-// epart, inverse FFT.
-template <class C, bool FUSE, int L_CT>
-__global__ void __launch_bounds__(BLOCK) k_fft_strided(Geom g, FftPlan pl,
+// Copies the twiddle table W_L^k (k < L) into LDS.
+template <class C>
+__device__ inline void load_twiddles(C* __restrict__ twl, const C* __restrict__ tw, int L) {
+  for (int e = threadIdx.x; e < L; e += blockDim.x) twl[e] = tw[e];
+}
+
+// LDS layout of the x passes (in units of C): b0 = P (L+1), b1 = max(P (L+1),
+// M (H+1)) (also the C2R staging area), then the L twiddles.
+__host__ __device__ inline int xpass_b1(int M, int L) {
+  const int P = M / 2, H = L / 2 + 1;
+  return P * (L + 1) > M * (H + 1) ? P * (L + 1) : M * (H + 1);
+}
+
+// K7a: R2C along x.  Block (bz, y) transforms the M real lines
+// (y, z = bz*M .. bz*M+M-1), two per complex FFT (a + i b), and writes
+// kx = 0..Kx/2 of each line, M consecutive kz per kx.
+//   A[k] = (Z[k] + conj(Z[-k])) / 2,  B[k] = (Z[k] - conj(Z[-k])) / (2i).
+template <class C, int L_CT>
+__global__ void __launch_bounds__(BLOCK) k_fft_x_r2c(Geom g, FftPlan pl, const C* __restrict__ tw,
+                                                     const decltype(C{}.x)* __restrict__ qgrid,
+                                                     C* __restrict__ spec, int M) {
+  using R = decltype(C{}.x);
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = g.K[0], H = g.H, Ky = g.K[1];
+  const int P = M / 2, sx = L + 1;
+  C* b0 = reinterpret_cast<C*>(smem);
+  C* b1 = b0 + P * sx;
+  C* twl = b1 + xpass_b1(M, L);
+  load_twiddles(twl, tw, L);
+  const int z0 = blockIdx.x * M, y = blockIdx.y;
+  staged<8, C>(
+      P * L,
+      [&](int e) {
+        const int p = e / L, xk = e - p * L;
+        const size_t row = (size_t(z0 + 2 * p) * Ky + y) * L;
+        C v;
+        v.x = qgrid[row + xk];
+        v.y = qgrid[row + size_t(Ky) * L + xk];
+        return v;
+      },
+      [&](int e, C v) {
+        const int p = e / L, xk = e - p * L;
+        b0[p * sx + xk] = v;
+      });
+  __syncthreads();
+  const C* res = lds_fft_any<C, L_CT>(b0, b1, P, L, sx, pl, twl, -1);
+  for (int e = threadIdx.x; e < M * H; e += blockDim.x) {
+    const int kx = e / M, t = e - kx * M;
+    const int p = t >> 1;
+    const C z = res[p * sx + kx];
+    const C w = res[p * sx + (kx == 0 ? 0 : L - kx)];
+    C v;
+    if ((t & 1) == 0) {
+      v.x = R(0.5) * (z.x + w.x);
+      v.y = R(0.5) * (z.y - w.y);
+    } else {
+      v.x = R(0.5) * (z.y + w.y);
+      v.y = R(0.5) * (w.x - z.x);
+    }
+    spec[spec_index(g, kx, y, z0 + t)] = v;
+  }
+}
+
+// K7b: FFT along y of the half spectrum.  Block (bz, kx) transforms the
+// columns kz = bz*M .. bz*M+M-1 (contiguous, coalesced) at that kx.
+// LDS: two buffers of M (L+1), then the L twiddles.
+template <class C, int L_CT>
+__global__ void __launch_bounds__(BLOCK) k_fft_y(Geom g, FftPlan pl, const C* __restrict__ tw,
+                                                 C* __restrict__ spec, int M, int stride, int sign) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = g.K[1], Kz = g.K[2];
+  C* b0 = reinterpret_cast<C*>(smem);
+  C* b1 = b0 + M * stride;
+  C* twl = b1 + M * stride;
+  load_twiddles(twl, tw, L);
+  const int z0 = blockIdx.x * M;
+  const size_t base = size_t(blockIdx.y) * L * Kz + z0;
+  staged<8, C>(
+      M * L,
+      [&](int e) {
+        const int l = e / M, t = e - l * M;
+        return spec[base + size_t(l) * Kz + t];
+      },
+      [&](int e, C v) {
+        const int l = e / M, t = e - l * M;
+        b0[t * stride + l] = v;
+      });
+  __syncthreads();
+  const C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, twl, sign);
+  for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
+    const int l = e / M, t = e - l * M;
+    spec[base + size_t(l) * Kz + t] = res[t * stride + l];
+  }
+}
+
+// K7c/K8: along z (contiguous lines of the half spectrum), fused: forward
+// FFT, multiply by the influence table G (same layout, 1/Ng folded in),
+// accumulate sum w_kx G |X|^2 into epart[block], inverse FFT.
+// LDS: two buffers of M L, then the L twiddles.
+template <class C, int L_CT>
+__global__ void __launch_bounds__(BLOCK) k_fft_z_fused(Geom g, FftPlan pl,
                                                        const C* __restrict__ tw,
-                                                       C* __restrict__ spec, int es, int os,
-                                                       int M, int stride, int sign,
+                                                       C* __restrict__ spec, int M,
                                                        const decltype(C{}.x)* __restrict__ G,
                                                        double* __restrict__ epart) {
   using R = decltype(C{}.x);
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double lds_red[BLOCK / WAVE];
+  const int L = g.K[2], Ky = g.K[1];
   C* b0 = reinterpret_cast<C*>(smem);
-  C* b1 = b0 + M * stride;
-  const int L = pl.L, H = g.H;
-  const int x0 = blockIdx.x * M;
-  const size_t obase = size_t(blockIdx.y) * os;
-  for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-    const int l = e / M, t = e - l * M;
-    const int xx = x0 + t;
-    C v; v.x = R(0); v.y = R(0);
-    if (xx < H) v = spec[obase + size_t(l) * es + xx];
-    b0[t * stride + l] = v;
-  }
+  C* b1 = b0 + M * L;
+  C* twl = b1 + M * L;
+  load_twiddles(twl, tw, L);
+  const int nlines = g.H * Ky;
+  const int line0 = blockIdx.x * M;
+  staged<8, C>(
+      M * L,
+      [&](int e) {
+        const int t = e / L, k = e - t * L;
+        C v; v.x = R(0); v.y = R(0);
+        if (line0 + t < nlines) v = spec[size_t(line0 + t) * L + k];
+        return v;
+      },
+      [&](int e, C v) { b0[e] = v; });
   __syncthreads();
-  C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, FUSE ? -1 : sign);
-  if (FUSE) {
-    double en[1] = {0.0};
-    for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-      const int l = e / M, t = e - l * M;
-      const int xx = x0 + t;
-      C v = res[t * stride + l];
-      R gv = R(0);
-      if (xx < H) gv = G[obase + size_t(l) * es + xx];
-      const double wx = (xx == 0 || 2 * xx == g.K[0]) ? 1.0 : 2.0;
-      if (xx < H) en[0] += wx * double(gv) * (double(v.x) * double(v.x) + double(v.y) * double(v.y));
-      v.x *= gv; v.y *= gv;
-      res[t * stride + l] = v;
-    }
-    __syncthreads();
-    C* other = (res == b0) ? b1 : b0;
-    res = lds_fft_any<C, L_CT>(res, other, M, L, stride, pl, tw, +1);
-    block_sum<1>(en, lds_red);
-    if (threadIdx.x == 0) epart[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = en[0];
-  }
+  C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, L, pl, twl, -1);
+  double en[1] = {0.0};
+  staged<8, R>(
+      M * L,
+      [&](int e) {
+        const int line = line0 + e / L;
+        return line < nlines ? G[size_t(line0) * L + e] : R(0);
+      },
+      [&](int e, R gv) {
+        const int line = line0 + e / L;
+        C v = res[e];
+        if (line < nlines) {
+          const int kx = line / Ky;
+          const double wx = (kx == 0 || 2 * kx == g.K[0]) ? 1.0 : 2.0;
+          en[0] += wx * double(gv) * (double(v.x) * double(v.x) + double(v.y) * double(v.y));
+        }
+        v.x *= gv; v.y *= gv;
+        res[e] = v;
+      });
+  __syncthreads();
+  C* other = (res == b0) ? b1 : b0;
+  res = lds_fft_any<C, L_CT>(res, other, M, L, L, pl, twl, +1);
+  block_sum<1>(en, lds_red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = en[0];
   for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-    const int l = e / M, t = e - l * M;
-    const int xx = x0 + t;
-    if (xx < H) spec[obase + size_t(l) * es + xx] = res[t * stride + l];
+    if (line0 + e / L < nlines) spec[size_t(line0) * L + e] = res[e];
   }
 }
 
-// K7c: C2R along x (inverse, unnormalised); rebuilds the Hermitian line.
+// K7d: C2R along x (inverse, unnormalised).  Block (bz, y) gathers the half
+// lines (y, z = bz*M ..), stages them in LDS, forms Z = A + i B over the full
+// Hermitian line (A[L-k] = conj(A[k])), inverse-transforms, and writes Re Z
+// and Im Z as the two real lines.
 template <class C, int L_CT>
 __global__ void __launch_bounds__(BLOCK) k_fft_x_c2r(Geom g, FftPlan pl, const C* __restrict__ tw,
                                                      const C* __restrict__ spec,
-                                                     decltype(C{}.x)* __restrict__ out, int M,
-                                                     int stride) {
-  using R = decltype(C{}.x);
+                                                     decltype(C{}.x)* __restrict__ out, int M) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const int L = g.K[0], H = g.H, Ky = g.K[1];
+  const int P = M / 2, sx = L + 1, hs = H + 1;
   C* b0 = reinterpret_cast<C*>(smem);
-  C* b1 = b0 + M * stride;
-  const int L = g.K[0], H = g.H, nlines = g.K[1] * g.K[2];
-  const int line0 = blockIdx.x * M;
-  for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-    const int t = e / L, k = e - t * L;
-    const int line = line0 + t;
-    C v; v.x = R(0); v.y = R(0);
-    if (line < nlines) {
-      if (k < H) {
-        v = spec[size_t(line) * H + k];
-      } else {
-        C w = spec[size_t(line) * H + (L - k)];
-        v.x = w.x; v.y = -w.y;
-      }
+  C* b1 = b0 + P * sx;
+  C* twl = b1 + xpass_b1(M, L);
+  load_twiddles(twl, tw, L);
+  const int z0 = blockIdx.x * M, y = blockIdx.y;
+  staged<8, C>(
+      M * H,
+      [&](int e) {
+        const int kx = e / M, t = e - kx * M;
+        return spec[spec_index(g, kx, y, z0 + t)];
+      },
+      [&](int e, C v) {
+        const int kx = e / M, t = e - kx * M;
+        b1[t * hs + kx] = v;
+      });
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * L; e += blockDim.x) {
+    const int p = e / L, k = e - p * L;
+    C a, b;
+    if (k < H) {
+      a = b1[(2 * p) * hs + k];
+      b = b1[(2 * p + 1) * hs + k];
+    } else {
+      a = b1[(2 * p) * hs + (L - k)]; a.y = -a.y;
+      b = b1[(2 * p + 1) * hs + (L - k)]; b.y = -b.y;
     }
-    b0[t * stride + k] = v;
+    C z;
+    z.x = a.x - b.y;
+    z.y = a.y + b.x;
+    b0[p * sx + k] = z;
   }
   __syncthreads();
-  const C* res = lds_fft_any<C, L_CT>(b0, b1, M, L, stride, pl, tw, +1);
-  for (int e = threadIdx.x; e < M * L; e += blockDim.x) {
-    const int t = e / L, k = e - t * L;
-    const int line = line0 + t;
-    if (line < nlines) out[size_t(line) * L + k] = res[t * stride + k].x;
+  const C* res = lds_fft_any<C, L_CT>(b0, b1, P, L, sx, pl, twl, +1);
+  for (int e = threadIdx.x; e < P * L; e += blockDim.x) {
+    const int p = e / L, xk = e - p * L;
+    const size_t row = (size_t(z0 + 2 * p) * Ky + y) * L;
+    const C v = res[p * sx + xk];
+    out[row + xk] = v.x;
+    out[row + size_t(Ky) * L + xk] = v.y;
   }
 }
 
@@ -367,9 +511,9 @@ __global__ void k_influence(Geom g, double a2, R* __restrict__ G) {
   const size_t total = size_t(g.H) * g.K[1] * g.K[2];
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int kx = int(t % g.H);
-  const int ky = int((t / g.H) % g.K[1]);
-  const int kz = int(t / (size_t(g.H) * g.K[1]));
+  const int kz = int(t % g.K[2]);
+  const int ky = int((t / g.K[2]) % g.K[1]);
+  const int kx = int(t / (size_t(g.K[2]) * g.K[1]));
   const int k3[3] = {kx, ky, kz};
   int m[3];
   double zf = 1.0;
@@ -418,7 +562,9 @@ __global__ void __launch_bounds__(BLOCK) k_gather_force(Geom g, const double* __
       iy[a] = wrapk(b[1], a, g.K[1]);
       iz[a] = wrapk(b[2], a, g.K[2]);
     }
+#pragma unroll
     for (int c = 0; c < 4; ++c)
+#pragma unroll
       for (int bb = 0; bb < 4; ++bb) {
         const size_t row = size_t(g.K[0]) * (size_t(iy[bb]) + size_t(g.K[1]) * size_t(iz[c]));
 #pragma unroll
@@ -469,7 +615,9 @@ __device__ inline double gather_phi(const Geom& g, const double* xi, const Rg* _
     iz[a] = wrapk(b[2], a, g.K[2]);
   }
   double e = 0.0;
+#pragma unroll
   for (int a = 0; a < 4; ++a)
+#pragma unroll
     for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
       for (int c = 0; c < 4; ++c) {

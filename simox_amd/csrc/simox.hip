@@ -50,6 +50,14 @@ struct TimedLaunch { int which; hipEvent_t a, b; };
 
 }  // namespace
 
+// Launch shapes of the FFT passes for one precision: mx real x-lines per
+// R2C / C2R workgroup, my kz-columns per y workgroup, mz z-lines per fused z
+// workgroup, and their dynamic LDS bytes.
+struct FftCfg {
+  int mx = 0, my = 0, mz = 0, zblocks = 0;
+  size_t shx = 0, shy = 0, shz = 0;
+};
+
 struct simox_ctx {
   int n = 0;
   int mode = SIMOX_MODE_FAITHFUL;
@@ -88,7 +96,7 @@ struct simox_ctx {
   double2* tw64[3] = {nullptr, nullptr, nullptr};
   float2* tw32[3] = {nullptr, nullptr, nullptr};
   FftPlan pl[3]{};
-  int M64[3] = {0, 0, 0}, M32[3] = {0, 0, 0}, stride[3] = {0, 0, 0};
+  FftCfg fc64{}, fc32{};
   // CTIP
   int cap = NNB_MAX;
   int* nb_cnt = nullptr;
@@ -434,19 +442,19 @@ template <class C, int L>
 struct FftKernels {
   static constexpr auto r2c = &k_fft_x_r2c<C, L>;
   static constexpr auto c2r = &k_fft_x_c2r<C, L>;
-  static constexpr auto str = &k_fft_strided<C, false, L>;
-  static constexpr auto fus = &k_fft_strided<C, true, L>;
+  static constexpr auto y = &k_fft_y<C, L>;
+  static constexpr auto fus = &k_fft_z_fused<C, L>;
 };
 template <class C>
 struct FftSet {
   decltype(&k_fft_x_r2c<C, 0>) r2c;
   decltype(&k_fft_x_c2r<C, 0>) c2r;
-  decltype(&k_fft_strided<C, false, 0>) str;
-  decltype(&k_fft_strided<C, true, 0>) fus;
+  decltype(&k_fft_y<C, 0>) y;
+  decltype(&k_fft_z_fused<C, 0>) fus;
 };
 template <class C, int L>
 FftSet<C> fft_set_of() {
-  return {FftKernels<C, L>::r2c, FftKernels<C, L>::c2r, FftKernels<C, L>::str, FftKernels<C, L>::fus};
+  return {FftKernels<C, L>::r2c, FftKernels<C, L>::c2r, FftKernels<C, L>::y, FftKernels<C, L>::fus};
 }
 template <class C>
 FftSet<C> fft_set(int L) {
@@ -467,59 +475,70 @@ FftSet<C> fft_set(int L) {
 }
 
 template <class C>
-int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const int* M,
+int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const FftCfg& fc,
               const decltype(C{}.x)* G, decltype(C{}.x)* greal) {
   const Geom& g = ctx->g;
   const int H = g.H, Ky = g.K[1], Kz = g.K[2];
-  const size_t csz = sizeof(C);
   const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
   hipEvent_t ev{};
-  {
-    const int nl = Ky * Kz;
-    const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
-    tstart(ctx, 1, &ev);
-    hipLaunchKernelGGL(fx.r2c, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[0], tw[0], greal, spec, M[0], ctx->stride[0]);
-    tstop(ctx, 1, ev);
-  }
-  {
-    const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
-    tstart(ctx, 2, &ev);
-    hipLaunchKernelGGL(fy.str, dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1], -1,
-                       (const decltype(C{}.x)*)nullptr, (double*)nullptr);
-    tstop(ctx, 2, ev);
-  }
-  {
-    const size_t sh = 2 * size_t(M[2]) * ctx->stride[2] * csz;
-    tstart(ctx, 3, &ev);
-    hipLaunchKernelGGL(fz.fus, dim3((H + M[2] - 1) / M[2], Ky), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[2], tw[2], spec, H * Ky, H, M[2], ctx->stride[2], 0, G, ctx->part);
-    tstop(ctx, 3, ev);
-  }
-  {
-    const size_t sh = 2 * size_t(M[1]) * ctx->stride[1] * csz;
-    tstart(ctx, 2, &ev);
-    hipLaunchKernelGGL(fy.str, dim3((H + M[1] - 1) / M[1], Kz), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[1], tw[1], spec, H, H * Ky, M[1], ctx->stride[1], +1,
-                       (const decltype(C{}.x)*)nullptr, (double*)nullptr);
-    tstop(ctx, 2, ev);
-  }
-  {
-    const int nl = Ky * Kz;
-    const size_t sh = 2 * size_t(M[0]) * ctx->stride[0] * csz;
-    tstart(ctx, 4, &ev);
-    hipLaunchKernelGGL(fx.c2r, dim3((nl + M[0] - 1) / M[0]), dim3(BLOCK), sh, ctx->st, g,
-                       ctx->pl[0], tw[0], spec, greal, M[0], ctx->stride[0]);
-    tstop(ctx, 4, ev);
-  }
+  tstart(ctx, 1, &ev);
+  hipLaunchKernelGGL(fx.r2c, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, ctx->pl[0],
+                     tw[0], greal, spec, fc.mx);
+  tstop(ctx, 1, ev);
+  tstart(ctx, 2, &ev);
+  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, ctx->pl[1], tw[1],
+                     spec, fc.my, Ky + 1, -1);
+  tstop(ctx, 2, ev);
+  tstart(ctx, 3, &ev);
+  hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, ctx->pl[2], tw[2],
+                     spec, fc.mz, G, ctx->part);
+  tstop(ctx, 3, ev);
+  tstart(ctx, 2, &ev);
+  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, ctx->pl[1], tw[1],
+                     spec, fc.my, Ky + 1, +1);
+  tstop(ctx, 2, ev);
+  tstart(ctx, 4, &ev);
+  hipLaunchKernelGGL(fx.c2r, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, ctx->pl[0],
+                     tw[0], spec, greal, fc.mx);
+  tstop(ctx, 4, ev);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int zpass_blocks(const simox_ctx* ctx) {
-  const int m = std::min(ctx->M32[2], ctx->M64[2]);
-  return ((ctx->g.H + m - 1) / m) * ctx->g.K[1];
+// Tile sizes of the three FFT passes for one precision: the largest that
+// fit FFT_LDS_CAP bytes of LDS per workgroup (two workgroups per CU).
+constexpr size_t FFT_LDS_CAP = 80 * 1024;
+template <class C>
+int choose_fft(simox_ctx* ctx, const Geom& g, FftCfg* c) {
+  const size_t csz = sizeof(C);
+  const int Kx = g.K[0], Ky = g.K[1], Kz = g.K[2], H = g.H;
+  auto shx = [&](int m) { return (size_t(m / 2) * (Kx + 1) + xpass_b1(m, Kx) + Kx) * csz; };
+  auto shy = [&](int m) { return (2 * size_t(m) * (Ky + 1) + Ky) * csz; };
+  auto shz = [&](int m) { return (2 * size_t(m) * Kz + Kz) * csz; };
+  *c = FftCfg{};
+  for (int m = 32; m >= 2 && !c->mx; m /= 2)
+    if (Kz % m == 0 && shx(m) <= FFT_LDS_CAP) c->mx = m;
+  for (int m = 32; m >= 1 && !c->my; m /= 2)
+    if (Kz % m == 0 && shy(m) <= FFT_LDS_CAP) c->my = m;
+  for (int m = 8; m >= 1 && !c->mz; m /= 2)
+    if (shz(m) <= FFT_LDS_CAP) c->mz = m;
+  if (!c->mx || !c->my || !c->mz) return SIMOX_ERR_ARG;
+  c->shx = shx(c->mx); c->shy = shy(c->my); c->shz = shz(c->mz);
+  c->zblocks = (H * Ky + c->mz - 1) / c->mz;
+  const FftSet<C> fx = fft_set<C>(Kx), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fx.r2c),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shx)));
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fx.c2r),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shx)));
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fy.y),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shy)));
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fz.fus),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shz)));
+  return 0;
+}
+
+int zpass_blocks(const simox_ctx* ctx, bool f32) {
+  return f32 ? ctx->fc32.zblocks : ctx->fc64.zblocks;
 }
 
 // Brick binning of the atoms for the tiled spread (once per set of positions).
@@ -574,9 +593,9 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   tstop(ctx, 0, ev);
   HIPCHK(hipGetLastError());
   if (f32)
-    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->M32, filter ? ctx->F32 : ctx->G32,
+    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->fc32, filter ? ctx->F32 : ctx->G32,
                              ctx->greal32);
-  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->M64, filter ? ctx->F64 : ctx->G64,
+  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->fc64, filter ? ctx->F64 : ctx->G64,
                             ctx->greal64);
 }
 
@@ -584,7 +603,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
 int do_pme(simox_ctx* ctx, bool fetch_energy) {
   const int n = ctx->n, nb = grid_blocks(n);
   CHK(pme_eval(ctx, ctx->q, nullptr, 0.0, use32(ctx)));
-  const int zb = zpass_blocks(ctx);
+  const int zb = zpass_blocks(ctx, use32(ctx));
   CHK(reduce(ctx, zb, 1, 0, 4));  // energy sum before the part buffer is reused
   CHK(ensure_part(ctx, size_t(nb) * 3));
   const double Ng = double(ctx->g.K[0]) * double(ctx->g.K[1]) * double(ctx->g.K[2]);
@@ -1171,14 +1190,11 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     CHK(dalloc(ctx, &ctx->tw64[d], L)); CHK(dalloc(ctx, &ctx->tw32[d], L));
     HIPCHK(hipMemcpy(ctx->tw64[d], t64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tw32[d], t32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
-    ctx->stride[d] = d == 0 ? L : L + 1;
-    const size_t lds_cap = 60 * 1024;
-    ctx->M64[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(double2)))));
-    ctx->M32[d] = int(std::min<size_t>(16, std::max<size_t>(1, lds_cap / (2 * ctx->stride[d] * sizeof(float2)))));
-    if (2 * size_t(ctx->stride[d]) * sizeof(double2) > lds_cap) {
-      ctx->err = "PME grid dimension too large for the LDS-resident FFT";
-      return SIMOX_ERR_ARG;
-    }
+  }
+  if (choose_fft<double2>(ctx, g, &ctx->fc64) ||
+      (mode == SIMOX_MODE_MIXED && choose_fft<float2>(ctx, g, &ctx->fc32))) {
+    ctx->err = "PME grid dimension too large for the LDS-resident FFT";
+    return SIMOX_ERR_ARG;
   }
   {
     const double a2 = g.alpha * g.alpha;
@@ -1196,7 +1212,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   HIPCHK(hipHostMalloc((void**)&ctx->herr, sizeof(int), hipHostMallocDefault));
   {
     // partial-sum buffer sized once for every launch that uses it
-    const size_t zb = size_t(zpass_blocks(ctx));
+    const size_t zb = size_t(std::max(ctx->fc64.zblocks, ctx->fc32.zblocks));
     CHK(ensure_part(ctx, std::max(size_t(grid_blocks(n)) * 4, zb) + 16));
   }
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->order);

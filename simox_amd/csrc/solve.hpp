@@ -12,6 +12,33 @@ namespace sx {
 //         - b_i - sum_j sym_ij q_j
 // part[3*block] = sum g, part[3*block+1] = -sum g.p (Brent's f, ctip.f90:236),
 // part[3*block+2] = sum g^2.
+// sum_k val[k n + i] vec[col[k n + i]] over row i of the ELL matrix
+// (column-major, k < c), accumulated in Tv in ascending k.  Eight columns and
+// values are loaded before their vector entries, so each thread keeps several
+// gathers in flight.
+template <class Tv>
+__device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
+                             const Tv* __restrict__ val, const double* __restrict__ vec) {
+  constexpr int U = 8;
+  Tv s = Tv(0);
+  int k = 0;
+  for (; k + U <= c; k += U) {
+    int cc[U];
+    Tv vv[U], pp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cc[u] = col[size_t(k + u) * n + i];
+      vv[u] = val[size_t(k + u) * n + i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pp[u] = Tv(vec[cc[u]]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += vv[u] * pp[u];
+  }
+  for (; k < c; ++k) s += val[size_t(k) * n + i] * Tv(vec[col[size_t(k) * n + i]]);
+  return s;
+}
+
 template <class Rg, class Tv>
 __global__ void __launch_bounds__(BLOCK) k_dudq(Geom g, const Tables* __restrict__ tb,
                                                 const double* __restrict__ x,
@@ -40,8 +67,7 @@ __global__ void __launch_bounds__(BLOCK) k_dudq(Geom g, const Tables* __restrict
     gi -= b[i];
     const int c = cnt[i];
     const size_t n = size_t(g.n);
-    Tv s = Tv(0);
-    for (int k = 0; k < c; ++k) s += val[size_t(k) * n + i] * Tv(ql[col[size_t(k) * n + i]]);
+    const Tv s = ell_dot(c, n, i, col, val, ql);
     gi -= double(s);
     gout[i] = gi;
     if (phi_out) phi_out[i] = phi;
@@ -257,8 +283,7 @@ __global__ void __launch_bounds__(BLOCK) k_hmatvec(Geom g, const Tables* __restr
     const double pi = p[i];
     const int c = cnt[i];
     const size_t n = size_t(g.n);
-    Tv s = Tv(0);
-    for (int k = 0; k < c; ++k) s += val[size_t(k) * n + i] * Tv(p[col[size_t(k) * n + i]]);
+    const Tv s = ell_dot(c, n, i, col, val, p);
     const double yi = diag_h(sc, q[i]) * pi + phi + double(s);
     y[i] = yi;
     acc[0] = pi * yi;
@@ -406,8 +431,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
     const double pi = p[i];
     const int c = cnt[i];
     const size_t n = size_t(g.n);
-    Tv s = Tv(0);
-    for (int k = 0; k < c; ++k) s += val[size_t(k) * n + i] * Tv(p[col[size_t(k) * n + i]]);
+    const Tv s = ell_dot(c, n, i, col, val, p);
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double yi = sc.Jp * pi + phi + double(s);
     y[i] = yi;
@@ -557,9 +581,9 @@ __global__ void k_filter_table(Geom g, const double* __restrict__ G64, double si
   const size_t total = size_t(g.H) * g.K[1] * g.K[2];
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int kx = int(t % g.H);
-  const int ky = int((t / g.H) % g.K[1]);
-  const int kz = int(t / (size_t(g.H) * g.K[1]));
+  const int kz = int(t % g.K[2]);
+  const int ky = int((t / g.K[2]) % g.K[1]);
+  const int kx = int(t / (size_t(g.K[2]) * g.K[1]));
   const int k3[3] = {kx, ky, kz};
   double m2 = 0.0;
 #pragma unroll
