@@ -97,6 +97,13 @@ const char* simox_last_error(const simox_ctx* ctx);
    touching atoms or cells: the drop-in FORCE / CTIP_SOLVER pass their eam,
    oxygn, param arguments through here (force.f90:3, ctip.f90:3). */
 int simox_set_params(simox_ctx* ctx, const simox_params* params);
+/* FFT launch shapes.  simox_create picks them by timing the candidate x
+   (R2C + C2R) and y tile sizes on the context's own grid (environment
+   SIMOX_AUTOTUNE=0 keeps the default rule); simox_autotune re-runs that
+   search.  simox_fft_shapes reports {mx, my, mz} of the FP64 chain, then of
+   the FP32 chain (zeros outside MIXED mode).  Shapes change no result. */
+int simox_autotune(simox_ctx* ctx);
+int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]);
 int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
 int simox_set_mode(simox_ctx* ctx, int mode);
 /* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged

@@ -22,6 +22,7 @@ EXPORTS = [
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
+    "simox_autotune", "simox_fft_shapes",
 ]
 
 # simox_kernel_timing categories
@@ -79,6 +80,8 @@ def load_library(path=LIB_PATH):
     lib.simox_set_mode.argtypes = [vp, i]
     lib.simox_set_charge_tol.argtypes = [vp, d, i]
     lib.simox_set_params.argtypes = [vp, vp]
+    lib.simox_autotune.argtypes = [vp]
+    lib.simox_fft_shapes.argtypes = [vp, P(i)]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
     lib.simox_cell_sort.argtypes = [vp]
@@ -168,6 +171,16 @@ class Simox:
         a = C.c_double()
         self._check(self.lib.simox_get_geometry(self.h, nc, lc, C.byref(a)))
         return tuple(nc), tuple(lc), a.value
+
+    def autotune(self):
+        """Re-time the FFT tile shapes on this context (done at creation)."""
+        self._check(self.lib.simox_autotune(self.h))
+
+    def fft_shapes(self):
+        """{"fp64": (mx, my, mz), "fp32": (mx, my, mz)} FFT tile shapes."""
+        a = (C.c_int * 6)()
+        self._check(self.lib.simox_fft_shapes(self.h, a))
+        return {"fp64": tuple(a[0:3]), "fp32": tuple(a[3:6])}
 
     def set_params(self, params):
         """Replace the potential parameters (a Params from potential_setup())."""

@@ -642,9 +642,10 @@ namespace sx {
 // of the tile exactly once.  No global atomics and no zeroing pass.
 // ---------------------------------------------------------------------------
 constexpr int BRK = 4;
-constexpr int TX = 16, TY = 16, TZ = 8;
+constexpr int TX = 32, TY = 16, TZ = 8;
 constexpr int TILE = TX * TY * TZ;
-constexpr int MAXB = 6 * 6 * 4;  // candidate bricks per tile (TX/4+2)(TY/4+2)(TZ/4+2)
+constexpr int MAXB = (TX / 4 + 2) * (TY / 4 + 2) * (TZ / 4 + 2);  // candidate bricks per tile
+static_assert(MAXB <= BLOCK, "one thread per candidate brick");
 
 __global__ void k_brick_assign(Geom g, const double* __restrict__ x, int* __restrict__ brick,
                                int* __restrict__ count) {
@@ -699,6 +700,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   __shared__ unsigned long long acc[TILE];
   __shared__ int bofs[MAXB + 1];
   __shared__ int bbeg[MAXB];
+  __shared__ int wsum[BLOCK / WAVE];
   const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
   const int t = blockIdx.x;
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
@@ -719,6 +721,9 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   }
   const int ncand = cb[0] * cb[1] * cb[2];
   for (int k = threadIdx.x; k < TILE; k += blockDim.x) acc[k] = 0ull;
+  // atom counts of the candidate bricks and their prefix sum (wave scans,
+  // then the totals of the preceding waves)
+  int cntk = 0;
   if (threadIdx.x < ncand) {
     const int k = threadIdx.x;
     const int ix = k % cb[0], iy = (k / cb[0]) % cb[1], iz = k / (cb[0] * cb[1]);
@@ -726,13 +731,21 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
     int by = (fb[1] + iy) % nb[1]; if (by < 0) by += nb[1];
     int bz = (fb[2] + iz) % nb[2]; if (bz < 0) bz += nb[2];
     const int bid = bx + nb[0] * (by + nb[1] * bz);
-    bbeg[k] = bstart[bid];
-    bofs[k + 1] = bstart[bid + 1] - bstart[bid];
+    const int b0 = bstart[bid];
+    bbeg[k] = b0;
+    cntk = bstart[bid + 1] - b0;
   }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int t = __shfl_up(cntk, o, WAVE);
+    if (lane >= o) cntk += t;
+  }
+  if (lane == WAVE - 1) wsum[wid] = cntk;
   if (threadIdx.x == 0) bofs[0] = 0;
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 1; k <= ncand; ++k) bofs[k] += bofs[k - 1];
+  for (int w = 0; w < wid; ++w) cntk += wsum[w];
+  if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
   __syncthreads();
   const int total = bofs[ncand];
   for (int f = threadIdx.x; f < total; f += blockDim.x) {

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -106,6 +107,7 @@ struct simox_ctx {
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
   double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
+  float* pv32 = nullptr;  // FP32 copy of the CG direction (MIXED)
   double *qh = nullptr, *qh2 = nullptr;  // charges before the last solve (predictor history)
   bool hist_valid = false;
   int last_inner = 0, last_outer = 0;
@@ -505,35 +507,120 @@ int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const FftCfg& fc,
   return 0;
 }
 
-// Tile sizes of the three FFT passes for one precision: the largest that
-// fit FFT_LDS_CAP bytes of LDS per workgroup (two workgroups per CU).
-constexpr size_t FFT_LDS_CAP = 80 * 1024;
+// Launch shapes of the FFT passes.  LDS bytes per workgroup of each pass for
+// tile size m (see the kernels' LDS layouts).
+constexpr size_t FFT_LDS_CAP = 80 * 1024;    // default rule: two workgroups per CU
+constexpr size_t FFT_LDS_MAX = 159 * 1024;   // gfx950: 160 KB per workgroup, less static LDS
+template <class C>
+size_t fft_shx(const Geom& g, int m) {
+  return (size_t(m / 2) * (g.K[0] + 1) + xpass_b1(m, g.K[0]) + g.K[0]) * sizeof(C);
+}
+template <class C>
+size_t fft_shy(const Geom& g, int m) { return (2 * size_t(m) * (g.K[1] + 1) + g.K[1]) * sizeof(C); }
+template <class C>
+size_t fft_shz(const Geom& g, int m) { return (2 * size_t(m) * g.K[2] + g.K[2]) * sizeof(C); }
+
+template <class C>
+void fft_finish(const Geom& g, FftCfg* c) {
+  c->shx = fft_shx<C>(g, c->mx);
+  c->shy = fft_shy<C>(g, c->my);
+  c->shz = fft_shz<C>(g, c->mz);
+  c->zblocks = (g.H * g.K[1] + c->mz - 1) / c->mz;
+}
+
+// Default shapes: the largest x / y tiles within FFT_LDS_CAP (they divide
+// Kz), z tiles of at most 4 lines and 40 KB (SIMOX_FFT_MZ overrides).  The kernels may use up to the
+// device's LDS per workgroup, which the tuner below can choose.
 template <class C>
 int choose_fft(simox_ctx* ctx, const Geom& g, FftCfg* c) {
-  const size_t csz = sizeof(C);
-  const int Kx = g.K[0], Ky = g.K[1], Kz = g.K[2], H = g.H;
-  auto shx = [&](int m) { return (size_t(m / 2) * (Kx + 1) + xpass_b1(m, Kx) + Kx) * csz; };
-  auto shy = [&](int m) { return (2 * size_t(m) * (Ky + 1) + Ky) * csz; };
-  auto shz = [&](int m) { return (2 * size_t(m) * Kz + Kz) * csz; };
+  const int Kz = g.K[2];
   *c = FftCfg{};
   for (int m = 32; m >= 2 && !c->mx; m /= 2)
-    if (Kz % m == 0 && shx(m) <= FFT_LDS_CAP) c->mx = m;
+    if (Kz % m == 0 && fft_shx<C>(g, m) <= FFT_LDS_CAP) c->mx = m;
   for (int m = 32; m >= 1 && !c->my; m /= 2)
-    if (Kz % m == 0 && shy(m) <= FFT_LDS_CAP) c->my = m;
-  for (int m = 8; m >= 1 && !c->mz; m /= 2)
-    if (shz(m) <= FFT_LDS_CAP) c->mz = m;
+    if (Kz % m == 0 && fft_shy<C>(g, m) <= FFT_LDS_CAP) c->my = m;
+  // z: small tiles (many workgroups per CU hide the latency of the fused pass)
+  for (int m = 4; m >= 1 && !c->mz; m /= 2)
+    if (fft_shz<C>(g, m) <= 40 * 1024) c->mz = m;
+  if (const char* e = std::getenv("SIMOX_FFT_MZ")) {
+    const int m = std::atoi(e);
+    if (m >= 1 && m <= 64 && fft_shz<C>(g, m) <= FFT_LDS_MAX) c->mz = m;
+  }
   if (!c->mx || !c->my || !c->mz) return SIMOX_ERR_ARG;
-  c->shx = shx(c->mx); c->shy = shy(c->my); c->shz = shz(c->mz);
-  c->zblocks = (H * Ky + c->mz - 1) / c->mz;
-  const FftSet<C> fx = fft_set<C>(Kx), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fx.r2c),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shx)));
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fx.c2r),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shx)));
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fy.y),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shy)));
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(fz.fus),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(c->shz)));
+  fft_finish<C>(g, c);
+  const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(g.K[1]), fz = fft_set<C>(Kz);
+  const void* fns[4] = {reinterpret_cast<const void*>(fx.r2c), reinterpret_cast<const void*>(fx.c2r),
+                        reinterpret_cast<const void*>(fy.y), reinterpret_cast<const void*>(fz.fus)};
+  for (const void* f : fns)
+    HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(FFT_LDS_MAX)));
+  return 0;
+}
+
+// Average time (ms) of `launch` over three runs after one warm-up run.
+template <class F>
+int time_launch(simox_ctx* ctx, F launch, float* ms) {
+  hipEvent_t a{}, b{};
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  launch();
+  HIPCHK(hipEventRecord(a, ctx->st));
+  for (int r = 0; r < 3; ++r) launch();
+  HIPCHK(hipEventRecord(b, ctx->st));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  *ms /= 3.0f;
+  HIPCHK(hipEventDestroy(a));
+  HIPCHK(hipEventDestroy(b));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Autotuner of the x (R2C + C2R) and y tile sizes on this context's grid and
+// buffers: every power-of-two tile that divides Kz and fits the device's LDS
+// is timed, the fastest kept.  The tile sizes do not change any result (each
+// line is transformed by the same arithmetic); the z tile, which groups the
+// energy partial sums, keeps its default so energies stay reproducible.
+template <class C>
+int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* greal) {
+  const Geom& g = ctx->g;
+  const int Ky = g.K[1], Kz = g.K[2], H = g.H;
+  const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky);
+  float best = 1e30f;
+  int pick = c->mx;
+  for (int m = 2; m <= 64; m *= 2) {
+    if (Kz % m || fft_shx<C>(g, m) > FFT_LDS_MAX) continue;
+    const size_t sh = fft_shx<C>(g, m);
+    float ms = 0.f;
+    CHK(time_launch(ctx, [&] {
+      hipLaunchKernelGGL(fx.r2c, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pl[0], tw[0],
+                         greal, spec, m);
+      hipLaunchKernelGGL(fx.c2r, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pl[0], tw[0],
+                         spec, greal, m);
+    }, &ms));
+    if (ms < best) { best = ms; pick = m; }
+  }
+  c->mx = pick;
+  best = 1e30f;
+  pick = c->my;
+  for (int m = 1; m <= 64; m *= 2) {
+    if (Kz % m || fft_shy<C>(g, m) > FFT_LDS_MAX) continue;
+    const size_t sh = fft_shy<C>(g, m);
+    float ms = 0.f;
+    CHK(time_launch(ctx, [&] {
+      hipLaunchKernelGGL(fy.y, dim3(Kz / m, H), dim3(BLOCK), sh, ctx->st, g, ctx->pl[1], tw[1], spec,
+                         m, Ky + 1, -1);
+    }, &ms));
+    if (ms < best) { best = ms; pick = m; }
+  }
+  c->my = pick;
+  fft_finish<C>(g, c);
+  return 0;
+}
+
+int autotune(simox_ctx* ctx) {
+  CHK(tune_fft<double2>(ctx, &ctx->fc64, ctx->spec64, ctx->tw64, ctx->greal64));
+  if (ctx->mode == SIMOX_MODE_MIXED)
+    CHK(tune_fft<float2>(ctx, &ctx->fc32, ctx->spec32, ctx->tw32, ctx->greal32));
   return 0;
 }
 
@@ -869,7 +956,8 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
     ++restarts;
     const double target = f32 ? std::max(0.3 * ctx->tol, 1e-4 * rms_true) : 0.3 * ctx->tol;
     double rz = s6[3];
-    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv);
+    float* p32 = f32 ? ctx->pv32 : nullptr;
+    hipLaunchKernelGGL(k_set_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv, p32);
     int it = 0;
     double rms = rms_true;
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, rz, ctx->dscal + 46);  // ls[6]
@@ -881,11 +969,11 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
       tstart(ctx, 6, &ev);
       if (f32)
         hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->gv, ctx->greal32,
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->greal32,
                            ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
       else
         hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->gv, ctx->greal64,
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->greal64,
                            ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
       tstop(ctx, 6, ev);
       CHK(reduce(ctx, nb, 3, 0, 22));
@@ -906,7 +994,8 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
       CHK(reduce(ctx, nb, 3, 0, 16));
       CHK(apply_prec(ctx, ctx->rv, f32));
       hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 19, ls);
-      hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv);
+      hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
+                         p32);
       HIPCHK(hipGetLastError());
       double st2[2];
       ++it;
@@ -1162,7 +1251,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->bvec, n)); CHK(dalloc(ctx, &ctx->nb_cnt, n));
   CHK(dalloc(ctx, &ctx->nb_col, size_t(ctx->cap) * n));
   CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
-  if (mode == SIMOX_MODE_MIXED) CHK(dalloc(ctx, &ctx->nb_val32, size_t(ctx->cap) * n));
+  if (mode == SIMOX_MODE_MIXED) {
+    CHK(dalloc(ctx, &ctx->nb_val32, size_t(ctx->cap) * n));
+    CHK(dalloc(ctx, &ctx->pv32, n));
+  }
   // PME
   const size_t ng = size_t(fft[0]) * fft[1] * fft[2];
   const size_t nh = size_t(g.H) * fft[1] * fft[2];
@@ -1191,10 +1283,11 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     HIPCHK(hipMemcpy(ctx->tw64[d], t64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tw32[d], t32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
   }
-  if (choose_fft<double2>(ctx, g, &ctx->fc64) ||
-      (mode == SIMOX_MODE_MIXED && choose_fft<float2>(ctx, g, &ctx->fc32))) {
-    ctx->err = "PME grid dimension too large for the LDS-resident FFT";
-    return SIMOX_ERR_ARG;
+  {
+    int rc = choose_fft<double2>(ctx, g, &ctx->fc64);
+    if (!rc && mode == SIMOX_MODE_MIXED) rc = choose_fft<float2>(ctx, g, &ctx->fc32);
+    if (rc == SIMOX_ERR_ARG) ctx->err = "PME grid dimension too large for the LDS-resident FFT";
+    if (rc) return rc;
   }
   {
     const double a2 = g.alpha * g.alpha;
@@ -1217,7 +1310,29 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   }
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->order);
   HIPCHK(hipGetLastError());
+  {
+    const char* e = std::getenv("SIMOX_AUTOTUNE");
+    if (!e || std::strcmp(e, "0") != 0) CHK(autotune(ctx));
+  }
   HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_autotune(simox_ctx* ctx) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  CHK(autotune(ctx));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]) {
+  if (!ctx || !shapes) return SIMOX_ERR_ARG;
+  const FftCfg* c[2] = {&ctx->fc64, &ctx->fc32};
+  for (int k = 0; k < 2; ++k) {
+    shapes[3 * k] = c[k]->mx;
+    shapes[3 * k + 1] = c[k]->my;
+    shapes[3 * k + 2] = c[k]->mz;
+  }
   return SIMOX_OK;
 }
 
@@ -1232,7 +1347,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
                   ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
-                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qh, ctx->qh2,
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qh, ctx->qh2, ctx->pv32,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);

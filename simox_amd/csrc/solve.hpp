@@ -16,9 +16,9 @@ namespace sx {
 // (column-major, k < c), accumulated in Tv in ascending k.  Eight columns and
 // values are loaded before their vector entries, so each thread keeps several
 // gathers in flight.
-template <class Tv>
+template <class Tv, class Tp>
 __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
-                             const Tv* __restrict__ val, const double* __restrict__ vec) {
+                             const Tv* __restrict__ val, const Tp* __restrict__ vec) {
   constexpr int U = 8;
   Tv s = Tv(0);
   int k = 0;
@@ -409,13 +409,16 @@ __device__ inline double pen_d1(const SpecCoef& sc, double q) {
   return 2.0 * WPEN * (lo + hi);
 }
 
-// y = H0 p (no penalty term); part[3k..] = p.y, p.gr, sum p^2 pen''(q)
+// y = H0 p (no penalty term); part[3k..] = p.y, p.gr, sum p^2 pen''(q).
+// The sparse part reads pt, the direction rounded to the operator's type
+// (p itself in FP64, the FP32 copy in MIXED: half the gathered bytes).
 template <class Rg, class Tv>
 __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __restrict__ tb,
                                                     const double* __restrict__ x,
                                                     const int* __restrict__ id,
                                                     const double* __restrict__ q,
                                                     const double* __restrict__ p,
+                                                    const Tv* __restrict__ pt,
                                                     const double* __restrict__ gr,
                                                     const Rg* __restrict__ grid, double c_pme,
                                                     const int* __restrict__ cnt,
@@ -431,7 +434,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
     const double pi = p[i];
     const int c = cnt[i];
     const size_t n = size_t(g.n);
-    const Tv s = ell_dot(c, n, i, col, val, p);
+    const Tv s = ell_dot(c, n, i, col, val, pt);  // pt = p rounded to Tv
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double yi = sc.Jp * pi + phi + double(s);
     y[i] = yi;
@@ -823,9 +826,22 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __r
 }
 
 __global__ void k_pcg_dir_dev(int n, const double* __restrict__ ls, const double* __restrict__ z,
-                              double* __restrict__ p) {
+                              double* __restrict__ p, float* __restrict__ p32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + ls[7] * p[i];
+  if (i < n) {
+    const double v = z[i] + ls[7] * p[i];
+    p[i] = v;
+    if (p32) p32[i] = float(v);
+  }
+}
+// p = z (restart of the direction), with its FP32 copy when p32 != null
+__global__ void k_set_dir(int n, const double* __restrict__ z, double* __restrict__ p,
+                          float* __restrict__ p32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i] = z[i];
+    if (p32) p32[i] = float(z[i]);
+  }
 }
 
 }  // namespace sx
