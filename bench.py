@@ -23,9 +23,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = {0: "k_spread", 1: "k_fft_x_r2c", 2: "k_fft_strided<fwd/inv y>",
-           3: "k_fft_strided<z, fused influence>", 4: "k_fft_x_c2r", 5: "k_gather_force",
-           6: "k_dudq", 7: "k_force", 8: "k_density", 9: "k_ctip_matrix"}
+KERNELS = {0: "k_spread_tile", 1: "k_fft_x_r2c", 2: "k_fft_y (fwd + inv)",
+           3: "k_fft_z_fused", 4: "k_fft_x_c2r", 5: "k_gather_force",
+           6: "k_h0matvec", 7: "k_force", 8: "k_nlist_density", 9: "k_ctip_matrix"}
+# timing category 6 is k_dudq in the faithful solve, k_h0matvec in the converged ones
+
+
+def kernel_name(w, mode):
+    return "k_dudq" if (w == 6 and mode == "faithful") else KERNELS[w]
+
+
+def measured_traffic(kernel, cfg, mode):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE x fetch_scale + WRITE_SIZE x
+    write_scale, KiB, scaled by a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            tab = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    e = tab.get("%s|%s|%s" % (kernel, cfg, mode))
+    if not e:
+        return None
+    return (e["fetch_kib"] * e["fetch_scale"] + e["write_kib"] * e["write_scale"]) * 1024.0
 WORKLOADS = {
     "c1": "C1: Al slab 6x6x8 cells (1,152) + 64 O, PME 32x32x96, Rcut 8 A, 300 K",
     "c2": "C2: Al slab 24x24x32 cells (73,728) + 4,096 O, PME 96x96x192, Rcut 10 A, 600 K",
@@ -115,14 +136,14 @@ def run_mode(s, mode_name, args, timed=True):
 
 def accuracy_vs_fp64(s, g_mixed, args):
     """Force / charge / energy error of the mixed path against the FP64 path on
-    the same state (positions and velocities of the mixed run; FP64 charges
-    solved to the same tolerance, warm-started from the mixed charges)."""
+    the same state: positions and velocities of the mixed run, FP64 charges
+    solved independently (from q = 0) to the same tolerance."""
     from simox_amd import MODE_FP64, Simox
     x, v, f_m, q_m = g_mixed.get_state()
     e_m = g_mixed.force()
     g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
-    g.set_charge_tol(args.tol, 5000)
-    g.set_state(s.ids, x, v, None, q_m)
+    g.set_charge_tol(args.tol, 20000)
+    g.set_state(s.ids, x, v, None, np.zeros(s.n))
     g.ctip_solve()
     e_d = g.force()
     g.pme()
@@ -137,7 +158,8 @@ def accuracy_vs_fp64(s, g_mixed, args):
     e_E = max(abs(e_m[k] - e_d[k]) / abs(e_d[k]) for k in ("Epair", "Egroup", "Edir", "Ectip"))
     g.close()
     return {"force_rel_err_max": eF, "force_rel_err_frob": eF_frob, "charge_abs_err_max": e_q,
-            "energy_rel_err_max": float(e_E), "reference": "FP64 GPU path, same state"}
+            "energy_rel_err_max": float(e_E),
+            "reference": "FP64 GPU path, same positions, charges solved from q = 0"}
 
 
 def main():
@@ -151,6 +173,9 @@ def main():
                     help="skip the FP64 comparison run (mixed mode only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (a known-size reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="converged charge solve: RMS projected gradient (eV/e)")
     args = ap.parse_args()
@@ -159,11 +184,22 @@ def main():
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)  # 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        del a, b
+        torch.cuda.empty_cache()
     from simox_amd.systems import config
     print("bench: building %s inputs" % args.config, file=sys.stderr, flush=True)
     s = config(args.config)
     g, st, elapsed, times = run_mode(s, args.mode, args)
     cg_iters, cg_restarts = g.charge_stats()
+    fft_tiles = g.fft_shapes()["fp32" if args.mode == "mixed" else "fp64"]
     cnt, _ = g.neighbors(stride=0)
     nb_entries = int(cnt.sum())
     # dominant kernel: largest total time in the timed region
@@ -172,6 +208,7 @@ def main():
     per_launch_s = ms / 1e3 / max(launches, 1)
     bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries)
     achieved = bytes_launch / per_launch_s / 1e9
+    traffic = measured_traffic(kernel_name(dom, args.mode), args.config, args.mode)
     value = s.n * args.steps / elapsed
     out = {
         "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
@@ -183,13 +220,14 @@ def main():
                    "ns_per_day": args.steps / elapsed * 1.0 * 86400 * 1e-6,
                    "neighbor_entries": nb_entries, "E_total_last": st["Etotal"],
                    "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
-                   "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts},
-        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved,
+                   "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
+                   "fft_tiles": fft_tiles},
+        "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_launch,
+                     "traffic": traffic, "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": per_launch_s * 1e3, "launches": launches},
-        "kernels_ms": {KERNELS[w]: round(times[w][0], 3) for w in KERNELS},
-        "kernels_launches": {KERNELS[w]: times[w][1] for w in KERNELS},
+        "kernels_ms": {kernel_name(w, args.mode): round(times[w][0], 3) for w in KERNELS},
+        "kernels_launches": {kernel_name(w, args.mode): times[w][1] for w in KERNELS},
     }
     if args.mode == "mixed" and not args.no_compare:
         print("bench: accuracy vs fp64", file=sys.stderr, flush=True)

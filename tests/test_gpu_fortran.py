@@ -7,9 +7,9 @@ and writes temporal_status.dat in PRINT_STAT's format (util.f90:12-18).  Its
 full-precision status file must equal the Python binding driving the same
 library with the same step flags, bit for bit.
 
-The drop-in build (the reference's main.f90 / util / vverlet / cellsort with
-simox_dropin.f90 in place of force.f90 and ctip.f90) exists only where the
-reference sources were present at build time; when it was built, its
+The drop-in build (the reference's main.f90 / util / vverlet / eam / datafmt
+with simox_dropin.f90 in place of force.f90, ctip.f90 and cellsort.f90) exists
+only where the reference sources were present at build time; when it was built, its
 temporal_status.dat must agree with simox_md's to the file's 5 significant
 digits.
 """

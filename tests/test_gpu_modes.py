@@ -113,3 +113,22 @@ def test_mixed_nve_energy_tracks_fp64():
         hist[mode] = [g.step(1, s.dt, 1, 1)["Etotal"] for _ in range(20)]
     a, b = np.array(hist[MODE_MIXED]), np.array(hist[MODE_FP64])
     assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("mode", [MODE_FP64, MODE_MIXED])
+def test_steps_are_deterministic(mode):
+    """Two fresh contexts, same input, 3 steps with sort + charge solve every
+    step: bit-identical positions, velocities, forces, charges and energies."""
+    s = tiny(seed=4)
+    out = []
+    for _ in range(2):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_charge_tol(1e-10, 2000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        g.force()
+        g.pme()
+        st = g.step(3, s.dt, 1, 1)
+        out.append((g.get_state(), st))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][1] == out[1][1]
