@@ -108,8 +108,12 @@ struct simox_ctx {
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
   double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
   float* pv32 = nullptr;  // FP32 copy of the CG direction (MIXED)
-  double *qh = nullptr, *qh2 = nullptr;  // charges before the last solve (predictor history)
-  bool hist_valid = false;
+  // predictor history: qhist[0] = charges before the last solve, [1], [2] the
+  // ones before; nhist of them valid (0 after any external change of q)
+  double* qhist[3] = {nullptr, nullptr, nullptr};
+  double* qtmp = nullptr;
+  int nhist = 0;
+  int pred_order = 1;
   int last_inner = 0, last_outer = 0;
   bool matrix_valid = false;
   bool nlist_valid = false;
@@ -366,10 +370,10 @@ int do_cell_sort(simox_ctx* ctx) {
   hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order, ctx->cell,
                      ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->x2, ctx->v2,
                      ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2);
-  if (ctx->hist_valid) {
-    hipLaunchKernelGGL(k_gather_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order, ctx->qh,
-                       ctx->qh2);
-    std::swap(ctx->qh, ctx->qh2);
+  for (int h = 0; h < ctx->nhist; ++h) {
+    hipLaunchKernelGGL(k_gather_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->order,
+                       ctx->qhist[h], ctx->qtmp);
+    std::swap(ctx->qhist[h], ctx->qtmp);
   }
   HIPCHK(hipGetLastError());
   std::swap(ctx->x, ctx->x2); std::swap(ctx->v, ctx->v2); std::swap(ctx->f, ctx->f2);
@@ -897,14 +901,17 @@ int build_prec(simox_ctx* ctx) {
   return 0;
 }
 
-// z = P M gv (preconditioned, projected on sum = 0); dscal[19..20] = {z.g, z.g_old}
-int apply_prec(simox_ctx* ctx, const double* g_old, bool f32) {
+// z = P M gv (preconditioned, projected on sum = 0); dscal[19..20] = {z.g, z.g_old};
+// ls != null: also the PR+ beta into ls[7] (and rz into ls[6])
+int apply_prec(simox_ctx* ctx, const double* g_old, bool f32, double* ls = nullptr) {
   const int n = ctx->n, nb = grid_blocks(n);
   if (!ctx->prec_on) {
     hipLaunchKernelGGL(k_ncg_z, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->gv, g_old, ctx->mv,
                        ctx->dscal + 16, ctx->dscal + 17, ctx->zv, ctx->part);
+    hipLaunchKernelGGL(k_reduce_op<3>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
+                       ctx->dscal + 19, ls);
     HIPCHK(hipGetLastError());
-    return reduce(ctx, nb, 2, 0, 19);
+    return 0;
   }
   hipLaunchKernelGGL(k_prec_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ctx->qc, ctx->gv, ctx->cv, ctx->sv);
@@ -920,7 +927,10 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32) {
   hipLaunchKernelGGL(k_prec_project, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 26, ctx->gv,
                      g_old, ctx->zv, ctx->part);
   HIPCHK(hipGetLastError());
-  return reduce(ctx, nb, 2, 0, 19);
+  CHK(reduce(ctx, nb, 2, 0, 19));
+  if (ls) hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 19, ls);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 // Converged charge solve (modes FP64 / MIXED): preconditioned nonlinear CG
@@ -932,10 +942,16 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32) {
 int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
   const int n = ctx->n, nb = grid_blocks(n);
   const bool f32 = use32(ctx);
-  if (ctx->hist_valid)
-    hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qh, ctx->qc);
-  else
-    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qc);
+  {
+    // polynomial extrapolation in time through q(t), q(t-1), ... of order
+    // min(pred_order, nhist); coefficients sum to 1, so sum(q) is kept
+    static const double coef[4][4] = {
+        {1.0, 0.0, 0.0, 0.0}, {2.0, -1.0, 0.0, 0.0}, {3.0, -3.0, 1.0, 0.0}, {4.0, -6.0, 4.0, -1.0}};
+    const int ord = std::min(ctx->pred_order, ctx->nhist);
+    const double* c = coef[ord];
+    hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ord, c[0], c[1], c[2],
+                       c[3], ctx->q, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qc);
+  }
   HIPCHK(hipGetLastError());
   int total_it = 0, nev = 0, restarts = 0;
   bool ok = false;
@@ -976,15 +992,15 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
                            ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->greal64,
                            ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
       tstop(ctx, 6, ev);
-      CHK(reduce(ctx, nb, 3, 0, 22));
-      hipLaunchKernelGGL(k_ls_init, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 22, ls);
-      // exact line search, Newton with bracketing, on the device
+      hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
+                         ctx->dscal + 22, ls);
+      // exact line search, Newton with bracketing, on the device (rounds after
+      // convergence return at once)
       for (int k = 0; k < 8; ++k) {
         hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                            ctx->qc, ctx->pv, ls, ctx->part);
-        hipLaunchKernelGGL(k_reduce_if, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2, ls,
-                           ctx->dscal + 24);
-        hipLaunchKernelGGL(k_ls_update, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 24, ls);
+        hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
+                           ctx->dscal + 24, ls);
       }
       hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                          ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
@@ -992,8 +1008,7 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
       hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                          ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
       CHK(reduce(ctx, nb, 3, 0, 16));
-      CHK(apply_prec(ctx, ctx->rv, f32));
-      hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, ctx->st, ctx->dscal + 19, ls);
+      CHK(apply_prec(ctx, ctx->rv, f32, ls));
       hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
                          p32);
       HIPCHK(hipGetLastError());
@@ -1019,11 +1034,13 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
     ctx->err = "charge solve did not reach the requested tolerance";
     return SIMOX_ERR_NOT_CONVERGED;
   }
-  // history for the next predictor: qh = charges before this solve
-  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qh);
+  // history for the next predictor: shift, qhist[0] = charges before this solve
+  std::swap(ctx->qhist[2], ctx->qhist[1]);
+  std::swap(ctx->qhist[1], ctx->qhist[0]);
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qhist[0]);
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->qc, ctx->q);
   HIPCHK(hipGetLastError());
-  ctx->hist_valid = true;
+  ctx->nhist = std::min(ctx->nhist + 1, 3);
   return 0;
 }
 
@@ -1247,7 +1264,12 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
   CHK(dalloc(ctx, &ctx->rv, n)); CHK(dalloc(ctx, &ctx->zv, n)); CHK(dalloc(ctx, &ctx->apv, n));
   CHK(dalloc(ctx, &ctx->mv, n));
-  CHK(dalloc(ctx, &ctx->qh, n)); CHK(dalloc(ctx, &ctx->qh2, n));
+  for (int h = 0; h < 3; ++h) CHK(dalloc(ctx, &ctx->qhist[h], n));
+  CHK(dalloc(ctx, &ctx->qtmp, n));
+  if (const char* e = std::getenv("SIMOX_PREDICTOR_ORDER")) {
+    const int o = std::atoi(e);
+    if (o >= 0 && o <= 3) ctx->pred_order = o;
+  }
   CHK(dalloc(ctx, &ctx->bvec, n)); CHK(dalloc(ctx, &ctx->nb_cnt, n));
   CHK(dalloc(ctx, &ctx->nb_col, size_t(ctx->cap) * n));
   CHK(dalloc(ctx, &ctx->nb_val64, size_t(ctx->cap) * n));
@@ -1347,7 +1369,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
                   ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
-                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qh, ctx->qh2, ctx->pv32,
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1435,7 +1457,7 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
     if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
     if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
     ctx->have_state = true;
-    ctx->hist_valid = false;
+    ctx->nhist = 0;
     ctx->prec_ready = false;
     CHK(do_cell_sort(ctx));
     HIPCHK(hipStreamSynchronize(ctx->st));
@@ -1445,7 +1467,7 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
   if (x) CHK(upload_caller(ctx, x, 3, ctx->x));
   if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
   if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
-  if (q) { CHK(upload_caller(ctx, q, 1, ctx->q)); ctx->hist_valid = false; }
+  if (q) { CHK(upload_caller(ctx, q, 1, ctx->q)); ctx->nhist = 0; }
   if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->bricks_valid = false; }
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;

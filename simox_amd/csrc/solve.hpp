@@ -16,6 +16,11 @@ namespace sx {
 // (column-major, k < c), accumulated in Tv in ascending k.  Eight columns and
 // values are loaded before their vector entries, so each thread keeps several
 // gathers in flight.
+// scalar steps of the device-side CG (defined with the line-search kernels)
+__device__ inline void ls_init(const double* red, double* ls);
+__device__ inline void ls_update(const double* red, double* ls);
+__device__ inline void beta_update(const double* red, double* ls);
+
 template <class Tv, class Tp>
 __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
                              const Tv* __restrict__ val, const Tp* __restrict__ vec) {
@@ -450,12 +455,20 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
   }
 }
 
-// charge predictor q_guess = 2 q - q_prev (linear extrapolation in time;
+// charge predictor: out = c0 q + c1 h0 + c2 h1 + c3 h2 with the first ord+1
+// terms (polynomial extrapolation in time; the coefficients sum to 1, which
 // preserves the total charge)
-__global__ void k_extrapolate(int n, const double* __restrict__ q, const double* __restrict__ qh,
+__global__ void k_extrapolate(int n, int ord, double c0, double c1, double c2, double c3,
+                              const double* __restrict__ q, const double* __restrict__ h0,
+                              const double* __restrict__ h1, const double* __restrict__ h2,
                               double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = 2.0 * q[i] - qh[i];
+  if (i >= n) return;
+  double v = c0 * q[i];
+  if (ord >= 1) v += c1 * h0[i];
+  if (ord >= 2) v += c2 * h1[i];
+  if (ord >= 3) v += c3 * h2[i];
+  out[i] = v;
 }
 
 // Line-search terms at step a: part[2k] = sum p (pen'(q + a p) - pen'(q)),
@@ -717,7 +730,7 @@ namespace sx {
 // ls layout: [0] g.p  [1] p.H0p  [2] a  [3] lo  [4] hi  [5] done  [6] rz
 //            [7] beta  [8] descent-failure flag
 // red: reduction outputs; red[0] = p.H0p, red[1] = p.g, red[2] = sum p^2 pen''(q)
-__global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ ls) {
+__device__ inline void ls_init(const double* red, double* ls) {
   const double pHp = red[0], gp = red[1], s2 = red[2];
   ls[0] = gp;
   ls[1] = pHp;
@@ -729,8 +742,10 @@ __global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ l
   ls[2] = bad ? 0.0 : gp / (pHp + s2);
 }
 
+__global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ ls) { ls_init(red, ls); }
+
 // Newton step on phi'(a) = -gp + a pHp + D1(a), bracketed; red = {D1(a), S2(a)}
-__global__ void k_ls_update(const double* __restrict__ red, double* __restrict__ ls) {
+__device__ inline void ls_update(const double* red, double* ls) {
   if (ls[5] != 0.0) return;
   const double gp = ls[0], pHp = ls[1], a = ls[2];
   const double fprime = -gp + a * pHp + red[0];
@@ -745,12 +760,15 @@ __global__ void k_ls_update(const double* __restrict__ red, double* __restrict__
   ls[4] = hi;
 }
 
+__global__ void k_ls_update(const double* __restrict__ red, double* __restrict__ ls) { ls_update(red, ls); }
+
 // beta = max(0, (z.g - z.g_old)/rz); rz = z.g; red = {z.g, z.g_old}
-__global__ void k_beta(const double* __restrict__ red, double* __restrict__ ls) {
+__device__ inline void beta_update(const double* red, double* ls) {
   const double b = (red[0] - red[1]) / ls[6];
   ls[7] = b > 0.0 ? b : 0.0;
   ls[6] = red[0];
 }
+__global__ void k_beta(const double* __restrict__ red, double* __restrict__ ls) { beta_update(red, ls); }
 
 // line-search terms with the step read from device memory (skipped once done)
 __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __restrict__ tb,
@@ -774,6 +792,37 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = acc[0];
     part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+// Final reduction (as k_reduce_final, one block, fixed order) followed by a
+// scalar step of the device-side CG on the sums, in the same launch:
+// OP 1 = line-search start (ls_init), 2 = Newton update of the line search
+// (ls_update; nothing once ls[5] marks it done), 3 = PR+ beta (beta_update,
+// when ls != null), 0 = none.
+template <int OP>
+__global__ void k_reduce_op(const double* __restrict__ part, int nparts, int nv,
+                            double* __restrict__ out, double* __restrict__ ls) {
+  __shared__ double lds[1024 / WAVE];
+  if (OP == 2 && ls[5] != 0.0) return;
+  for (int v = 0; v < nv; ++v) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(nv) * k + v];
+    s = wave_sum(s);
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    if (lane == 0) lds[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = lds[0];
+      for (int q = 1; q < int(blockDim.x / WAVE); ++q) t += lds[q];
+      out[v] = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && ls) {
+    if (OP == 1) ls_init(out, ls);
+    if (OP == 2) ls_update(out, ls);
+    if (OP == 3) beta_update(out, ls);
   }
 }
 
