@@ -97,6 +97,11 @@ const char* simox_last_error(const simox_ctx* ctx);
    touching atoms or cells: the drop-in FORCE / CTIP_SOLVER pass their eam,
    oxygn, param arguments through here (force.f90:3, ctip.f90:3). */
 int simox_set_params(simox_ctx* ctx, const simox_params* params);
+/* Split cutoff (build-only; the reference has one Rcut for EAM, Ewald and
+   CTIP, main.f90:172-174, quirk Q9): EAM density, pair and embedding terms
+   only for r < r_eam <= Rcut; real-space Ewald, CTIP and the cells keep Rcut.
+   Default r_eam = Rcut. */
+int simox_set_eam_cutoff(simox_ctx* ctx, double r_eam);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x
    (R2C + C2R) and y tile sizes on the context's own grid (environment
    SIMOX_AUTOTUNE=0 keeps the default rule); simox_autotune re-runs that

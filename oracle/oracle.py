@@ -80,6 +80,11 @@ def set_spread_rounding(on):
     lib().orc_set_spread_rounding(C.c_int(1 if on else 0))
 
 
+def set_eam_cutoff(r_eam):
+    """Split cutoff: EAM terms only for r < r_eam (None or <= 0: the common Rcut)."""
+    lib().orc_set_eam_cutoff(C.c_double(r_eam if r_eam else 0.0))
+
+
 def params():
     p = Params()
     lib().orc_potential_setup(C.byref(p))

@@ -313,6 +313,8 @@ int orc_embed(const orc_params* p, int id, double rho, double* F, double* dF) {
 /* ------------------------------------------------------------------------ */
 /* FORCE, force.f90:3-361                                                    */
 /* ------------------------------------------------------------------------ */
+static double g_ream2 = -1.0;  /* EAM cutoff squared; < 0: the common Rcut */
+
 typedef struct {
   const orc_sys* s; const orc_params* p; const int* id; const double* x; const double* q;
   double* f; double* rho; double* drhodx; int* nlist; int* list; double* drho_list;
@@ -337,6 +339,9 @@ static void force_pair(force_ws* w, int li, int lj) {
       (id_i < 4 && id_j < 4 && id_i != id_j)) { w->err = ORC_ERR_UNKNOWN_PAIR; return; }
   double fj[2], dfjdr[2], phi, df;
   orc_eam_pair(p, id_i, id_j, r, fj, dfjdr, &phi, &df);
+  if (g_ream2 > 0.0 && !(r2 < g_ream2)) {  /* split cutoff: no EAM terms past r_eam */
+    fj[0] = fj[1] = dfjdr[0] = dfjdr[1] = phi = df = 0.0;
+  }
   w->nlist[li]++; w->nlist[lj]++;
   if (w->nlist[li] > ORC_NNB_MAX || w->nlist[lj] > ORC_NNB_MAX) { w->err = ORC_ERR_NB_OVERFLOW; return; }
   int ni = w->nlist[li] - 1, nj = w->nlist[lj] - 1;
@@ -586,6 +591,7 @@ static int wrapk(int b, int nn, int K) { return (b + nn - 1 + K) % K; } /* MOD(I
    0 = exact FP64 weights (the build's FP64 / MIXED modes). */
 static int g_round_f32 = 1;
 void orc_set_spread_rounding(int on) { g_round_f32 = on ? 1 : 0; }
+void orc_set_eam_cutoff(double r_eam) { g_ream2 = r_eam > 0.0 ? r_eam * r_eam : -1.0; }
 
 static void spread(const orc_sys* s, const double* x, const double* q, double complex* Q) {
   const int* K = s->fft;

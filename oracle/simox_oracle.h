@@ -102,6 +102,10 @@ void orc_fft3d(double* data, int nx, int ny, int nz, int sign);
 /* 1 (default): FP32-rounded spreading weights as the reference (quirk Q1);
    0: exact FP64 weights. */
 void orc_set_spread_rounding(int on);
+/* Build-only split cutoff (no reference counterpart; the reference uses one
+   Rcut for EAM and Ewald, quirk Q9): EAM density, pair and embedding terms only
+   for r < r_eam; Ewald real space and CTIP keep Rcut.  r_eam <= 0: off. */
+void orc_set_eam_cutoff(double r_eam);
 /* PME (force.f90:364-464): f += reciprocal forces (mean removed), returns Erec. */
 double orc_pme(const orc_sys* s, const int* id, const double* x, const double* q,
                const double* psi, double* f);

@@ -22,7 +22,7 @@ EXPORTS = [
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
-    "simox_autotune", "simox_fft_shapes",
+    "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
 ]
 
 # simox_kernel_timing categories
@@ -81,6 +81,7 @@ def load_library(path=LIB_PATH):
     lib.simox_set_charge_tol.argtypes = [vp, d, i]
     lib.simox_set_params.argtypes = [vp, vp]
     lib.simox_autotune.argtypes = [vp]
+    lib.simox_set_eam_cutoff.argtypes = [vp, d]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
@@ -171,6 +172,10 @@ class Simox:
         a = C.c_double()
         self._check(self.lib.simox_get_geometry(self.h, nc, lc, C.byref(a)))
         return tuple(nc), tuple(lc), a.value
+
+    def set_eam_cutoff(self, r_eam):
+        """Split cutoff: EAM terms only for r < r_eam (<= Rcut)."""
+        self._check(self.lib.simox_set_eam_cutoff(self.h, float(r_eam)))
 
     def autotune(self):
         """Re-time the FFT tile shapes on this context (done at creation)."""

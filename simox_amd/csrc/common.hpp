@@ -72,6 +72,7 @@ struct Geom {
   double qbox[3];      // box/4: |dx| below it cannot wrap
   double lcell[3];
   double rcut2;
+  double ream2;        // EAM cutoff squared (= rcut2 unless split)
   double alpha;        // Ewald a = 3.8/Rcut
   int K[3];            // PME grid
   int H;               // Kx/2 + 1

@@ -217,9 +217,11 @@ __global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* _
         ++k;
         const int sj = id[j];
         if (bad || sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid) { bad = true; return; }
-        T f, dfr;
-        dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
-        rho += f;
+        if (r2 < g.ream2) {  // split cutoff: EAM only inside r_eam
+          T f, dfr;
+          dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
+          rho += f;
+        }
       }
     });
     if (bad) SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR);
@@ -277,8 +279,10 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
         dens_eval<T>(pc.di, r, fi_, dfi);
         dens_eval<T>(pc.dj, r, fj_, dfj);
         phi_eval<T>(pc.ph, r, phi, dfp);
-        // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356)
-        T s = dfp - dFi * dfj - dF[j] * dfi;
+        // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356),
+        // inside the EAM cutoff only (= Rcut unless split)
+        const bool eam = r2d < g.ream2;
+        T s = eam ? dfp - dFi * dfj - dF[j] * dfi : T(0);
         // real-space Ewald (force.f90:94-96)
         T ar = a * r;
         T tmp = erfc(ar);
@@ -289,7 +293,7 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
         ctip_force_terms<T>(pc, r, r2, qi, qj, dfc, e);
         s += dfc;
         ec += e;
-        ep += phi;
+        if (eam) ep += phi;
         fx += T(rxd[0]) * s;
         fy += T(rxd[1]) * s;
         fz += T(rxd[2]) * s;

@@ -1237,6 +1237,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   }
   g.ncall = g.nc[0] * g.nc[1] * g.nc[2];
   g.rcut2 = rcut * rcut;
+  g.ream2 = g.rcut2;
   g.alpha = 3.8 / rcut;
   g.H = fft[0] / 2 + 1;
   ctx->V = box[0] * box[1] * box[2];
@@ -1479,6 +1480,17 @@ int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) 
   if (v) CHK(download_caller(ctx, ctx->v, 3, v));
   if (f) CHK(download_caller(ctx, ctx->f, 3, f));
   if (q) CHK(download_caller(ctx, ctx->q, 1, q));
+  return SIMOX_OK;
+}
+
+int simox_set_eam_cutoff(simox_ctx* ctx, double r_eam) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  if (!(r_eam > 0.0) || r_eam * r_eam > ctx->g.rcut2) {
+    ctx->err = "EAM cutoff must be in (0, Rcut]";
+    return SIMOX_ERR_ARG;
+  }
+  ctx->g.ream2 = r_eam * r_eam;
+  ctx->nlist_valid = false;
   return SIMOX_OK;
 }
 

@@ -82,6 +82,12 @@ module simox_gpu
       integer(c_int), value :: link_stride
     end function simox_get_cells
 
+    integer(c_int) function simox_set_eam_cutoff(ctx, r_eam) bind(c, name='simox_set_eam_cutoff')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: r_eam
+    end function simox_set_eam_cutoff
+
     integer(c_int) function simox_set_params(ctx, params) bind(c, name='simox_set_params')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, params

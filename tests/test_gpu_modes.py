@@ -132,3 +132,32 @@ def test_steps_are_deterministic(mode):
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a, b)
     assert out[0][1] == out[1][1]
+
+
+def test_split_cutoff_matches_oracle(oracle_mod):
+    """Build-only split cutoff (EAM 6 A inside Rcut 8 A): FAITHFUL forces and
+    energies against the oracle with the same split, 1e-10 relative."""
+    from simox_amd import MODE_FAITHFUL
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_math import _split_case
+    s = _split_case()
+    rng = np.random.default_rng(3)
+    q = rng.normal(size=s.n) * 0.2
+    q -= q.mean()
+    g = Simox(s.n, s.box, s.fft, 8.0, mode=MODE_FAITHFUL)
+    g.set_eam_cutoff(6.0)
+    g.set_state(s.ids, s.x, s.v, None, q)
+    e = g.force()
+    f = g.get_state()[2]
+    o = oracle_mod.Oracle(s.n, s.box, s.fft, 8.0)
+    o.cell_sort(s.x)
+    oracle_mod.set_eam_cutoff(6.0)
+    try:
+        ref = o.force(s.ids, s.x, q)
+    finally:
+        oracle_mod.set_eam_cutoff(None)
+    assert np.abs(f - ref["f"]).max() <= 1e-10 * np.abs(ref["f"]).max()
+    for k in ("Epair", "Egroup", "Edir", "Ectip"):
+        assert e[k] == pytest.approx(ref[k], rel=1e-10, abs=1e-12)
+    with pytest.raises(Exception):
+        g.set_eam_cutoff(9.0)

@@ -210,3 +210,53 @@ def test_ctip_solution_satisfies_kkt(oracle_mod):
     assert (U - Uk) < 0.05 * (U0 - Uk)
     assert np.abs(q_kkt - q).max() < 0.1
     assert iters >= 1 and nev >= 3
+
+
+def _split_case():
+    from simox_amd.systems import al_o_slab
+    # 24.3 A box: Ncell >= 3 for both cutoffs below, so no duplicated pairs
+    return al_o_slab(6, 6, 3, 12, 5, 300.0, (24, 24, 36), 8.0, vacuum=14.0, o_layer=(2.0, 5.0))
+
+
+def test_split_cutoff_eam_terms_equal_short_cutoff_run(oracle_mod):
+    """Build-only split cutoff (quirk Q9): EAM with r_eam = 6 A inside an 8 A
+    Rcut gives the EAM energies, densities and (at q = 0) forces of a plain
+    run with Rcut = 6 A, where the two pair sets coincide by construction."""
+    s = _split_case()
+    q0 = np.zeros(s.n)
+    o8 = oracle_mod.Oracle(s.n, s.box, s.fft, 8.0)
+    o8.cell_sort(s.x)
+    oracle_mod.set_eam_cutoff(6.0)
+    try:
+        a = o8.force(s.ids, s.x, q0)
+    finally:
+        oracle_mod.set_eam_cutoff(None)
+    o6 = oracle_mod.Oracle(s.n, s.box, s.fft, 6.0)
+    o6.cell_sort(s.x)
+    b = o6.force(s.ids, s.x, q0)
+    full = o8.force(s.ids, s.x, q0)
+    assert a["Epair"] == pytest.approx(b["Epair"], rel=1e-12)
+    assert a["Egroup"] == pytest.approx(b["Egroup"], rel=1e-12)
+    np.testing.assert_allclose(a["rho"], b["rho"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(a["f"], b["f"], rtol=0, atol=1e-10 * np.abs(b["f"]).max())
+    # and the split really changes the answer relative to one 8 A cutoff
+    assert abs(full["Epair"] - a["Epair"]) > 1e-3
+
+
+def test_split_cutoff_keeps_coulomb_terms(oracle_mod):
+    """With charges, the split run's Ewald and CTIP energies are those of the
+    single 8 A run (only EAM terms are cut at r_eam)."""
+    s = _split_case()
+    rng = np.random.default_rng(3)
+    q = rng.normal(size=s.n) * 0.2
+    q -= q.mean()
+    o8 = oracle_mod.Oracle(s.n, s.box, s.fft, 8.0)
+    o8.cell_sort(s.x)
+    full = o8.force(s.ids, s.x, q)
+    oracle_mod.set_eam_cutoff(6.0)
+    try:
+        a = o8.force(s.ids, s.x, q)
+    finally:
+        oracle_mod.set_eam_cutoff(None)
+    assert a["Edir"] == full["Edir"]
+    assert a["Ectip"] == full["Ectip"]
