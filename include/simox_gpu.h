@@ -102,6 +102,25 @@ int simox_set_params(simox_ctx* ctx, const simox_params* params);
    only for r < r_eam <= Rcut; real-space Ewald, CTIP and the cells keep Rcut.
    Default r_eam = Rcut. */
 int simox_set_eam_cutoff(simox_ctx* ctx, double r_eam);
+/* Thermostat of simox_vv1 / simox_vv2 / simox_step (main.f90:44-58):
+   kind 0 = NVE (VV_NVE1/2), 1 = Berendsen (VV_NVE1 + VV_NVT2_BEREND,
+   vverlet.f90:160-208, coupling 0.025), 2 = Langevin (VV_NVT1_STOCH +
+   VV_NVT2_STOCH, vverlet.f90:237-296, friction alpha; alpha and the noise
+   amplitude in single precision as there).  t_kelvin = T_given (K).  The
+   Langevin noise is N(0,1) from a counter-based generator keyed by (seed,
+   step, atom, component); the reference draws from the unseeded RANDOM_NUMBER,
+   so the noise sequence is not the reference's, its distribution is. */
+int simox_set_thermostat(simox_ctx* ctx, int kind, double t_kelvin, double alpha,
+                         unsigned long long seed);
+/* Pair histogram of FORCE (pair%rdf, force.f90:55,76,159,180; Nrdf = 200
+   bins of Rcut/200).  While on, every simox_force / simox_step adds one
+   count per direction of each listed pair to bin INT(r/dr).  simox_get_rdf
+   returns hist[(bin-1)*25 + (sj-1)*5 + (si-1)], bin = 1..201 (the layout of
+   rdf(5,5,201)); hist[i,j] + hist[j,i] (i != j) and hist[i,i] equal the
+   reference's rdf(i,j) + rdf(j,i) and rdf(i,i), the sums PRINT_RDF writes
+   (util.f90:270-277).  reset != 0 zeroes it after the copy. */
+int simox_set_rdf(simox_ctx* ctx, int on);
+int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x
    (R2C + C2R) and y tile sizes on the context's own grid (environment
    SIMOX_AUTOTUNE=0 keeps the default rule); simox_autotune re-runs that

@@ -253,8 +253,10 @@ class Oracle:
                     Edir=out["Edir"], Ectip=out["Ectip"], Egroup=out["Egroup"])
 
 
-def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None):
-    """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90."""
+def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None,
+                  t_given=600.0, alpha=0.1):
+    """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90.
+    t_given (K) is the Berendsen target; alpha the Langevin friction (run at T = 0)."""
     import tempfile
     if not os.path.exists(REF_DRIVER):
         raise RuntimeError("reference build absent (oracle/_ref/ref_driver)")
@@ -265,7 +267,7 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
     nnb = NNB_MAX if dudq_inputs is not None else 0
     with open(fin, "wb") as fh:
         np.array([n, *fft, nnb], dtype=np.int32).tofile(fh)
-        np.array([*box, rcut, dt], dtype=np.float64).tofile(fh)
+        np.array([*box, rcut, dt, t_given, alpha], dtype=np.float64).tofile(fh)
         np.asarray(ids, dtype=np.int32).tofile(fh)
         for a in (x, v, f):
             np.asarray(a, dtype=np.float64).reshape(n, 3).tofile(fh)
@@ -306,6 +308,13 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
     out["mv2"] = take(np.float64, 1)[0]
     out["npt_new"] = take(np.int32, 1)[0]
     out["rdf"] = take(np.int32, 5 * 5 * 201).reshape(201, 5, 5)
+    out["vb"] = take(np.float64, 3 * n).reshape(n, 3)
+    out["mv2b"] = take(np.float64, 1)[0]
+    out["xs"] = take(np.float64, 3 * n).reshape(n, 3)
+    out["vs"] = take(np.float64, 3 * n).reshape(n, 3)
+    out["vs2"] = take(np.float64, 3 * n).reshape(n, 3)
+    out["fs2"] = take(np.float64, 3 * n).reshape(n, 3)
+    out["mv2s"] = take(np.float64, 1)[0]
     if dudq_inputs is not None:
         out["g"] = take(np.float64, n)
     return out

@@ -23,6 +23,7 @@ EXPORTS = [
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
+    "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf",
 ]
 
 # simox_kernel_timing categories
@@ -82,6 +83,9 @@ def load_library(path=LIB_PATH):
     lib.simox_set_params.argtypes = [vp, vp]
     lib.simox_autotune.argtypes = [vp]
     lib.simox_set_eam_cutoff.argtypes = [vp, d]
+    lib.simox_set_thermostat.argtypes = [vp, i, d, d, C.c_ulonglong]
+    lib.simox_set_rdf.argtypes = [vp, i]
+    lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
@@ -172,6 +176,22 @@ class Simox:
         a = C.c_double()
         self._check(self.lib.simox_get_geometry(self.h, nc, lc, C.byref(a)))
         return tuple(nc), tuple(lc), a.value
+
+    def set_thermostat(self, kind, t_kelvin=0.0, alpha=0.0, seed=1):
+        """0 NVE, 1 Berendsen (VV_NVT2_BEREND), 2 Langevin (VV_NVT1/2_STOCH)."""
+        self._check(self.lib.simox_set_thermostat(self.h, int(kind), float(t_kelvin),
+                                                  float(alpha), int(seed)))
+
+    def set_rdf(self, on=True):
+        """Accumulate FORCE's pair histogram (pair%rdf) on every force call."""
+        self._check(self.lib.simox_set_rdf(self.h, 1 if on else 0))
+
+    def rdf(self, reset=False):
+        """Histogram [bin-1][sj-1][si-1] (201 x 5 x 5), one count per pair direction."""
+        h = np.zeros(201 * 25, dtype=np.int64)
+        self._check(self.lib.simox_get_rdf(self.h, h.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                           1 if reset else 0))
+        return h.reshape(201, 5, 5)
 
     def set_eam_cutoff(self, r_eam):
         """Split cutoff: EAM terms only for r < r_eam (<= Rcut)."""

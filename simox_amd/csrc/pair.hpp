@@ -241,6 +241,15 @@ __global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* _
 // over the neighbour list.  Overwrites f (force.f90:28);
 // e_part[3*blockIdx.x + {0,1,2}] = half-pair sums of phi, q_i q_j erfc(ar)/r
 // and the CTIP energy (FORCE's Epair, Edir/kc, Ectip/kc).
+// rdf != null (the launch then brings RDF_BINS unsigned ints of dynamic LDS):
+// also the pair histogram of FORCE (force.f90:55,76,159,180), bin INT(r/dr),
+// one count per list entry, i.e. per direction of each pair, into
+// rdf[(bin-1)*25 + (sj-1)*5 + (si-1)] for bin >= 1 (the layout of the
+// reference's rdf(5,5,201) from its second bin on).  The reference adds 2 per
+// pair under one orientation (id_i, id_j); its output uses rdf(i,j)+rdf(j,i)
+// (util.f90:270-277), which these counts reproduce.
+constexpr int NRDF = 200;  // datafmt.f90:31
+constexpr int RDF_BINS = 25 * (NRDF + 1);
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restrict__ tb,
                                                  const double* __restrict__ x,
@@ -250,8 +259,15 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
                                                  const int* __restrict__ col,
                                                  const T* __restrict__ dF,
                                                  double* __restrict__ f,
-                                                 double* __restrict__ e_part) {
+                                                 double* __restrict__ e_part,
+                                                 unsigned long long* __restrict__ rdf,
+                                                 double dr) {
   __shared__ double lds[3 * BLOCK / WAVE];
+  extern __shared__ unsigned int hist[];
+  if (rdf) {
+    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x) hist[k] = 0u;
+    __syncthreads();
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double es[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
@@ -272,6 +288,11 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
         if (sj < 1 || sj > NSPEC) continue;
         const PairCoef& pc = tb->pc[pidx(si, sj)];
         if (!pc.valid) continue;
+        if (rdf) {
+          const int bin = int(sqrt(r2d) / dr);  // INT(r/pair%dr)
+          if (bin >= 1 && bin <= NRDF + 1)
+            atomicAdd(&hist[(bin - 1) * 25 + (sj - 1) * 5 + (si - 1)], 1u);
+        }
         const T r2 = T(r2d);
         const T r = T(sqrt(r2d));
         const T qj = T(q[j]);
@@ -311,6 +332,10 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
     e_part[3 * blockIdx.x] = es[0];
     e_part[3 * blockIdx.x + 1] = es[1];
     e_part[3 * blockIdx.x + 2] = es[2];
+  }
+  if (rdf) {  // block_sum ended with a barrier: the histogram is complete
+    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x)
+      if (hist[k]) atomicAdd(&rdf[k], (unsigned long long)hist[k]);
   }
 }
 

@@ -114,6 +114,14 @@ struct simox_ctx {
   double* qtmp = nullptr;
   int nhist = 0;
   int pred_order = 1;
+  // thermostat of the step (0 NVE, 1 Berendsen, 2 Langevin), target kB*T in eV
+  int thermo = 0;
+  double t_given = 0.0, alpha = 0.0;
+  unsigned long long seed = 1, nvt_step = 0;
+  // pair histogram of FORCE (RDF), accumulated while rdf_on
+  bool rdf_on = false;
+  unsigned long long* rdf = nullptr;
+  double rdf_dr = 0.0;  // Rcut / Nrdf (main.f90:254)
   int last_inner = 0, last_outer = 0;
   bool matrix_valid = false;
   bool nlist_valid = false;
@@ -415,14 +423,17 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
                      ctx->dscal + 0);
   hipEvent_t ev{};
   tstart(ctx, 7, &ev);
+  unsigned long long* rdf = ctx->rdf_on ? ctx->rdf : nullptr;
+  const size_t sh = rdf ? RDF_BINS * sizeof(unsigned) : 0;
+  const double dr = ctx->rdf_dr;
   if (use32(ctx))
-    hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+    hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
                        ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF32, ctx->f,
-                       ctx->part + nb);
+                       ctx->part + nb, rdf, dr);
   else
-    hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+    hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
                        ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF64, ctx->f,
-                       ctx->part + nb);
+                       ctx->part + nb, rdf, dr);
   tstop(ctx, 7, ev);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part + nb, nb, 3, 0,
@@ -1058,9 +1069,14 @@ int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
 }
 
 int do_vv1(simox_ctx* ctx, double dt) {
-  hipLaunchKernelGGL(k_vv1, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab,
-                     ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt, ctx->x, ctx->v,
-                     ctx->f);
+  if (ctx->thermo == 2)  // VV_NVT1_STOCH
+    hipLaunchKernelGGL(k_vv1_stoch, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n,
+                       ctx->dtab, ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt,
+                       ctx->alpha, ctx->x, ctx->v, ctx->f);
+  else  // VV_NVE1
+    hipLaunchKernelGGL(k_vv1, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab,
+                       ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt, ctx->x, ctx->v,
+                       ctx->f);
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
@@ -1070,9 +1086,23 @@ int do_vv1(simox_ctx* ctx, double dt) {
 
 int do_vv2(simox_ctx* ctx, double dt, bool fetch_mv2) {
   const int nb = grid_blocks(ctx->n);
-  CHK(ensure_part(ctx, size_t(nb)));
-  hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id, dt,
-                     ctx->v, ctx->f, ctx->part);
+  CHK(ensure_part(ctx, size_t(nb) * 10));
+  if (ctx->thermo == 1) {  // VV_NVT2_BEREND
+    hipLaunchKernelGGL(k_vv2_berend1, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id,
+                       dt, ctx->v, ctx->f, ctx->part);
+    CHK(reduce(ctx, nb, 10, 0, 50));
+    hipLaunchKernelGGL(k_vv2_berend2, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id,
+                       ctx->dscal + 50, ctx->t_given, ctx->v, ctx->part);
+  } else if (ctx->thermo == 2) {  // VV_NVT2_STOCH
+    const float a32 = float(ctx->alpha);
+    const float b32 = float(std::sqrt(2.0 * double(a32) * ctx->t_given / dt));
+    hipLaunchKernelGGL(k_vv2_stoch, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id,
+                       ctx->perm, dt, a32, b32, ctx->seed, ctx->nvt_step, ctx->v, ctx->f, ctx->part);
+    ++ctx->nvt_step;
+  } else {  // VV_NVE2
+    hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->dtab, ctx->id, dt,
+                       ctx->v, ctx->f, ctx->part);
+  }
   HIPCHK(hipGetLastError());
   CHK(reduce(ctx, nb, 1, 0, 12));
   if (fetch_mv2) CHK(fetch(ctx, 12, 1, &ctx->mv2));
@@ -1237,6 +1267,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   }
   g.ncall = g.nc[0] * g.nc[1] * g.nc[2];
   g.rcut2 = rcut * rcut;
+  ctx->rdf_dr = rcut / double(NRDF);
   g.ream2 = g.rcut2;
   g.alpha = 3.8 / rcut;
   g.H = fft[0] / 2 + 1;
@@ -1370,7 +1401,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
                   ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
-                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32,
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1480,6 +1511,40 @@ int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) 
   if (v) CHK(download_caller(ctx, ctx->v, 3, v));
   if (f) CHK(download_caller(ctx, ctx->f, 3, f));
   if (q) CHK(download_caller(ctx, ctx->q, 1, q));
+  return SIMOX_OK;
+}
+
+int simox_set_rdf(simox_ctx* ctx, int on) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  if (on && !ctx->rdf) {
+    CHK(dalloc(ctx, &ctx->rdf, RDF_BINS));
+    HIPCHK(hipMemsetAsync(ctx->rdf, 0, RDF_BINS * sizeof(unsigned long long), ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  }
+  ctx->rdf_on = on != 0;
+  return SIMOX_OK;
+}
+
+int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset) {
+  if (!ctx || !hist) return SIMOX_ERR_ARG;
+  if (!ctx->rdf) {
+    for (int k = 0; k < RDF_BINS; ++k) hist[k] = 0;
+    return SIMOX_OK;
+  }
+  HIPCHK(hipMemcpyAsync(hist, ctx->rdf, RDF_BINS * sizeof(long long), hipMemcpyDeviceToHost, ctx->st));
+  if (reset) HIPCHK(hipMemsetAsync(ctx->rdf, 0, RDF_BINS * sizeof(unsigned long long), ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return SIMOX_OK;
+}
+
+int simox_set_thermostat(simox_ctx* ctx, int kind, double t_kelvin, double alpha,
+                         unsigned long long seed) {
+  if (!ctx || kind < 0 || kind > 2 || !(t_kelvin >= 0.0) || !(alpha >= 0.0)) return SIMOX_ERR_ARG;
+  ctx->thermo = kind;
+  ctx->t_given = t_kelvin * 8.617343e-5;  // main.f90:160, kB of datafmt.f90:27
+  ctx->alpha = alpha;
+  ctx->seed = seed;
+  ctx->nvt_step = 0;
   return SIMOX_OK;
 }
 

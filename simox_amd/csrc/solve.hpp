@@ -215,6 +215,142 @@ __global__ void k_vv1(int n, const Tables* __restrict__ tb, const int* __restric
   }
 }
 
+// Langevin first half (VV_NVT1_STOCH, vverlet.f90:237-257):
+//   v = v (1 - alpha dt / 2m) + dt f / 2m,  x += dt v, wrapped (DNINT).
+__global__ void k_vv1_stoch(int n, const Tables* __restrict__ tb, const int* __restrict__ id,
+                            double b0, double b1, double b2, double dt, double alpha,
+                            double* __restrict__ x, double* __restrict__ v,
+                            const double* __restrict__ f) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xm = tb->sc[id[i] - 1].xm;
+  const double box[3] = {b0, b1, b2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double vn = v[3 * i + d] * (1.0 - 0.5 * alpha * dt / xm) + 0.5 * dt * f[3 * i + d] / xm;
+    v[3 * i + d] = vn;
+    const double xn = x[3 * i + d] + dt * vn;
+    x[3 * i + d] = xn - box[d] * round(xn / box[d]);
+  }
+}
+
+// Counter-based normal deviates for the Langevin force: splitmix64 of
+// (seed, step, atom, component) gives two uniforms in (0, 1], Box-Muller gives
+// N(0, 1).  Independent of launch shape and atom order (the caller's atom
+// index keys the stream).
+__device__ inline unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double gauss(unsigned long long seed, unsigned long long step, int atom, int k) {
+  const unsigned long long key =
+      splitmix64(seed ^ splitmix64(step * 0x100000001B3ull + (unsigned long long)(3 * atom + k)));
+  const unsigned long long k2 = splitmix64(key);
+  const double u1 = (double((key >> 11) + 1)) * (1.0 / 9007199254740992.0);  // (0, 1]
+  const double u2 = double(k2 >> 11) * (1.0 / 9007199254740992.0);         // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * PI * u2);
+}
+
+// Langevin second half (VV_NVT2_STOCH, vverlet.f90:260-296): per component
+// f += eta beta (the random force stays in f for the next first half),
+// v = (v + dt f / 2m) / (1 + alpha dt / 2m), part = m v^2 sums.  alpha and
+// beta are single precision there (no IMPLICIT NONE: implicit REAL), as here.
+__global__ void __launch_bounds__(BLOCK) k_vv2_stoch(int n, const Tables* __restrict__ tb,
+                                                     const int* __restrict__ id,
+                                                     const int* __restrict__ perm, double dt,
+                                                     float alpha, float beta,
+                                                     unsigned long long seed,
+                                                     unsigned long long step,
+                                                     double* __restrict__ v,
+                                                     double* __restrict__ f,
+                                                     double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const double xm = tb->sc[id[i] - 1].xm;
+    double m2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double eta = beta != 0.0f ? gauss(seed, step, perm[i], k) : 0.0;
+      const double fk = f[3 * i + k] + eta * double(beta);
+      f[3 * i + k] = fk;
+      const double vk = (v[3 * i + k] + 0.5 * dt * fk / xm) / (1.0 + 0.5 * double(alpha) * dt / xm);
+      v[3 * i + k] = vk;
+      m2 = m2 + xm * (vk * vk);
+    }
+    acc[0] = m2;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Berendsen second half (VV_NVT2_BEREND, vverlet.f90:160-208), pass 1: half
+// kick and per-species sums of m v^2 (summed per component) and counts;
+// part[11k + s] = sum over species s+1, part[11k + 5 + s] = count.
+__global__ void __launch_bounds__(BLOCK) k_vv2_berend1(int n, const Tables* __restrict__ tb,
+                                                       const int* __restrict__ id, double dt,
+                                                       double* __restrict__ v,
+                                                       const double* __restrict__ f,
+                                                       double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double lds[10 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const int s = id[i] - 1;
+    const double xm = tb->sc[s].xm;
+    double m2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double vk = v[3 * i + k] + 0.5 * dt * f[3 * i + k] / xm;
+      v[3 * i + k] = vk;
+      m2 = m2 + xm * (vk * vk);
+    }
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      acc[t] = (t == s) ? m2 : 0.0;
+      acc[5 + t] = (t == s) ? 1.0 : 0.0;
+    }
+  }
+  block_sum<10>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int t = 0; t < 10; ++t) part[10 * blockIdx.x + t] = acc[t];
+}
+
+// pass 2: lambda_s = sqrt(1 + C (T_given / T0_s - 1)), T0_s = sum_s / (3 N_s)
+// (C_BEREND = 0.025), v *= lambda, part = m v^2 sums.
+__global__ void __launch_bounds__(BLOCK) k_vv2_berend2(int n, const Tables* __restrict__ tb,
+                                                       const int* __restrict__ id,
+                                                       const double* __restrict__ sums,
+                                                       double t_given, double* __restrict__ v,
+                                                       double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const int s = id[i] - 1;
+    const double xm = tb->sc[s].xm;
+    const double t0 = sums[s] / (3.0 * sums[5 + s]);
+    const double lam = sqrt(1.0 + 0.025 * (t_given / t0 - 1.0));
+    double m2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double vk = v[3 * i + k] * lam;
+      v[3 * i + k] = vk;
+      m2 = m2 + xm * (vk * vk);
+    }
+    acc[0] = m2;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
 // K13: VV_NVE2 (vverlet.f90:145-154): half kick and m v^2 partial sums.
 __global__ void __launch_bounds__(BLOCK) k_vv2(int n, const Tables* __restrict__ tb,
                                                const int* __restrict__ id, double dt,

@@ -63,6 +63,8 @@ contains
     rc = simox_create(int(n, c_int), box, fft, real(sys%rcut, c_double), env_mode(), &
                       c_loc(cur), ctx)
     call simox_check(ctx, rc)
+    rc = simox_set_rdf(ctx, 1)   ! FORCE also fills pair%rdf
+    call simox_check(ctx, rc)
     if (allocated(xb)) deallocate(xb, fb, qb, idb)
     allocate(xb(3, n), fb(3, n), qb(n), idb(n))
     n_ctx = n
@@ -259,6 +261,7 @@ subroutine force(nset, eam, oxygn, sys, param, pair, q, cell)
   real(c_double) :: e(4)
   integer(c_int) :: rc
   integer :: i
+  integer(c_long_long), target :: hist(5, 5, 201)
   call simox_dropin_ensure(nset, sys, q)
   call simox_dropin_params(eam, oxygn, param, pc)
   call simox_dropin_params_update(pc)
@@ -272,6 +275,10 @@ subroutine force(nset, eam, oxygn, sys, param, pair, q, cell)
   end do
   sys%epair = e(1); sys%edir = e(2); sys%ectip = e(3); sys%egroup = e(4)
   pair%npt_new = nset%npt_all
+  ! pair histogram: counts per pair direction; PRINT_RDF reads rdf(i,j) + rdf(j,i)
+  rc = simox_get_rdf(ctx, hist, 1)
+  call simox_check(ctx, rc)
+  pair%rdf(:, :, :) = pair%rdf(:, :, :) + int(hist(:, :, :), gi)
 end subroutine force
 
 ! PME (force.f90:364): adds the reciprocal forces to q%ff, sets Erec.

@@ -82,6 +82,29 @@ module simox_gpu
       integer(c_int), value :: link_stride
     end function simox_get_cells
 
+    integer(c_int) function simox_set_thermostat(ctx, kind, t_kelvin, alpha, seed) &
+        bind(c, name='simox_set_thermostat')
+      import :: c_int, c_double, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      real(c_double), value :: t_kelvin, alpha
+      integer(c_long_long), value :: seed
+    end function simox_set_thermostat
+
+    integer(c_int) function simox_set_rdf(ctx, on) bind(c, name='simox_set_rdf')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function simox_set_rdf
+
+    ! hist(5, 5, 201): counts per pair direction (see include/simox_gpu.h)
+    integer(c_int) function simox_get_rdf(ctx, hist, reset) bind(c, name='simox_get_rdf')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: hist(5, 5, 201)
+      integer(c_int), value :: reset
+    end function simox_get_rdf
+
     integer(c_int) function simox_set_eam_cutoff(ctx, r_eam) bind(c, name='simox_set_eam_cutoff')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -10,12 +10,15 @@
 ! temporal_status_full.dat with the same columns at full precision, and
 ! snapshotNNN.xyz restart files in PRINT_SNAPSHOT's format (util.f90:23-81).
 !
+! Thermostat YES selects Berendsen (VV_NVT2_BEREND) or, with freq_new < 0,
+! Langevin (VV_NVT1/2_STOCH) as main.f90:44-58 does; the pair histogram is
+! written every time_rdf as PRINT_RDF does (util.f90:246-307).
+!
 ! A non-restart run initialises velocities as VINIT does (vverlet.f90:2-71):
 ! uniform random components, rigid motion removed, each species scaled to
 ! T_given, and the first forces evaluated at x - dt*v.  The generator is
 ! seeded from SIMOX_SEED (default 1) where the reference's RANDOM_SEED is
-! unseeded.  Not covered (the reference's other paths): thermostats,
-! OXIDATION mode, RDF output.
+! unseeded.  Not covered (the reference's other path): OXIDATION mode.
 !
 ! Usage: simox_md [mode]   (mode = faithful | fp64 | mixed; default faithful)
 program simox_md
@@ -27,8 +30,9 @@ program simox_md
   real(c_double) :: time_max, time_snap, time_rdf, time_stat, dt, box(3), t_given, alpha_l
   real(c_double) :: rcut, z_new, q_crit, stats(8)
   integer(c_int) :: fft(3), freq_new, freq_sort, n_o2, mode, rc, n
-  integer :: freq_max, freq_snap, freq_stat, nloop, i, ios
-  logical :: restart, do_ctip, do_sort, do_stat
+  integer :: freq_max, freq_snap, freq_stat, freq_rdf, nloop, i, ios
+  logical :: restart, do_ctip, do_sort, do_stat, thermostat
+  integer(c_long_long), target :: hist(5, 5, 201)
   real(c_double), allocatable, target :: x(:,:), v(:,:), f(:,:), q(:)
   integer(c_int), allocatable, target :: id(:)
   character(len=3), allocatable :: el(:)
@@ -53,6 +57,7 @@ program simox_md
   freq_max = nint(time_max / dt)
   freq_snap = nint(time_snap / dt)
   freq_stat = nint(time_stat / dt)
+  freq_rdf = nint(time_rdf / dt)
   dt = dt / tfm
   read(10, *) dummy
   read(10, *) box
@@ -60,7 +65,13 @@ program simox_md
   read(10, *) fft
   read(10, *) dummy
   read(10, *) tstat, t_given, alpha_l
-  if (to_upper(tstat) /= 'NO') stop "=== thermostats are not supported by simox_md ==="
+  if (to_upper(tstat) == 'YES') then
+    thermostat = .true.
+  else if (to_upper(tstat) == 'NO') then
+    thermostat = .false.
+  else
+    stop "=== Temperature control is unknown ==="
+  end if
   read(10, *) dummy
   read(10, *) rstart
   restart = to_upper(rstart) == 'YES'
@@ -107,6 +118,16 @@ program simox_md
   rc = simox_pme_self(ctx, stats(1))   ! main.f90:33
   call simox_check(ctx, rc)
   if (.not. restart) call vinit()
+  if (thermostat) then
+    if (freq_new < 0) then
+      rc = simox_set_thermostat(ctx, 2, t_given, alpha_l, 1_c_long_long)
+    else
+      rc = simox_set_thermostat(ctx, 1, t_given, alpha_l, 1_c_long_long)
+    end if
+    call simox_check(ctx, rc)
+  end if
+  rc = simox_set_rdf(ctx, 1)
+  call simox_check(ctx, rc)
 
   open(15, file='temporal_status.dat')
   open(16, file='temporal_status_full.dat')
@@ -123,8 +144,9 @@ program simox_md
       rc = simox_step(ctx, 1, dt, merge(1, 0, do_sort), merge(1, 0, do_ctip), c_null_ptr)
     end if
     call simox_check(ctx, rc)
-    if (do_stat) call print_stat(nloop)
     if (mod(nloop, freq_snap) == 0) call print_snapshot(nloop / freq_snap)
+    if (do_stat) call print_stat(nloop)
+    if (mod(nloop, freq_rdf) == 0) call print_rdf(nloop)
   end do
   close(15)
   close(16)
@@ -210,6 +232,51 @@ contains
     write(16, '(7(ES24.16, 1X))') nl * tfm * dt, temp, e_tot, e_es, e_eam, &
          & st(5), st(6)
   end subroutine print_stat
+
+  ! PRINT_RDF (util.f90:246-307): histogram of the FORCE calls since the last
+  ! print, averaged over freq_rdf, normalised as there; then cleared.
+  subroutine print_rdf(nl)
+    integer, intent(in) :: nl
+    real(c_double), parameter :: pi = 3.1415926535897931d0
+    real(c_double) :: res(15, 200), dr, r, cst, loop_sum, xrdf
+    character(len=64) :: fname
+    integer :: a, b, m, k, npt_old, npt_new
+    rc = simox_get_rdf(ctx, hist, 1)
+    call simox_check(ctx, rc)
+    xrdf = dble(freq_rdf)
+    m = 0
+    do a = 1, 5
+      do b = a, 5
+        m = m + 1
+        if (a /= b) then
+          res(m, :) = dble(hist(a, b, 1:200) + hist(b, a, 1:200)) / xrdf
+        else
+          res(m, :) = dble(hist(a, a, 1:200)) / xrdf
+        end if
+      end do
+    end do
+    dr = rcut / 200.0d0
+    npt_old = n
+    npt_new = n
+    loop_sum = dble(npt_old * npt_new) * 4.d0 * pi / 3.0d0 / (box(1) * box(2) * box(3))
+    write(fname, '("rdf", I3.3, ".dat")') nl / freq_rdf
+    open(22, file=trim(fname), action='write')
+    write(22, '("# Header: current loop = ",I7," Npt_old = ", I5, " Npt_new = ", I5)') nl, &
+         & npt_old, npt_new
+    write(22, '(A)') "# Radius(A) / Al-Al(2) / Al-Ni(3) / Al-Fe(4) / Al-O(5) / Al-H(6) " // &
+         & "/ Ni-Ni(7) / Ni-Fe(8) / Ni-O(9) / Ni-H(10) / Fe-Fe(11) / Fe-O(12)" // &
+         & " / Fe-H(13) / O-O(14) / O-H(15) / H-H(16) "
+    do k = 1, 200
+      r = dble(k) * dr
+      cst = ((r + dr)**3 - dr**3) * loop_sum
+      write(22, '(ES10.3, 1X)', advance='no') dr * (dble(k) - 0.5d0)
+      do m = 1, 15
+        write(22, '(ES10.3, 1X)', advance='no') res(m, k) / cst
+      end do
+      write(22, *)
+    end do
+    close(22)
+  end subroutine print_rdf
 
   subroutine print_snapshot(nframe)
     integer, intent(in) :: nframe

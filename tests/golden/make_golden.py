@@ -18,6 +18,10 @@ from oracle.oracle import Oracle, run_reference  # noqa: E402
 from simox_amd.systems import config, tiny  # noqa: E402
 
 
+T_BEREND = 600.0      # K, target of VV_NVT2_BEREND
+ALPHA_LANGEVIN = 0.1  # friction of VV_NVT1/2_STOCH (run at T_given = 0)
+
+
 def one(name, s, seed, with_dudq=True):
     rng = np.random.default_rng(seed)
     q = rng.normal(size=s.n) * 0.3
@@ -30,13 +34,18 @@ def one(name, s, seed, with_dudq=True):
         nlist, lst, asym, sym = o.ctip_matrix(s.ids, s.x)
         phi = rng.normal(size=s.n)
         dud = (nlist, lst, asym, sym, phi)
-    ref = run_reference(s.ids, s.x, s.v, f0, q, s.box, s.fft, s.rcut, s.dt, dudq_inputs=dud)
+    ref = run_reference(s.ids, s.x, s.v, f0, q, s.box, s.fft, s.rcut, s.dt, dudq_inputs=dud,
+                        t_given=T_BEREND, alpha=ALPHA_LANGEVIN)
     out = dict(ids=s.ids, x=s.x, v=s.v, f0=f0, q=q, box=s.box, fft=np.array(s.fft),
                rcut=np.array(s.rcut), dt=np.array(s.dt),
                ref_npt=ref["npt"], ref_pair=ref["pair"], ref_f=ref["f"],
                ref_E=np.array([ref["Epair"], ref["Edir"], ref["Ectip"], ref["Egroup"], ref["Eself"]]),
                ref_x1=ref["x1"], ref_v1=ref["v1"], ref_v2=ref["v2"], ref_mv2=np.array(ref["mv2"]),
-               ref_rdf=ref["rdf"])
+               ref_rdf=ref["rdf"],
+               t_given=np.array(T_BEREND), alpha=np.array(ALPHA_LANGEVIN),
+               ref_vb=ref["vb"], ref_mv2b=np.array(ref["mv2b"]), ref_xs=ref["xs"],
+               ref_vs=ref["vs"], ref_vs2=ref["vs2"], ref_fs2=ref["fs2"],
+               ref_mv2s=np.array(ref["mv2s"]))
     # links only for occupied slots (compact)
     out["ref_link"] = np.concatenate([ref["link"][c, :ref["npt"][c]] for c in range(len(ref["npt"]))])
     if s.n <= 400:

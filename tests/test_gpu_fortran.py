@@ -152,3 +152,47 @@ def test_simox_md_vinit(case, tmp_path):
     np.testing.assert_array_equal(rows[0], rows[1])
     assert abs(rows[0][0, 1] - 300.0) < 30.0
     assert np.all(np.isfinite(rows[0]))
+
+
+def test_simox_md_berendsen_and_rdf(case, tmp_path):
+    """thermostat YES (Berendsen, freq_new >= 0): simox_md's full-precision
+    status equals the Python binding with set_thermostat(1, T); its rdf001.dat
+    (PRINT_RDF format, util.f90:246-307) holds the histogram of the same run."""
+    s, f, q, _ = case
+    d = str(tmp_path)
+    write_inputs(d, s, f, q)
+    cfg = open(os.path.join(d, "config.prm")).read().replace("NO 300.0 0.1", "YES 450.0 0.1")
+    nrdf_t = NSTEPS * DT_FS
+    cfg = cfg.replace(f"{100 * NSTEPS * DT_FS:.17e}", f"{nrdf_t:.17e}")
+    open(os.path.join(d, "config.prm"), "w").write(cfg)
+    run(SIMOX_MD, d)
+    full = load_status(os.path.join(d, "temporal_status_full.dat"))
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FAITHFUL)
+    g.set_state(s.ids, s.x, s.v, f, q)
+    g.pme_self()
+    g.set_thermostat(1, 450.0, 0.1, 1)
+    g.set_rdf(True)
+    rows = []
+    for nloop in range(1, NSTEPS + 1):
+        st = g.step(1, DT, 1 if nloop % FREQ_SORT == 0 else 0, 1 if nloop % FREQ_NEW == 1 else 0)
+        e_es = st["Eself"] + st["Edir"] + st["Erec"] + st["Ectip"]
+        e_eam = st["Epair"] + st["Egroup"]
+        rows.append([nloop * TFM * DT, st["mv2"] / (3 * s.n) / 8.617343e-5,
+                     e_es + e_eam + st["mv2"] / 2, e_es, e_eam, st["Epair"], st["Egroup"]])
+    np.testing.assert_allclose(full, np.array(rows), rtol=1e-15, atol=0)
+    h = g.rdf().transpose(2, 1, 0)  # hist(si, sj, bin) as Fortran sees it
+    res = []
+    for a in range(5):
+        for b in range(a, 5):
+            res.append((h[a, b, :200] + h[b, a, :200]) / NSTEPS if a != b else h[a, a, :200] / NSTEPS)
+    res = np.array(res)
+    dr = s.rcut / 200.0
+    loop_sum = float(s.n * s.n) * 4.0 * np.pi / 3.0 / np.prod(s.box)
+    r = np.arange(1, 201) * dr
+    cst = ((r + dr) ** 3 - dr ** 3) * loop_sum
+    lines = [l for l in open(os.path.join(d, "rdf001.dat")).read().splitlines() if not l.startswith("#")]
+    got = np.array([[float(t) for t in l.split()] for l in lines])
+    assert got.shape == (200, 16)
+    np.testing.assert_allclose(got[:, 0], dr * (np.arange(1, 201) - 0.5), rtol=1e-3)
+    np.testing.assert_allclose(got[:, 1:], (res / cst).T, rtol=1e-3, atol=1e-300)
+    assert got[:, 1].max() > 0
