@@ -120,6 +120,12 @@ int simox_set_thermostat(simox_ctx* ctx, int kind, double t_kelvin, double alpha
    reference's rdf(i,j) + rdf(j,i) and rdf(i,i), the sums PRINT_RDF writes
    (util.f90:270-277).  reset != 0 zeroes it after the copy. */
 int simox_set_rdf(simox_ctx* ctx, int on);
+/* OXIDATION mode (main.f90:62-86): on != 0 makes simox_vv1 / simox_step use
+   VV_NVE1_REFLECT (vverlet.f90:100-129: x, y periodic, reflecting wall at
+   z = box_z/2).  simox_remove_rigid_motion = REMOVE_RIGID_MOTION
+   (vverlet.f90:211-234): subtracts the mean velocity and the mean z. */
+int simox_set_reflect(simox_ctx* ctx, int on);
+int simox_remove_rigid_motion(simox_ctx* ctx);
 int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x
    (R2C + C2R) and y tile sizes on the context's own grid (environment

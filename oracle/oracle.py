@@ -254,9 +254,10 @@ class Oracle:
 
 
 def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None,
-                  t_given=600.0, alpha=0.1):
+                  t_given=600.0, alpha=0.1, zshift=0.0):
     """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90.
-    t_given (K) is the Berendsen target; alpha the Langevin friction (run at T = 0)."""
+    t_given (K) is the Berendsen target; alpha the Langevin friction (run at T = 0);
+    zshift moves z before VV_NVE1_REFLECT."""
     import tempfile
     if not os.path.exists(REF_DRIVER):
         raise RuntimeError("reference build absent (oracle/_ref/ref_driver)")
@@ -267,7 +268,7 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
     nnb = NNB_MAX if dudq_inputs is not None else 0
     with open(fin, "wb") as fh:
         np.array([n, *fft, nnb], dtype=np.int32).tofile(fh)
-        np.array([*box, rcut, dt, t_given, alpha], dtype=np.float64).tofile(fh)
+        np.array([*box, rcut, dt, t_given, alpha, zshift], dtype=np.float64).tofile(fh)
         np.asarray(ids, dtype=np.int32).tofile(fh)
         for a in (x, v, f):
             np.asarray(a, dtype=np.float64).reshape(n, 3).tofile(fh)
@@ -315,6 +316,8 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
     out["vs2"] = take(np.float64, 3 * n).reshape(n, 3)
     out["fs2"] = take(np.float64, 3 * n).reshape(n, 3)
     out["mv2s"] = take(np.float64, 1)[0]
+    for k in ("xr", "vr", "xr2", "vr2"):
+        out[k] = take(np.float64, 3 * n).reshape(n, 3)
     if dudq_inputs is not None:
         out["g"] = take(np.float64, n)
     return out

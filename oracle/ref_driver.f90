@@ -15,12 +15,14 @@
 !   VV_NVT2_BEREND (on a copy after VV_NVE1, T_given)   vverlet.f90:160-208
 !   VV_NVT1_STOCH / VV_NVT2_STOCH (on a copy, T_given = 0, so the random
 !     force is zero and the result deterministic)        vverlet.f90:237-296
+!   VV_NVE1_REFLECT then REMOVE_RIGID_MOTION (on a copy with z shifted by
+!     zshift, so that the top atoms cross the reflecting wall) vverlet.f90:100-129,211-234
 !   dUdq (on matrices read from input)   ctip.f90:307-333
 ! PME, PHI_MESH, CTIP_SOLVER and BRENT_SOLVER call FFTE's ZFFT3D and are not
 ! linked (link-time optimisation drops them because nothing here calls them).
 !
 ! Input  (stream file, native endian):
-!   int32 n, fft(3), nnb; real64 box(3), rcut, dt, t_given (K), alpha
+!   int32 n, fft(3), nnb; real64 box(3), rcut, dt, t_given (K), alpha, zshift
 !   int32 id(n); real64 x(3,n), v(3,n), f(3,n), q(n)
 !   int32 nlist(n), list(n,500); real64 asym(n,500), sym(n,500), phi(n)
 !   (nnb = 0 skips the dUdq block)
@@ -32,6 +34,8 @@
 !   real64 vb(3,n), mv2b after VV_NVT2_BEREND (from the VV_NVE1 state)
 !   real64 xs(3,n), vs(3,n) after VV_NVT1_STOCH; vs2(3,n), fs2(3,n), mv2s
 !     after VV_NVT2_STOCH with T_given = 0
+!   real64 xr(3,n), vr(3,n) after VV_NVE1_REFLECT; xr2(3,n), vr2(3,n) after
+!     REMOVE_RIGID_MOTION
 !   real64 g(n) (if nnb > 0)
 program ref_driver
 use datafmt
@@ -42,20 +46,20 @@ type(metl) :: eam(nmetal)
 type(potn) :: param(natom)
 type(oxid) :: oxygn
 type(corr) :: pair
-type(ptcl), allocatable :: q(:), q1(:), qb(:), qs(:)
+type(ptcl), allocatable :: q(:), q1(:), qb(:), qs(:), qr(:)
 type(sort), allocatable :: cell(:)
 real(dp), allocatable :: psi(:,:,:), xyz(:,:), vel(:,:), frc(:,:), chg(:)
 real(dp), allocatable :: xjz(:,:), asym(:,:), sym(:,:), phi(:), qmx(:,:), g(:)
 integer(gi), allocatable :: ids(:), list(:,:), nlist(:)
 integer(gi) :: n, fft(3), nnb, i, m
-real(dp) :: box(3), rcut, dt, t_given, alpha
+real(dp) :: box(3), rcut, dt, t_given, alpha, zshift
 character(len=512) :: fin, fout
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
 open(10, file=trim(fin), access='stream', form='unformatted', status='old')
 read(10) n, fft, nnb
-read(10) box, rcut, dt, t_given, alpha
+read(10) box, rcut, dt, t_given, alpha, zshift
 allocate(ids(n), xyz(3,n), vel(3,n), frc(3,n), chg(n))
 read(10) ids, xyz, vel, frc, chg
 
@@ -74,7 +78,7 @@ pair%dr = rcut/dble(nrdf)
 pair%v = nset%v
 pair%rdf(:,:,:) = 0
 
-allocate(q(n), q1(n), qb(n), qs(n), cell(nset%ncell_all))
+allocate(q(n), q1(n), qb(n), qs(n), qr(n), cell(nset%ncell_all))
 nset%alpha = alpha
 sys%t_given = t_given*kb
 allocate(psi(fft(1),fft(2),fft(3)))
@@ -95,6 +99,10 @@ call pme_self(nset, sys, param, q, psi)
 ! integrator on copies, using the input forces
 q1(:) = q(:)
 qs(:) = q(:)
+qr(:) = q(:)
+do i = 1, n
+   qr(i)%xx(3) = qr(i)%xx(3) + zshift
+end do
 call vv_nve1(nset, param, q1, dt)
 qb(:) = q1(:)
 
@@ -126,6 +134,12 @@ call vv_nvt2_stoch(nset, param, sys, qs, dt)
 write(11) (qs(i)%xv(:), i = 1, n)
 write(11) (qs(i)%ff(:), i = 1, n)
 write(11) sys%mv2
+call vv_nve1_reflect(nset, param, qr, dt)
+write(11) (qr(i)%xx(:), i = 1, n)
+write(11) (qr(i)%xv(:), i = 1, n)
+call remove_rigid_motion(nset, qr)
+write(11) (qr(i)%xx(:), i = 1, n)
+write(11) (qr(i)%xv(:), i = 1, n)
 
 if (nnb > 0) then
    allocate(xjz(n,3), asym(n,nnb_max), sym(n,nnb_max), phi(n), qmx(n,2), g(n))

@@ -23,7 +23,8 @@ EXPORTS = [
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
-    "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf",
+    "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
+    "simox_remove_rigid_motion",
 ]
 
 # simox_kernel_timing categories
@@ -85,6 +86,8 @@ def load_library(path=LIB_PATH):
     lib.simox_set_eam_cutoff.argtypes = [vp, d]
     lib.simox_set_thermostat.argtypes = [vp, i, d, d, C.c_ulonglong]
     lib.simox_set_rdf.argtypes = [vp, i]
+    lib.simox_set_reflect.argtypes = [vp, i]
+    lib.simox_remove_rigid_motion.argtypes = [vp]
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -181,6 +184,14 @@ class Simox:
         """0 NVE, 1 Berendsen (VV_NVT2_BEREND), 2 Langevin (VV_NVT1/2_STOCH)."""
         self._check(self.lib.simox_set_thermostat(self.h, int(kind), float(t_kelvin),
                                                   float(alpha), int(seed)))
+
+    def set_reflect(self, on=True):
+        """OXIDATION mode first half: VV_NVE1_REFLECT instead of VV_NVE1."""
+        self._check(self.lib.simox_set_reflect(self.h, 1 if on else 0))
+
+    def remove_rigid_motion(self):
+        """REMOVE_RIGID_MOTION: subtract the mean velocity and the mean z."""
+        self._check(self.lib.simox_remove_rigid_motion(self.h))
 
     def set_rdf(self, on=True):
         """Accumulate FORCE's pair histogram (pair%rdf) on every force call."""

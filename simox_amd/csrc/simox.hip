@@ -116,6 +116,7 @@ struct simox_ctx {
   int pred_order = 1;
   // thermostat of the step (0 NVE, 1 Berendsen, 2 Langevin), target kB*T in eV
   int thermo = 0;
+  bool reflect = false;  // OXIDATION-mode first half (VV_NVE1_REFLECT)
   double t_given = 0.0, alpha = 0.0;
   unsigned long long seed = 1, nvt_step = 0;
   // pair histogram of FORCE (RDF), accumulated while rdf_on
@@ -1069,7 +1070,11 @@ int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
 }
 
 int do_vv1(simox_ctx* ctx, double dt) {
-  if (ctx->thermo == 2)  // VV_NVT1_STOCH
+  if (ctx->reflect)  // VV_NVE1_REFLECT
+    hipLaunchKernelGGL(k_vv1_reflect, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n,
+                       ctx->dtab, ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt, ctx->x,
+                       ctx->v, ctx->f);
+  else if (ctx->thermo == 2)  // VV_NVT1_STOCH
     hipLaunchKernelGGL(k_vv1_stoch, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n,
                        ctx->dtab, ctx->id, ctx->g.box[0], ctx->g.box[1], ctx->g.box[2], dt,
                        ctx->alpha, ctx->x, ctx->v, ctx->f);
@@ -1511,6 +1516,28 @@ int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) 
   if (v) CHK(download_caller(ctx, ctx->v, 3, v));
   if (f) CHK(download_caller(ctx, ctx->f, 3, f));
   if (q) CHK(download_caller(ctx, ctx->q, 1, q));
+  return SIMOX_OK;
+}
+
+int simox_set_reflect(simox_ctx* ctx, int on) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  ctx->reflect = on != 0;
+  return SIMOX_OK;
+}
+
+int simox_remove_rigid_motion(simox_ctx* ctx) {
+  if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
+  const int n = ctx->n, nb = grid_blocks(n);
+  CHK(ensure_part(ctx, size_t(nb) * 4));
+  hipLaunchKernelGGL(k_rigid_sums, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->x, ctx->v, ctx->part);
+  CHK(reduce(ctx, nb, 4, 0, 60));
+  hipLaunchKernelGGL(k_rigid_apply, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 60, ctx->x,
+                     ctx->v);
+  HIPCHK(hipGetLastError());
+  ctx->matrix_valid = false;
+  ctx->nlist_valid = false;
+  ctx->bricks_valid = false;
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }
 

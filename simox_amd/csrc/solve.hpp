@@ -215,6 +215,57 @@ __global__ void k_vv1(int n, const Tables* __restrict__ tb, const int* __restric
   }
 }
 
+// OXIDATION-mode first half (VV_NVE1_REFLECT, vverlet.f90:100-129): as
+// VV_NVE1 with x, y periodic and a reflecting wall at z = box_z / 2:
+// z > box_z / 2 -> z = box_z - z, v_z = -|v_z|.
+__global__ void k_vv1_reflect(int n, const Tables* __restrict__ tb, const int* __restrict__ id,
+                              double b0, double b1, double b2, double dt, double* __restrict__ x,
+                              double* __restrict__ v, const double* __restrict__ f) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xm = tb->sc[id[i] - 1].xm;
+  const double box[3] = {b0, b1, b2};
+  double xn[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double vn = v[3 * i + d] + 0.5 * dt * f[3 * i + d] / xm;
+    v[3 * i + d] = vn;
+    xn[d] = x[3 * i + d] + dt * vn;
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d) x[3 * i + d] = xn[d] - box[d] * round(xn[d] / box[d]);
+  if (xn[2] > b2 / 2.0) {
+    xn[2] = b2 - xn[2];
+    v[3 * i + 2] = -fabs(v[3 * i + 2]);
+  }
+  x[3 * i + 2] = xn[2];
+}
+
+// REMOVE_RIGID_MOTION (vverlet.f90:211-234), pass 1: part[4k..] = sums of
+// v_x, v_y, v_z and z; pass 2 subtracts the means (v, and z of the positions).
+__global__ void __launch_bounds__(BLOCK) k_rigid_sums(int n, const double* __restrict__ x,
+                                                      const double* __restrict__ v,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    acc[0] = v[3 * i]; acc[1] = v[3 * i + 1]; acc[2] = v[3 * i + 2]; acc[3] = x[3 * i + 2];
+  }
+  block_sum<4>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = acc[k];
+}
+__global__ void k_rigid_apply(int n, const double* __restrict__ sums, double* __restrict__ x,
+                              double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dn = double(n);
+  for (int d = 0; d < 3; ++d) v[3 * i + d] = v[3 * i + d] - sums[d] / dn;
+  x[3 * i + 2] = x[3 * i + 2] - sums[3] / dn;
+}
+
 // Langevin first half (VV_NVT1_STOCH, vverlet.f90:237-257):
 //   v = v (1 - alpha dt / 2m) + dt f / 2m,  x += dt v, wrapped (DNINT).
 __global__ void k_vv1_stoch(int n, const Tables* __restrict__ tb, const int* __restrict__ id,

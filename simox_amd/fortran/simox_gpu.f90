@@ -91,6 +91,17 @@ module simox_gpu
       integer(c_long_long), value :: seed
     end function simox_set_thermostat
 
+    integer(c_int) function simox_set_reflect(ctx, on) bind(c, name='simox_set_reflect')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function simox_set_reflect
+
+    integer(c_int) function simox_remove_rigid_motion(ctx) bind(c, name='simox_remove_rigid_motion')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function simox_remove_rigid_motion
+
     integer(c_int) function simox_set_rdf(ctx, on) bind(c, name='simox_set_rdf')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

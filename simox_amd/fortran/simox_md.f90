@@ -1,5 +1,4 @@
-! simox_md: Fortran host program for the REGULAR NVE run of SIMOX on one
-! MI355X.  Reads the reference's input files (config.prm layout of
+! simox_md: Fortran host program for SIMOX runs on one MI355X.  Reads the reference's input files (config.prm layout of
 ! main.f90:122-180; restart.xyz "El x y z vx vy vz fx fy fz q" or input.xyz
 ! "El x y z q", main.f90:205-245) and runs the step loop of main.f90:41-61
 ! device-resident through the C-ABI (simox_step), with TAG_SET's cadence
@@ -14,11 +13,17 @@
 ! Langevin (VV_NVT1/2_STOCH) as main.f90:44-58 does; the pair histogram is
 ! written every time_rdf as PRINT_RDF does (util.f90:246-307).
 !
+! OXIDATION runs follow main.f90:62-86 call by call: VV_NVE1_REFLECT, cell
+! sort, and on charge steps REMOVE_RIGID_MOTION, OXIDE_CHECK (util.f90:
+! 152-188) and NEW_O2 (datafmt.f90:98-139, positions of RANDOM_POSITION_O2,
+! util.f90:104-149) before the charge solve; the device context is rebuilt
+! for the grown atom count and the new atoms are binned into cells at once.
+!
 ! A non-restart run initialises velocities as VINIT does (vverlet.f90:2-71):
 ! uniform random components, rigid motion removed, each species scaled to
-! T_given, and the first forces evaluated at x - dt*v.  The generator is
-! seeded from SIMOX_SEED (default 1) where the reference's RANDOM_SEED is
-! unseeded.  Not covered (the reference's other path): OXIDATION mode.
+! T_given, and the first forces evaluated at x - dt*v.  The generator (VINIT,
+! new O2 positions) is seeded from SIMOX_SEED (default 1) where the
+! reference's RANDOM_SEED is unseeded.
 !
 ! Usage: simox_md [mode]   (mode = faithful | fp64 | mixed; default faithful)
 program simox_md
@@ -31,7 +36,10 @@ program simox_md
   real(c_double) :: rcut, z_new, q_crit, stats(8)
   integer(c_int) :: fft(3), freq_new, freq_sort, n_o2, mode, rc, n
   integer :: freq_max, freq_snap, freq_stat, freq_rdf, nloop, i, ios
-  logical :: restart, do_ctip, do_sort, do_stat, thermostat
+  logical :: restart, do_ctip, do_sort, do_stat, thermostat, regular, new_o2
+  real(c_double) :: e_self, e_rec, e4(4), mv2
+  integer(c_long_long) :: hist_carry(5, 5, 201)
+  integer :: npt_old
   integer(c_long_long), target :: hist(5, 5, 201)
   real(c_double), allocatable, target :: x(:,:), v(:,:), f(:,:), q(:)
   integer(c_int), allocatable, target :: id(:)
@@ -51,7 +59,13 @@ program simox_md
   read(10, *) dummy
   read(10, *) dummy
   read(10, *) runtype
-  if (to_upper(runtype) /= 'REGULAR') stop "=== only REGULAR runs are supported by simox_md ==="
+  if (to_upper(runtype) == 'REGULAR') then
+    regular = .true.
+  else if (to_upper(runtype) == 'OXIDATION') then
+    regular = .false.
+  else
+    stop "=== Simulation type is unknown ==="
+  end if
   read(10, *) dummy
   read(10, *) time_max, time_snap, time_rdf, time_stat, dt
   freq_max = nint(time_max / dt)
@@ -115,11 +129,14 @@ program simox_md
   call simox_check(ctx, rc)
   rc = simox_set_state(ctx, c_loc(id), c_loc(x), c_loc(v), c_loc(f), c_loc(q))
   call simox_check(ctx, rc)
-  rc = simox_pme_self(ctx, stats(1))   ! main.f90:33
+  rc = simox_pme_self(ctx, e_self)   ! main.f90:33
   call simox_check(ctx, rc)
+  call seed_rng()
+  hist_carry = 0
+  npt_old = n
   if (.not. restart) call vinit()
   if (thermostat) then
-    if (freq_new < 0) then
+    if (freq_new < 0 .and. regular) then
       rc = simox_set_thermostat(ctx, 2, t_given, alpha_l, 1_c_long_long)
     else
       rc = simox_set_thermostat(ctx, 1, t_given, alpha_l, 1_c_long_long)
@@ -134,16 +151,22 @@ program simox_md
   write(15, *) "# time(fs) - T(K) - E_total -   E_Ewald   -   E_eam  -   ", &
        & "Epair   -  Egroup (eV)"
   write(16, '(A)') "# time(fs) T(K) E_total E_es E_eam Epair Egroup (full precision)"
+  if (.not. regular) then
+    rc = simox_set_reflect(ctx, 1)
+    call simox_check(ctx, rc)
+  end if
   do nloop = 1, freq_max
     do_ctip = freq_new >= 0 .and. mod(nloop, freq_new) == 1
     do_sort = mod(nloop, freq_sort) == 0
     do_stat = mod(nloop, freq_stat) == 0
-    if (do_stat) then
+    if (.not. regular) then
+      call oxidation_step()
+    else if (do_stat) then
       rc = simox_step(ctx, 1, dt, merge(1, 0, do_sort), merge(1, 0, do_ctip), c_loc(st))
     else
       rc = simox_step(ctx, 1, dt, merge(1, 0, do_sort), merge(1, 0, do_ctip), c_null_ptr)
     end if
-    call simox_check(ctx, rc)
+    if (regular) call simox_check(ctx, rc)
     if (mod(nloop, freq_snap) == 0) call print_snapshot(nloop / freq_snap)
     if (do_stat) call print_stat(nloop)
     if (mod(nloop, freq_rdf) == 0) call print_rdf(nloop)
@@ -154,22 +177,144 @@ program simox_md
 
 contains
 
-  ! VINIT (vverlet.f90:2-71): random velocities at T_given, forces at x - dt*v
-  subroutine vinit()
-    real(c_double), allocatable, target :: x0(:,:), v0(:,:)
-    real(c_double) :: sumv(3), mv2s(5), xm(5), r, lam(5), e4(4), erec
-    integer :: nsp(5), k, j, nseed
+  subroutine seed_rng()
+    integer :: nseed
     integer, allocatable :: seed(:)
     character(len=32) :: sv
-    type(c_params) :: prm
-    call simox_potential_setup(prm)
-    xm = prm%prm(:)%xm
     call random_seed(size=nseed)
     allocate(seed(nseed))
     seed = 1
     call get_environment_variable('SIMOX_SEED', sv)
     if (len_trim(sv) > 0) read(sv, *) seed(1)
     call random_seed(put=seed)
+  end subroutine seed_rng
+
+  ! One OXIDATION step (main.f90:63-80), call by call; fills st(1:7).
+  subroutine oxidation_step()
+    integer(c_int) :: it, nev
+    rc = simox_vv1(ctx, dt)                 ! VV_NVE1_REFLECT
+    call simox_check(ctx, rc)
+    if (do_sort) then
+      rc = simox_cell_sort(ctx)
+      call simox_check(ctx, rc)
+    end if
+    if (do_ctip) then
+      rc = simox_remove_rigid_motion(ctx)
+      call simox_check(ctx, rc)
+      call oxide_check()
+      if (new_o2) call add_o2()
+      rc = simox_ctip_solve(ctx, e_self, it, nev)
+      call simox_check(ctx, rc)
+    end if
+    rc = simox_force(ctx, e4)
+    call simox_check(ctx, rc)
+    rc = simox_pme(ctx, e_rec)
+    call simox_check(ctx, rc)
+    rc = simox_vv2(ctx, dt, mv2)            ! VV_NVE2 or VV_NVT2_BEREND
+    call simox_check(ctx, rc)
+    st(1:7) = [e_self, e4(2), e_rec, e4(3), e4(1), e4(4), mv2]
+  end subroutine oxidation_step
+
+  ! OXIDE_CHECK (util.f90:152-188): a new dimer when the highest Al is above
+  ! the last inserted oxygens or their largest charge is below q_crit.
+  subroutine oxide_check()
+    real(c_double) :: m_height, o_height, q_max
+    integer :: k
+    rc = simox_get_state(ctx, c_loc(x), c_null_ptr, c_null_ptr, c_loc(q))
+    call simox_check(ctx, rc)
+    m_height = 0.0d0
+    do k = 1, n
+      if (id(k) == 1) m_height = max(x(3, k), m_height)
+    end do
+    o_height = 0.0d0
+    q_max = -10.0d0
+    do k = n, n - 2 * n_o2 + 1, -1
+      if (id(k) == 4) then
+        o_height = max(x(3, k), o_height)
+        q_max = max(q(k), q_max)
+      end if
+    end do
+    new_o2 = m_height > o_height .or. q_max < q_crit
+  end subroutine oxide_check
+
+  ! NEW_O2 (datafmt.f90:98-139) with RANDOM_POSITION_O2 (util.f90:104-149):
+  ! n_o2 dimers (O-O 2 A) at height z_new, centres at least
+  ! sqrt(Lx Ly / n_o2) / 2 apart, random orientation (the y component uses
+  ! cos(phi) as there), v = (0, 0, -sqrt(3 kB T / 16)), f = 0, q = 0.  The
+  ! context is rebuilt for the new atom count.
+  subroutine add_o2()
+    real(c_double), parameter :: pi = 3.1415926535897931d0, l0 = 2.0d0
+    real(c_double), allocatable, target :: xn(:,:), vn(:,:), fn(:,:), qn(:)
+    integer(c_int), allocatable, target :: idn(:)
+    character(len=3), allocatable :: eln(:)
+    real(c_double) :: cx(n_o2, 3), r, rmin2, phi, theta, d(3)
+    integer :: k, j, m, nn
+    logical :: close_pair
+    integer(c_long_long), target :: h(5, 5, 201)
+    if (n_o2 <= 0) return
+    rmin2 = (sqrt(box(1) * box(2) / dble(n_o2)) * 0.5d0)**2
+    if (rmin2 < l0**2) stop "=== Too many new oxygens ==="
+    do k = 1, n_o2
+      close_pair = .true.
+      do while (close_pair)
+        call random_number(r); cx(k, 1) = (r - 0.5d0) * box(1)
+        call random_number(r); cx(k, 2) = (r - 0.5d0) * box(2)
+        close_pair = .false.
+        do j = 1, k - 1
+          if ((cx(k, 1) - cx(j, 1))**2 + (cx(k, 2) - cx(j, 2))**2 < rmin2) close_pair = .true.
+        end do
+      end do
+    end do
+    cx(:, 3) = z_new
+    rc = simox_get_state(ctx, c_loc(x), c_loc(v), c_loc(f), c_loc(q))
+    call simox_check(ctx, rc)
+    rc = simox_get_rdf(ctx, h, 1)
+    call simox_check(ctx, rc)
+    hist_carry = hist_carry + h
+    nn = n + 2 * n_o2
+    allocate(xn(3, nn), vn(3, nn), fn(3, nn), qn(nn), idn(nn), eln(nn))
+    xn(:, 1:n) = x; vn(:, 1:n) = v; fn(:, 1:n) = f; qn(1:n) = q; idn(1:n) = id; eln(1:n) = el
+    do k = 1, n_o2
+      call random_number(r); phi = r * pi * 2.0d0
+      call random_number(r); theta = r * pi
+      d = [l0 * sin(theta) * cos(phi), l0 * sin(theta) * cos(phi), l0 * cos(theta)]
+      m = n + 2 * k - 1
+      xn(:, m) = cx(k, :) + d
+      xn(:, m + 1) = cx(k, :) - d
+    end do
+    do m = n + 1, nn
+      vn(:, m) = [0.0d0, 0.0d0, -sqrt(t_given * kb * 3.0d0 / 16.0d0)]
+      fn(:, m) = 0.0d0
+      qn(m) = 0.0d0
+      idn(m) = 4
+      eln(m) = 'O'
+    end do
+    call move_alloc(xn, x); call move_alloc(vn, v); call move_alloc(fn, f)
+    call move_alloc(qn, q); call move_alloc(idn, id); call move_alloc(eln, el)
+    n = nn
+    rc = simox_destroy(ctx)
+    rc = simox_create(n, box, fft, rcut, mode, c_null_ptr, ctx)
+    call simox_check(ctx, rc)
+    rc = simox_set_state(ctx, c_loc(id), c_loc(x), c_loc(v), c_loc(f), c_loc(q))
+    call simox_check(ctx, rc)
+    rc = simox_set_reflect(ctx, 1)
+    call simox_check(ctx, rc)
+    if (thermostat) then
+      rc = simox_set_thermostat(ctx, 1, t_given, alpha_l, 1_c_long_long)
+      call simox_check(ctx, rc)
+    end if
+    rc = simox_set_rdf(ctx, 1)
+    call simox_check(ctx, rc)
+  end subroutine add_o2
+
+  ! VINIT (vverlet.f90:2-71): random velocities at T_given, forces at x - dt*v
+  subroutine vinit()
+    real(c_double), allocatable, target :: x0(:,:), v0(:,:)
+    real(c_double) :: sumv(3), mv2s(5), xm(5), r, lam(5), e4(4), erec
+    integer :: nsp(5), k, j
+    type(c_params) :: prm
+    call simox_potential_setup(prm)
+    xm = prm%prm(:)%xm
     sumv = 0.0d0
     do k = 1, n
       do j = 1, 3
@@ -240,9 +385,11 @@ contains
     real(c_double), parameter :: pi = 3.1415926535897931d0
     real(c_double) :: res(15, 200), dr, r, cst, loop_sum, xrdf
     character(len=64) :: fname
-    integer :: a, b, m, k, npt_old, npt_new
+    integer :: a, b, m, k, npt_new
     rc = simox_get_rdf(ctx, hist, 1)
     call simox_check(ctx, rc)
+    hist = hist + hist_carry
+    hist_carry = 0
     xrdf = dble(freq_rdf)
     m = 0
     do a = 1, 5
@@ -256,7 +403,6 @@ contains
       end do
     end do
     dr = rcut / 200.0d0
-    npt_old = n
     npt_new = n
     loop_sum = dble(npt_old * npt_new) * 4.d0 * pi / 3.0d0 / (box(1) * box(2) * box(3))
     write(fname, '("rdf", I3.3, ".dat")') nl / freq_rdf
@@ -276,6 +422,7 @@ contains
       write(22, *)
     end do
     close(22)
+    npt_old = npt_new
   end subroutine print_rdf
 
   subroutine print_snapshot(nframe)

@@ -34,8 +34,11 @@ def one(name, s, seed, with_dudq=True):
         nlist, lst, asym, sym = o.ctip_matrix(s.ids, s.x)
         phi = rng.normal(size=s.n)
         dud = (nlist, lst, asym, sym, phi)
+    # VV_NVE1_REFLECT case: z shifted so that the 5 highest atoms sit on or
+    # above the reflecting wall at box_z / 2
+    zshift = 0.5 * s.box[2] - np.sort(s.x[:, 2])[-5]
     ref = run_reference(s.ids, s.x, s.v, f0, q, s.box, s.fft, s.rcut, s.dt, dudq_inputs=dud,
-                        t_given=T_BEREND, alpha=ALPHA_LANGEVIN)
+                        t_given=T_BEREND, alpha=ALPHA_LANGEVIN, zshift=zshift)
     out = dict(ids=s.ids, x=s.x, v=s.v, f0=f0, q=q, box=s.box, fft=np.array(s.fft),
                rcut=np.array(s.rcut), dt=np.array(s.dt),
                ref_npt=ref["npt"], ref_pair=ref["pair"], ref_f=ref["f"],
@@ -45,7 +48,8 @@ def one(name, s, seed, with_dudq=True):
                t_given=np.array(T_BEREND), alpha=np.array(ALPHA_LANGEVIN),
                ref_vb=ref["vb"], ref_mv2b=np.array(ref["mv2b"]), ref_xs=ref["xs"],
                ref_vs=ref["vs"], ref_vs2=ref["vs2"], ref_fs2=ref["fs2"],
-               ref_mv2s=np.array(ref["mv2s"]))
+               ref_mv2s=np.array(ref["mv2s"]), zshift=np.array(zshift),
+               ref_xr=ref["xr"], ref_vr=ref["vr"], ref_xr2=ref["xr2"], ref_vr2=ref["vr2"])
     # links only for occupied slots (compact)
     out["ref_link"] = np.concatenate([ref["link"][c, :ref["npt"][c]] for c in range(len(ref["npt"]))])
     if s.n <= 400:

@@ -196,3 +196,26 @@ def test_simox_md_berendsen_and_rdf(case, tmp_path):
     np.testing.assert_allclose(got[:, 0], dr * (np.arange(1, 201) - 0.5), rtol=1e-3)
     np.testing.assert_allclose(got[:, 1:], (res / cst).T, rtol=1e-3, atol=1e-300)
     assert got[:, 1].max() > 0
+
+
+def test_simox_md_oxidation(case, tmp_path):
+    """OXIDATION run (main.f90:62-86): with q_crit above any charge every
+    charge step inserts a dimer (OXIDE_CHECK, NEW_O2), so the atom count grows
+    by 2 at steps 1, 6 and 11; the reflecting wall keeps every atom at
+    z <= box_z / 2; the run stays finite."""
+    s, f, q, _ = case
+    d = str(tmp_path)
+    write_inputs(d, s, f, q)
+    cfg = open(os.path.join(d, "config.prm")).read()
+    cfg = cfg.replace("REGULAR", "OXIDATION").replace("# z_new N_o2\n0.0 0\n# q_crit\n0.0\n",
+                                                       "# z_new N_o2\n12.0 1\n# q_crit\n10.0\n")
+    open(os.path.join(d, "config.prm"), "w").write(cfg)
+    run(SIMOX_MD, d)
+    rows = load_status(os.path.join(d, "temporal_status_full.dat"))
+    assert rows.shape == (NSTEPS, 7) and np.all(np.isfinite(rows))
+    lines = open(os.path.join(d, "snapshot001.xyz")).read().splitlines()
+    n6 = int(lines[0])
+    assert n6 == s.n + 4                       # dimers added at steps 1 and 6
+    z = np.array([float(l.split()[3]) for l in lines[2:]])
+    assert len(z) == n6 and z.max() <= 0.5 * s.box[2] + 1e-9
+    assert [l.split()[0] for l in lines[-4:]] == ["O"] * 4

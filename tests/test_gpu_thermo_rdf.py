@@ -99,3 +99,24 @@ def test_langevin_relaxes_to_target_and_is_reproducible():
 def test_berendsen_heats_toward_target():
     t = _temps(MODE_FP64, 1, 900.0, 0.0, 1, 200)
     assert t[-20:].mean() > t[:5].mean() + 100.0
+
+
+def test_reflect_and_rigid_motion_match_reference_build(case):
+    """OXIDATION-mode pieces: VV_NVE1_REFLECT (z shifted so the top atoms
+    cross the wall) bit-exact, then REMOVE_RIGID_MOTION (its means are sums
+    over all atoms: 1e-13)."""
+    d = case
+    x = d["x"].copy()
+    x[:, 2] += float(d["zshift"])
+    g = Simox(d["n"], d["box"], d["fft"], d["rcut"], mode=MODE_FAITHFUL)
+    g.set_state(d["ids"], x, d["v"], d["f0"], d["q"])
+    g.set_reflect(True)
+    g.vv1(d["dt"])
+    xr, vr, _, _ = g.get_state()
+    np.testing.assert_array_equal(xr, d["ref_xr"])
+    np.testing.assert_array_equal(vr, d["ref_vr"])
+    assert (vr[:, 2] < 0).sum() >= 1
+    g.remove_rigid_motion()
+    x2, v2, _, _ = g.get_state()
+    np.testing.assert_allclose(x2, d["ref_xr2"], rtol=1e-13, atol=1e-12)
+    np.testing.assert_allclose(v2, d["ref_vr2"], rtol=1e-13, atol=1e-15)
