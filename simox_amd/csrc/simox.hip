@@ -995,14 +995,19 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
       CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
       hipEvent_t ev{};
       tstart(ctx, 6, &ev);
-      if (f32)
+      if (f32) {
+        hipLaunchKernelGGL(k_phi_gather<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                           ctx->greal32, ctx->c_pme, ctx->phi);
         hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->greal32,
-                           ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
-      else
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
+                           ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
+      } else {
+        hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                           ctx->greal64, ctx->c_pme, ctx->phi);
         hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->greal64,
-                           ctx->c_pme, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
+                           ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+      }
       tstop(ctx, 6, ev);
       hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
                          ctx->dscal + 22, ls);

@@ -13,9 +13,9 @@ namespace sx {
 // part[3*block] = sum g, part[3*block+1] = -sum g.p (Brent's f, ctip.f90:236),
 // part[3*block+2] = sum g^2.
 // sum_k val[k n + i] vec[col[k n + i]] over row i of the ELL matrix
-// (column-major, k < c), accumulated in Tv in ascending k.  Eight columns and
-// values are loaded before their vector entries, so each thread keeps several
-// gathers in flight.
+// (column-major, k < c), accumulated in Tv in ascending k.  Sixteen columns
+// and values are loaded before their vector entries, so each thread keeps
+// many gathers in flight.
 // scalar steps of the device-side CG (defined with the line-search kernels)
 __device__ inline void ls_init(const double* red, double* ls);
 __device__ inline void ls_update(const double* red, double* ls);
@@ -24,7 +24,7 @@ __device__ inline void beta_update(const double* red, double* ls);
 template <class Tv, class Tp>
 __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
                              const Tv* __restrict__ val, const Tp* __restrict__ vec) {
-  constexpr int U = 8;
+  constexpr int U = 16;
   Tv s = Tv(0);
   int k = 0;
   for (; k + U <= c; k += U) {
@@ -601,6 +601,17 @@ __device__ inline double pen_d1(const SpecCoef& sc, double q) {
   return 2.0 * WPEN * (lo + hi);
 }
 
+// Potential at the atoms from the real grid: phi_i = c_pme sum w Qfin
+// (PHI_MESH's gather, ctip.f90:381-393), as its own pass so that the matvec
+// below keeps few registers and many waves in flight.
+template <class Rg>
+__global__ void __launch_bounds__(BLOCK) k_phi_gather(Geom g, const double* __restrict__ x,
+                                                      const Rg* __restrict__ grid, double c_pme,
+                                                      double* __restrict__ phi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.n) phi[i] = c_pme * gather_phi(g, x + 3 * i, grid);
+}
+
 // y = H0 p (no penalty term); part[3k..] = p.y, p.gr, sum p^2 pen''(q).
 // The sparse part reads pt, the direction rounded to the operator's type
 // (p itself in FP64, the FP32 copy in MIXED: half the gathered bytes).
@@ -612,7 +623,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
                                                     const double* __restrict__ p,
                                                     const Tv* __restrict__ pt,
                                                     const double* __restrict__ gr,
-                                                    const Rg* __restrict__ grid, double c_pme,
+                                                    const double* __restrict__ phi_in,
                                                     const int* __restrict__ cnt,
                                                     const int* __restrict__ col,
                                                     const Tv* __restrict__ val,
@@ -622,7 +633,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
-    const double phi = c_pme * gather_phi(g, x + 3 * i, grid);
+    const double phi = phi_in[i];
     const double pi = p[i];
     const int c = cnt[i];
     const size_t n = size_t(g.n);
