@@ -125,6 +125,15 @@ int simox_set_rdf(simox_ctx* ctx, int on);
    z = box_z/2).  simox_remove_rigid_motion = REMOVE_RIGID_MOTION
    (vverlet.f90:211-234): subtracts the mean velocity and the mean z. */
 int simox_set_reflect(simox_ctx* ctx, int on);
+/* Build-only parameter gradients (the reference has none): at the converged
+   charges of the last simox_ctip_solve (modes FP64, MIXED), the derivatives of
+   E = Eself + Edir + Erec + Ectip (the energies the step reports) with respect
+   to chi_s and J_s of the five species, through the charge equilibration by
+   an adjoint solve (H lambda = P dE/dq, sum lambda = 0, FP64 PCG to an RMS
+   residual of 1e-12):  grad[s] = dE/dchi_{s+1}, grad[5+s] = dE/dJ_{s+1}.
+   lambda (caller order, may be NULL) receives the adjoint; *iters its CG
+   iterations. */
+int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters);
 int simox_remove_rigid_motion(simox_ctx* ctx);
 int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x

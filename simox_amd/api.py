@@ -24,7 +24,7 @@ EXPORTS = [
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
-    "simox_remove_rigid_motion",
+    "simox_remove_rigid_motion", "simox_param_grad",
 ]
 
 # simox_kernel_timing categories
@@ -87,6 +87,7 @@ def load_library(path=LIB_PATH):
     lib.simox_set_thermostat.argtypes = [vp, i, d, d, C.c_ulonglong]
     lib.simox_set_rdf.argtypes = [vp, i]
     lib.simox_set_reflect.argtypes = [vp, i]
+    lib.simox_param_grad.argtypes = [vp, P(d), vp, P(i)]
     lib.simox_remove_rigid_motion.argtypes = [vp]
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
@@ -184,6 +185,18 @@ class Simox:
         """0 NVE, 1 Berendsen (VV_NVT2_BEREND), 2 Langevin (VV_NVT1/2_STOCH)."""
         self._check(self.lib.simox_set_thermostat(self.h, int(kind), float(t_kelvin),
                                                   float(alpha), int(seed)))
+
+    def param_grad(self, want_lambda=False):
+        """dE/dchi (5), dE/dJ (5) of E = Eself + Edir + Erec + Ectip at the
+        converged charges, through the charge equilibration (adjoint solve)."""
+        g = (C.c_double * 10)()
+        lam = np.zeros(self.n) if want_lambda else None
+        it = C.c_int()
+        self._check(self.lib.simox_param_grad(self.h, g, _ptr(lam), C.byref(it)))
+        out = dict(dchi=np.array(g[0:5]), dJ=np.array(g[5:10]), iters=it.value)
+        if want_lambda:
+            out["lambda"] = lam
+        return out
 
     def set_reflect(self, on=True):
         """OXIDATION mode first half: VV_NVE1_REFLECT instead of VV_NVE1."""

@@ -1,0 +1,93 @@
+"""CTIP parameter gradients as a torch.autograd.Function, with a thin
+gradcheck / fitting harness (build-only; the reference has no derivatives).
+
+E(theta) = Eself + Edir + Erec + Ectip of the MD step at the charges the
+converged solve returns, theta = (chi_1..5, J_1..5) for Al, Ni, Fe, O, H.
+The backward pass uses the library's adjoint solve through the charge
+equilibration (simox_param_grad); torch is only the autograd plumbing here.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .api import MODE_FP64, Simox, potential_setup
+
+
+def params_with(theta, base=None):
+    """A copy of the potential parameters with chi, J replaced by theta."""
+    p = potential_setup() if base is None else _copy(base)
+    t = np.asarray(theta, dtype=np.float64)
+    for s in range(5):
+        p.prm[s].chi = float(t[s])
+        p.prm[s].J = float(t[5 + s])
+    return p
+
+
+def theta_of(params):
+    return np.array([params.prm[s].chi for s in range(5)] + [params.prm[s].J for s in range(5)])
+
+
+def _copy(p):
+    q = type(p)()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(p))
+    return q
+
+
+def solve_energy(system, theta, mode=MODE_FP64, tol=1e-12, q0=None):
+    """(E, context) at theta: charges solved from q0 (default 0) to tol."""
+    s = system
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode, params=params_with(theta))
+    g.set_charge_tol(tol, 20000)
+    g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n) if q0 is None else q0)
+    r = g.ctip_solve()
+    e = g.force()
+    erec = g.pme()
+    return r["Eself"] + e["Edir"] + erec + e["Ectip"], g
+
+
+class CtipEnergy(torch.autograd.Function):
+    """theta (float64 tensor [10]) -> E (scalar tensor) for a fixed system."""
+
+    @staticmethod
+    def forward(ctx, theta, system, mode=MODE_FP64, tol=1e-12):
+        E, g = solve_energy(system, theta.detach().cpu().numpy(), mode, tol)
+        gr = g.param_grad()
+        g.close()
+        ctx.save_for_backward(torch.tensor(np.concatenate([gr["dchi"], gr["dJ"]]),
+                                           dtype=theta.dtype))
+        return torch.tensor(E, dtype=theta.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return grad_out * grad, None, None, None
+
+
+def gradcheck(system, theta=None, eps=1e-5, atol=1e-6, rtol=1e-4):
+    """torch.autograd.gradcheck of CtipEnergy (central differences of E)."""
+    t0 = theta_of(potential_setup()) if theta is None else np.asarray(theta)
+    t = torch.tensor(t0, dtype=torch.float64, requires_grad=True)
+    return torch.autograd.gradcheck(lambda x: CtipEnergy.apply(x, system), (t,), eps=eps,
+                                    atol=atol, rtol=rtol)
+
+
+def fit(system, target, theta=None, mask=None, steps=10):
+    """Gauss-Newton fit of E(theta) to a target energy over the entries of
+    theta selected by mask (the gradient comes from CtipEnergy's backward);
+    returns (theta, history of (E - target)^2)."""
+    t0 = theta_of(potential_setup()) if theta is None else np.asarray(theta, dtype=np.float64)
+    m = torch.ones(10, dtype=torch.float64) if mask is None else torch.tensor(mask, dtype=torch.float64)
+    t = torch.tensor(t0, dtype=torch.float64, requires_grad=True)
+    hist = []
+    for _ in range(steps):
+        resid = CtipEnergy.apply(t, system) - target
+        resid.backward()
+        with torch.no_grad():
+            g = m * t.grad
+            gg = float(g @ g)
+            if gg > 0.0:
+                t -= float(resid) * g / gg
+            t.grad.zero_()
+        hist.append(float(resid.detach()) ** 2)
+    return t.detach().numpy(), hist

@@ -1,0 +1,186 @@
+// Parameter gradients of the CTIP energy through the charge equilibration
+// (build-only; the reference has no derivatives).
+//
+// E(theta) = Eself + Edir + Erec + Ectip at the converged charges q*(theta),
+// theta = (chi_s, J_s) per species.  q* satisfies G(q*, theta) = mu 1,
+// sum q* = Q, with G = dU/dq the solver's gradient (ctip.f90:307-333).  With
+// the adjoint lambda (sum lambda = 0) of H lambda = P dE/dq (H = dG/dq on the
+// active set, P the projection on sum = 0):
+//   dE/dtheta = dE/dtheta|_q - lambda . dG/dtheta,
+//   dG_i/dchi_s = [s_i = s],  dG_i/dJ_s = q_i [s_i = s].
+// dE/dq follows the energies the MD step reports: Eself of CTIP_SOLVER's end
+// (ctip.f90:194-209: chi q + J' q^2 / 2 - kc a / sqrt(pi) q^2 + w pen), Edir
+// and Ectip of FORCE (force.f90:94-140, with its same-species Slater form, Q2)
+// and Erec of PME, whose q-gradient is the potential phi.
+#pragma once
+#include "common.hpp"
+#include "pair.hpp"
+#include "solve.hpp"
+
+namespace sx {
+
+// d/dq_i of Ectip / kc for the pair (i, j) of FORCE's CTIP terms
+// (derivative of e = q_i Z_j (jfi - fifj) + q_j Z_i (ifj - fifj) + q_i q_j fifj).
+__device__ inline double ctip_energy_dqi(const PairCoef& pc, double r, double r2, double qj) {
+  const double xi_i = pc.xi_i, xi_j = pc.xi_j, Zj = pc.Z_j;
+  double jfi, fifj;
+  const double ir = 1.0 / r;
+  if (pc.same) {
+    const double ei = exp(-2.0 * xi_i * r);
+    jfi = -ei * (xi_i + ir);
+    fifj = -ei * (1.0 + pc.c[1] * r + pc.c[2] * r2 + pc.c[3] * r2 * r);
+  } else {
+    const double ei = exp(-2.0 * xi_i * r), ej = exp(-2.0 * xi_j * r);
+    jfi = -ei * (xi_i + ir);
+    fifj = -pc.c[0] * ei - pc.c[1] * ej - pc.c[2] * ei / r - pc.c[3] * ej / r;
+  }
+  return Zj * (jfi - fifj) + qj * fifj;
+}
+
+// out_i = dE/dq_i with phi_i (PME potential of q, c_pme scaled) given.
+__global__ void __launch_bounds__(BLOCK) k_denergy_dq(Geom g, const Tables* __restrict__ tb,
+                                                      const double* __restrict__ x,
+                                                      const int* __restrict__ id,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ phi,
+                                                      const int* __restrict__ cnt,
+                                                      const int* __restrict__ col,
+                                                      double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int si = id[i];
+  const double qi = q[i];
+  const size_t n = size_t(g.n);
+  double s = 0.0;
+  for (int k = 0; k < cnt[i]; ++k) {
+    const int j = col[size_t(k) * n + i];
+    double rx[3], r2;
+    pair_sep(g, x, i, j, rx, r2);
+    const PairCoef& pc = tb->pc[pidx(si, id[j])];
+    const double r = sqrt(r2), qj = q[j];
+    s += qj * erfc(g.alpha * r) / r + ctip_energy_dqi(pc, r, r2, qj);
+  }
+  const SpecCoef& sc = tb->sc[si - 1];
+  const double self = sc.chi + sc.Jp * qi - 2.0 * (KC * g.alpha / SQRTPI) * qi + pen_d1(sc, qi);
+  out[i] = KC * s + self + phi[i];
+}
+
+// y += 2w p on the active set (the penalty part of H); part = p.y partials
+__global__ void __launch_bounds__(BLOCK) k_add_penalty(int n, const Tables* __restrict__ tb,
+                                                       const int* __restrict__ id,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ p,
+                                                       double* __restrict__ y,
+                                                       double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    double yi = y[i];
+    if (outside(sc, q[i])) yi += 2.0 * WPEN * p[i];
+    y[i] = yi;
+    acc[0] = p[i] * yi;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// part[3k..] = sum v, sum m, sum m v with m = 1 / diag_m (Jacobi weights)
+__global__ void __launch_bounds__(BLOCK) k_adj_sums(int n, const Tables* __restrict__ tb,
+                                                    const int* __restrict__ id,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const double m = 1.0 / diag_m(tb->sc[id[i] - 1], q[i]);
+    acc[0] = v[i]; acc[1] = m; acc[2] = m * v[i];
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = acc[k];
+}
+
+// r = v - mean(v) (projection on sum = 0), r written in place of v
+__global__ void k_adj_center(int n, const double* __restrict__ sums, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] -= sums[0] / double(n);
+}
+
+// z = M^-1 r - M^-1 1 (1' M^-1 r / 1' M^-1 1) (sum z = 0); part = r.z partials
+__global__ void __launch_bounds__(BLOCK) k_adj_prec(int n, const Tables* __restrict__ tb,
+                                                    const int* __restrict__ id,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ r,
+                                                    const double* __restrict__ sums,
+                                                    double* __restrict__ z,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < n) {
+    const double m = 1.0 / diag_m(tb->sc[id[i] - 1], q[i]);
+    const double zi = m * r[i] - m * (sums[2] / sums[1]);
+    z[i] = zi;
+    acc[0] = r[i] * zi;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// lambda += a p; r -= a y; part[2k..] = r.r, sum r partials
+__global__ void __launch_bounds__(BLOCK) k_adj_update(int n, double a, const double* __restrict__ p,
+                                                      const double* __restrict__ y,
+                                                      double* __restrict__ lam,
+                                                      double* __restrict__ r,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < n) {
+    lam[i] += a * p[i];
+    const double ri = r[i] - a * y[i];
+    r[i] = ri;
+    acc[0] = ri * ri;
+    acc[1] = ri;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+// p = z + b p
+__global__ void k_adj_dir(int n, double b, const double* __restrict__ z, double* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + b * p[i];
+}
+
+// part[10k + s] = sum over species s+1 of (q - lambda) (dE/dchi),
+// part[10k + 5 + s] = sum of (q^2 / 2 - lambda q) (dE/dJ)
+__global__ void __launch_bounds__(BLOCK) k_param_grad(int n, const int* __restrict__ id,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ lam,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[10 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const int s = id[i] - 1;
+    const double qi = q[i], li = lam[i];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      acc[t] = (t == s) ? qi - li : 0.0;
+      acc[5 + t] = (t == s) ? 0.5 * qi * qi - li * qi : 0.0;
+    }
+  }
+  block_sum<10>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int t = 0; t < 10; ++t) part[10 * blockIdx.x + t] = acc[t];
+}
+
+}  // namespace sx

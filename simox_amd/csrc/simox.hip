@@ -17,6 +17,7 @@
 #include "pair.hpp"
 #include "pme.hpp"
 #include "solve.hpp"
+#include "adjoint.hpp"
 
 using namespace sx;
 
@@ -121,6 +122,7 @@ struct simox_ctx {
   unsigned long long seed = 1, nvt_step = 0;
   // pair histogram of FORCE (RDF), accumulated while rdf_on
   bool rdf_on = false;
+  double* adj = nullptr;  // adjoint work vectors b, lambda, r, z, p, y (6 n)
   unsigned long long* rdf = nullptr;
   double rdf_dr = 0.0;  // Rcut / Nrdf (main.f90:254)
   int last_inner = 0, last_outer = 0;
@@ -1411,7 +1413,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
                   ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
-                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf,
+                  ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1521,6 +1523,102 @@ int simox_get_state(simox_ctx* ctx, double* x, double* v, double* f, double* q) 
   if (v) CHK(download_caller(ctx, ctx->v, 3, v));
   if (f) CHK(download_caller(ctx, ctx->f, 3, f));
   if (q) CHK(download_caller(ctx, ctx->q, 1, q));
+  return SIMOX_OK;
+}
+
+// Adjoint of the converged charge solve: H lam = P b on sum(lam) = 0 (H the
+// solver's Hessian on the active set at ctx->q), Jacobi-preconditioned
+// projected CG in FP64; b = adj[0:n], lam = adj[n:2n].
+int adjoint_solve(simox_ctx* ctx, double tol, int maxit, int* iters) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  CHK(ensure_part(ctx, size_t(nb) * 10));
+  double* b = ctx->adj;
+  double* lam = b + n;
+  double* r = b + 2 * size_t(n);
+  double* z = b + 3 * size_t(n);
+  double* p = b + 4 * size_t(n);
+  double* y = b + 5 * size_t(n);
+  double s[3];
+  auto project = [&](double* v) -> int {
+    hipLaunchKernelGGL(k_adj_sums, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->q,
+                       v, ctx->part);
+    CHK(reduce(ctx, nb, 3, 0, 28));
+    hipLaunchKernelGGL(k_adj_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 28, v);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  auto precond = [&](double* rz) -> int {  // z = P_M M^-1 r, *rz = r.z
+    hipLaunchKernelGGL(k_adj_sums, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->q,
+                       r, ctx->part);
+    CHK(reduce(ctx, nb, 3, 0, 28));
+    hipLaunchKernelGGL(k_adj_prec, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->q,
+                       r, ctx->dscal + 28, z, ctx->part);
+    CHK(reduce(ctx, nb, 1, 0, 31));
+    return fetch(ctx, 31, 1, rz);
+  };
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, b, r);
+  hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, 0.0, lam);
+  CHK(project(r));
+  double rz = 0.0;
+  CHK(precond(&rz));
+  hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, z, p);
+  int it = 0;
+  for (; it < maxit; ++it) {
+    // y = P H p
+    CHK(pme_eval(ctx, p, nullptr, 0.0, false));
+    hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                       ctx->greal64, ctx->c_pme, ctx->phi);
+    hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->q, p, p, p, ctx->phi, ctx->nb_cnt,
+                       ctx->nb_col, ctx->nb_val64, y, ctx->part);
+    hipLaunchKernelGGL(k_add_penalty, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id, ctx->q,
+                       p, y, ctx->part);
+    CHK(reduce(ctx, nb, 1, 0, 32));
+    CHK(fetch(ctx, 32, 1, s));
+    if (!(s[0] > 0.0)) break;
+    CHK(project(y));
+    const double a = rz / s[0];
+    hipLaunchKernelGGL(k_adj_update, dim3(nb), dim3(BLOCK), 0, ctx->st, n, a, p, y, lam, r, ctx->part);
+    CHK(reduce(ctx, nb, 2, 0, 33));
+    CHK(fetch(ctx, 33, 1, s));
+    if (std::sqrt(s[0] / n) <= tol) { ++it; break; }
+    double rz_new = 0.0;
+    CHK(precond(&rz_new));
+    hipLaunchKernelGGL(k_adj_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, rz_new / rz, z, p);
+    HIPCHK(hipGetLastError());
+    rz = rz_new;
+  }
+  if (iters) *iters = it;
+  if (it >= maxit) {
+    ctx->err = "adjoint charge solve did not reach the requested tolerance";
+    return SIMOX_ERR_NOT_CONVERGED;
+  }
+  return 0;
+}
+
+int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters) {
+  if (!ctx || !need_state(ctx) || !grad) return SIMOX_ERR_ARG;
+  if (ctx->mode == SIMOX_MODE_FAITHFUL) {
+    ctx->err = "parameter gradients need a converged charge solve (modes FP64, MIXED)";
+    return SIMOX_ERR_ARG;
+  }
+  const int n = ctx->n, nb = grid_blocks(n);
+  if (!ctx->adj) CHK(dalloc(ctx, &ctx->adj, 6 * size_t(n)));
+  if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  // b = dE/dq at the current charges (PME potential with exact weights)
+  CHK(pme_eval(ctx, ctx->q, nullptr, 0.0, false));
+  hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                     ctx->greal64, ctx->c_pme, ctx->phi);
+  hipLaunchKernelGGL(k_denergy_dq, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                     ctx->id, ctx->q, ctx->phi, ctx->nb_cnt, ctx->nb_col, ctx->adj);
+  HIPCHK(hipGetLastError());
+  CHK(adjoint_solve(ctx, 1e-12, 20000, iters));
+  CHK(ensure_part(ctx, size_t(nb) * 10));
+  hipLaunchKernelGGL(k_param_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->id, ctx->q,
+                     ctx->adj + n, ctx->part);
+  CHK(reduce(ctx, nb, 10, 0, 50));
+  CHK(fetch(ctx, 50, 10, grad));
+  if (lambda) CHK(download_caller(ctx, ctx->adj + n, 1, lambda));
   return SIMOX_OK;
 }
 
