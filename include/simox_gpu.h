@@ -211,6 +211,16 @@ int simox_vv2(simox_ctx* ctx, double dt, double* mv2);
 int simox_step(simox_ctx* ctx, int nsteps, double dt, int sort_every, int ctip_every,
                double* stats);
 
+/* Batched replicas (build-only): nrep independent contexts (each its own box,
+   grid, atoms and HIP stream) advanced by the same simox_step sequence in
+   lockstep.  The converged charge solves of all replicas proceed together:
+   the host waits on one replica's stream while the others' work runs, and
+   each inner iteration is replayed from a captured HIP graph.  Results equal
+   simox_step on each context.  stats (may be NULL): 8 doubles per replica as
+   simox_step.  FAITHFUL contexts are stepped one after another. */
+int simox_batch_step(simox_ctx** ctxs, int nrep, int nsteps, double dt, int sort_every,
+                     int ctip_every, double* stats);
+
 /* Per-kernel timing (HIP events recorded on the context stream around each
    launch while enabled): which = 0 spread, 1 FFT x R2C, 2 FFT y (both
    directions), 3 FFT z + influence multiply, 4 FFT x C2R, 5 PME force gather,

@@ -7,8 +7,8 @@ VV_NVE1, VV_NVE2) and raise SimoxError with the reference's STOP text where
 the reference would stop.  There is no CPU fallback: without the built
 library every call fails loudly.
 """
-from .api import (MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox, SimoxError, load_library,
-                  potential_setup)
+from .api import (MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox, SimoxError, batch_step,
+                  load_library, potential_setup)
 
-__all__ = ["Simox", "SimoxError", "load_library", "potential_setup", "MODE_FAITHFUL",
-           "MODE_FP64", "MODE_MIXED"]
+__all__ = ["Simox", "SimoxError", "batch_step", "load_library", "potential_setup",
+           "MODE_FAITHFUL", "MODE_FP64", "MODE_MIXED"]

@@ -24,7 +24,7 @@ EXPORTS = [
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
-    "simox_remove_rigid_motion", "simox_param_grad",
+    "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
 ]
 
 # simox_kernel_timing categories
@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
     lib.simox_set_rdf.argtypes = [vp, i]
     lib.simox_set_reflect.argtypes = [vp, i]
     lib.simox_param_grad.argtypes = [vp, P(d), vp, P(i)]
+    lib.simox_batch_step.argtypes = [P(vp), i, i, d, i, i, vp]
     lib.simox_remove_rigid_motion.argtypes = [vp]
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
@@ -348,3 +349,24 @@ class Simox:
         cnt = C.c_longlong()
         self._check(self.lib.simox_kernel_timing(self.h, int(which), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+def batch_step(sims, nsteps, dt, sort_every=1, ctip_every=1, stats=True):
+    """simox_batch_step: the same steps on every Simox of sims in lockstep
+    (independent systems; each keeps its own box, grid and stream).  Returns
+    a list of stats dicts (as Simox.step) or None."""
+    lib = load_library()
+    arr = (C.c_void_p * len(sims))(*[s.h.value for s in sims])
+    st = np.zeros(8 * len(sims)) if stats else None
+    rc = lib.simox_batch_step(arr, len(sims), int(nsteps), float(dt), int(sort_every),
+                              int(ctip_every), _ptr(st))
+    if rc:
+        for s in sims:  # report the replica that holds the message
+            txt = lib.simox_last_error(s.h)
+            if txt:
+                raise SimoxError(rc, txt.decode())
+        raise SimoxError(rc, "batch step failed")
+    if st is None:
+        return None
+    keys = ["Eself", "Edir", "Erec", "Ectip", "Epair", "Egroup", "mv2", "Etotal"]
+    return [dict(zip(keys, st[8 * k:8 * k + 8].tolist())) for k in range(len(sims))]

@@ -52,6 +52,15 @@ struct TimedLaunch { int which; hipEvent_t a, b; };
 
 }  // namespace
 
+// Progress of a converged charge solve between host decisions.
+enum { NCG_IDLE = 0, NCG_WAIT_RESTART = 1, NCG_WAIT_ITER = 2, NCG_DONE = 3 };
+struct NcgRun {
+  int phase = NCG_IDLE;
+  int total_it = 0, nev = 0, restarts = 0, it = 0;
+  double rms = 0.0, target = 0.0;
+  bool ok = false;
+};
+
 // Launch shapes of the FFT passes for one precision: mx real x-lines per
 // R2C / C2R workgroup, my kz-columns per y workgroup, mz z-lines per fused z
 // workgroup, and their dynamic LDS bytes.
@@ -115,6 +124,11 @@ struct simox_ctx {
   double* qtmp = nullptr;
   int nhist = 0;
   int pred_order = 1;
+  NcgRun ncg;
+  // captured inner iteration of the charge solve (batched stepping)
+  bool use_graph = false;
+  hipGraphExec_t iter_graph = nullptr;
+  int part_gen = 0, iter_graph_gen = -1;
   // thermostat of the step (0 NVE, 1 Berendsen, 2 Langevin), target kB*T in eV
   int thermo = 0;
   bool reflect = false;  // OXIDATION-mode first half (VV_NVE1_REFLECT)
@@ -277,6 +291,7 @@ int ensure_part(simox_ctx* ctx, size_t count) {
   if (count <= ctx->part_cap) return 0;
   if (ctx->part) HIPCHK(hipFree(ctx->part));
   ctx->part_cap = count;
+  ++ctx->part_gen;  // a captured graph holding the old buffer is stale
   return dalloc(ctx, &ctx->part, count);
 }
 
@@ -953,9 +968,106 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32, double* ls = nullp
 // gradient recurrence in FP64.  Restarts recompute the true gradient with the
 // FP64 operator (iterative refinement); the solve ends when the RMS of the
 // projected true gradient (eV/e) is below ctx->tol.
-int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
+//
+// The solve is a state machine with one host decision per restart or
+// iteration (ncg_begin, then ncg_check after the context's stream has
+// drained), so that several contexts can be advanced in turn
+// (simox_batch_step) while the GPU works on the others.
+
+// One inner iteration: H0 p, exact line search, step, new direction.
+int ncg_iter_body(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
   const bool f32 = use32(ctx);
+  float* p32 = f32 ? ctx->pv32 : nullptr;
+  double* ls = ctx->dscal + 40;
+  // v = H0 p with p.v, p.g, sum p^2 pen''(q)
+  CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
+  hipEvent_t ev{};
+  tstart(ctx, 6, &ev);
+  if (f32) {
+    hipLaunchKernelGGL(k_phi_gather<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                       ctx->greal32, ctx->c_pme, ctx->phi);
+    hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
+                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
+  } else {
+    hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                       ctx->greal64, ctx->c_pme, ctx->phi);
+    hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
+                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+  }
+  tstop(ctx, 6, ev);
+  hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
+                     ctx->dscal + 22, ls);
+  // exact line search, Newton with bracketing, on the device (rounds after
+  // convergence return at once)
+  for (int k = 0; k < 8; ++k) {
+    hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                       ctx->qc, ctx->pv, ls, ctx->part);
+    hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
+                       ctx->dscal + 24, ls);
+  }
+  hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
+  CHK(reduce(ctx, nb, 1, 0, 0));
+  hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
+  CHK(reduce(ctx, nb, 3, 0, 16));
+  CHK(apply_prec(ctx, ctx->rv, f32, ls));
+  hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
+                     p32);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Launches one inner iteration: replays the captured graph of the body when
+// ctx->use_graph (batched stepping), else enqueues the kernels.
+int ncg_launch_iter(simox_ctx* ctx) {
+  if (!ctx->use_graph || ctx->timing) return ncg_iter_body(ctx);
+  if (!ctx->iter_graph || ctx->iter_graph_gen != ctx->part_gen) {
+    if (ctx->iter_graph) HIPCHK(hipGraphExecDestroy(ctx->iter_graph));
+    ctx->iter_graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    const int rc = ncg_iter_body(ctx);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->st, &graph);
+    if (rc) return rc;
+    HIPCHK(e);
+    HIPCHK(hipGraphInstantiate(&ctx->iter_graph, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    ctx->iter_graph_gen = ctx->part_gen;
+  }
+  HIPCHK(hipGraphLaunch(ctx->iter_graph, ctx->st));
+  return 0;
+}
+
+int ncg_issue_iter(simox_ctx* ctx) {
+  CHK(ncg_launch_iter(ctx));
+  // the convergence and descent scalars of this iteration, for ncg_check
+  HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->hscal + 48, ctx->dscal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  ctx->ncg.phase = NCG_WAIT_ITER;
+  return 0;
+}
+
+// true gradient (FP64 operator) and its projected RMS
+int ncg_issue_restart(simox_ctx* ctx) {
+  const int n = ctx->n, nb = grid_blocks(n);
+  CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false, false));
+  ++ctx->ncg.nev;
+  hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
+  CHK(reduce(ctx, nb, 3, 0, 16));  // {sum m g, sum m, sum g^2} at 16..18
+  CHK(apply_prec(ctx, nullptr, use32(ctx)));  // {z.g, -} at 19..20
+  HIPCHK(hipMemcpyAsync(ctx->hscal + 16, ctx->dscal + 16, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  ctx->ncg.phase = NCG_WAIT_RESTART;
+  return 0;
+}
+
+int ncg_begin(simox_ctx* ctx) {
+  const int n = ctx->n, nb = grid_blocks(n);
   {
     // polynomial extrapolation in time through q(t), q(t-1), ... of order
     // min(pred_order, nhist); coefficients sum to 1, so sum(q) is kept
@@ -967,93 +1079,17 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
                        c[3], ctx->q, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qc);
   }
   HIPCHK(hipGetLastError());
-  int total_it = 0, nev = 0, restarts = 0;
-  bool ok = false;
-  double s6[6];
+  ctx->ncg = NcgRun{};
+  // the cell sort swaps the atom buffers (x, id, cell, ...): a graph captured
+  // in an earlier solve may name the other buffer of each pair, so capture anew
+  ctx->iter_graph_gen = -1;
   if (ctx->prec_on) CHK(build_prec(ctx));
-  for (;;) {
-    // true gradient (FP64 operator) and its projected RMS
-    CHK(grad_eval(ctx, ctx->qc, nullptr, 0.0, false, false));
-    ++nev;
-    hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                       ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
-    CHK(reduce(ctx, nb, 3, 0, 16));  // {sum m g, sum m, sum g^2} at 16..18
-    CHK(apply_prec(ctx, nullptr, f32));  // {z.g, -} at 19..20
-    CHK(fetch(ctx, 16, 4, s6));
-    const double rms_true = std::sqrt(s6[2] / n);
-    if (rms_true <= ctx->tol) { ok = true; break; }
-    if (restarts >= 60) break;
-    ++restarts;
-    const double target = f32 ? std::max(0.3 * ctx->tol, 1e-4 * rms_true) : 0.3 * ctx->tol;
-    double rz = s6[3];
-    float* p32 = f32 ? ctx->pv32 : nullptr;
-    hipLaunchKernelGGL(k_set_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv, p32);
-    int it = 0;
-    double rms = rms_true;
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, rz, ctx->dscal + 46);  // ls[6]
-    while (rms > target && it < ctx->maxit) {
-      double* ls = ctx->dscal + 40;
-      // v = H0 p with p.v, p.g, sum p^2 pen''(q)
-      CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
-      hipEvent_t ev{};
-      tstart(ctx, 6, &ev);
-      if (f32) {
-        hipLaunchKernelGGL(k_phi_gather<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                           ctx->greal32, ctx->c_pme, ctx->phi);
-        hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
-                           ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
-      } else {
-        hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                           ctx->greal64, ctx->c_pme, ctx->phi);
-        hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                           ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
-                           ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
-      }
-      tstop(ctx, 6, ev);
-      hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
-                         ctx->dscal + 22, ls);
-      // exact line search, Newton with bracketing, on the device (rounds after
-      // convergence return at once)
-      for (int k = 0; k < 8; ++k) {
-        hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                           ctx->qc, ctx->pv, ls, ctx->part);
-        hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
-                           ctx->dscal + 24, ls);
-      }
-      hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                         ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
-      CHK(reduce(ctx, nb, 1, 0, 0));
-      hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                         ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
-      CHK(reduce(ctx, nb, 3, 0, 16));
-      CHK(apply_prec(ctx, ctx->rv, f32, ls));
-      hipLaunchKernelGGL(k_pcg_dir_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
-                         p32);
-      HIPCHK(hipGetLastError());
-      double st2[2];
-      ++it;
-      // one host round trip per iteration: convergence and descent check
-      HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-      HIPCHK(hipMemcpyAsync(ctx->hscal + 48, ctx->dscal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-      HIPCHK(hipStreamSynchronize(ctx->st));
-      st2[0] = ctx->hscal[18];
-      st2[1] = ctx->hscal[48];
-      rms = std::sqrt(st2[0] / n);
-      if (st2[1] != 0.0) break;  // not a descent direction: restart from the true gradient
-    }
-    total_it += it;
-    nev += it;
-  }
-  ctx->last_inner = total_it;
-  ctx->last_outer = restarts;
-  if (iters) *iters = total_it;
-  if (n_eval) *n_eval = nev;
-  if (!ok) {
-    ctx->err = "charge solve did not reach the requested tolerance";
-    return SIMOX_ERR_NOT_CONVERGED;
-  }
-  // history for the next predictor: shift, qhist[0] = charges before this solve
+  return ncg_issue_restart(ctx);
+}
+
+// The accepted charges become the state; shift the predictor history.
+int ncg_finish(simox_ctx* ctx) {
+  const int n = ctx->n, nb = grid_blocks(n);
   std::swap(ctx->qhist[2], ctx->qhist[1]);
   std::swap(ctx->qhist[1], ctx->qhist[0]);
   hipLaunchKernelGGL(k_copy, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->q, ctx->qhist[0]);
@@ -1061,6 +1097,65 @@ int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
   HIPCHK(hipGetLastError());
   ctx->nhist = std::min(ctx->nhist + 1, 3);
   return 0;
+}
+
+// Host decision after the context's stream has drained: read the scalars of
+// the restart or iteration just run and issue the next piece of work.
+int ncg_check(simox_ctx* ctx) {
+  NcgRun& r = ctx->ncg;
+  const int n = ctx->n, nb = grid_blocks(n);
+  if (r.phase == NCG_WAIT_RESTART) {
+    const double rms_true = std::sqrt(ctx->hscal[18] / n);
+    if (rms_true <= ctx->tol) {
+      r.ok = true;
+      r.phase = NCG_DONE;
+      return ncg_finish(ctx);
+    }
+    if (r.restarts >= 60) { r.phase = NCG_DONE; return 0; }
+    ++r.restarts;
+    r.target = use32(ctx) ? std::max(0.3 * ctx->tol, 1e-4 * rms_true) : 0.3 * ctx->tol;
+    float* p32 = use32(ctx) ? ctx->pv32 : nullptr;
+    hipLaunchKernelGGL(k_set_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv, p32);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, ctx->hscal[19],
+                       ctx->dscal + 46);  // ls[6] = rz
+    HIPCHK(hipGetLastError());
+    r.it = 0;
+    r.rms = rms_true;
+    if (r.rms > r.target && r.it < ctx->maxit) return ncg_issue_iter(ctx);
+    return ncg_issue_restart(ctx);
+  }
+  if (r.phase == NCG_WAIT_ITER) {
+    ++r.it;
+    r.rms = std::sqrt(ctx->hscal[18] / n);
+    const bool descent = ctx->hscal[48] == 0.0;  // else restart from the true gradient
+    if (descent && r.rms > r.target && r.it < ctx->maxit) return ncg_issue_iter(ctx);
+    r.total_it += r.it;
+    r.nev += r.it;
+    return ncg_issue_restart(ctx);
+  }
+  return 0;
+}
+
+int ncg_end(simox_ctx* ctx, int* iters, int* n_eval) {
+  ctx->last_inner = ctx->ncg.total_it;
+  ctx->last_outer = ctx->ncg.restarts;
+  if (iters) *iters = ctx->ncg.total_it;
+  if (n_eval) *n_eval = ctx->ncg.nev;
+  ctx->ncg.phase = NCG_IDLE;
+  if (!ctx->ncg.ok) {
+    ctx->err = "charge solve did not reach the requested tolerance";
+    return SIMOX_ERR_NOT_CONVERGED;
+  }
+  return 0;
+}
+
+int ctip_ncg(simox_ctx* ctx, int* iters, int* n_eval) {
+  CHK(ncg_begin(ctx));
+  while (ctx->ncg.phase != NCG_DONE) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    CHK(ncg_check(ctx));
+  }
+  return ncg_end(ctx, iters, n_eval);
 }
 
 int do_ctip(simox_ctx* ctx, int* iters, int* n_eval) {
@@ -1372,7 +1467,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   {
     // partial-sum buffer sized once for every launch that uses it
     const size_t zb = size_t(std::max(ctx->fc64.zblocks, ctx->fc32.zblocks));
-    CHK(ensure_part(ctx, std::max(size_t(grid_blocks(n)) * 4, zb) + 16));
+    CHK(ensure_part(ctx, std::max(size_t(grid_blocks(n)) * 10, zb) + 16));
   }
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->order);
   HIPCHK(hipGetLastError());
@@ -1406,6 +1501,7 @@ int simox_destroy(simox_ctx* ctx) {
   if (!ctx) return SIMOX_OK;
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   tresolve(ctx);
+  if (ctx->iter_graph) (void)hipGraphExecDestroy(ctx->iter_graph);
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
@@ -1824,6 +1920,77 @@ int simox_step(simox_ctx* ctx, int nsteps, double dt, int sort_every, int ctip_e
     stats[0] = ctx->Eself; stats[1] = ctx->Edir; stats[2] = ctx->Erec; stats[3] = ctx->Ectip;
     stats[4] = ctx->Epair; stats[5] = ctx->Egroup; stats[6] = ctx->mv2;
     stats[7] = ctx->Eself + ctx->Edir + ctx->Erec + ctx->Ectip + ctx->Epair + ctx->Egroup + ctx->mv2 / 2.0;
+  }
+  return SIMOX_OK;
+}
+
+int simox_batch_step(simox_ctx** ctxs, int nrep, int nsteps, double dt, int sort_every,
+                     int ctip_every, double* stats) {
+  if (!ctxs || nrep < 0 || nsteps < 0) return SIMOX_ERR_ARG;
+  for (int r = 0; r < nrep; ++r)
+    if (!ctxs[r] || !need_state(ctxs[r])) return SIMOX_ERR_ARG;
+  for (int r = 0; r < nrep; ++r)
+    if (ctxs[r]->mode == SIMOX_MODE_FAITHFUL) {  // host Brent: no interleaving, step in turn
+      for (int k = 0; k < nrep; ++k) {
+        const int rc = simox_step(ctxs[k], nsteps, dt, sort_every, ctip_every,
+                                  stats ? stats + 8 * size_t(k) : nullptr);
+        if (rc) return rc;
+      }
+      return SIMOX_OK;
+    }
+  struct GraphMode {  // graph replay of the inner iteration for this call only
+    simox_ctx** c; int n;
+    ~GraphMode() { for (int r = 0; r < n; ++r) c[r]->use_graph = false; }
+  } gm{ctxs, nrep};
+  for (int r = 0; r < nrep; ++r) ctxs[r]->use_graph = true;
+  for (int s = 0; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    for (int r = 0; r < nrep; ++r) {
+      simox_ctx* ctx = ctxs[r];
+      CHK(do_vv1(ctx, dt));
+      if (sort_every > 0 && s % sort_every == 0) CHK(do_cell_sort(ctx));
+    }
+    if (ctip_every > 0 && s % ctip_every == 0) {
+      for (int r = 0; r < nrep; ++r) {
+        simox_ctx* ctx = ctxs[r];
+        CHK(build_matrix(ctx));
+        CHK(ncg_begin(ctx));
+      }
+      // round robin: wait for one context while the others' work runs
+      for (bool busy = true; busy;) {
+        busy = false;
+        for (int r = 0; r < nrep; ++r) {
+          simox_ctx* ctx = ctxs[r];
+          if (ctx->ncg.phase == NCG_DONE) continue;
+          HIPCHK(hipStreamSynchronize(ctx->st));
+          CHK(ncg_check(ctx));
+          busy = busy || ctx->ncg.phase != NCG_DONE;
+        }
+      }
+      for (int r = 0; r < nrep; ++r) {
+        simox_ctx* ctx = ctxs[r];
+        CHK(ncg_end(ctx, nullptr, nullptr));
+        double e3[3];
+        CHK(self_energy(ctx, ctx->q, 1, WPEN, e3));  // ctip.f90:194-209
+        ctx->Eself = e3[0] * KC * ctx->g.alpha / SQRTPI + (e3[1] + e3[2]);
+      }
+    }
+    for (int r = 0; r < nrep; ++r) {
+      simox_ctx* ctx = ctxs[r];
+      CHK(do_force(ctx, last && stats));
+      CHK(do_pme(ctx, last && stats));
+      CHK(do_vv2(ctx, dt, last && stats));
+    }
+  }
+  for (int r = 0; r < nrep; ++r) {
+    simox_ctx* ctx = ctxs[r];
+    CHK(check_err(ctx, 0));
+    if (stats) {
+      double* st = stats + 8 * size_t(r);
+      st[0] = ctx->Eself; st[1] = ctx->Edir; st[2] = ctx->Erec; st[3] = ctx->Ectip;
+      st[4] = ctx->Epair; st[5] = ctx->Egroup; st[6] = ctx->mv2;
+      st[7] = st[0] + st[1] + st[2] + st[3] + st[4] + st[5] + st[6] / 2.0;
+    }
   }
   return SIMOX_OK;
 }
