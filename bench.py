@@ -51,6 +51,8 @@ WORKLOADS = {
     "c1": "C1: Al slab 6x6x8 cells (1,152) + 64 O, PME 32x32x96, Rcut 8 A, 300 K",
     "c2": "C2: Al slab 24x24x32 cells (73,728) + 4,096 O, PME 96x96x192, Rcut 10 A, 600 K",
     "c3": "C3: Al slab 64x64x60 cells (983,040) + 65,536 O, PME 256x256x512, Rcut 10 A, 900 K",
+    "c4": "C4: 256 independent Al/O slabs, 4x4x6 to 10x10x12 cells (410-4,904 atoms), "
+          "0.1-1.0 ML O, 300-1200 K, per-replica PME grid, batched lockstep steps",
 }
 
 
@@ -113,11 +115,15 @@ def cpu_baseline(seconds=15.0):
                       "CTIP from q=0 then warm, FORCE, PME, Verlet) of C1 (1,216 atoms)" % k}
 
 
-def run_mode(s, mode_name, args, timed=True):
+def run_mode(s, mode_name, args, timed=True, precond=None):
+    """precond None: the mode's default preconditioner (Jacobi in FP64, the
+    plain method; long-range Woodbury in MIXED)."""
     from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox
     mode_id = {"faithful": MODE_FAITHFUL, "fp64": MODE_FP64, "mixed": MODE_MIXED}[mode_name]
     g = Simox.from_system(s, mode=mode_id)
     g.set_charge_tol(args.tol, 5000)
+    if precond is not None:
+        g.set_charge_precond(precond)
     # initial forces at x(0) with q = 0 (the first step re-equilibrates charges)
     g.force()
     g.pme()
@@ -132,6 +138,85 @@ def run_mode(s, mode_name, args, timed=True):
     times = {w: g.kernel_time(w) for w in KERNELS}
     g.timing(False)
     return g, st, elapsed, times
+
+
+def bench_replicas(args):
+    """Aggregate atom-steps/s of the batched replicas (simox_batch_step, mixed,
+    solver iterations replayed from captured HIP graphs) next to the same
+    contexts stepped one after another with simox_step."""
+    from simox_amd import MODE_MIXED, Simox, batch_step
+    from simox_amd.systems import replicas
+    rs = replicas(args.replicas)
+    n_tot = sum(s.n for s in rs)
+    print("bench: creating %d replica contexts (%d atoms)" % (len(rs), n_tot), file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    sims = []
+    for r, s in enumerate(rs):
+        g = Simox.from_system(s, mode=MODE_MIXED)
+        g.set_charge_tol(args.tol, 5000)
+        g.force()
+        g.pme()
+        sims.append(g)
+        if (r + 1) % 64 == 0:
+            print("bench: %d contexts ready" % (r + 1), file=sys.stderr, flush=True)
+    t_create = time.perf_counter() - t0
+    dt = rs[0].dt
+    print("bench: batched warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
+    if args.warmup:
+        batch_step(sims, args.warmup, dt, 1, 1, stats=False)
+    print("bench: batched timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch_step(sims, args.steps, dt, 1, 1, stats=True)
+    el_b = time.perf_counter() - t0
+    print("bench: looped timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for r, g in enumerate(sims):
+        g.step(args.steps, dt, 1, 1, stats=True)
+        if (r + 1) % 64 == 0:
+            print("bench: looped %d replicas" % (r + 1), file=sys.stderr, flush=True)
+    el_l = time.perf_counter() - t0
+    # kernel durations: one more looped pass with HIP-event timing per context
+    print("bench: looped kernel timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    tot = {w: [0.0, 0, 0.0] for w in KERNELS}  # ms, launches, algorithmic bytes
+    for s, g in zip(rs, sims):
+        g.timing(True)
+        g.step(args.steps, dt, 1, 1, stats=False)
+        cnt, _ = g.neighbors(stride=0)
+        nb = int(cnt.sum())
+        for w in KERNELS:
+            ms, k = g.kernel_time(w)
+            tot[w][0] += ms
+            tot[w][1] += k
+            tot[w][2] += k * algorithmic_bytes(w, s, "mixed", nb)
+        g.timing(False)
+    iters = [g.charge_stats()[0] for g in sims]
+    for g in sims:
+        g.close()
+    dom = max(tot, key=lambda w: tot[w][0])
+    ms, launches, nbytes = tot[dom]
+    achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    value = n_tot * args.steps / el_b
+    looped = n_tot * args.steps / el_l
+    return {
+        "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": el_b / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64 mixed", "data": "synthetic",
+        "config": {"workload": WORKLOADS["c4"], "mode": "mixed", "replicas": len(rs),
+                   "n_atoms_total": n_tot, "n_atoms_min": min(s.n for s in rs),
+                   "n_atoms_max": max(s.n for s in rs), "ctip_every": 1, "sort_every": 1,
+                   "charge_tol": args.tol, "charge_iters_last_step_mean": float(np.mean(iters)),
+                   "context_create_s": t_create},
+        "looped": {"value": looped, "ms_per_step": el_l / args.steps * 1e3},
+        "speedup_vs_looped": value / looped,
+        "roofline": {"bound": "hbm", "kernel": kernel_name(dom, "mixed"), "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": nbytes / max(launches, 1),
+                     "avg_launch_ms": ms / max(launches, 1), "launches": launches,
+                     "timed_in": "looped pass with per-kernel HIP events"},
+        "kernels_ms": {kernel_name(w, "mixed"): round(tot[w][0], 3) for w in KERNELS},
+    }
 
 
 def accuracy_vs_fp64(s, g_mixed, args):
@@ -178,6 +263,7 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="converged charge solve: RMS projected gradient (eV/e)")
+    ap.add_argument("--replicas", type=int, default=256, help="replica count of the c4 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU hot path "
@@ -194,6 +280,15 @@ def main():
         torch.cuda.synchronize()
         del a, b
         torch.cuda.empty_cache()
+    if args.config == "c4":
+        out = bench_replicas(args)
+        if not args.no_cpu:
+            print("bench: cpu baseline", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        else:
+            out["cpu_baseline"] = None
+        print(json.dumps(out))
+        return 0
     from simox_amd.systems import config
     print("bench: building %s inputs" % args.config, file=sys.stderr, flush=True)
     s = config(args.config)
@@ -221,6 +316,7 @@ def main():
                    "neighbor_entries": nb_entries, "E_total_last": st["Etotal"],
                    "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
                    "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
+                   "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
                    "fft_tiles": fft_tiles},
         "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -238,8 +334,17 @@ def main():
         g2.close()
         v2 = s.n * args.steps / el2
         out["fp64_path"] = {"value": v2, "ms_per_step": el2 / args.steps * 1e3,
-                            "charge_iters_last_step": it2}
+                            "charge_iters_last_step": it2, "precond": "jacobi"}
         out["speedup_vs_fp64"] = value / v2
+        # the same FP64 kernels with the mixed path's preconditioner: the part of
+        # the speed-up that comes from precision alone
+        g3, _, el3, _ = run_mode(s, "fp64", args, precond=1)
+        it3, _ = g3.charge_stats()
+        g3.close()
+        v3 = s.n * args.steps / el3
+        out["fp64_precond_path"] = {"value": v3, "ms_per_step": el3 / args.steps * 1e3,
+                                    "charge_iters_last_step": it3, "precond": "long-range"}
+        out["speedup_vs_fp64_precond"] = value / v3
     else:
         g.close()
     if not args.no_cpu:

@@ -148,9 +148,13 @@ int simox_set_mode(simox_ctx* ctx, int mode);
 /* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged
    charge solve (modes FP64, MIXED). */
 int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
-/* Preconditioner of the converged charge solve: 0 = Jacobi (default),
-   1 = Coulomb screening (one extra PME-grid pass per iteration; fewer
-   iterations from cold starts, not for warm-started MD steps). */
+/* Preconditioner of the converged charge solve: 0 = Jacobi (the default in
+   FP64 mode, the plain method), 1 = long-range Woodbury preconditioner (the
+   default in MIXED mode): M^-1 = D^-1 - D^-1 W^T F W D^-1 with W the B-spline
+   spread onto a PME grid of half the resolution and F = Gc / (1 + Gc rho) the
+   Coulomb kernel screened by the density rho of 1/J.  One extra spread, FFT
+   pair and gather on the coarse grid per iteration; about half the iterations
+   for 1M-atom slabs (the Jacobi count grows with the box, this one does not). */
 int simox_set_charge_precond(simox_ctx* ctx, int on);
 /* Iterations and FP64 restarts of the last converged charge solve. */
 int simox_charge_stats(simox_ctx* ctx, int* iters, int* restarts);

@@ -69,6 +69,31 @@ struct FftCfg {
   size_t shx = 0, shy = 0, shz = 0;
 };
 
+// One PME grid: sizes, FFT plans and launch shapes, twiddles, the real grid,
+// the half spectrum, the influence and filter tables (same layout as the
+// spectrum) and the brick lists of the spread (atoms grouped by the 4^3 grid
+// brick of their base point).
+struct PmeGrid {
+  int K[3] = {0, 0, 0}, H = 0;
+  FftPlan pl[3]{};
+  FftCfg fc64{}, fc32{};
+  double2* tw64[3] = {nullptr, nullptr, nullptr};
+  float2* tw32[3] = {nullptr, nullptr, nullptr};
+  double* greal64 = nullptr;
+  float* greal32 = nullptr;
+  double2* spec64 = nullptr;
+  float2* spec32 = nullptr;
+  double* G64 = nullptr;
+  float* G32 = nullptr;
+  double* F64 = nullptr;
+  float* F32 = nullptr;
+  int nbrick = 0;
+  int *brick = nullptr, *bcount = nullptr, *bstart = nullptr, *bfill = nullptr;
+  int *bunord = nullptr, *border = nullptr, *bsum = nullptr, *btotal = nullptr;
+  bool bricks_valid = false;
+  bool ready = false;
+};
+
 struct simox_ctx {
   int n = 0;
   int mode = SIMOX_MODE_FAITHFUL;
@@ -86,28 +111,13 @@ struct simox_ctx {
   int *count = nullptr, *start = nullptr, *fill = nullptr, *slot = nullptr, *order = nullptr;
   double *rho = nullptr, *dF64 = nullptr, *frec = nullptr;
   float* dF32 = nullptr;
-  // PME
-  int nbrick = 0;
-  int *brick = nullptr, *bcount = nullptr, *bstart = nullptr, *bfill = nullptr;
-  int *bunord = nullptr, *border = nullptr, *bsum = nullptr, *btotal = nullptr;
-  bool bricks_valid = false;
-  double* greal64 = nullptr;
-  float* greal32 = nullptr;
-  double2* spec64 = nullptr;
-  float2* spec32 = nullptr;
-  double* G64 = nullptr;
-  float* G32 = nullptr;
-  double* F64 = nullptr;  // preconditioner filter tables (same layout as G)
-  float* F32 = nullptr;
+  // PME: the reference's grid (pm) and the half-resolution grid of the charge
+  // solver's long-range preconditioner (pc, set up on first use)
+  PmeGrid pm, pc;
   bool prec_on = false;
-  bool prec_ready = false;
-  double prec_j0 = 1.0;
+  double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* sv = nullptr;
-  double* cv = nullptr;  // per-atom density scaling of the preconditioner
-  double2* tw64[3] = {nullptr, nullptr, nullptr};
-  float2* tw32[3] = {nullptr, nullptr, nullptr};
-  FftPlan pl[3]{};
-  FftCfg fc64{}, fc32{};
+  double* cv = nullptr;  // 1/sqrt(J) per atom, the preconditioner's fixed scaling
   // CTIP
   int cap = NNB_MAX;
   int* nb_cnt = nullptr;
@@ -407,7 +417,7 @@ int do_cell_sort(simox_ctx* ctx) {
   std::swap(ctx->cell, ctx->cell2);
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
   return 0;
 }
 
@@ -509,33 +519,45 @@ FftSet<C> fft_set(int L) {
   }
 }
 
+// Geometry of the kernels on grid pm: the context's, with pm's sizes.
+Geom geom_of(const simox_ctx* ctx, const PmeGrid& pm) {
+  Geom g = ctx->g;
+  for (int d = 0; d < 3; ++d) g.K[d] = pm.K[d];
+  g.H = pm.H;
+  return g;
+}
+
+// Event timing of the PME passes covers the reference's grid only.
+bool timed_grid(const simox_ctx* ctx, const PmeGrid& pm) { return &pm == &ctx->pm; }
+
 template <class C>
-int pme_chain(simox_ctx* ctx, C* spec, C* const* tw, const FftCfg& fc,
+int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& fc,
               const decltype(C{}.x)* G, decltype(C{}.x)* greal) {
-  const Geom& g = ctx->g;
+  const Geom g = geom_of(ctx, pm);
   const int H = g.H, Ky = g.K[1], Kz = g.K[2];
   const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
+  const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
-  tstart(ctx, 1, &ev);
-  hipLaunchKernelGGL(fx.r2c, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, ctx->pl[0],
+  if (timed) tstart(ctx, 1, &ev);
+  hipLaunchKernelGGL(fx.r2c, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, pm.pl[0],
                      tw[0], greal, spec, fc.mx);
-  tstop(ctx, 1, ev);
-  tstart(ctx, 2, &ev);
-  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, ctx->pl[1], tw[1],
+  if (timed) tstop(ctx, 1, ev);
+  if (timed) tstart(ctx, 2, &ev);
+  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, pm.pl[1], tw[1],
                      spec, fc.my, Ky + 1, -1);
-  tstop(ctx, 2, ev);
-  tstart(ctx, 3, &ev);
-  hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, ctx->pl[2], tw[2],
+  if (timed) tstop(ctx, 2, ev);
+  if (timed) tstart(ctx, 3, &ev);
+  hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, pm.pl[2], tw[2],
                      spec, fc.mz, G, ctx->part);
-  tstop(ctx, 3, ev);
-  tstart(ctx, 2, &ev);
-  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, ctx->pl[1], tw[1],
+  if (timed) tstop(ctx, 3, ev);
+  if (timed) tstart(ctx, 2, &ev);
+  hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, pm.pl[1], tw[1],
                      spec, fc.my, Ky + 1, +1);
-  tstop(ctx, 2, ev);
-  tstart(ctx, 4, &ev);
-  hipLaunchKernelGGL(fx.c2r, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, ctx->pl[0],
+  if (timed) tstop(ctx, 2, ev);
+  if (timed) tstart(ctx, 4, &ev);
+  hipLaunchKernelGGL(fx.c2r, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, pm.pl[0],
                      tw[0], spec, greal, fc.mx);
-  tstop(ctx, 4, ev);
+  if (timed) tstop(ctx, 4, ev);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -625,9 +647,9 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
     const size_t sh = fft_shx<C>(g, m);
     float ms = 0.f;
     CHK(time_launch(ctx, [&] {
-      hipLaunchKernelGGL(fx.r2c, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pl[0], tw[0],
+      hipLaunchKernelGGL(fx.r2c, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[0], tw[0],
                          greal, spec, m);
-      hipLaunchKernelGGL(fx.c2r, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pl[0], tw[0],
+      hipLaunchKernelGGL(fx.c2r, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[0], tw[0],
                          spec, greal, m);
     }, &ms));
     if (ms < best) { best = ms; pick = m; }
@@ -640,7 +662,7 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
     const size_t sh = fft_shy<C>(g, m);
     float ms = 0.f;
     CHK(time_launch(ctx, [&] {
-      hipLaunchKernelGGL(fy.y, dim3(Kz / m, H), dim3(BLOCK), sh, ctx->st, g, ctx->pl[1], tw[1], spec,
+      hipLaunchKernelGGL(fy.y, dim3(Kz / m, H), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[1], tw[1], spec,
                          m, Ky + 1, -1);
     }, &ms));
     if (ms < best) { best = ms; pick = m; }
@@ -651,72 +673,141 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
 }
 
 int autotune(simox_ctx* ctx) {
-  CHK(tune_fft<double2>(ctx, &ctx->fc64, ctx->spec64, ctx->tw64, ctx->greal64));
+  CHK(tune_fft<double2>(ctx, &ctx->pm.fc64, ctx->pm.spec64, ctx->pm.tw64, ctx->pm.greal64));
   if (ctx->mode == SIMOX_MODE_MIXED)
-    CHK(tune_fft<float2>(ctx, &ctx->fc32, ctx->spec32, ctx->tw32, ctx->greal32));
+    CHK(tune_fft<float2>(ctx, &ctx->pm.fc32, ctx->pm.spec32, ctx->pm.tw32, ctx->pm.greal32));
   return 0;
 }
 
 int zpass_blocks(const simox_ctx* ctx, bool f32) {
-  return f32 ? ctx->fc32.zblocks : ctx->fc64.zblocks;
+  return f32 ? ctx->pm.fc32.zblocks : ctx->pm.fc64.zblocks;
 }
 
 // Brick binning of the atoms for the tiled spread (once per set of positions).
-int ensure_bricks(simox_ctx* ctx) {
-  if (ctx->bricks_valid) return 0;
-  const int n = ctx->n, nb = grid_blocks(n), B = ctx->nbrick;
-  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, ctx->bcount);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, ctx->bfill);
-  hipLaunchKernelGGL(k_brick_assign, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, ctx->brick,
-                     ctx->bcount);
+int ensure_bricks(simox_ctx* ctx, PmeGrid& pm) {
+  if (pm.bricks_valid) return 0;
+  const int n = ctx->n, nb = grid_blocks(n), B = pm.nbrick;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, pm.bcount);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(B)), dim3(BLOCK), 0, ctx->st, B, 0, pm.bfill);
+  hipLaunchKernelGGL(k_brick_assign, dim3(nb), dim3(BLOCK), 0, ctx->st, geom_of(ctx, pm), ctx->x, pm.brick,
+                     pm.bcount);
   const int nsb = (B + 1023) / 1024;
-  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(1024), 0, ctx->st, B, ctx->bcount, ctx->bstart,
-                     ctx->bsum);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->st, nsb, ctx->bsum, ctx->btotal);
-  hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(1024), 0, ctx->st, B, ctx->bstart, ctx->bsum,
-                     ctx->btotal);
-  hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->brick, ctx->bstart,
-                     ctx->bfill, ctx->bunord);
-  hipLaunchKernelGGL(k_brick_place, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->brick, ctx->bstart,
-                     ctx->bfill, ctx->bunord, ctx->border);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(1024), 0, ctx->st, B, pm.bcount, pm.bstart,
+                     pm.bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->st, nsb, pm.bsum, pm.btotal);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(1024), 0, ctx->st, B, pm.bstart, pm.bsum,
+                     pm.btotal);
+  hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(BLOCK), 0, ctx->st, n, pm.brick, pm.bstart,
+                     pm.bfill, pm.bunord);
+  hipLaunchKernelGGL(k_brick_place, dim3(nb), dim3(BLOCK), 0, ctx->st, n, pm.brick, pm.bstart,
+                     pm.bfill, pm.bunord, pm.border);
   HIPCHK(hipGetLastError());
-  ctx->bricks_valid = true;
+  pm.bricks_valid = true;
   return 0;
 }
 
 // Spread charges q (or ql = q + step p when p != null) and convolve, with the
 // FFT chain in FP32 (f32) or FP64.  FAITHFUL mode rounds the spreading weights
 // to FP32 as the reference does (quirk Q1).
+// filter: convolve with the grid's filter table F instead of the influence
+// table G.  grid: the PME grid (default the reference's).
 int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool f32,
-             bool filter = false) {
+             bool filter = false, PmeGrid* grid = nullptr) {
+  PmeGrid& pm = grid ? *grid : ctx->pm;
+  const Geom g = geom_of(ctx, pm);
   const int nb = grid_blocks(ctx->n);
-  CHK(ensure_bricks(ctx));
+  CHK(ensure_bricks(ctx, pm));
   const double* src = q;
   if (p) {
     hipLaunchKernelGGL(k_lincomb, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, q, p, step, ctx->ql);
     src = ctx->ql;
   }
   const bool f32w = ctx->mode == SIMOX_MODE_FAITHFUL;
-  const int ntile = ((ctx->g.K[0] + TX - 1) / TX) * ((ctx->g.K[1] + TY - 1) / TY) *
-                    ((ctx->g.K[2] + TZ - 1) / TZ);
+  const int ntile = ((g.K[0] + TX - 1) / TX) * ((g.K[1] + TY - 1) / TY) * ((g.K[2] + TZ - 1) / TZ);
+  const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
-  tstart(ctx, 0, &ev);
+  if (timed) tstart(ctx, 0, &ev);
   if (f32)
-    hipLaunchKernelGGL((k_spread_tile<false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal32);
+    hipLaunchKernelGGL((k_spread_tile<false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+                       ctx->x, src, pm.bstart, pm.border, pm.greal32);
   else if (f32w)
-    hipLaunchKernelGGL((k_spread_tile<true, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal64);
+    hipLaunchKernelGGL((k_spread_tile<true, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+                       ctx->x, src, pm.bstart, pm.border, pm.greal64);
   else
-    hipLaunchKernelGGL((k_spread_tile<false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->x, src, ctx->bstart, ctx->border, ctx->greal64);
-  tstop(ctx, 0, ev);
+    hipLaunchKernelGGL((k_spread_tile<false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+                       ctx->x, src, pm.bstart, pm.border, pm.greal64);
+  if (timed) tstop(ctx, 0, ev);
   HIPCHK(hipGetLastError());
   if (f32)
-    return pme_chain<float2>(ctx, ctx->spec32, ctx->tw32, ctx->fc32, filter ? ctx->F32 : ctx->G32,
-                             ctx->greal32);
-  return pme_chain<double2>(ctx, ctx->spec64, ctx->tw64, ctx->fc64, filter ? ctx->F64 : ctx->G64,
-                            ctx->greal64);
+    return pme_chain<float2>(ctx, pm, pm.spec32, pm.tw32, pm.fc32, filter ? pm.F32 : pm.G32,
+                             pm.greal32);
+  return pme_chain<double2>(ctx, pm, pm.spec64, pm.tw64, pm.fc64, filter ? pm.F64 : pm.G64,
+                            pm.greal64);
+}
+
+// Buffers, twiddles, plans and launch shapes of PME grid pm with sizes K (each
+// a product of 2, 3, 5 and a multiple of BRK), and its influence table when
+// with_G (the reference's grid; the preconditioner's grid has filter tables
+// only).  The FP32 chain exists in MIXED mode.
+int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G) {
+  const bool f32 = ctx->mode == SIMOX_MODE_MIXED;
+  for (int d = 0; d < 3; ++d) {
+    if (K[d] < BRK || K[d] % BRK || !make_plan(K[d], &pm.pl[d])) return SIMOX_ERR_ARG;
+    pm.K[d] = K[d];
+  }
+  pm.H = K[0] / 2 + 1;
+  const Geom g = geom_of(ctx, pm);
+  const size_t n = size_t(ctx->n);
+  const size_t ng = size_t(K[0]) * K[1] * K[2];
+  const size_t nh = size_t(pm.H) * K[1] * K[2];
+  pm.nbrick = (K[0] / BRK) * (K[1] / BRK) * (K[2] / BRK);
+  CHK(dalloc(ctx, &pm.brick, n)); CHK(dalloc(ctx, &pm.bunord, n)); CHK(dalloc(ctx, &pm.border, n));
+  CHK(dalloc(ctx, &pm.bcount, pm.nbrick)); CHK(dalloc(ctx, &pm.bfill, pm.nbrick));
+  CHK(dalloc(ctx, &pm.bstart, size_t(pm.nbrick) + 1));
+  CHK(dalloc(ctx, &pm.bsum, (pm.nbrick + 1023) / 1024 + 1)); CHK(dalloc(ctx, &pm.btotal, 1));
+  // FP64 chain always (faithful / FP64 modes, MIXED outer residual); FP32 chain for MIXED
+  CHK(dalloc(ctx, &pm.greal64, ng)); CHK(dalloc(ctx, &pm.spec64, nh)); CHK(dalloc(ctx, &pm.G64, nh));
+  CHK(dalloc(ctx, &pm.F64, nh));
+  if (f32) {
+    CHK(dalloc(ctx, &pm.greal32, ng)); CHK(dalloc(ctx, &pm.spec32, nh));
+    if (with_G) CHK(dalloc(ctx, &pm.G32, nh));
+    CHK(dalloc(ctx, &pm.F32, nh));
+  }
+  for (int d = 0; d < 3; ++d) {
+    const int L = K[d];
+    std::vector<double2> t64(L);
+    std::vector<float2> t32(L);
+    for (int k = 0; k < L; ++k) {
+      const double ang = 2.0 * PI * double(k) / double(L);
+      t64[k] = make_double2(std::cos(ang), -std::sin(ang));
+      t32[k] = make_float2(float(std::cos(ang)), float(-std::sin(ang)));
+    }
+    CHK(dalloc(ctx, &pm.tw64[d], L)); CHK(dalloc(ctx, &pm.tw32[d], L));
+    HIPCHK(hipMemcpy(pm.tw64[d], t64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pm.tw32[d], t32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  CHK(choose_fft<double2>(ctx, g, &pm.fc64));
+  if (f32) CHK(choose_fft<float2>(ctx, g, &pm.fc32));
+  // G64 is kept for every grid: the preconditioner's filter is built from it
+  const double a2 = g.alpha * g.alpha;
+  const int nbk = int((nh + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G64);
+  if (f32 && with_G)
+    hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G32);
+  HIPCHK(hipGetLastError());
+  pm.bricks_valid = false;
+  pm.ready = true;
+  return 0;
+}
+
+void free_grid(PmeGrid& pm) {
+  void* bufs[] = {pm.brick, pm.bunord, pm.border, pm.bcount, pm.bfill, pm.bstart, pm.bsum,
+                  pm.btotal, pm.greal64, pm.greal32, pm.spec64, pm.spec32, pm.G64, pm.G32,
+                  pm.F64, pm.F32, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
+                  pm.tw32[2]};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  pm = PmeGrid{};
 }
 
 // PME (force.f90:364-464): reciprocal forces added to ctx->f, E_rec.
@@ -732,10 +823,10 @@ int do_pme(simox_ctx* ctx, bool fetch_energy) {
   tstart(ctx, 5, &ev);
   if (use32(ctx))
     hipLaunchKernelGGL(k_gather_force<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                       ctx->q, ctx->greal32, cst, ctx->frec, ctx->part);
+                       ctx->q, ctx->pm.greal32, cst, ctx->frec, ctx->part);
   else
     hipLaunchKernelGGL(k_gather_force<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                       ctx->q, ctx->greal64, cst, ctx->frec, ctx->part);
+                       ctx->q, ctx->pm.greal64, cst, ctx->frec, ctx->part);
   tstop(ctx, 5, ev);
   HIPCHK(hipGetLastError());
   CHK(reduce(ctx, nb, 3, 0, 5));
@@ -781,12 +872,12 @@ int grad_eval(simox_ctx* ctx, const double* q, const double* p, double step, boo
   tstart(ctx, 6, &ev);
   if (f32)
     hipLaunchKernelGGL((k_dudq<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->greal32, ctx->c_pme, ctx->nb_cnt,
+                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->pm.greal32, ctx->c_pme, ctx->nb_cnt,
                        ctx->nb_col, ctx->nb_val32, ctx->bvec, p, ctx->gv,
                        want_phi ? ctx->phi : nullptr, ctx->part);
   else
     hipLaunchKernelGGL((k_dudq<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->greal64, ctx->c_pme, ctx->nb_cnt,
+                       ctx->dtab, ctx->x, ctx->id, qeval, ctx->pm.greal64, ctx->c_pme, ctx->nb_cnt,
                        ctx->nb_col, ctx->nb_val64, ctx->bvec, p, ctx->gv,
                        want_phi ? ctx->phi : nullptr, ctx->part);
   tstop(ctx, 6, ev);
@@ -889,44 +980,73 @@ int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
   return 0;
 }
 
-// Coulomb-screening preconditioner set-up for the current positions (once per
-// solve): low-passed atom density (a spread of ones through the FFT chain with
-// the Gaussian table, sigma = 1.5 A), per-atom scaling c_i, filter table F.
+// The preconditioner's grid: half the resolution of the reference's grid in
+// every dimension, set up on first use, where those sizes are valid grids
+// (multiples of 4 with factors 2, 3, 5); else, or with SIMOX_PREC_FINE=1,
+// the reference's grid itself.  The filter acts on long wavelengths, which
+// the coarse grid resolves, and its FFTs cost an eighth.
+int prec_grid(simox_ctx* ctx, PmeGrid** out) {
+  *out = &ctx->pm;
+  if (ctx->pc.ready) { *out = &ctx->pc; return 0; }
+  const char* e = std::getenv("SIMOX_PREC_FINE");
+  if (e && std::strcmp(e, "1") == 0) return 0;
+  int K[3];
+  FftPlan tmp;
+  for (int d = 0; d < 3; ++d) {
+    K[d] = ctx->pm.K[d] / 2;
+    if (ctx->pm.K[d] % 2 || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) return 0;
+  }
+  const int rc = init_grid(ctx, ctx->pc, K, false);
+  if (rc) {
+    free_grid(ctx->pc);
+    if (rc == SIMOX_ERR_ARG) return 0;  // no launch shape for these sizes: fine grid
+    return rc;
+  }
+  *out = &ctx->pc;
+  return 0;
+}
+
+// Long-range preconditioner set-up for the current positions (once per
+// solve) on its grid: rho = 1.03 x the maximum over the grid of the
+// low-passed density of 1/J (a spread of w = 1/J through the FFT chain with
+// the Gaussian table, sigma = 2 A), then the filter table F = Gc / (1 + Gc rho)
+// with Gc = c Ng G, c = kc Ng / (pi V) for that grid's Ng.
 int build_prec(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
-  if (!ctx->prec_ready) {
-    std::vector<int> ids(n);
-    HIPCHK(hipMemcpy(ids.data(), ctx->id, sizeof(int) * n, hipMemcpyDeviceToHost));
-    double jsum = 0.0;
-    for (int i = 0; i < n; ++i) jsum += ctx->htab.sc[ids[i] - 1].J;
-    ctx->prec_j0 = jsum / n;
-    ctx->prec_ready = true;
-  }
-  const double sigma2 = 1.5 * 1.5;
-  const double ng = double(ctx->g.K[0]) * ctx->g.K[1] * ctx->g.K[2];
-  const double h3 = ctx->V / ng;
-  const size_t nh = size_t(ctx->g.H) * ctx->g.K[1] * ctx->g.K[2];
+  PmeGrid* pp = nullptr;
+  CHK(prec_grid(ctx, &pp));
+  PmeGrid& pc = *pp;
+  const Geom g = geom_of(ctx, pc);
+  const size_t nh = size_t(g.H) * g.K[1] * g.K[2];
+  const size_t ng = size_t(g.K[0]) * g.K[1] * g.K[2];
   const int nbk = int((nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->G64,
-                     sigma2, 0.0, 0.0, 0.0, 0, ctx->F64);
-  hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(BLOCK), 0, ctx->st, n, 1.0, ctx->sv);
+  hipLaunchKernelGGL(k_prec_weights, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ctx->sv, ctx->cv);
+  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, pc.G64,
+                     2.0 * 2.0, 0.0, 0.0, 0, pc.F64);
   HIPCHK(hipGetLastError());
-  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, false, true));
-  hipLaunchKernelGGL(k_prec_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                     ctx->greal64, ctx->cv, ctx->part);
+  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, false, true, &pc));
+  const int nbm = int(std::min<size_t>(1024, (ng + BLOCK - 1) / BLOCK));
+  CHK(ensure_part(ctx, size_t(nbm)));
+  hipLaunchKernelGGL(k_grid_max<double>, dim3(nbm), dim3(BLOCK), 0, ctx->st, ng, pc.greal64,
+                     ctx->part);
   HIPCHK(hipGetLastError());
-  CHK(reduce(ctx, nb, 1, 0, 27));
-  hipLaunchKernelGGL(k_prec_c, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 27, ctx->cv);
-  double rsum;
-  CHK(fetch(ctx, 27, 1, &rsum));
-  const double rho_ref = rsum / n;
-  const double lam_scale = KC / PI * (rho_ref / h3);  // (kc/pi) n_ref
-  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->G64,
-                     sigma2, lam_scale, 1.0 / ctx->prec_j0, 1.0 / rho_ref, 1, ctx->F64);
+  CHK(reduce(ctx, nbm, 1, 1, 27));
+  double rmax;
+  CHK(fetch(ctx, 27, 1, &rmax));
+  if (!(rmax > 0.0)) {
+    ctx->err = "charge preconditioner: empty density grid";
+    return SIMOX_ERR_ARG;
+  }
+  const double rho = 1.03 * rmax;
+  const double c_grid = KC * double(ng) / ctx->V / PI;
+  hipLaunchKernelGGL(k_filter_table<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, pc.G64, 0.0,
+                     c_grid, rho, 1, pc.F64);
   if (use32(ctx))
-    hipLaunchKernelGGL(k_filter_table<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->G64, sigma2, lam_scale, 1.0 / ctx->prec_j0, 1.0 / rho_ref, 1, ctx->F32);
+    hipLaunchKernelGGL(k_filter_table<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, pc.G64, 0.0,
+                       c_grid, rho, 1, pc.F32);
   HIPCHK(hipGetLastError());
+  ctx->prec_rho = rho;
   return 0;
 }
 
@@ -942,15 +1062,17 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32, double* ls = nullp
     HIPCHK(hipGetLastError());
     return 0;
   }
+  PmeGrid& pc = ctx->pc.ready ? ctx->pc : ctx->pm;  // chosen by build_prec
+  const Geom g = geom_of(ctx, pc);
   hipLaunchKernelGGL(k_prec_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ctx->qc, ctx->gv, ctx->cv, ctx->sv);
-  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true));
+  CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
   if (f32)
-    hipLaunchKernelGGL(k_prec_finish<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, ctx->greal32, ctx->zv, ctx->part);
+    hipLaunchKernelGGL(k_prec_finish<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal32, ctx->zv, ctx->part);
   else
-    hipLaunchKernelGGL(k_prec_finish<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, ctx->greal64, ctx->zv, ctx->part);
+    hipLaunchKernelGGL(k_prec_finish<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal64, ctx->zv, ctx->part);
   HIPCHK(hipGetLastError());
   CHK(reduce(ctx, nb, 1, 0, 26));
   hipLaunchKernelGGL(k_prec_project, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dscal + 26, ctx->gv,
@@ -986,13 +1108,13 @@ int ncg_iter_body(simox_ctx* ctx) {
   tstart(ctx, 6, &ev);
   if (f32) {
     hipLaunchKernelGGL(k_phi_gather<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                       ctx->greal32, ctx->c_pme, ctx->phi);
+                       ctx->pm.greal32, ctx->c_pme, ctx->phi);
     hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
                        ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
   } else {
     hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                       ctx->greal64, ctx->c_pme, ctx->phi);
+                       ctx->pm.greal64, ctx->c_pme, ctx->phi);
     hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
                        ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
@@ -1187,7 +1309,7 @@ int do_vv1(simox_ctx* ctx, double dt) {
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
   return 0;
 }
 
@@ -1350,6 +1472,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   ctx->n = n;
   ctx->mode = mode;
   ctx->rcut = rcut;
+  ctx->prec_on = mode == SIMOX_MODE_MIXED;
   Geom& g = ctx->g;
   g.n = n;
   for (int d = 0; d < 3; ++d) {
@@ -1363,7 +1486,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
       return SIMOX_ERR_ARG;
     }
     g.lcell[d] = box[d] / double(g.nc[d]);  // main.f90:190
-    if (fft[d] < 2 || !make_plan(fft[d], &ctx->pl[d])) {
+    if (fft[d] < 2 || !make_plan(fft[d], &ctx->pm.pl[d])) {
       ctx->err = "PME grid sizes must be products of 2, 3 and 5";
       return SIMOX_ERR_ARG;
     }
@@ -1416,47 +1539,11 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     CHK(dalloc(ctx, &ctx->nb_val32, size_t(ctx->cap) * n));
     CHK(dalloc(ctx, &ctx->pv32, n));
   }
-  // PME
-  const size_t ng = size_t(fft[0]) * fft[1] * fft[2];
-  const size_t nh = size_t(g.H) * fft[1] * fft[2];
-  ctx->nbrick = (fft[0] / BRK) * (fft[1] / BRK) * (fft[2] / BRK);
-  CHK(dalloc(ctx, &ctx->brick, n)); CHK(dalloc(ctx, &ctx->bunord, n)); CHK(dalloc(ctx, &ctx->border, n));
-  CHK(dalloc(ctx, &ctx->bcount, ctx->nbrick)); CHK(dalloc(ctx, &ctx->bfill, ctx->nbrick));
-  CHK(dalloc(ctx, &ctx->bstart, size_t(ctx->nbrick) + 1));
-  CHK(dalloc(ctx, &ctx->bsum, (ctx->nbrick + 1023) / 1024 + 1)); CHK(dalloc(ctx, &ctx->btotal, 1));
-  // FP64 chain always (faithful / FP64 modes, MIXED outer residual); FP32 chain for MIXED
-  CHK(dalloc(ctx, &ctx->greal64, ng)); CHK(dalloc(ctx, &ctx->spec64, nh)); CHK(dalloc(ctx, &ctx->G64, nh));
-  CHK(dalloc(ctx, &ctx->F64, nh)); CHK(dalloc(ctx, &ctx->sv, n)); CHK(dalloc(ctx, &ctx->cv, n));
-  if (mode == SIMOX_MODE_MIXED) {
-    CHK(dalloc(ctx, &ctx->greal32, ng)); CHK(dalloc(ctx, &ctx->spec32, nh)); CHK(dalloc(ctx, &ctx->G32, nh));
-    CHK(dalloc(ctx, &ctx->F32, nh));
-  }
-  for (int d = 0; d < 3; ++d) {
-    const int L = fft[d];
-    std::vector<double2> t64(L);
-    std::vector<float2> t32(L);
-    for (int k = 0; k < L; ++k) {
-      const double ang = 2.0 * PI * double(k) / double(L);
-      t64[k] = make_double2(std::cos(ang), -std::sin(ang));
-      t32[k] = make_float2(float(std::cos(ang)), float(-std::sin(ang)));
-    }
-    CHK(dalloc(ctx, &ctx->tw64[d], L)); CHK(dalloc(ctx, &ctx->tw32[d], L));
-    HIPCHK(hipMemcpy(ctx->tw64[d], t64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->tw32[d], t32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
-  }
+  CHK(dalloc(ctx, &ctx->sv, n)); CHK(dalloc(ctx, &ctx->cv, n));
   {
-    int rc = choose_fft<double2>(ctx, g, &ctx->fc64);
-    if (!rc && mode == SIMOX_MODE_MIXED) rc = choose_fft<float2>(ctx, g, &ctx->fc32);
+    const int rc = init_grid(ctx, ctx->pm, fft, true);
     if (rc == SIMOX_ERR_ARG) ctx->err = "PME grid dimension too large for the LDS-resident FFT";
     if (rc) return rc;
-  }
-  {
-    const double a2 = g.alpha * g.alpha;
-    const int nbk = int((nh + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G64);
-    if (mode == SIMOX_MODE_MIXED)
-      hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, ctx->G32);
-    HIPCHK(hipGetLastError());
   }
   CHK(dalloc(ctx, &ctx->dscal, 64));
   CHK(dalloc(ctx, &ctx->part_eg, grid_blocks(n)));
@@ -1466,7 +1553,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   HIPCHK(hipHostMalloc((void**)&ctx->herr, sizeof(int), hipHostMallocDefault));
   {
     // partial-sum buffer sized once for every launch that uses it
-    const size_t zb = size_t(std::max(ctx->fc64.zblocks, ctx->fc32.zblocks));
+    const size_t zb = size_t(std::max(ctx->pm.fc64.zblocks, ctx->pm.fc32.zblocks));
     CHK(ensure_part(ctx, std::max(size_t(grid_blocks(n)) * 10, zb) + 16));
   }
   hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->order);
@@ -1488,7 +1575,7 @@ int simox_autotune(simox_ctx* ctx) {
 
 int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]) {
   if (!ctx || !shapes) return SIMOX_ERR_ARG;
-  const FftCfg* c[2] = {&ctx->fc64, &ctx->fc32};
+  const FftCfg* c[2] = {&ctx->pm.fc64, &ctx->pm.fc32};
   for (int k = 0; k < 2; ++k) {
     shapes[3 * k] = c[k]->mx;
     shapes[3 * k + 1] = c[k]->my;
@@ -1505,18 +1592,14 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->brick, ctx->bunord, ctx->border, ctx->bcount,
-                  ctx->bfill, ctx->bstart, ctx->bsum, ctx->btotal, ctx->greal64, ctx->greal32, ctx->spec64,
-                  ctx->spec32, ctx->G64, ctx->G32, ctx->F64, ctx->F32, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  for (int d = 0; d < 3; ++d) {
-    if (ctx->tw64[d]) (void)hipFree(ctx->tw64[d]);
-    if (ctx->tw32[d]) (void)hipFree(ctx->tw32[d]);
-  }
+  free_grid(ctx->pm);
+  free_grid(ctx->pc);
   if (ctx->hscal) (void)hipHostFree(ctx->hscal);
   if (ctx->herr) (void)hipHostFree(ctx->herr);
   if (ctx->hstage) (void)hipHostFree(ctx->hstage);
@@ -1598,7 +1681,6 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
     if (q) CHK(upload_caller(ctx, q, 1, ctx->q));
     ctx->have_state = true;
     ctx->nhist = 0;
-    ctx->prec_ready = false;
     CHK(do_cell_sort(ctx));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return check_err(ctx, 0);
@@ -1608,7 +1690,7 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
   if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
   if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
   if (q) { CHK(upload_caller(ctx, q, 1, ctx->q)); ctx->nhist = 0; }
-  if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->bricks_valid = false; }
+  if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->pm.bricks_valid = ctx->pc.bricks_valid = false; }
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }
@@ -1663,7 +1745,7 @@ int adjoint_solve(simox_ctx* ctx, double tol, int maxit, int* iters) {
     // y = P H p
     CHK(pme_eval(ctx, p, nullptr, 0.0, false));
     hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                       ctx->greal64, ctx->c_pme, ctx->phi);
+                       ctx->pm.greal64, ctx->c_pme, ctx->phi);
     hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->q, p, p, p, ctx->phi, ctx->nb_cnt,
                        ctx->nb_col, ctx->nb_val64, y, ctx->part);
@@ -1704,7 +1786,7 @@ int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters
   // b = dE/dq at the current charges (PME potential with exact weights)
   CHK(pme_eval(ctx, ctx->q, nullptr, 0.0, false));
   hipLaunchKernelGGL(k_phi_gather<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
-                     ctx->greal64, ctx->c_pme, ctx->phi);
+                     ctx->pm.greal64, ctx->c_pme, ctx->phi);
   hipLaunchKernelGGL(k_denergy_dq, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
                      ctx->id, ctx->q, ctx->phi, ctx->nb_cnt, ctx->nb_col, ctx->adj);
   HIPCHK(hipGetLastError());
@@ -1735,7 +1817,7 @@ int simox_remove_rigid_motion(simox_ctx* ctx) {
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }
@@ -1793,7 +1875,6 @@ int simox_set_params(simox_ctx* ctx, const simox_params* params) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->prec_ready = false;
   return SIMOX_OK;
 }
 

@@ -772,46 +772,52 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_z(int n, const double* __restrict
 namespace sx {
 
 // ---------------------------------------------------------------------------
-// Coulomb-screening preconditioner for the charge solve.
-//   M^-1 = D^-1/2 (I - W^T F W) D^-1/2,   D = J + 2w [q outside (qmin, qmax)]
-// W spreads atom values onto the PME grid with the B-spline weights, F is a
-// k-space filter.  For a smooth charge mode of wave vector k the Hessian acts
-// as J + lambda(k), lambda(k) = (kc/pi) n0 G(k) (n0 = atoms per volume, G the
-// PME influence function), which grows like 1/k^2 and makes plain Jacobi CG
-// slow for large boxes.  With f = (lambda/J0)/(1 + lambda/J0) and
-// F = f/(n0 h^3), W^T F W ~ f on smooth modes, so M^-1 ~ 1/(J + lambda) there
-// and ~ 1/J on short wavelengths.  Neutrality: z = M g - u (u.g)/(1.u), u = M 1.
+// Long-range (Woodbury) preconditioner for the charge solve.
+// The Hessian is H = D + S + W^T Gc W: D the diagonal J (+2w where the
+// penalty is active), S the short-range pair matrix, W the B-spline spread
+// onto the PME grid and Gc the reciprocal-space kernel (c_pme times the PME
+// influence table).  For a slab of N atoms the W^T Gc W part grows like 1/k^2
+// on long wavelengths, so Jacobi CG needs iterations that grow with the box.
+// Dropping S, Woodbury gives
+//   (D + W^T Gc W)^-1 = D^-1 - D^-1 W^T (Gc^-1 + W D^-1 W^T)^-1 W D^-1,
+// and on smooth grid fields W D^-1 W^T acts as the local density rho of 1/J
+// per grid point, so the middle factor is the k-space filter
+//   F(k) = Gc(k) / (1 + Gc(k) rho).
+// For a smooth mode M^-1 ~ (1/J)(1 - rho_true/rho) + 1/(J + Gc rho_true):
+// rho below the true density makes M indefinite on long waves (and CG
+// stalls), rho above it only weakens the filter.  rho is therefore the
+// maximum of the low-passed (sigma = 2 A) density of 1/J on the grid, which
+// for a crystalline slab matches the bulk value, times 1.03.
+// Applied as M^-1 = D^-1/2 (I - C W^T F W C) D^-1/2 with C = diag(1/sqrt(J)),
+// fixed for the solve, so M stays symmetric when the penalty set changes.
 // ---------------------------------------------------------------------------
 // Filter tables on the half spectrum (same layout as G; values / Ng):
-//   stage 0: S(k) = exp(-k^2 sigma^2)                       (low-pass)
-//   stage 1: F(k) = f(k) S(k) / rho_ref,  f = l/(1+l),  l = lam_scale G(k)/J0
-// rho_ref = mean low-passed atom density per grid point at the atoms; with the
-// per-atom scaling c_i = sqrt(rho_ref/rho_i), C W^T F W C ~ f on smooth modes
-// in every region (slab, oxide layer), so 0 < M.
+//   stage 0: S(k) = exp(-4 pi^2 m^2 sigma^2)            (Gaussian low-pass)
+//   stage 1: F(k) = s / (1 + s rho),  s = c_pme Ng G(k)  (G holds psi B / Ng)
 template <class R>
 __global__ void k_filter_table(Geom g, const double* __restrict__ G64, double sigma2,
-                               double lam_scale, double inv_j0, double inv_rho, int stage,
-                               R* __restrict__ F) {
+                               double c_pme, double rho, int stage, R* __restrict__ F) {
   const size_t total = size_t(g.H) * g.K[1] * g.K[2];
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int kz = int(t % g.K[2]);
-  const int ky = int((t / g.K[2]) % g.K[1]);
-  const int kx = int(t / (size_t(g.K[2]) * g.K[1]));
-  const int k3[3] = {kx, ky, kz};
-  double m2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int K = g.K[d];
-    const int m = (k3[d] + K / 2) % K - K / 2;
-    m2 += double(m * m) / (g.box[d] * g.box[d]);
-  }
   const double ng = double(g.K[0]) * g.K[1] * g.K[2];
-  const double S = exp(-4.0 * PI * PI * m2 * sigma2);
-  double v = S;
-  if (stage == 1) {
-    const double lam = lam_scale * G64[t] * ng * inv_j0;  // G64 holds G/Ng
-    v = lam / (1.0 + lam) * S * inv_rho;
+  double v;
+  if (stage == 0) {
+    const int kz = int(t % g.K[2]);
+    const int ky = int((t / g.K[2]) % g.K[1]);
+    const int kx = int(t / (size_t(g.K[2]) * g.K[1]));
+    const int k3[3] = {kx, ky, kz};
+    double m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int K = g.K[d];
+      const int m = (k3[d] + K / 2) % K - K / 2;
+      m2 += double(m * m) / (g.box[d] * g.box[d]);
+    }
+    v = exp(-4.0 * PI * PI * m2 * sigma2);
+  } else {
+    const double sk = c_pme * G64[t] * ng;
+    v = sk / (1.0 + sk * rho);
   }
   F[t] = R(v / ng);
 }
@@ -841,28 +847,15 @@ __global__ void k_fill_f64(int n, double v, double* __restrict__ a) {
   if (i < n) a[i] = v;
 }
 
-// local smoothed atom density at each atom, rho_i = W^T (S * W 1); part = sum rho
-template <class Rg>
-__global__ void __launch_bounds__(BLOCK) k_prec_density(Geom g, const double* __restrict__ x,
-                                                        const Rg* __restrict__ grid,
-                                                        double* __restrict__ rho,
-                                                        double* __restrict__ part) {
-  __shared__ double lds[BLOCK / WAVE];
+// w_i = 1/J_i (the density spread to find rho), c_i = 1/sqrt(J_i)
+__global__ void k_prec_weights(int n, const Tables* __restrict__ tb, const int* __restrict__ id,
+                               double* __restrict__ w, double* __restrict__ c) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[1] = {0.0};
-  if (i < g.n) {
-    const double r = fmax(gather_phi(g, x + 3 * i, grid), 1e-12);
-    rho[i] = r;
-    acc[0] = r;
+  if (i < n) {
+    const double J = tb->sc[id[i] - 1].J;
+    w[i] = 1.0 / J;
+    c[i] = 1.0 / sqrt(J);
   }
-  block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-}
-
-// c_i = sqrt(rho_ref / rho_i), rho_ref = mean rho (from its sum)
-__global__ void k_prec_c(int n, const double* __restrict__ rho_sum, double* __restrict__ c) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) c[i] = sqrt(rho_sum[0] / double(n) / c[i]);
 }
 
 // s = c g / sqrt(D)

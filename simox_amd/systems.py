@@ -144,6 +144,42 @@ def config(name, seed=1):
     raise KeyError(name)
 
 
+def fft_size(n):
+    """Smallest K >= n that is a multiple of 4 with no prime factor above 5."""
+    k = max(4, int(np.ceil(n / 4.0)) * 4)
+    while True:
+        m = k
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return k
+        k += 4
+
+
+def replicas(n_rep=256, seed=1):
+    """Independent Al/O slabs of uneven size: slab cells drawn from 4x4x6 to
+    10x10x12 (384-4,800 Al), O coverage 0.1-1.0 monolayer of the (001) surface
+    (2 sites per a^2), temperature 300-1200 K.  Rcut = min(8, L_xy / 3) keeps
+    3 or more cells per dimension; PME grid spacing is Rcut / 10 per dimension."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(n_rep):
+        nx, ny = (int(v) for v in rng.integers(4, 11, size=2))
+        nz = int(rng.integers(6, 13))
+        cov = float(rng.uniform(0.1, 1.0))
+        T = float(rng.uniform(300.0, 1200.0))
+        n_o = max(1, int(round(cov * 2 * nx * ny)))
+        lxy = min(nx, ny) * A_AL
+        rcut = min(8.0, np.floor(lxy / 3.0 * 100.0) / 100.0)
+        vacuum = 20.0
+        box = np.array([nx * A_AL, ny * A_AL, nz * A_AL + vacuum])
+        fft = tuple(fft_size(b / (rcut / 10.0)) for b in box)
+        out.append(al_o_slab(nx, ny, nz, n_o, seed * 100003 + r, T, fft, rcut, vacuum=vacuum,
+                             o_layer=(2.0, 6.0), o_dmin=2.0, name="c4_%d" % r))
+    return out
+
+
 def tiny(seed=1, nx=4, ny=4, nz=4, n_o=16, T=300.0, fft=(16, 16, 32), rcut=6.0, vacuum=12.0):
     """A small system for fast parity tests."""
     return al_o_slab(nx, ny, nz, n_o, seed, T, fft, rcut, vacuum=vacuum, o_layer=(2.0, 5.0),
