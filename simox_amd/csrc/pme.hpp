@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_x_c2r(Geom g, FftPlan pl, const C
 // G = psi(k) 36^3 / prod_d (4 + 2 cos(2 pi k_d / K_d))^2 (force.f90:414-420,
 // 500-514); G(0) = 0 (quirk Q5).
 template <class R>
-__global__ void k_influence(Geom g, double a2, R* __restrict__ G) {
+__global__ void k_influence(Geom g, double a2, R* __restrict__ G, int order) {
   const size_t total = size_t(g.H) * g.K[1] * g.K[2];
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -521,7 +521,8 @@ __global__ void k_influence(Geom g, double a2, R* __restrict__ G) {
   for (int d = 0; d < 3; ++d) {
     const int K = g.K[d];
     m[d] = (k3[d] + K / 2) % K - K / 2;
-    double z = 4.0 + 2.0 * cos(2.0 * PI * double(k3[d]) / double(K));
+    // |b(k)|^-2 of the order-4 B-spline; 1 for order 2 (linear weights)
+    double z = order == 4 ? 4.0 + 2.0 * cos(2.0 * PI * double(k3[d]) / double(K)) : 6.0;
     zf *= z * z;
   }
   double val = 0.0;
@@ -627,6 +628,49 @@ __device__ inline double gather_phi(const Geom& g, const double* xi, const Rg* _
   return e;
 }
 
+// Linear (order-2, cloud-in-cell) assignment, used on the charge solver's
+// preconditioner grid: weights (1 - f, f) at base points b and b + 1,
+// b = INT(u), f = u - b.
+__device__ inline void cic_weights(const Geom& g, const double* xi, double W[3][2], int ix[3][2]) {
+  double u[3];
+  int b[3];
+  grid_coord(g, xi, u, b);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double f = u[d] - double(b[d]);
+    W[d][0] = 1.0 - f;
+    W[d][1] = f;
+    int r = b[d] % g.K[d];
+    if (r < 0) r += g.K[d];
+    ix[d][0] = r;
+    ix[d][1] = r + 1 == g.K[d] ? 0 : r + 1;
+  }
+}
+
+template <class Rg>
+__device__ inline double gather_cic(const Geom& g, const double* xi, const Rg* __restrict__ grid) {
+  double W[3][2];
+  int ix[3][2];
+  cic_weights(g, xi, W, ix);
+  double e = 0.0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      const size_t row = size_t(g.K[0]) * (size_t(ix[1][bb]) + size_t(g.K[1]) * size_t(ix[2][c]));
+      e += W[1][bb] * W[2][c] * (W[0][0] * double(grid[row + ix[0][0]]) +
+                                 W[0][1] * double(grid[row + ix[0][1]]));
+    }
+  return e;
+}
+
+// gather of the given assignment order (4: the reference's B-splines, 2: linear)
+template <int ORD, class Rg>
+__device__ inline double gather_ord(const Geom& g, const double* xi, const Rg* __restrict__ grid) {
+  if constexpr (ORD == 4) return gather_phi(g, xi, grid);
+  else return gather_cic(g, xi, grid);
+}
+
 }  // namespace sx
 
 namespace sx {
@@ -691,7 +735,9 @@ __global__ void k_brick_scatter(int n, const int* __restrict__ brick, const int*
 }
 
 // K6 (tiled): grid[tile] = sum over atoms of q w, written as R (double/float).
-template <bool F32W, class R>
+// ORD 4: the reference's order-4 B-splines (points base-1 .. base+2); ORD 2:
+// linear weights (points base, base+1) for the preconditioner grid.
+template <int ORD, bool F32W, class R>
 __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __restrict__ x,
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ bstart,
@@ -711,7 +757,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   int fb[3], cb[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int a = lo[d] - 2, b = lo[d] + ext[d];  // base residues lo-2 .. hi+1
+    // bases whose stencil meets the tile: lo-2 .. hi+1 (ORD 4), lo-1 .. hi (ORD 2)
+    const int a = lo[d] - (ORD == 4 ? 2 : 1), b = lo[d] + ext[d] - (ORD == 4 ? 0 : 1);
     int f0 = (a >= 0) ? a / BRK : -((-a + BRK - 1) / BRK);
     int l0 = b / BRK;
     int c = l0 - f0 + 1;
@@ -757,6 +804,33 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
     }
     const int ai = border[bbeg[l] + (f - bofs[l])];
     const double qi = q[ai];
+    if constexpr (ORD == 2) {
+      double W[3][2];
+      int ix[3][2];
+      cic_weights(g, x + 3 * ai, W, ix);
+      int o[3][2];
+      bool in[3][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          int p = ix[d][a] - lo[d];
+          if (p < 0) p += g.K[d];
+          o[d][a] = p;
+          in[d][a] = p < ext[d];
+        }
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+            if (in[2][c] && in[1][bb] && in[0][a]) {
+              const long long v = llrint(qi * (W[0][a] * W[1][bb] * W[2][c]) * GRID_SCALE);
+              atomicAdd(&acc[TX * (o[1][bb] + TY * o[2][c]) + o[0][a]], (unsigned long long)v);
+            }
+      continue;
+    }
     double u[3], AX[3][4], AXX[3][4];
     int b[3];
     grid_coord(g, x + 3 * ai, u, b);

@@ -75,6 +75,7 @@ struct FftCfg {
 // brick of their base point).
 struct PmeGrid {
   int K[3] = {0, 0, 0}, H = 0;
+  int order = 4;  // assignment order: 4 (B-splines) or 2 (linear, preconditioner grid)
   FftPlan pl[3]{};
   FftCfg fc64{}, fc32{};
   double2* tw64[3] = {nullptr, nullptr, nullptr};
@@ -727,14 +728,20 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
   if (timed) tstart(ctx, 0, &ev);
-  if (f32)
-    hipLaunchKernelGGL((k_spread_tile<false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+  if (pm.order == 2 && f32)
+    hipLaunchKernelGGL((k_spread_tile<2, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+                       ctx->x, src, pm.bstart, pm.border, pm.greal32);
+  else if (pm.order == 2)
+    hipLaunchKernelGGL((k_spread_tile<2, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+                       ctx->x, src, pm.bstart, pm.border, pm.greal64);
+  else if (f32)
+    hipLaunchKernelGGL((k_spread_tile<4, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
   else if (f32w)
-    hipLaunchKernelGGL((k_spread_tile<true, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+    hipLaunchKernelGGL((k_spread_tile<4, true, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
   else
-    hipLaunchKernelGGL((k_spread_tile<false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
+    hipLaunchKernelGGL((k_spread_tile<4, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
   if (timed) tstop(ctx, 0, ev);
   HIPCHK(hipGetLastError());
@@ -749,13 +756,14 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
 // a product of 2, 3, 5 and a multiple of BRK), and its influence table when
 // with_G (the reference's grid; the preconditioner's grid has filter tables
 // only).  The FP32 chain exists in MIXED mode.
-int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G) {
+int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int order) {
   const bool f32 = ctx->mode == SIMOX_MODE_MIXED;
   for (int d = 0; d < 3; ++d) {
     if (K[d] < BRK || K[d] % BRK || !make_plan(K[d], &pm.pl[d])) return SIMOX_ERR_ARG;
     pm.K[d] = K[d];
   }
   pm.H = K[0] / 2 + 1;
+  pm.order = order;
   const Geom g = geom_of(ctx, pm);
   const size_t n = size_t(ctx->n);
   const size_t ng = size_t(K[0]) * K[1] * K[2];
@@ -791,9 +799,9 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G) {
   // G64 is kept for every grid: the preconditioner's filter is built from it
   const double a2 = g.alpha * g.alpha;
   const int nbk = int((nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G64);
+  hipLaunchKernelGGL(k_influence<double>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G64, order);
   if (f32 && with_G)
-    hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G32);
+    hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G32, order);
   HIPCHK(hipGetLastError());
   pm.bricks_valid = false;
   pm.ready = true;
@@ -980,6 +988,14 @@ int ctip_faithful(simox_ctx* ctx, int* iters, int* n_eval) {
   return 0;
 }
 
+// Assignment order on the preconditioner's own grid: linear weights (8
+// points per atom, an eighth of the spread and gather work of the B-splines),
+// or the B-splines with SIMOX_PREC_ORDER=4.
+int prec_order(const simox_ctx*) {
+  const char* e = std::getenv("SIMOX_PREC_ORDER");
+  return (e && std::strcmp(e, "4") == 0) ? 4 : 2;
+}
+
 // The preconditioner's grid: half the resolution of the reference's grid in
 // every dimension, set up on first use, where those sizes are valid grids
 // (multiples of 4 with factors 2, 3, 5); else, or with SIMOX_PREC_FINE=1,
@@ -996,7 +1012,7 @@ int prec_grid(simox_ctx* ctx, PmeGrid** out) {
     K[d] = ctx->pm.K[d] / 2;
     if (ctx->pm.K[d] % 2 || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) return 0;
   }
-  const int rc = init_grid(ctx, ctx->pc, K, false);
+  const int rc = init_grid(ctx, ctx->pc, K, false, prec_order(ctx));
   if (rc) {
     free_grid(ctx->pc);
     if (rc == SIMOX_ERR_ARG) return 0;  // no launch shape for these sizes: fine grid
@@ -1067,11 +1083,17 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32, double* ls = nullp
   hipLaunchKernelGGL(k_prec_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ctx->qc, ctx->gv, ctx->cv, ctx->sv);
   CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
-  if (f32)
-    hipLaunchKernelGGL(k_prec_finish<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+  if (f32 && pc.order == 2)
+    hipLaunchKernelGGL((k_prec_finish<2, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal32, ctx->zv, ctx->part);
+  else if (pc.order == 2)
+    hipLaunchKernelGGL((k_prec_finish<2, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal64, ctx->zv, ctx->part);
+  else if (f32)
+    hipLaunchKernelGGL((k_prec_finish<4, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
                        ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal32, ctx->zv, ctx->part);
   else
-    hipLaunchKernelGGL(k_prec_finish<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
+    hipLaunchKernelGGL((k_prec_finish<4, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, g, ctx->dtab,
                        ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->cv, pc.greal64, ctx->zv, ctx->part);
   HIPCHK(hipGetLastError());
   CHK(reduce(ctx, nb, 1, 0, 26));
@@ -1541,7 +1563,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   }
   CHK(dalloc(ctx, &ctx->sv, n)); CHK(dalloc(ctx, &ctx->cv, n));
   {
-    const int rc = init_grid(ctx, ctx->pm, fft, true);
+    const int rc = init_grid(ctx, ctx->pm, fft, true, 4);
     if (rc == SIMOX_ERR_ARG) ctx->err = "PME grid dimension too large for the LDS-resident FFT";
     if (rc) return rc;
   }

@@ -867,7 +867,7 @@ __global__ void k_prec_scale(int n, const Tables* __restrict__ tb, const int* __
 }
 
 // z = (g/sqrt(D) - c W^T grid)/sqrt(D); part[k] = sum z
-template <class Rg>
+template <int ORD, class Rg>
 __global__ void __launch_bounds__(BLOCK) k_prec_finish(Geom g, const Tables* __restrict__ tb,
                                                        const double* __restrict__ x,
                                                        const int* __restrict__ id,
@@ -882,7 +882,7 @@ __global__ void __launch_bounds__(BLOCK) k_prec_finish(Geom g, const Tables* __r
   double acc[1] = {0.0};
   if (i < g.n) {
     const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
-    const double zi = (gr[i] / sd - c[i] * gather_phi(g, x + 3 * i, grid)) / sd;
+    const double zi = (gr[i] / sd - c[i] * gather_ord<ORD>(g, x + 3 * i, grid)) / sd;
     z[i] = zi;
     acc[0] = zi;
   }
