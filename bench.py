@@ -25,7 +25,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNELS = {0: "k_spread_tile", 1: "k_fft_x_r2c", 2: "k_fft_y (fwd + inv)",
            3: "k_fft_z_fused", 4: "k_fft_x_c2r", 5: "k_gather_force",
-           6: "k_h0matvec", 7: "k_force", 8: "k_nlist_density", 9: "k_ctip_matrix"}
+           6: "k_h0matvec", 7: "k_force", 8: "k_nlist_density", 9: "k_ctip_matrix",
+           10: "k_gather_tile"}
 # timing category 6 is k_dudq in the faithful solve, k_h0matvec in the converged ones
 
 
@@ -77,8 +78,13 @@ def algorithmic_bytes(which, s, mode, nb_entries):
         return nh * c + ng * r
     if which == 5:   # force gather: grid read, x q in, frec out
         return ng * r + n * (32 + 24)
-    if which == 6:   # dUdq: grid read, x id q b p in, g out, ELL (col + value) rows
-        return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
+    if which == 6:
+        if mode == "faithful":  # dUdq: grid read, x id q b p in, g out, ELL rows
+            return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
+        # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows
+        return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + r)
+    if which == 10:  # tiled potential gather: grid read once, x in, phi out
+        return ng * r + n * (24 + 8)
     if which == 7:   # pair force: x q id dF in, f out
         return n * (24 + 8 + 4 + r + 24)
     if which == 8:   # density: x id in, rho dF out

@@ -228,9 +228,10 @@ int simox_batch_step(simox_ctx** ctxs, int nrep, int nsteps, double dt, int sort
 /* Per-kernel timing (HIP events recorded on the context stream around each
    launch while enabled): which = 0 spread, 1 FFT x R2C, 2 FFT y (both
    directions), 3 FFT z + influence multiply, 4 FFT x C2R, 5 PME force gather,
-   6 dUdq (potential gather + sparse matvec), 7 pair force, 8 EAM density +
-   embedding, 9 CTIP matrix build.  Returns accumulated ms and launch count
-   since the last simox_timing_enable. */
+   6 dUdq (faithful mode: potential gather + sparse matvec; converged modes:
+   the sparse matvec H0 p), 7 pair force, 8 EAM density + embedding, 9 CTIP
+   matrix build, 10 tiled potential gather of the converged solve.  Returns
+   accumulated ms and launch count since the last simox_timing_enable. */
 int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launches);
 int simox_timing_enable(simox_ctx* ctx, int on);
 

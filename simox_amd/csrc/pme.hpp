@@ -876,6 +876,125 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   }
 }
 
+// K9 (tiled): out_i = scale * sum w Qfin over the order-4 stencil of atom i
+// (the gather of PHI_MESH, ctip.f90:381-393).  One workgroup per tile of
+// GX x GY x GZ grid points: the tile and its stencil halo (one point below,
+// two above) are staged in LDS with row-contiguous loads, then the atoms whose
+// base point lies in the tile's bricks (the spread's brick lists) read their
+// 64 weights' points from LDS.  Same products and summation order as
+// gather_phi, so the result is that of the per-atom gather.
+template <class Rg, int GX, int GY, int GZ>
+__global__ void __launch_bounds__(BLOCK) k_gather_tile(Geom g, const double* __restrict__ x,
+                                                       const int* __restrict__ bstart,
+                                                       const int* __restrict__ border,
+                                                       const Rg* __restrict__ grid, double scale,
+                                                       double* __restrict__ out) {
+  constexpr int GLX = GX + 3, GLY = GY + 3, GLZ = GZ + 3;
+  constexpr int GBRICKS = (GX / BRK) * (GY / BRK) * (GZ / BRK);
+  static_assert(GBRICKS <= BLOCK, "one thread per brick of the tile");
+  extern __shared__ unsigned char gather_lds[];
+  Rg* tile = reinterpret_cast<Rg*>(gather_lds);
+  __shared__ int bofs[GBRICKS + 1];
+  __shared__ int bbeg[GBRICKS];
+  __shared__ int wsum[BLOCK / WAVE];
+  const int ntx = (g.K[0] + GX - 1) / GX, nty = (g.K[1] + GY - 1) / GY;
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int lo[3] = {tx * GX, ty * GY, tz * GZ};
+  const int ext[3] = {min(GX, g.K[0] - lo[0]), min(GY, g.K[1] - lo[1]), min(GZ, g.K[2] - lo[2])};
+  // grid points lo-1 .. lo+ext+1 (periodic) into LDS, eight loads in flight
+  // per thread (the staging is latency-bound otherwise); LDS index k =
+  // px + GLX (py + GLY pz) with compile-time strides
+  const int lx = ext[0] + 3, ly = ext[1] + 3, lz = ext[2] + 3;
+  constexpr int GL = GLX * GLY * GLZ;
+  constexpr int U = 8;
+  for (int k0 = threadIdx.x; k0 < GL; k0 += U * BLOCK) {
+    Rg v[U];
+#pragma unroll
+    for (int w = 0; w < U; ++w) {
+      const int k = k0 + w * BLOCK;
+      v[w] = Rg(0);
+      const int px = k % GLX, py = (k / GLX) % GLY, pz = k / (GLX * GLY);
+      if (k < GL && px < lx && py < ly && pz < lz) {
+        int gx = lo[0] - 1 + px, gy = lo[1] - 1 + py, gz = lo[2] - 1 + pz;
+        gx += gx < 0 ? g.K[0] : (gx >= g.K[0] ? -g.K[0] : 0);
+        gy += gy < 0 ? g.K[1] : (gy >= g.K[1] ? -g.K[1] : 0);
+        gz += gz < 0 ? g.K[2] : (gz >= g.K[2] ? -g.K[2] : 0);
+        v[w] = grid[size_t(gx) + size_t(g.K[0]) * (size_t(gy) + size_t(g.K[1]) * size_t(gz))];
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < U; ++w) {
+      const int k = k0 + w * BLOCK;
+      if (k < GL) tile[k] = v[w];
+    }
+  }
+  // atom counts of the tile's bricks and their prefix sum
+  const int nbx = g.K[0] / BRK, nby = g.K[1] / BRK;
+  const int cbx = ext[0] / BRK, cby = ext[1] / BRK, cbz = ext[2] / BRK;
+  const int ncand = cbx * cby * cbz;
+  int cntk = 0;
+  if (threadIdx.x < ncand) {
+    const int k = threadIdx.x;
+    const int bx = lo[0] / BRK + k % cbx, by = lo[1] / BRK + (k / cbx) % cby,
+              bz = lo[2] / BRK + k / (cbx * cby);
+    const int bid = bx + nbx * (by + nby * bz);
+    const int b0 = bstart[bid];
+    bbeg[k] = b0;
+    cntk = bstart[bid + 1] - b0;
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int v = __shfl_up(cntk, o, WAVE);
+    if (lane >= o) cntk += v;
+  }
+  if (lane == WAVE - 1) wsum[wid] = cntk;
+  if (threadIdx.x == 0) bofs[0] = 0;
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) cntk += wsum[w];
+  if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
+  __syncthreads();
+  const int total = bofs[ncand];
+  for (int f = threadIdx.x; f < total; f += blockDim.x) {
+    int l = 0, h = ncand - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (bofs[mid] <= f) l = mid; else h = mid - 1;
+    }
+    const int ai = border[bbeg[l] + (f - bofs[l])];
+    double u[3], AX[3][4], AXX[3][4];
+    int b[3];
+    grid_coord(g, x + 3 * ai, u, b);
+    int o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      bspline4(u[d], AX[d], AXX[d]);
+      int r = b[d] % g.K[d];
+      if (r < 0) r += g.K[d];
+      o[d] = r - lo[d];  // stencil point a sits at LDS index o + a (0 .. ext + 2)
+    }
+    double e = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          e += AX[0][a] * AX[1][bb] * AX[2][c] *
+               double(tile[(o[0] + a) + GLX * ((o[1] + bb) + GLY * (o[2] + c))]);
+    out[ai] = scale * e;
+  }
+}
+
+template <int GX, int GY, int GZ>
+struct GatherTile {
+  static constexpr size_t lds(size_t elem) { return size_t(GX + 3) * (GY + 3) * (GZ + 3) * elem; }
+  static int tiles(const Geom& g) {
+    return ((g.K[0] + GX - 1) / GX) * ((g.K[1] + GY - 1) / GY) * ((g.K[2] + GZ - 1) / GZ);
+  }
+};
+
 // ql = q + step p (the trial charges of a line search)
 __global__ void k_lincomb(int n, const double* __restrict__ q, const double* __restrict__ p,
                           double step, double* __restrict__ ql) {
