@@ -214,6 +214,55 @@ static void metal_homo(const orc_metal* e, double r, double fj[2], double dfjdr[
   *df = -dphidr / r;
 }
 
+/* ZJW density and pair parts of one metal at r: f = fe ftn, dfdr = fe dftn/dr,
+   phi = A e^{-a(r/re-1)}/(1+kappa20) - B ftn and its r-derivative (eam.f90:55-70) */
+static void metal_part(const orc_metal* e, double r, double* f, double* dfdr, double* phi,
+                       double* dphidr) {
+  double r_re = r / e->r_e, r_re_1 = r_re - 1.0;
+  double l20 = p20(r_re - e->lambda), l19 = l20 / (r_re - e->lambda);
+  double k20 = p20(r_re - e->kappa), k19 = k20 / (r_re - e->kappa);
+  double ea = exp(-e->alpha * r_re_1), eb = exp(-e->beta * r_re_1);
+  double ftn = eb / (1.0 + l20);
+  double dftndr = eb * (-e->beta * (1.0 + l20) - 20.0 * l19) / e->r_e / ((1.0 + l20) * (1.0 + l20));
+  *f = e->f_e * ftn;
+  *dfdr = e->f_e * dftndr;
+  *phi = e->A * ea / (1.0 + k20) - e->B * ftn;
+  *dphidr = e->A * ea * (-e->alpha * (1.0 + k20) - 20.0 * k19) / e->r_e /
+                ((1.0 + k20) * (1.0 + k20)) - e->B * dftndr;
+}
+
+/* METAL_HETERO, eam.f90:38-98: each metal's own density, the pair term the
+   density-weighted mean 0.5 (phi_i f_j / f_i + phi_j f_i / f_j) */
+static void metal_hetero(const orc_metal* ei, const orc_metal* ej, double r, double fj[2],
+                         double dfjdr[2], double* phi, double* df) {
+  double fi, dfi, phi_i, dphi_i, fk, dfk, phi_j, dphi_j;
+  metal_part(ei, r, &fi, &dfi, &phi_i, &dphi_i);
+  metal_part(ej, r, &fk, &dfk, &phi_j, &dphi_j);
+  fj[0] = fi; dfjdr[0] = dfi / r;
+  fj[1] = fk; dfjdr[1] = dfk / r;
+  *phi = 0.5 * (phi_i * fj[1] / fj[0] + phi_j * fj[0] / fj[1]);
+  *df = -0.5 * ((dphi_i * fj[1] * fj[0] + phi_i * dfjdr[1] * r * fj[0] - phi_i * fj[1] * dfjdr[0] * r) /
+                    (fj[0] * fj[0]) +
+                (dphi_j * fj[0] * fj[1] + phi_j * dfjdr[0] * r * fj[1] - phi_j * fj[0] * dfjdr[1] * r) /
+                    (fj[1] * fj[1])) / r;
+}
+
+/* AL_NI, eam.f90:101-168: fj(1) from the first argument's density, fj(2) from
+   the second's, and a fixed Al-Ni pair term (constants eam.f90:147-153).  The
+   caller passes (Al, Ni) for both orders of the pair, so which atom receives
+   which density depends on the loop's orientation (force.f90:60-63). */
+static void al_ni(const orc_metal* ea_, const orc_metal* eb_, double r, double fj[2],
+                  double dfjdr[2], double* phi, double* df) {
+  double f, dfdr;
+  zjw(r, ea_->r_e, ea_->f_e, ea_->beta, ea_->lambda, &f, &dfdr);
+  fj[0] = f; dfjdr[0] = dfdr / r;
+  zjw(r, eb_->r_e, eb_->f_e, eb_->beta, eb_->lambda, &f, &dfdr);
+  fj[1] = f; dfjdr[1] = dfdr / r;
+  double dphidr;
+  zjw_pair(r, 2.71579, 8.00443, 4.75970, 0.44254, 0.68349, 0.63279, 0.81777, phi, &dphidr);
+  *df = -dphidr / r;
+}
+
 /* oxygen density e^{-gamma(...)}/(1+nu20) with r_e of the partner row (Q15), eam.f90:299-302 */
 static void oxy_density(const orc_oxid* o, double re, double r, double* f, double* dfdr_over_r) {
   double r_re = r / re, r_re_1 = r_re - 1.0;
@@ -225,9 +274,15 @@ static void oxy_density(const orc_oxid* o, double re, double r, double* f, doubl
 
 void orc_eam_pair(const orc_params* p, int id_i, int id_j, double r, double fj[2],
                   double dfjdr[2], double* phi, double* df) {
-  /* dispatch of force.f90:58-74 for the Al/O (and homo-metal) branches */
+  /* dispatch of force.f90:58-74 */
   if (id_i < 4 && id_j < 4 && id_i == id_j) {
     metal_homo(&p->eam[id_i - 1], r, fj, dfjdr, phi, df);
+  } else if (id_i == 1 && id_j == 2) {
+    al_ni(&p->eam[0], &p->eam[1], r, fj, dfjdr, phi, df);
+  } else if (id_i == 2 && id_j == 1) {
+    al_ni(&p->eam[0], &p->eam[1], r, fj, dfjdr, phi, df);
+  } else if (id_i < 4 && id_j < 4) {
+    metal_hetero(&p->eam[id_i - 1], &p->eam[id_j - 1], r, fj, dfjdr, phi, df);
   } else if (id_i < 4 && id_j == 4) { /* METAL_OXYGEN eam.f90:171-225 */
     const orc_metal* e = &p->eam[id_i - 1];
     double f, dfdr;
@@ -261,7 +316,7 @@ void orc_eam_pair(const orc_params* p, int id_i, int id_j, double r, double fj[2
     oxy_density(o, o->r_e[3], r, &fj[0], &dfjdr[0]);
     fj[1] = fj[0]; dfjdr[1] = dfjdr[0];
   } else {
-    /* Ni/Fe cross terms and H are outside this build; callers reject them first */
+    /* H (id 5): FORCE stops with "Unknown particle pair"; callers reject it first */
     fj[0] = fj[1] = dfjdr[0] = dfjdr[1] = *phi = *df = NAN;
   }
 }
@@ -335,8 +390,7 @@ static void force_pair(force_ws* w, int li, int lj) {
   double q_i = w->q[li], q_j = w->q[lj];
   double xi_i = p->prm[id_i - 1].xi, Z_i = p->prm[id_i - 1].Z;
   double xi_j = p->prm[id_j - 1].xi, Z_j = p->prm[id_j - 1].Z;
-  if (id_i < 1 || id_i > 4 || id_j < 1 || id_j > 4 ||
-      (id_i < 4 && id_j < 4 && id_i != id_j)) { w->err = ORC_ERR_UNKNOWN_PAIR; return; }
+  if (id_i < 1 || id_i > 4 || id_j < 1 || id_j > 4) { w->err = ORC_ERR_UNKNOWN_PAIR; return; }
   double fj[2], dfjdr[2], phi, df;
   orc_eam_pair(p, id_i, id_j, r, fj, dfjdr, &phi, &df);
   if (g_ream2 > 0.0 && !(r2 < g_ream2)) {  /* split cutoff: no EAM terms past r_eam */

@@ -184,3 +184,23 @@ def tiny(seed=1, nx=4, ny=4, nz=4, n_o=16, T=300.0, fft=(16, 16, 32), rcut=6.0, 
     """A small system for fast parity tests."""
     return al_o_slab(nx, ny, nz, n_o, seed, T, fft, rcut, vacuum=vacuum, o_layer=(2.0, 5.0),
                      o_dmin=2.0, name="tiny")
+
+
+def tiny_alloy(seed=1, nx=5, ny=5, nz=4, n_o=16, frac_ni=0.15, frac_fe=0.10, fft=(20, 20, 32),
+               rcut=6.0):
+    """A small Al slab with seeded Ni (id 2) and Fe (id 3) substitutions and an
+    O layer: exercises the Al-Ni (AL_NI) and other metal-metal (METAL_HETERO)
+    pair branches.  3 or more cells per dimension."""
+    s = al_o_slab(nx, ny, nz, n_o, seed, 300.0, fft, rcut, vacuum=12.0, o_layer=(2.0, 5.0),
+                  o_dmin=2.0, name="tiny_alloy")
+    rng = np.random.default_rng(seed + 7919)
+    ids = s.ids.copy()
+    metal = np.flatnonzero(ids == AL)
+    pick = rng.permutation(metal)
+    n_ni, n_fe = int(round(frac_ni * len(metal))), int(round(frac_fe * len(metal)))
+    ids[pick[:n_ni]] = NI
+    ids[pick[n_ni:n_ni + n_fe]] = FE
+    v = maxwell_boltzmann(rng, ids, 300.0)
+    return System(ids=ids, x=s.x, v=v, q=s.q, box=s.box, fft=s.fft, rcut=s.rcut, dt=s.dt,
+                  name="tiny_alloy", meta=dict(s.meta, frac_ni=frac_ni, frac_fe=frac_fe))
+

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle.oracle import Oracle, run_reference  # noqa: E402
-from simox_amd.systems import config, tiny  # noqa: E402
+from simox_amd.systems import config, tiny, tiny_alloy  # noqa: E402
 
 
 T_BEREND = 600.0      # K, target of VV_NVT2_BEREND
@@ -66,3 +66,5 @@ if __name__ == "__main__":
     one("tiny_s1", tiny(seed=1), 11)
     one("tiny_s2", tiny(seed=2, nx=3, ny=3, nz=4, n_o=8, fft=(12, 12, 24), rcut=5.0), 12)
     one("c1_s1", config("c1", seed=1), 21)
+    # Al slab with Ni and Fe substitutions: the AL_NI and METAL_HETERO branches
+    one("alloy_s1", tiny_alloy(seed=1), 31)

@@ -1,0 +1,73 @@
+"""Ni/Fe cross pairs on the GPU: METAL_HETERO (eam.f90:38-98) and AL_NI
+(eam.f90:101-168) against the reference build (golden alloy_s1: an Al slab
+with 60 Ni and 40 Fe substitutions and 16 O, written by the reference's own
+FORCE).  AL_NI hands each atom of an Al-Ni pair the density of the
+reference's loop orientation (force.f90:60-63), so the densities pin that
+orientation too.  Bars as the other FAITHFUL parity tests: forces and energies
+1e-10 relative; MIXED against FP64 e_F <= 1e-5."""
+import numpy as np
+import pytest
+
+from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox, SimoxError
+from simox_amd.systems import tiny_alloy
+
+from test_gpu_parity import gpu_for, load, oracle_for, relmax
+
+# The device branches for Ni/Fe cross pairs faulted the GPU on their first
+# run (an illegal memory access during FORCE of alloy_s1).  Reading the new
+# indexing (pair table (id_j, id_i), cell ids of both atoms) found nothing out
+# of bounds, so no cause is shown yet: the branches stay off on the device
+# (FORCE returns SIMOX_ERR_UNKNOWN_PAIR for Ni/Fe cross pairs, as before) and
+# these tests are skipped.  The oracle's METAL_HETERO / AL_NI restatement is
+# pinned against the reference build by tests/test_oracle_ref.py (alloy_s1).
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="Ni/Fe cross-pair device path faulted the GPU on its first "
+                                      "run; cause not found, path switched off on the device")]
+
+
+@pytest.fixture(scope="module")
+def alloy():
+    return load("alloy_s1")
+
+
+def test_alloy_force_matches_reference_build(alloy):
+    d = alloy
+    assert set(np.unique(d["ids"])) == {1, 2, 3, 4}
+    g = gpu_for(d, MODE_FAITHFUL)
+    e = g.force()
+    f = g.get_state()[2]
+    assert relmax(f, d["ref_f"]) <= 1e-10
+    got = np.array([e["Epair"], e["Edir"], e["Ectip"], e["Egroup"]])
+    np.testing.assert_allclose(got, d["ref_E"][:4], rtol=1e-10, atol=1e-10)
+
+
+def test_alloy_densities_match_oracle(alloy, oracle_mod):
+    d = alloy
+    g = gpu_for(d, MODE_FAITHFUL)
+    g.force()
+    rho = g.rho()
+    o = oracle_for(oracle_mod, d)
+    ref = o.force(d["ids"], d["x"], d["q"])
+    np.testing.assert_allclose(rho, ref["rho"], rtol=1e-12, atol=1e-12)
+
+
+def test_alloy_mixed_matches_fp64(alloy):
+    d = alloy
+    out = {}
+    for mode in (MODE_FP64, MODE_MIXED):
+        g = gpu_for(d, mode)
+        g.set_charge_tol(1e-10, 3000)
+        g.ctip_solve()
+        g.force()
+        g.pme()
+        out[mode] = g.get_state()[2]
+    ref = out[MODE_FP64]
+    rms = np.sqrt((ref ** 2).sum(1).mean())
+    assert np.sqrt(((out[MODE_MIXED] - ref) ** 2).sum(1)).max() / rms <= 1e-5
+
+
+def test_al_ni_needs_three_cells_per_axis():
+    s = tiny_alloy(seed=2, nx=4, ny=4, nz=4, rcut=8.0, fft=(16, 16, 32))
+    g = Simox(s.n, s.box, s.fft, s.rcut)
+    with pytest.raises(SimoxError, match="three or more cells"):
+        g.set_state(s.ids, s.x, s.v, None, s.q)
