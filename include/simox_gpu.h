@@ -215,6 +215,20 @@ int simox_vv2(simox_ctx* ctx, double dt, double* mv2);
 int simox_step(simox_ctx* ctx, int nsteps, double dt, int sort_every, int ctip_every,
                double* stats);
 
+/* EAM in spline form (build-only; no counterpart in the reference, whose EAM
+   is analytic).  Eight uniform cubic B-splines of m coefficients each
+   (4 <= m <= 64), coef[f*m + k], function f valid on [lo[f], lo[f] + (m-3) h[f]]:
+   f = 0 density of an Al source, 1 density of an O source into Al, 2 into O,
+   3 pair term Al-Al, 4 Al-O, 5 O-O, 6 embedding F of Al, 7 of O.  They replace
+   the ZJW forms in FORCE's density, embedding and pair terms (Al/O systems
+   only: other species return SIMOX_ERR_UNKNOWN_PAIR).  coef = NULL switches
+   back to the analytic forms. */
+int simox_set_eam_splines(simox_ctx* ctx, int m, const double* lo, const double* h,
+                          const double* coef);
+/* grad[f*m + k] = dE/dcoef[f*m + k] of E = Epair + Egroup (the EAM energy of
+   FORCE) at the current positions, spline form set. */
+int simox_eam_spline_grad(simox_ctx* ctx, double* grad);
+
 /* Batched replicas (build-only): nrep independent contexts (each its own box,
    grid, atoms and HIP stream) advanced by the same simox_step sequence in
    lockstep.  The converged charge solves of all replicas proceed together:

@@ -22,7 +22,8 @@ EXPORTS = [
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
-    "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff",
+    "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
+    "simox_eam_spline_grad",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
 ]
@@ -33,6 +34,8 @@ T_SPREAD, T_FFT, T_GATHER, T_SPMV, T_FORCE, T_DENSITY = range(6)
 
 class SimoxError(RuntimeError):
     """Raised where the reference would STOP; carries the reference's text."""
+
+    _m = 0  # coefficients per spline function once set_eam_splines ran
 
     def __init__(self, code, text):
         super().__init__(text)
@@ -83,6 +86,8 @@ def load_library(path=LIB_PATH):
     lib.simox_set_charge_tol.argtypes = [vp, d, i]
     lib.simox_set_params.argtypes = [vp, vp]
     lib.simox_autotune.argtypes = [vp]
+    lib.simox_set_eam_splines.argtypes = [vp, i, vp, vp, vp]
+    lib.simox_eam_spline_grad.argtypes = [vp, vp]
     lib.simox_set_eam_cutoff.argtypes = [vp, d]
     lib.simox_set_thermostat.argtypes = [vp, i, d, d, C.c_ulonglong]
     lib.simox_set_rdf.argtypes = [vp, i]
@@ -340,6 +345,25 @@ class Simox:
             return None
         keys = ["Eself", "Edir", "Erec", "Ectip", "Epair", "Egroup", "mv2", "Etotal"]
         return dict(zip(keys, st.tolist()))
+
+    def set_eam_splines(self, lo, h, coef):
+        """Spline-form EAM (coef [8, m]); coef None: back to the analytic forms."""
+        if coef is None:
+            self._check(self.lib.simox_set_eam_splines(self.h, 0, None, None, None))
+            self._m = 0
+            return
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self._check(self.lib.simox_set_eam_splines(self.h, coef.shape[1], _ptr(lo), _ptr(h),
+                                                   _ptr(coef)))
+        self._m = coef.shape[1]
+
+    def eam_spline_grad(self):
+        """dE/dcoef [8, m] of E = Epair + Egroup at the current positions."""
+        g = np.zeros((8, self._m))
+        self._check(self.lib.simox_eam_spline_grad(self.h, _ptr(g)))
+        return g
 
     def timing(self, on=True):
         self._check(self.lib.simox_timing_enable(self.h, 1 if on else 0))

@@ -56,10 +56,25 @@ struct SpecCoef {      // per-species CTIP data (ctip.f90:33-39) and mass
   double chi, J, Jp, Z, qmin, qmax, xm, xi;  // Jp = J - 2 kc a / sqrt(pi)
 };
 
+// EAM in spline form (build-only, Al/O): eight functions, each a uniform
+// cubic B-spline s(x) = sum_k c_k B((x - lo)/h + 1 - k), k = 0..m-1, valid on
+// [lo, lo + (m - 3) h]:
+//   0 density of an Al source (into any atom), 1 density of an O source into
+//   Al, 2 density of an O source into O, 3 pair term Al-Al, 4 Al-O, 5 O-O,
+//   6 embedding F of Al, 7 of O.
+constexpr int NSPL = 8;
+constexpr int SPL_MAX = 64;
+struct SplineSet {
+  int on, m;
+  double lo[NSPL], h[NSPL];
+  double c[NSPL][SPL_MAX];
+};
+
 struct Tables {
   PairCoef pc[NSPEC * NSPEC];
   EmbedCoef ec[NSPEC];
   SpecCoef sc[NSPEC];
+  SplineSet sp;
 };
 
 // Geometry passed by value to kernels.

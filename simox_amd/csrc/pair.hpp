@@ -147,6 +147,58 @@ __device__ inline void embed(const EmbedCoef& e, T rho, T& F, T& dF) {
 }
 
 // ---------------------------------------------------------------------------
+// EAM in spline form (build-only, Al/O; the layout is SplineSet's).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int spl_dens(int si, int sj) {  // density into si from sj
+  return sj == 1 ? 0 : (si == 1 ? 1 : 2);
+}
+__host__ __device__ inline int spl_pair(int si, int sj) {
+  return (si == 1 && sj == 1) ? 3 : ((si == 4 && sj == 4) ? 5 : 4);
+}
+__host__ __device__ inline int spl_embed(int si) { return si == 1 ? 6 : 7; }
+__host__ __device__ inline bool spl_species_ok(int s) { return s == 1 || s == 4; }
+
+// Cubic B-spline weights of x on function f: returns the first of the four
+// coefficient indices; w = weights, dw = their x-derivatives.  x is clamped to
+// the valid range (derivatives 0 outside it).
+template <class T>
+__device__ inline int spl_basis(const SplineSet& sp, int f, T x, T w[4], T dw[4]) {
+  const T h = T(sp.h[f]);
+  T u = (x - T(sp.lo[f])) / h + T(1);
+  const T ulo = T(1), uhi = T(sp.m - 2) - T(1e-6);
+  bool clamped = false;
+  if (!(u >= ulo)) { u = ulo; clamped = true; }  // also catches NaN
+  if (!(u <= uhi)) { u = uhi; clamped = true; }
+  int k0 = int(u);  // 1 .. m-3, also where uhi rounds up in float
+  k0 = k0 < 1 ? 1 : (k0 > sp.m - 3 ? sp.m - 3 : k0);
+  const T t = u - T(k0), t2 = t * t, t3 = t2 * t, s1 = T(1) - t;
+  w[0] = s1 * s1 * s1 / T(6);
+  w[1] = (T(3) * t3 - T(6) * t2 + T(4)) / T(6);
+  w[2] = (-T(3) * t3 + T(3) * t2 + T(3) * t + T(1)) / T(6);
+  w[3] = t3 / T(6);
+  const T ih = clamped ? T(0) : T(1) / h;
+  dw[0] = -s1 * s1 / T(2) * ih;
+  dw[1] = (T(3) * t2 - T(4) * t) / T(2) * ih;
+  dw[2] = (-T(3) * t2 + T(2) * t + T(1)) / T(2) * ih;
+  dw[3] = t2 / T(2) * ih;
+  return k0 - 1;
+}
+
+template <class T>
+__device__ inline void spl_eval(const SplineSet& sp, int f, T x, T& v, T& dv) {
+  T w[4], dw[4];
+  const int k = spl_basis<T>(sp, f, x, w, dw);
+  v = T(0);
+  dv = T(0);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const T c = T(sp.c[f][k + m]);
+    v += c * w[m];
+    dv += c * dw[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Full-shell neighbour walk over the 27 cells around atom i's cell.
 // ---------------------------------------------------------------------------
 template <class Fn>
@@ -217,9 +269,13 @@ __global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* _
         ++k;
         const int sj = id[j];
         if (bad || sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid) { bad = true; return; }
+        if (tb->sp.on && !(spl_species_ok(si) && spl_species_ok(sj))) { bad = true; return; }
         if (r2 < g.ream2) {  // split cutoff: EAM only inside r_eam
           T f, dfr;
-          dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
+          if (tb->sp.on)
+            spl_eval<T>(tb->sp, spl_dens(si, sj), T(sqrt(r2)), f, dfr);
+          else
+            dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
           rho += f;
         }
       }
@@ -228,7 +284,12 @@ __global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* _
     if (k > NNB_MAX || k > cap) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
     cnt[i] = k < cap ? k : cap;
     T F = T(0), dF = T(0);
-    if (!bad) embed<T>(tb->ec[si - 1], rho, F, dF);
+    if (!bad) {
+      if (tb->sp.on)
+        spl_eval<T>(tb->sp, spl_embed(si), rho, F, dF);
+      else
+        embed<T>(tb->ec[si - 1], rho, F, dF);
+    }
     rho_out[i] = double(rho);
     dF_out[i] = dF;
     eg[0] = double(F);
@@ -297,9 +358,18 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
         const T r = T(sqrt(r2d));
         const T qj = T(q[j]);
         T fi_, dfi, fj_, dfj, phi, dfp;
-        dens_eval<T>(pc.di, r, fi_, dfi);
-        dens_eval<T>(pc.dj, r, fj_, dfj);
-        phi_eval<T>(pc.ph, r, phi, dfp);
+        if (tb->sp.on) {  // spline EAM: dfi, dfj, dfp in the analytic forms' units (/r)
+          spl_eval<T>(tb->sp, spl_dens(sj, si), r, fi_, dfi);
+          spl_eval<T>(tb->sp, spl_dens(si, sj), r, fj_, dfj);
+          spl_eval<T>(tb->sp, spl_pair(si, sj), r, phi, dfp);
+          dfi /= r;
+          dfj /= r;
+          dfp = -dfp / r;
+        } else {
+          dens_eval<T>(pc.di, r, fi_, dfi);
+          dens_eval<T>(pc.dj, r, fj_, dfj);
+          phi_eval<T>(pc.ph, r, phi, dfp);
+        }
         // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356),
         // inside the EAM cutoff only (= Rcut unless split)
         const bool eam = r2d < g.ream2;
@@ -337,6 +407,66 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
     for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x)
       if (hist[k]) atomicAdd(&rdf[k], (unsigned long long)hist[k]);
   }
+}
+
+// Gradient of the spline-form EAM energy E = sum_pairs phi(r) + sum_i F(rho_i)
+// with respect to the spline coefficients (build-only):
+//   dE/dc(pair)_k = sum over pairs of B_k(r),
+//   dE/dc(F)_k    = sum over atoms of B_k(rho_i),
+//   dE/dc(dens)_k = sum_i F'(rho_i) sum_j B_k(r_ij) over the sources j.
+// Each block accumulates its atoms' terms in LDS in 64-bit fixed point at
+// 2^32 (integer adds: order-independent, so deterministic) and writes one
+// partial row of NSPL * SPL_MAX values.
+constexpr double SPL_SCALE = 4294967296.0;
+__global__ void __launch_bounds__(BLOCK) k_eam_spline_grad(Geom g, const Tables* __restrict__ tb,
+                                                           const double* __restrict__ x,
+                                                           const int* __restrict__ id,
+                                                           const double* __restrict__ rho,
+                                                           const int* __restrict__ cnt,
+                                                           const int* __restrict__ col,
+                                                           double* __restrict__ part) {
+  __shared__ unsigned long long acc[NSPL * SPL_MAX];
+  for (int k = threadIdx.x; k < NSPL * SPL_MAX; k += blockDim.x) acc[k] = 0ull;
+  __syncthreads();
+  const SplineSet& sp = tb->sp;
+  auto add = [&](int f, int k0, const double w[4], double scale) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      atomicAdd(&acc[f * SPL_MAX + k0 + m], (unsigned long long)llrint(scale * w[m] * SPL_SCALE));
+  };
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.n) {
+    const int si = id[i];
+    double w[4], dw[4], F, dF;
+    const int fe = spl_embed(si);
+    add(fe, spl_basis<double>(sp, fe, rho[i], w, dw), w, 1.0);
+    spl_eval<double>(sp, fe, rho[i], F, dF);
+    const size_t n = size_t(g.n);
+    for (int k = 0; k < cnt[i]; ++k) {
+      const int j = col[size_t(k) * n + i];
+      double rx[3], r2;
+      pair_sep(g, x, i, j, rx, r2);
+      if (!(r2 < g.ream2)) continue;
+      const double r = sqrt(r2);
+      const int sj = id[j];
+      const int fp = spl_pair(si, sj), fd = spl_dens(si, sj);
+      add(fp, spl_basis<double>(sp, fp, r, w, dw), w, 0.5);
+      add(fd, spl_basis<double>(sp, fd, r, w, dw), w, dF);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NSPL * SPL_MAX; k += blockDim.x)
+    part[size_t(blockIdx.x) * (NSPL * SPL_MAX) + k] = double((long long)acc[k]) / SPL_SCALE;
+}
+
+// out[v] = sum over the nparts rows of part[row * nv + v], rows in order
+__global__ void k_sum_rows(const double* __restrict__ part, int nparts, int nv,
+                           double* __restrict__ out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double s = 0.0;
+  for (int r = 0; r < nparts; ++r) s += part[size_t(r) * nv + v];
+  out[v] = s;
 }
 
 // K5: CTIP sparse matrix over the neighbour list: val[k*n + i] = sym_ij for

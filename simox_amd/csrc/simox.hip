@@ -117,6 +117,7 @@ struct simox_ctx {
   PmeGrid pm, pc;
   bool prec_on = false;
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
+  double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
   double* sv = nullptr;
   double* cv = nullptr;  // 1/sqrt(J) per atom, the preconditioner's fixed scaling
   // CTIP
@@ -1651,7 +1652,7 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
@@ -1964,6 +1965,62 @@ int simox_force(simox_ctx* ctx, double energies[4]) {
   if (energies) {
     energies[0] = ctx->Epair; energies[1] = ctx->Edir; energies[2] = ctx->Ectip; energies[3] = ctx->Egroup;
   }
+  return SIMOX_OK;
+}
+
+int simox_set_eam_splines(simox_ctx* ctx, int m, const double* lo, const double* h,
+                          const double* coef) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  SplineSet& sp = ctx->htab.sp;
+  if (!coef) {
+    sp.on = 0;
+  } else {
+    if (m < 4 || m > SPL_MAX || !lo || !h) {
+      ctx->err = "EAM splines: 4 <= m <= 64 coefficients per function, lo and h required";
+      return SIMOX_ERR_ARG;
+    }
+    for (int f = 0; f < NSPL; ++f)
+      if (!(h[f] > 0.0)) {
+        ctx->err = "EAM splines: knot spacing must be positive";
+        return SIMOX_ERR_ARG;
+      }
+    sp.on = 1;
+    sp.m = m;
+    for (int f = 0; f < NSPL; ++f) {
+      sp.lo[f] = lo[f];
+      sp.h[f] = h[f];
+      for (int k = 0; k < SPL_MAX; ++k) sp.c[f][k] = k < m ? coef[size_t(f) * m + k] : 0.0;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(ctx->dtab, &ctx->htab, sizeof(Tables), hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->nlist_valid = false;  // densities and embedding follow the new functions
+  return SIMOX_OK;
+}
+
+int simox_eam_spline_grad(simox_ctx* ctx, double* grad) {
+  if (!ctx || !grad || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (!ctx->htab.sp.on) {
+    ctx->err = "EAM splines are not set";
+    return SIMOX_ERR_ARG;
+  }
+  const int n = ctx->n, nb = grid_blocks(n), nv = NSPL * SPL_MAX;
+  CHK(ensure_nlist(ctx));
+  CHK(check_err(ctx, 0));
+  if (!ctx->spl_grad) CHK(dalloc(ctx, &ctx->spl_grad, nv));
+  CHK(ensure_part(ctx, size_t(nb) * nv));
+  hipLaunchKernelGGL(k_eam_spline_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                     ctx->x, ctx->id, ctx->rho, ctx->nb_cnt, ctx->nb_col, ctx->part);
+  hipLaunchKernelGGL(k_sum_rows, dim3((nv + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->st,
+                     ctx->part, nb, nv, ctx->spl_grad);
+  HIPCHK(hipGetLastError());
+  std::vector<double> all(nv);
+  HIPCHK(hipMemcpyAsync(all.data(), ctx->spl_grad, nv * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const int m = ctx->htab.sp.m;
+  for (int f = 0; f < NSPL; ++f)
+    for (int k = 0; k < m; ++k) grad[size_t(f) * m + k] = all[size_t(f) * SPL_MAX + k];
   return SIMOX_OK;
 }
 
