@@ -134,6 +134,11 @@ int simox_set_reflect(simox_ctx* ctx, int on);
    lambda (caller order, may be NULL) receives the adjoint; *iters its CG
    iterations. */
 int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters);
+/* As simox_param_grad, plus grad[10+s] = dE/dxi_{s+1}, the Slater exponent
+   (zeta) of each species, with the [fa|fb] coefficients (datafmt.f90:201-220)
+   following it: kc dEctip/dxi at fixed charges minus lambda . dG/dxi of the
+   solver's matrix, by forward-mode dual numbers through the same formulas. */
+int simox_param_grad_all(simox_ctx* ctx, double grad[15], double* lambda, int* iters);
 int simox_remove_rigid_motion(simox_ctx* ctx);
 int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x

@@ -23,7 +23,7 @@ EXPORTS = [
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
-    "simox_eam_spline_grad",
+    "simox_eam_spline_grad", "simox_param_grad_all",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
 ]
@@ -93,6 +93,7 @@ def load_library(path=LIB_PATH):
     lib.simox_set_rdf.argtypes = [vp, i]
     lib.simox_set_reflect.argtypes = [vp, i]
     lib.simox_param_grad.argtypes = [vp, P(d), vp, P(i)]
+    lib.simox_param_grad_all.argtypes = [vp, P(d), vp, P(i)]
     lib.simox_batch_step.argtypes = [P(vp), i, i, d, i, i, vp]
     lib.simox_remove_rigid_motion.argtypes = [vp]
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
@@ -192,14 +193,18 @@ class Simox:
         self._check(self.lib.simox_set_thermostat(self.h, int(kind), float(t_kelvin),
                                                   float(alpha), int(seed)))
 
-    def param_grad(self, want_lambda=False):
-        """dE/dchi (5), dE/dJ (5) of E = Eself + Edir + Erec + Ectip at the
-        converged charges, through the charge equilibration (adjoint solve)."""
-        g = (C.c_double * 10)()
+    def param_grad(self, want_lambda=False, with_xi=False):
+        """dE/dchi (5), dE/dJ (5) and, with_xi, dE/dxi (5) of E = Eself + Edir +
+        Erec + Ectip at the converged charges, through the charge
+        equilibration (adjoint solve)."""
+        g = (C.c_double * 15)()
         lam = np.zeros(self.n) if want_lambda else None
         it = C.c_int()
-        self._check(self.lib.simox_param_grad(self.h, g, _ptr(lam), C.byref(it)))
+        fn = self.lib.simox_param_grad_all if with_xi else self.lib.simox_param_grad
+        self._check(fn(self.h, g, _ptr(lam), C.byref(it)))
         out = dict(dchi=np.array(g[0:5]), dJ=np.array(g[5:10]), iters=it.value)
+        if with_xi:
+            out["dxi"] = np.array(g[10:15])
         if want_lambda:
             out["lambda"] = lam
         return out

@@ -2,7 +2,9 @@
 gradcheck / fitting harness (build-only; the reference has no derivatives).
 
 E(theta) = Eself + Edir + Erec + Ectip of the MD step at the charges the
-converged solve returns, theta = (chi_1..5, J_1..5) for Al, Ni, Fe, O, H.
+converged solve returns, theta = (chi_1..5, J_1..5) for Al, Ni, Fe, O, H, and
+optionally (xi_1..5), the Slater exponents (zeta), with the [fa|fb]
+coefficients recomputed from them (datafmt.f90:201-220).
 The backward pass uses the library's adjoint solve through the charge
 equilibration (simox_param_grad); torch is only the autograd plumbing here.
 """
@@ -11,21 +13,29 @@ import ctypes as C
 import numpy as np
 import torch
 
-from .api import MODE_FP64, Simox, potential_setup
+from .api import MODE_FP64, Simox, load_library, potential_setup
 
 
 def params_with(theta, base=None):
-    """A copy of the potential parameters with chi, J replaced by theta."""
+    """A copy of the potential parameters with chi, J (and, for 15 entries,
+    xi with the [fa|fb] coefficients that follow from it) replaced by theta."""
     p = potential_setup() if base is None else _copy(base)
     t = np.asarray(theta, dtype=np.float64)
     for s in range(5):
         p.prm[s].chi = float(t[s])
         p.prm[s].J = float(t[5 + s])
+    if len(t) == 15:
+        for s in range(5):
+            p.prm[s].xi = float(t[10 + s])
+        load_library().simox_fafb(C.byref(p))
     return p
 
 
-def theta_of(params):
-    return np.array([params.prm[s].chi for s in range(5)] + [params.prm[s].J for s in range(5)])
+def theta_of(params, with_xi=False):
+    t = [params.prm[s].chi for s in range(5)] + [params.prm[s].J for s in range(5)]
+    if with_xi:
+        t += [params.prm[s].xi for s in range(5)]
+    return np.array(t)
 
 
 def _copy(p):
@@ -47,15 +57,17 @@ def solve_energy(system, theta, mode=MODE_FP64, tol=1e-12, q0=None):
 
 
 class CtipEnergy(torch.autograd.Function):
-    """theta (float64 tensor [10]) -> E (scalar tensor) for a fixed system."""
+    """theta (float64 tensor [10], or [15] with xi) -> E (scalar tensor) for a
+    fixed system."""
 
     @staticmethod
     def forward(ctx, theta, system, mode=MODE_FP64, tol=1e-12):
-        E, g = solve_energy(system, theta.detach().cpu().numpy(), mode, tol)
-        gr = g.param_grad()
+        th = theta.detach().cpu().numpy()
+        E, g = solve_energy(system, th, mode, tol)
+        gr = g.param_grad(with_xi=len(th) == 15)
         g.close()
-        ctx.save_for_backward(torch.tensor(np.concatenate([gr["dchi"], gr["dJ"]]),
-                                           dtype=theta.dtype))
+        parts = [gr["dchi"], gr["dJ"]] + ([gr["dxi"]] if len(th) == 15 else [])
+        ctx.save_for_backward(torch.tensor(np.concatenate(parts), dtype=theta.dtype))
         return torch.tensor(E, dtype=theta.dtype)
 
     @staticmethod

@@ -183,4 +183,99 @@ __global__ void __launch_bounds__(BLOCK) k_param_grad(int n, const int* __restri
     for (int t = 0; t < 10; ++t) part[10 * blockIdx.x + t] = acc[t];
 }
 
+// ---------------------------------------------------------------------------
+// Derivatives in the Slater exponents xi (zeta) by forward-mode dual numbers
+// through the same CTIP formulas (ctip_force_core, ctip_matrix_core) and the
+// [fa|fb] coefficients of POTENTIAL_SETUP (datafmt.f90:201-220).
+// ---------------------------------------------------------------------------
+struct Dual {
+  double v, d;
+  __host__ __device__ Dual(double a = 0.0, double b = 0.0) : v(a), d(b) {}
+};
+__device__ inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline Dual operator/(Dual a, Dual b) {
+  return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v));
+}
+template <>
+__device__ inline Dual fexp<Dual>(Dual a) {
+  const double e = exp(a.v);
+  return Dual(e, a.d * e);
+}
+__device__ inline Dual erfc(Dual a) {
+  return Dual(::erfc(a.v), -a.d * 2.0 / SQRTPI * exp(-a.v * a.v));
+}
+
+// [fa|fb] coefficients of exponents a (row species) and b (column species)
+template <class S>
+__device__ inline void fafb_core(bool same, S a, S b, S c[4]) {
+  if (same) {
+    c[0] = S(1);
+    c[1] = S(11) * a / S(8);
+    c[2] = S(0.75) * (a * a);
+    c[3] = (a * a * a) / S(6);
+  } else {
+    const S s = a + b, da = a - b, db = b - a;
+    const S a2 = a * a, b2 = b * b, a4 = a2 * a2, b4 = b2 * b2;
+    c[0] = a * b4 / (s * s) / (da * da);
+    c[1] = b * a4 / (s * s) / (db * db);
+    c[2] = (S(3) * a2 * b4 - b4 * b2) / (s * s * s) / (da * da * da);
+    c[3] = (S(3) * b2 * a4 - a4 * a2) / (s * s * s) / (db * db * db);
+  }
+}
+
+// part[10k + s] = half the sum over the block's list entries of dEctip/dxi_s
+// / kc (FORCE's Q2 form), part[10k + 5 + s] = sum of lambda_i dG_i/dxi_s
+// (the solver's Q3 matrix and its b_i = sum_j Z_j asym_ij), s = species - 1.
+__global__ void __launch_bounds__(BLOCK) k_xi_grad(Geom g, const Tables* __restrict__ tb,
+                                                   const double* __restrict__ x,
+                                                   const int* __restrict__ id,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ lam,
+                                                   const int* __restrict__ cnt,
+                                                   const int* __restrict__ col,
+                                                   double* __restrict__ part) {
+  __shared__ double lds[10 * BLOCK / WAVE];
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.n) {
+    const int si = id[i];
+    const double qi = q[i], li = lam[i];
+    const size_t n = size_t(g.n);
+    for (int k = 0; k < cnt[i]; ++k) {
+      const int j = col[size_t(k) * n + i];
+      double rx[3], r2;
+      pair_sep(g, x, i, j, rx, r2);
+      const double r = sqrt(r2);
+      const int sj = id[j];
+      const double qj = q[j];
+      const PairCoef& pc = tb->pc[pidx(si, sj)];
+      const bool same = pc.same != 0;
+      // same species: one exponent, seeded in both places; else each in turn
+      for (int pass = 0; pass < (same ? 1 : 2); ++pass) {
+        const Dual xa(pc.xi_i, (same || pass == 0) ? 1.0 : 0.0);
+        const Dual xb(pc.xi_j, (same || pass == 1) ? 1.0 : 0.0);
+        Dual c[4];
+        fafb_core<Dual>(same, xa, xb, c);
+        Dual df, e, sym, asym;
+        ctip_force_core<Dual>(same, xa, xb, Dual(pc.Z_i), Dual(pc.Z_j), c, Dual(r), Dual(r2),
+                              Dual(qi), Dual(qj), df, e);
+        ctip_matrix_core<Dual>(same, xa, xb, c, Dual(r), Dual(g.alpha), sym, asym);
+        const int sp = (pass == 0 ? si : sj) - 1;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          if (t != sp) continue;
+          acc[t] += 0.5 * e.d;
+          acc[5 + t] += li * (sym.d * qj + pc.Z_j * asym.d);
+        }
+      }
+    }
+  }
+  block_sum<10>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int t = 0; t < 10; ++t) part[10 * blockIdx.x + t] = acc[t];
+}
+
 }  // namespace sx

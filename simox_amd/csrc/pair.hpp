@@ -47,57 +47,72 @@ __device__ inline void phi_eval(const PhiCoef& c, T r, T& phi, T& df) {
 }
 
 // CTIP short-range Slater terms of FORCE (force.f90:101-140).  Same-species
-// fifj lacks the 1/r factor (quirk Q2) and is reproduced as written.
+// fifj lacks the 1/r factor (quirk Q2) and is reproduced as written.  The
+// core takes the exponents and [fa|fb] coefficients as values of S (the
+// working type, or a dual number for derivatives in the exponents).
+template <class S>
+__device__ inline void ctip_force_core(bool same, S xi_i, S xi_j, S Zi, S Zj, const S c[4], S r,
+                                       S r2, S qi, S qj, S& df, S& e) {
+  S jfi, ifj, djfidr, difjdr, fifj, dfifjdr;
+  S ir = S(1) / r;
+  if (same) {
+    S xir = S(2) * xi_i * r, ei = fexp<S>(-xir);
+    jfi = -ei * (xi_i + ir);
+    ifj = jfi;
+    djfidr = ei * (S(2) * (xi_i * xi_i) + (S(1) + xir) / r2);
+    difjdr = djfidr;
+    const S c1 = c[1], c2 = c[2], c3 = c[3];
+    fifj = -ei * (S(1) + c1 * r + c2 * r2 + c3 * r2 * r);
+    dfifjdr = ei * ((xir + S(1)) * (S(1) / r2 + c1 * ir + c2 + c3 * r) -
+                    (c1 * ir + c2 * S(2) + c3 * S(3) * r));
+  } else {
+    S xir_i = S(2) * xi_i * r, ei = fexp<S>(-xir_i);
+    S xir_j = S(2) * xi_j * r, ej = fexp<S>(-xir_j);
+    jfi = -ei * (xi_i + ir);
+    ifj = -ej * (xi_j + ir);
+    djfidr = ei * (S(2) * (xi_i * xi_i) + (S(1) + xir_i) / r2);
+    difjdr = ej * (S(2) * (xi_j * xi_j) + (S(1) + xir_j) / r2);
+    const S c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    fifj = -c0 * ei - c1 * ej - c2 * ei / r - c3 * ej / r;
+    dfifjdr = c0 * ei * S(2) * xi_i + c1 * ej * S(2) * xi_j + c2 * ei * (S(1) + xir_i) / r2 +
+              c3 * ej * (S(1) + xir_j) / r2;
+  }
+  df = S(KC) * ((qi * Zj + qj * Zi - qi * qj) * dfifjdr - qi * Zj * djfidr - qj * Zi * difjdr) / r;
+  e = qi * Zj * (jfi - fifj) + qj * Zi * (ifj - fifj) + qi * qj * fifj;
+}
+
 template <class T>
 __device__ inline void ctip_force_terms(const PairCoef& pc, T r, T r2, T qi, T qj, T& df,
                                         T& e) {
-  const T xi_i = T(pc.xi_i), xi_j = T(pc.xi_j), Zi = T(pc.Z_i), Zj = T(pc.Z_j);
-  T jfi, ifj, djfidr, difjdr, fifj, dfifjdr;
-  T ir = T(1) / r;
-  if (pc.same) {
-    T xir = T(2) * xi_i * r, ei = fexp<T>(-xir);
-    jfi = -ei * (xi_i + ir);
-    ifj = jfi;
-    djfidr = ei * (T(2) * (xi_i * xi_i) + (T(1) + xir) / r2);
-    difjdr = djfidr;
-    const T c1 = T(pc.c[1]), c2 = T(pc.c[2]), c3 = T(pc.c[3]);
-    fifj = -ei * (T(1) + c1 * r + c2 * r2 + c3 * r2 * r);
-    dfifjdr = ei * ((xir + T(1)) * (T(1) / r2 + c1 * ir + c2 + c3 * r) -
-                    (c1 * ir + c2 * T(2) + c3 * T(3) * r));
-  } else {
-    T xir_i = T(2) * xi_i * r, ei = fexp<T>(-xir_i);
-    T xir_j = T(2) * xi_j * r, ej = fexp<T>(-xir_j);
-    jfi = -ei * (xi_i + ir);
-    ifj = -ej * (xi_j + ir);
-    djfidr = ei * (T(2) * (xi_i * xi_i) + (T(1) + xir_i) / r2);
-    difjdr = ej * (T(2) * (xi_j * xi_j) + (T(1) + xir_j) / r2);
-    const T c0 = T(pc.c[0]), c1 = T(pc.c[1]), c2 = T(pc.c[2]), c3 = T(pc.c[3]);
-    fifj = -c0 * ei - c1 * ej - c2 * ei / r - c3 * ej / r;
-    dfifjdr = c0 * ei * T(2) * xi_i + c1 * ej * T(2) * xi_j + c2 * ei * (T(1) + xir_i) / r2 +
-              c3 * ej * (T(1) + xir_j) / r2;
-  }
-  df = T(KC) * ((qi * Zj + qj * Zi - qi * qj) * dfifjdr - qi * Zj * djfidr - qj * Zi * difjdr) / r;
-  e = qi * Zj * (jfi - fifj) + qj * Zi * (ifj - fifj) + qi * qj * fifj;
+  const T c[4] = {T(pc.c[0]), T(pc.c[1]), T(pc.c[2]), T(pc.c[3])};
+  ctip_force_core<T>(pc.same != 0, T(pc.xi_i), T(pc.xi_j), T(pc.Z_i), T(pc.Z_j), c, r, r2, qi,
+                     qj, df, e);
 }
 
 // CTIP solver matrix entries for row i (ctip.f90:81-110): sym_ij and asym_ij.
 // Same-species [fi|fj] is the solver's own variant (quirk Q3).
+template <class S>
+__device__ inline void ctip_matrix_core(bool same, S xi_i, S xi_j, const S c[4], S r, S alpha,
+                                        S& sym, S& asym) {
+  S ej = fexp<S>(S(-2) * xi_j * r), ej_r = ej / r;
+  S afb_i, fifj;
+  if (same) {
+    afb_i = -xi_j * ej - ej_r;
+    S xir = xi_i * r;
+    fifj = -ej * (S(1) + xir * (c[1] + xir * (c[2] + xir * c[3]))) / r;
+  } else {
+    S ei = fexp<S>(S(-2) * xi_i * r), ei_r = ei / r;
+    afb_i = -xi_i * ei - ei_r;
+    fifj = -ei * (c[0] + c[2] / r) - ej * (c[1] + c[3] / r);
+  }
+  asym = S(KC) * (afb_i - fifj);
+  sym = S(KC) * (fifj + erfc(alpha * r) / r);
+}
+
 template <class T>
 __device__ inline void ctip_matrix_terms(const PairCoef& pc, T r, T alpha, T& sym, T& asym) {
-  const T xi_i = T(pc.xi_i), xi_j = T(pc.xi_j);
-  T ej = fexp<T>(T(-2) * xi_j * r), ej_r = ej / r;
-  T afb_i, fifj;
-  if (pc.same) {
-    afb_i = -xi_j * ej - ej_r;
-    T xir = xi_i * r;
-    fifj = -ej * (T(1) + xir * (T(pc.c[1]) + xir * (T(pc.c[2]) + xir * T(pc.c[3])))) / r;
-  } else {
-    T ei = fexp<T>(T(-2) * xi_i * r), ei_r = ei / r;
-    afb_i = -xi_i * ei - ei_r;
-    fifj = -ei * (T(pc.c[0]) + T(pc.c[2]) / r) - ej * (T(pc.c[1]) + T(pc.c[3]) / r);
-  }
-  asym = T(KC) * (afb_i - fifj);
-  sym = T(KC) * (fifj + erfc(alpha * r) / r);
+  const T c[4] = {T(pc.c[0]), T(pc.c[1]), T(pc.c[2]), T(pc.c[3])};
+  ctip_matrix_core<T>(pc.same != 0, T(pc.xi_i), T(pc.xi_j), c, r, alpha, sym, asym);
 }
 
 // ---------------------------------------------------------------------------

@@ -1834,7 +1834,7 @@ int adjoint_solve(simox_ctx* ctx, double tol, int maxit, int* iters) {
   return 0;
 }
 
-int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters) {
+int param_grad_impl(simox_ctx* ctx, double* grad, bool with_xi, double* lambda, int* iters) {
   if (!ctx || !need_state(ctx) || !grad) return SIMOX_ERR_ARG;
   if (ctx->mode == SIMOX_MODE_FAITHFUL) {
     ctx->err = "parameter gradients need a converged charge solve (modes FP64, MIXED)";
@@ -1856,8 +1856,26 @@ int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters
                      ctx->adj + n, ctx->part);
   CHK(reduce(ctx, nb, 10, 0, 50));
   CHK(fetch(ctx, 50, 10, grad));
+  if (with_xi) {
+    // dE/dxi_s = kc * dEctip/dxi_s (at fixed q) - lambda . dG/dxi_s
+    hipLaunchKernelGGL(k_xi_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                       ctx->id, ctx->q, ctx->adj + n, ctx->nb_cnt, ctx->nb_col, ctx->part);
+    HIPCHK(hipGetLastError());
+    CHK(reduce(ctx, nb, 10, 0, 50));
+    double t[10];
+    CHK(fetch(ctx, 50, 10, t));
+    for (int sp = 0; sp < 5; ++sp) grad[10 + sp] = KC * t[sp] - t[5 + sp];
+  }
   if (lambda) CHK(download_caller(ctx, ctx->adj + n, 1, lambda));
   return SIMOX_OK;
+}
+
+int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters) {
+  return param_grad_impl(ctx, grad, false, lambda, iters);
+}
+
+int simox_param_grad_all(simox_ctx* ctx, double grad[15], double* lambda, int* iters) {
+  return param_grad_impl(ctx, grad, true, lambda, iters);
 }
 
 int simox_set_reflect(simox_ctx* ctx, int on) {

@@ -47,3 +47,48 @@ def test_torch_gradcheck_and_fit(system):
     mask[[0, 3]] = 1.0  # fit chi_Al, chi_O to a shifted energy
     _, hist = fit(system, E0 - 0.05, t0, mask=mask, steps=4)
     assert hist[-1] < 1e-3 * hist[0]
+
+
+def test_xi_gradient_matches_central_differences(system):
+    """dE/dxi_s (zeta, with [fa|fb] recomputed, datafmt.f90:201-220) from the
+    dual-number kernel and the adjoint, against central differences of E over
+    re-converged solves; step 1e-6, relative 1e-6 above the rounding floor."""
+    from simox_amd.autograd import potential_setup, solve_energy, theta_of
+    t0 = theta_of(potential_setup(), with_xi=True)
+    E0, g = solve_energy(system, t0)
+    gr = g.param_grad(with_xi=True)
+    g.close()
+    grad = gr["dxi"]
+    for s in (0, 3):  # Al, O
+        k = 10 + s
+        h = 1e-6
+        tp, tm = t0.copy(), t0.copy()
+        tp[k] += h
+        tm[k] -= h
+        ep = solve_energy(system, tp)[0]
+        em = solve_energy(system, tm)[0]
+        fd = (ep - em) / (2 * h)
+        floor = 1e-12 * abs(E0) / h  # charges converged to 1e-12: E noise ~ 1e-12 |E|
+        assert abs(fd - grad[s]) <= 1e-6 * abs(grad[s]) + floor, (s, grad[s], fd)
+    assert np.all(grad[[1, 2, 4]] == 0.0)
+    # the chi / J part is the same with or without the xi derivatives
+    E1, g1 = solve_energy(system, t0[:10])
+    gr1 = g1.param_grad()
+    g1.close()
+    assert np.array_equal(gr1["dchi"], gr["dchi"]) and np.array_equal(gr1["dJ"], gr["dJ"])
+
+
+def test_torch_gradcheck_with_xi(system):
+    import torch
+    from simox_amd.autograd import CtipEnergy, potential_setup, theta_of
+    t0 = theta_of(potential_setup(), with_xi=True)
+    base = torch.tensor(t0, dtype=torch.float64)
+    idx = [0, 3, 10, 13]  # chi_Al, chi_O, xi_Al, xi_O
+
+    def energy(v):
+        full = base.clone()
+        full[idx] = v
+        return CtipEnergy.apply(full, system)
+
+    v0 = torch.tensor(t0[idx], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(energy, (v0,), eps=1e-6, atol=1e-4, rtol=1e-4)
