@@ -116,6 +116,7 @@ struct simox_ctx {
   // solver's long-range preconditioner (pc, set up on first use)
   PmeGrid pm, pc;
   bool prec_on = false;
+  bool small_path = false;  // single-workgroup vector passes (n <= LS_SMALL_MAX)
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
   double* sv = nullptr;
@@ -1181,13 +1182,53 @@ int ncg_iter_body(simox_ctx* ctx) {
   tstop(ctx, 6, ev);
   hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
                      ctx->dscal + 22, ls);
-  // exact line search, Newton with bracketing, on the device (rounds after
+  // exact line search, Newton with bracketing, on the device: one workgroup
+  // for small systems, else two launches per round (rounds after
   // convergence return at once)
-  for (int k = 0; k < 8; ++k) {
-    hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                       ctx->qc, ctx->pv, ls, ctx->part);
-    hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
-                       ctx->dscal + 24, ls);
+  if (ctx->small_path) {
+    // small systems: line search, step, centring and the preconditioner's
+    // per-atom passes each in one workgroup (launch latency bound)
+    hipLaunchKernelGGL(k_ncg_ls_small, dim3(1), dim3(1024), 0, ctx->st, n, ctx->dtab, ctx->id,
+                       ctx->qc, ctx->pv, ls, ctx->dscal + 24);
+    if (!ctx->prec_on) {
+      hipLaunchKernelGGL(k_small_step<true>, dim3(1), dim3(1024), 0, ctx->st, n, ctx->dtab, ctx->id,
+                         ls, ctx->apv, nullptr, ctx->qc, ctx->gv, ctx->rv, ctx->mv, ctx->zv,
+                         ctx->pv, p32, nullptr, ctx->dscal);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+    hipLaunchKernelGGL(k_small_step<false>, dim3(1), dim3(1024), 0, ctx->st, n, ctx->dtab, ctx->id,
+                       ls, ctx->apv, ctx->cv, ctx->qc, ctx->gv, ctx->rv, ctx->mv, nullptr, ctx->pv,
+                       nullptr, ctx->sv, ctx->dscal);
+    HIPCHK(hipGetLastError());
+    PmeGrid& pc = ctx->pc.ready ? ctx->pc : ctx->pm;  // chosen by build_prec
+    CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
+    const Geom gp = geom_of(ctx, pc);
+    if (f32 && pc.order == 2)
+      hipLaunchKernelGGL((k_small_prec_tail<2, float>), dim3(1), dim3(1024), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal32, ls, ctx->zv, ctx->pv, p32, ctx->dscal);
+    else if (pc.order == 2)
+      hipLaunchKernelGGL((k_small_prec_tail<2, double>), dim3(1), dim3(1024), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal64, ls, ctx->zv, ctx->pv, p32, ctx->dscal);
+    else if (f32)
+      hipLaunchKernelGGL((k_small_prec_tail<4, float>), dim3(1), dim3(1024), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal32, ls, ctx->zv, ctx->pv, p32, ctx->dscal);
+    else
+      hipLaunchKernelGGL((k_small_prec_tail<4, double>), dim3(1), dim3(1024), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal64, ls, ctx->zv, ctx->pv, p32, ctx->dscal);
+    HIPCHK(hipGetLastError());
+    return 0;
+  } else {
+    for (int k = 0; k < 8; ++k) {
+      hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                         ctx->qc, ctx->pv, ls, ctx->part);
+      hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
+                         ctx->dscal + 24, ls);
+    }
   }
   hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
@@ -1532,6 +1573,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   ctx->mode = mode;
   ctx->rcut = rcut;
   ctx->prec_on = mode == SIMOX_MODE_MIXED;
+  {
+    const char* e = std::getenv("SIMOX_SMALL_PATH");
+    ctx->small_path = n <= LS_SMALL_MAX && !(e && std::strcmp(e, "0") == 0);
+  }
   Geom& g = ctx->g;
   g.n = n;
   for (int d = 0; d < 3; ++d) {

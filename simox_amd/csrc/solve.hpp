@@ -986,6 +986,202 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
   }
 }
 
+// The whole line search in one workgroup (systems of up to LS_SMALL_MAX atoms;
+// at 78 k atoms one workgroup is already slower than the two-launch rounds):
+// up to 8 Newton rounds, each a fixed-order sum over the atoms by the 1024
+// threads and the scalar update, instead of two launches per round.  red
+// receives the last round's sums (as k_reduce_op<2> leaves them).
+constexpr int LS_SMALL_MAX = 16384;
+__global__ void __launch_bounds__(1024) k_ncg_ls_small(int n, const Tables* __restrict__ tb,
+                                                       const int* __restrict__ id,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ p,
+                                                       double* __restrict__ ls,
+                                                       double* __restrict__ red) {
+  __shared__ double lds[2 * 1024 / WAVE];
+  __shared__ double st[9];
+  __shared__ double sums[2];
+  if (threadIdx.x < 9) st[threadIdx.x] = ls[threadIdx.x];
+  if (threadIdx.x < 2) sums[threadIdx.x] = red[threadIdx.x];
+  __syncthreads();
+  for (int round = 0; round < 8; ++round) {
+    if (st[5] != 0.0) break;  // uniform: read after the barrier
+    const double a = st[2];
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const SpecCoef& sc = tb->sc[id[i] - 1];
+      const double pi = p[i], qi = q[i], qa = qi + a * pi;
+      acc[0] += pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
+      acc[1] += outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const double w = wave_sum(acc[v]);
+      if ((threadIdx.x & (WAVE - 1)) == 0) lds[v * (1024 / WAVE) + threadIdx.x / WAVE] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int v = 0; v < 2; ++v) {
+        double t = lds[v * (1024 / WAVE)];
+        for (int k = 1; k < int(blockDim.x / WAVE); ++k) t += lds[v * (1024 / WAVE) + k];
+        sums[v] = t;
+      }
+      ls_update(sums, st);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) ls[threadIdx.x] = st[threadIdx.x];
+  if (threadIdx.x < 2) red[threadIdx.x] = sums[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// Small systems (n <= LS_SMALL_MAX): the vector work of an iteration after the
+// line search in one 1024-thread workgroup each, with the global sums formed
+// inside (fixed order), instead of a kernel and a reduction launch per step.
+// Launch latency, not bandwidth, bounds these systems.
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ inline void blk_allsum(double (&v)[NV], double* lds, double* tot) {
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = v[k];
+  __syncthreads();
+}
+
+// Step q += a p, g -= a H0p + dpen', centring of g and m = 1/(J + pen'')
+// (k_ncg_step_dev, k_ncg_center and their reductions); then, JACOBI: z, the
+// PR+ beta and the new direction (k_ncg_z, beta_update, k_pcg_dir_dev);
+// else s = c g / sqrt(D) for the long-range preconditioner (k_prec_scale).
+// scal: the context's scalar slots ([0] sum g, [16..18] sum m g, sum m,
+// sum g^2, [19..20] z.g, z.g_old).
+template <bool JACOBI>
+__global__ void __launch_bounds__(1024) k_small_step(int n, const Tables* __restrict__ tb,
+                                                     const int* __restrict__ id,
+                                                     double* __restrict__ ls,
+                                                     const double* __restrict__ v,
+                                                     const double* __restrict__ c,
+                                                     double* __restrict__ q,
+                                                     double* __restrict__ g,
+                                                     double* __restrict__ g_old,
+                                                     double* __restrict__ m,
+                                                     double* __restrict__ z,
+                                                     double* __restrict__ p,
+                                                     float* __restrict__ p32,
+                                                     double* __restrict__ s_out,
+                                                     double* __restrict__ scal) {
+  __shared__ double lds[3 * 1024 / WAVE];
+  __shared__ double tot[3];
+  const double a = ls[8] != 0.0 ? 0.0 : ls[2];
+  double s1[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double qi = q[i], qn = qi + a * p[i];
+    const double gi = g[i];
+    g_old[i] = gi;
+    const double gn = gi - a * v[i] - (pen_d1(sc, qn) - pen_d1(sc, qi));
+    q[i] = qn;
+    g[i] = gn;
+    s1[0] += gn;
+  }
+  blk_allsum<1>(s1, lds, tot);
+  const double gmean = tot[0] / double(n);
+  if (threadIdx.x == 0) scal[0] = tot[0];
+  double s3[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double gi = g[i] - gmean;
+    g[i] = gi;
+    const double mi = 1.0 / diag_m(tb->sc[id[i] - 1], q[i]);
+    m[i] = mi;
+    s3[0] += mi * gi; s3[1] += mi; s3[2] += gi * gi;
+  }
+  __syncthreads();
+  blk_allsum<3>(s3, lds, tot);
+  if (threadIdx.x == 0) { scal[16] = tot[0]; scal[17] = tot[1]; scal[18] = tot[2]; }
+  if constexpr (JACOBI) {
+    const double ratio = tot[0] / tot[1];
+    double s2[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double zi = m[i] * g[i] - m[i] * ratio;
+      z[i] = zi;
+      s2[0] += zi * g[i];
+      s2[1] += zi * g_old[i];
+    }
+    __syncthreads();
+    blk_allsum<2>(s2, lds, tot);
+    if (threadIdx.x == 0) {
+      scal[19] = tot[0];
+      scal[20] = tot[1];
+      beta_update(tot, ls);
+      tot[2] = ls[7];
+    }
+    __syncthreads();
+    const double beta = tot[2];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const double pv = z[i] + beta * p[i];
+      p[i] = pv;
+      if (p32) p32[i] = float(pv);
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      s_out[i] = c[i] * g[i] / sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+  }
+}
+
+// Long-range preconditioner's tail for small systems: z = (g/sqrt(D) - c W^T
+// grid)/sqrt(D) (k_prec_finish), z -= mean z (k_prec_project), the PR+ beta
+// and the new direction, in one workgroup.
+template <int ORD, class Rg>
+__global__ void __launch_bounds__(1024) k_small_prec_tail(Geom gp, const Tables* __restrict__ tb,
+                                                          const double* __restrict__ x,
+                                                          const int* __restrict__ id,
+                                                          const double* __restrict__ q,
+                                                          const double* __restrict__ g,
+                                                          const double* __restrict__ g_old,
+                                                          const double* __restrict__ c,
+                                                          const Rg* __restrict__ grid,
+                                                          double* __restrict__ ls,
+                                                          double* __restrict__ z,
+                                                          double* __restrict__ p,
+                                                          float* __restrict__ p32,
+                                                          double* __restrict__ scal) {
+  __shared__ double lds[2 * 1024 / WAVE];
+  __shared__ double tot[3];
+  const int n = gp.n;
+  double s1[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+    const double zi = (g[i] / sd - c[i] * gather_ord<ORD>(gp, x + 3 * i, grid)) / sd;
+    z[i] = zi;
+    s1[0] += zi;
+  }
+  blk_allsum<1>(s1, lds, tot);
+  if (threadIdx.x == 0) scal[26] = tot[0];
+  const double zmean = tot[0] / double(n);
+  double s2[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double zi = z[i] - zmean;
+    z[i] = zi;
+    s2[0] += zi * g[i];
+    s2[1] += zi * g_old[i];
+  }
+  __syncthreads();
+  blk_allsum<2>(s2, lds, tot);
+  if (threadIdx.x == 0) {
+    scal[19] = tot[0];
+    scal[20] = tot[1];
+    beta_update(tot, ls);
+    tot[2] = ls[7];
+  }
+  __syncthreads();
+  const double beta = tot[2];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double pv = z[i] + beta * p[i];
+    p[i] = pv;
+    if (p32) p32[i] = float(pv);
+  }
+}
+
 // Final reduction (as k_reduce_final, one block, fixed order) followed by a
 // scalar step of the device-side CG on the sums, in the same launch:
 // OP 1 = line-search start (ls_init), 2 = Newton update of the line search
