@@ -225,6 +225,55 @@ def bench_replicas(args):
     }
 
 
+def nve_drift(args):
+    """E_total (util.f90:12-15) over args.drift NVE steps of C1 (1,216 atoms,
+    Rcut 8 A, 300 K, dt 1 fs, cell sort and charge solve every step) in MIXED
+    and in FP64: drift as the least-squares slope (meV/atom/ps) and the largest
+    excursion from E_total(0) per atom, both paths from the same start."""
+    from simox_amd import MODE_FP64, MODE_MIXED, Simox
+    from simox_amd.systems import al_o_slab, config
+    s = config("c1")
+    # the same slab without O and with q = 0 and no charge solve: only EAM
+    # acts, so E_total is the Hamiltonian of the forces and its drift is the
+    # integrator's (with O and charges, FORCE's Ectip of same-species pairs is
+    # not the potential of its forces, quirk Q2, so E_total is not conserved)
+    s0 = al_o_slab(6, 6, 8, 0, 1, 300.0, (32, 32, 96), 8.0, vacuum=20.0, name="c1_eam")
+    out = {"metric": "NVE drift of E_total", "config": {
+        "workload": "C1: Al slab 6x6x8 cells (1,152) + 64 O, PME 32x32x96, Rcut 8 A, 300 K",
+        "workload_eam_only": "the same Al slab without O, q = 0, no charge solve",
+        "steps": args.drift, "dt_fs": 1.0, "ctip_every": 1, "sort_every": 1,
+        "charge_tol": args.tol}}
+    for name, mode, sys_, ctip in (("mixed", MODE_MIXED, s, 1), ("fp64", MODE_FP64, s, 1),
+                                   ("mixed_eam_only", MODE_MIXED, s0, 0),
+                                   ("fp64_eam_only", MODE_FP64, s0, 0)):
+        s = sys_
+        g = Simox.from_system(s, mode=mode)
+        g.set_charge_tol(args.tol, 5000)
+        if ctip:
+            g.ctip_solve()
+        g.force()
+        g.pme()
+        e = np.empty(args.drift + 1)
+        e[0] = g.step(1, s.dt, 1, ctip)["Etotal"]
+        t0 = time.perf_counter()
+        chunk = 500
+        done = 0
+        while done < args.drift:
+            k = min(chunk, args.drift - done)
+            for j in range(k):
+                e[done + j + 1] = g.step(1, s.dt, 1, ctip)["Etotal"]
+            done += k
+            print("bench: drift %s %d steps" % (name, done), file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        g.close()
+        t_ps = np.arange(len(e)) * 1e-3
+        slope = np.polyfit(t_ps, e, 1)[0]
+        out[name] = {"slope_meV_per_atom_per_ps": slope / s.n * 1e3,
+                     "max_excursion_meV_per_atom": float(np.abs(e - e[0]).max() / s.n * 1e3),
+                     "E0": float(e[0]), "E_last": float(e[-1]), "ms_per_step": el / args.drift * 1e3}
+    return out
+
+
 def accuracy_vs_fp64(s, g_mixed, args):
     """Force / charge / energy error of the mixed path against the FP64 path on
     the same state: positions and velocities of the mixed run, FP64 charges
@@ -270,6 +319,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="converged charge solve: RMS projected gradient (eV/e)")
     ap.add_argument("--replicas", type=int, default=256, help="replica count of the c4 workload")
+    ap.add_argument("--drift", type=int, default=0,
+                    help="instead of the throughput line: NVE drift of E_total over this many "
+                         "C1 steps, MIXED and FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU hot path "
@@ -286,6 +338,9 @@ def main():
         torch.cuda.synchronize()
         del a, b
         torch.cuda.empty_cache()
+    if args.drift > 0:
+        print(json.dumps(nve_drift(args)))
+        return 0
     if args.config == "c4":
         out = bench_replicas(args)
         if not args.no_cpu:

@@ -160,7 +160,7 @@ def fft_size(n):
 def replicas(n_rep=256, seed=1):
     """Independent Al/O slabs of uneven size: slab cells drawn from 4x4x6 to
     10x10x12 (384-4,800 Al), O coverage 0.1-1.0 monolayer of the (001) surface
-    (2 sites per a^2), temperature 300-1200 K.  Rcut = min(8, L_xy / 3) keeps
+    (2 sites per a^2), temperature 300-1200 K.  Rcut = min(10, L_xy / 3) keeps
     3 or more cells per dimension; PME grid spacing is Rcut / 10 per dimension."""
     rng = np.random.default_rng(seed)
     out = []
@@ -171,7 +171,7 @@ def replicas(n_rep=256, seed=1):
         T = float(rng.uniform(300.0, 1200.0))
         n_o = max(1, int(round(cov * 2 * nx * ny)))
         lxy = min(nx, ny) * A_AL
-        rcut = min(8.0, np.floor(lxy / 3.0 * 100.0) / 100.0)
+        rcut = min(10.0, np.floor(lxy / 3.0 * 100.0) / 100.0)
         vacuum = 20.0
         box = np.array([nx * A_AL, ny * A_AL, nz * A_AL + vacuum])
         fft = tuple(fft_size(b / (rcut / 10.0)) for b in box)
