@@ -248,6 +248,22 @@ __device__ inline void bspline4(double u, double AX[4], double AXX[4]) {
   AXX[3] = (3.0 * (z * z) - 24.0 * z + 48.0) / 6.0;
 }
 
+// The weights alone, with the divisions by 6 as products by 1/6 (within an
+// ulp of bspline4's): the spread and the gather of the converged solves,
+// where the FAITHFUL mode's exact replay of the reference's expressions does
+// not apply.  Horner forms of the same cubics.
+__device__ inline void bspline4_w(double u, double AX[4]) {
+  constexpr double s6 = 1.0 / 6.0;
+  double z = trunc(u) - u + 1.0;
+  AX[0] = z * z * z * s6;
+  z += 1.0;
+  AX[1] = (((-3.0 * z + 12.0) * z - 12.0) * z + 4.0) * s6;
+  z += 1.0;
+  AX[2] = (((3.0 * z - 24.0) * z + 60.0) * z - 44.0) * s6;
+  z += 1.0;
+  AX[3] = (((-z + 12.0) * z - 48.0) * z + 64.0) * s6;
+}
+
 // u = K (x + L/2) / L, base = INT(u) (force.f90:388,404)
 __device__ inline void grid_coord(const Geom& g, const double* xi, double u[3], int b[3]) {
 #pragma clang fp contract(off)
@@ -835,7 +851,10 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
     int b[3];
     grid_coord(g, x + 3 * ai, u, b);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) bspline4(u[d], AX[d], AXX[d]);
+    for (int d = 0; d < 3; ++d) {
+      if constexpr (F32W) bspline4(u[d], AX[d], AXX[d]);  // the reference's expressions (Q1 replay)
+      else bspline4_w(u[d], AX[d]);
+    }
     int o[3][4];
     bool in[3][4];
 #pragma unroll
@@ -969,7 +988,7 @@ __global__ void __launch_bounds__(BLOCK) k_gather_tile(Geom g, const double* __r
     int o[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      bspline4(u[d], AX[d], AXX[d]);
+      bspline4_w(u[d], AX[d]);
       int r = b[d] % g.K[d];
       if (r < 0) r += g.K[d];
       o[d] = r - lo[d];  // stencil point a sits at LDS index o + a (0 .. ext + 2)
