@@ -233,6 +233,12 @@ int simox_set_eam_splines(simox_ctx* ctx, int m, const double* lo, const double*
 /* grad[f*m + k] = dE/dcoef[f*m + k] of E = Epair + Egroup (the EAM energy of
    FORCE) at the current positions, spline form set. */
 int simox_eam_spline_grad(simox_ctx* ctx, double* grad);
+/* grad[f*m + k] = dS/dcoef[f*m + k] for S = sum_i u_i . F_i, F the EAM part
+   of the forces at the current positions (spline form set): the vector-
+   Jacobian product of the forces.  u: caller order, 3 per atom; NULL uses the
+   forces the last simox_force / simox_pme left (the total forces), so that
+   dE/dcoef + 2 w grad is the gradient of E + w sum |F|^2. */
+int simox_eam_spline_force_vjp(simox_ctx* ctx, const double* u, double* grad);
 
 /* Batched replicas (build-only): nrep independent contexts (each its own box,
    grid, atoms and HIP stream) advanced by the same simox_step sequence in

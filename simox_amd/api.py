@@ -23,7 +23,7 @@ EXPORTS = [
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
-    "simox_eam_spline_grad", "simox_param_grad_all",
+    "simox_eam_spline_grad", "simox_param_grad_all", "simox_eam_spline_force_vjp",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
 ]
@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
     lib.simox_autotune.argtypes = [vp]
     lib.simox_set_eam_splines.argtypes = [vp, i, vp, vp, vp]
     lib.simox_eam_spline_grad.argtypes = [vp, vp]
+    lib.simox_eam_spline_force_vjp.argtypes = [vp, vp, vp]
     lib.simox_set_eam_cutoff.argtypes = [vp, d]
     lib.simox_set_thermostat.argtypes = [vp, i, d, d, C.c_ulonglong]
     lib.simox_set_rdf.argtypes = [vp, i]
@@ -368,6 +369,14 @@ class Simox:
         """dE/dcoef [8, m] of E = Epair + Egroup at the current positions."""
         g = np.zeros((8, self._m))
         self._check(self.lib.simox_eam_spline_grad(self.h, _ptr(g)))
+        return g
+
+    def eam_spline_force_vjp(self, u=None):
+        """dS/dcoef [8, m] for S = sum_i u_i . F_i(EAM); u None: the current
+        total forces (then dE/dcoef + 2 w this = d(E + w sum |F|^2)/dcoef)."""
+        g = np.zeros((8, self._m))
+        ua = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
+        self._check(self.lib.simox_eam_spline_force_vjp(self.h, _ptr(ua), _ptr(g)))
         return g
 
     def timing(self, on=True):

@@ -199,6 +199,28 @@ __device__ inline int spl_basis(const SplineSet& sp, int f, T x, T w[4], T dw[4]
   return k0 - 1;
 }
 
+// value, first and second x-derivatives (second derivatives of the cubic
+// B-splines: (1 - t), 3t - 2, 1 - 3t, t over h^2)
+__device__ inline void spl_eval2(const SplineSet& sp, int f, double x, double& v, double& dv,
+                                 double& d2v) {
+  double w[4], dw[4];
+  const int k = spl_basis<double>(sp, f, x, w, dw);
+  const double h = sp.h[f];
+  const double u = (x - sp.lo[f]) / h + 1.0;
+  const double t = u - double(k + 1);
+  const bool inside = dw[1] != 0.0 || dw[2] != 0.0 || dw[0] != 0.0 || dw[3] != 0.0;
+  const double ih2 = inside ? 1.0 / (h * h) : 0.0;
+  const double d2w[4] = {(1.0 - t) * ih2, (3.0 * t - 2.0) * ih2, (1.0 - 3.0 * t) * ih2, t * ih2};
+  v = dv = d2v = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double c = sp.c[f][k + m];
+    v += c * w[m];
+    dv += c * dw[m];
+    d2v += c * d2w[m];
+  }
+}
+
 template <class T>
 __device__ inline void spl_eval(const SplineSet& sp, int f, T x, T& v, T& dv) {
   T w[4], dw[4];
@@ -472,6 +494,91 @@ __global__ void __launch_bounds__(BLOCK) k_eam_spline_grad(Geom g, const Tables*
   __syncthreads();
   for (int k = threadIdx.x; k < NSPL * SPL_MAX; k += blockDim.x)
     part[size_t(blockIdx.x) * (NSPL * SPL_MAX) + k] = double((long long)acc[k]) / SPL_SCALE;
+}
+
+// Vector-Jacobian product of the spline-form EAM forces (build-only): for
+// S = sum_i u_i . F_i(EAM), with W_ij = (u_i - u_j) . r_ij / r (symmetric in
+// i, j) and T_i = sum_j f'_{j->i}(r_ij) W_ij,
+//   dS/dc(pair)_k = -1/2 sum_i sum_j B'_k(r_ij) W_ij,
+//   dS/dc(F)_k    = -sum_i B'_k(rho_i) T_i,
+//   dS/dc(dens)_k = -sum_i sum_j (F''(rho_i) T_i B_k(r_ij) + F'(rho_i) W_ij B'_k(r_ij)),
+// the sums over j running over the full shell (each pair from both sides).
+__device__ inline double vjp_w(const double* __restrict__ u, int i, int j, const double rx[3],
+                               double r) {
+  return ((u[3 * i] - u[3 * j]) * rx[0] + (u[3 * i + 1] - u[3 * j + 1]) * rx[1] +
+          (u[3 * i + 2] - u[3 * j + 2]) * rx[2]) / r;
+}
+
+// pass 1: T_i
+__global__ void k_eam_vjp_t(Geom g, const Tables* __restrict__ tb, const double* __restrict__ x,
+                            const int* __restrict__ id, const double* __restrict__ u,
+                            const int* __restrict__ cnt, const int* __restrict__ col,
+                            double* __restrict__ t_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int si = id[i];
+  const size_t n = size_t(g.n);
+  double t = 0.0;
+  for (int k = 0; k < cnt[i]; ++k) {
+    const int j = col[size_t(k) * n + i];
+    double rx[3], r2;
+    pair_sep(g, x, i, j, rx, r2);
+    if (!(r2 < g.ream2)) continue;
+    const double r = sqrt(r2);
+    double f, df;
+    spl_eval<double>(tb->sp, spl_dens(si, id[j]), r, f, df);
+    t += df * vjp_w(u, i, j, rx, r);
+  }
+  t_out[i] = t;
+}
+
+// pass 2: the coefficient sums, 64-bit fixed point at 2^28 in LDS per block
+constexpr double VJP_SCALE = 268435456.0;
+__global__ void __launch_bounds__(BLOCK) k_eam_vjp(Geom g, const Tables* __restrict__ tb,
+                                                   const double* __restrict__ x,
+                                                   const int* __restrict__ id,
+                                                   const double* __restrict__ rho,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ tv,
+                                                   const int* __restrict__ cnt,
+                                                   const int* __restrict__ col,
+                                                   double* __restrict__ part) {
+  __shared__ unsigned long long acc[NSPL * SPL_MAX];
+  for (int k = threadIdx.x; k < NSPL * SPL_MAX; k += blockDim.x) acc[k] = 0ull;
+  __syncthreads();
+  const SplineSet& sp = tb->sp;
+  auto add = [&](int f, int k0, const double w[4], double scale) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      atomicAdd(&acc[f * SPL_MAX + k0 + m], (unsigned long long)llrint(scale * w[m] * VJP_SCALE));
+  };
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.n) {
+    const int si = id[i];
+    const double ti = tv[i];
+    double w[4], dw[4], F, dF, d2F;
+    const int fe = spl_embed(si);
+    add(fe, spl_basis<double>(sp, fe, rho[i], w, dw), dw, -ti);
+    spl_eval2(sp, fe, rho[i], F, dF, d2F);
+    const size_t n = size_t(g.n);
+    for (int k = 0; k < cnt[i]; ++k) {
+      const int j = col[size_t(k) * n + i];
+      double rx[3], r2;
+      pair_sep(g, x, i, j, rx, r2);
+      if (!(r2 < g.ream2)) continue;
+      const double r = sqrt(r2);
+      const double wij = vjp_w(u, i, j, rx, r);
+      const int sj = id[j];
+      const int fp = spl_pair(si, sj), fd = spl_dens(si, sj);
+      add(fp, spl_basis<double>(sp, fp, r, w, dw), dw, -0.5 * wij);
+      const int k0 = spl_basis<double>(sp, fd, r, w, dw);
+      add(fd, k0, w, -d2F * ti);
+      add(fd, k0, dw, -dF * wij);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NSPL * SPL_MAX; k += blockDim.x)
+    part[size_t(blockIdx.x) * (NSPL * SPL_MAX) + k] = double((long long)acc[k]) / VJP_SCALE;
 }
 
 // out[v] = sum over the nparts rows of part[row * nv + v], rows in order

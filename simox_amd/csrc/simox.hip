@@ -119,6 +119,8 @@ struct simox_ctx {
   bool small_path = false;  // single-workgroup vector passes (n <= LS_SMALL_MAX)
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
+  double* vjp_u = nullptr;     // 3N weights of the force vector-Jacobian product
+  double* vjp_t = nullptr;     // N per-atom sums T_i of it
   double* sv = nullptr;
   double* cv = nullptr;  // 1/sqrt(J) per atom, the preconditioner's fixed scaling
   // CTIP
@@ -1697,7 +1699,7 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
@@ -2074,6 +2076,41 @@ int simox_eam_spline_grad(simox_ctx* ctx, double* grad) {
   CHK(ensure_part(ctx, size_t(nb) * nv));
   hipLaunchKernelGGL(k_eam_spline_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
                      ctx->x, ctx->id, ctx->rho, ctx->nb_cnt, ctx->nb_col, ctx->part);
+  hipLaunchKernelGGL(k_sum_rows, dim3((nv + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->st,
+                     ctx->part, nb, nv, ctx->spl_grad);
+  HIPCHK(hipGetLastError());
+  std::vector<double> all(nv);
+  HIPCHK(hipMemcpyAsync(all.data(), ctx->spl_grad, nv * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  const int m = ctx->htab.sp.m;
+  for (int f = 0; f < NSPL; ++f)
+    for (int k = 0; k < m; ++k) grad[size_t(f) * m + k] = all[size_t(f) * SPL_MAX + k];
+  return SIMOX_OK;
+}
+
+int simox_eam_spline_force_vjp(simox_ctx* ctx, const double* u, double* grad) {
+  if (!ctx || !grad || !need_state(ctx)) return SIMOX_ERR_ARG;
+  if (!ctx->htab.sp.on) {
+    ctx->err = "EAM splines are not set";
+    return SIMOX_ERR_ARG;
+  }
+  const int n = ctx->n, nb = grid_blocks(n), nv = NSPL * SPL_MAX;
+  CHK(ensure_nlist(ctx));
+  CHK(check_err(ctx, 0));
+  if (!ctx->spl_grad) CHK(dalloc(ctx, &ctx->spl_grad, nv));
+  if (!ctx->vjp_t) CHK(dalloc(ctx, &ctx->vjp_t, size_t(n)));
+  const double* ud = ctx->f;  // the forces of the last FORCE (+ PME)
+  if (u) {
+    if (!ctx->vjp_u) CHK(dalloc(ctx, &ctx->vjp_u, 3 * size_t(n)));
+    CHK(upload_caller(ctx, u, 3, ctx->vjp_u));
+    ud = ctx->vjp_u;
+  }
+  CHK(ensure_part(ctx, size_t(nb) * nv));
+  hipLaunchKernelGGL(k_eam_vjp_t, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                     ctx->id, ud, ctx->nb_cnt, ctx->nb_col, ctx->vjp_t);
+  hipLaunchKernelGGL(k_eam_vjp, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                     ctx->id, ctx->rho, ud, ctx->vjp_t, ctx->nb_cnt, ctx->nb_col, ctx->part);
   hipLaunchKernelGGL(k_sum_rows, dim3((nv + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->st,
                      ctx->part, nb, nv, ctx->spl_grad);
   HIPCHK(hipGetLastError());

@@ -155,3 +155,36 @@ class EamSplineEnergy(torch.autograd.Function):
     def backward(ctx, grad_out):
         (grad,) = ctx.saved_tensors
         return grad_out * grad, None, None, None
+
+
+def loss_and_grad(system, lo, h, coef, w, q=None, mode=MODE_FP64):
+    """L = Epair + Egroup + w sum_i |F_i|^2 at fixed charges (system.q or q),
+    F the total forces of FORCE + PME, and dL/dcoef [8, m] =
+    dE/dcoef + 2 w (vector-Jacobian product of the EAM forces with F)."""
+    s = system
+    g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+    g.set_state(s.ids, s.x, s.v, None, s.q if q is None else q)
+    g.set_eam_splines(lo, h, coef)
+    e = g.force()
+    g.pme()
+    f = g.get_state()[2]
+    loss = e["Epair"] + e["Egroup"] + w * float((f ** 2).sum())
+    grad = g.eam_spline_grad() + 2.0 * w * g.eam_spline_force_vjp()
+    g.close()
+    return loss, grad
+
+
+class EamSplineLoss(torch.autograd.Function):
+    """coef (float64 tensor [8, m]) -> L = Epair + Egroup + w sum |F|^2 of a
+    fixed system at fixed charges; backward from the library's gradients."""
+
+    @staticmethod
+    def forward(ctx, coef, system, lo, h, w, q=None):
+        loss, grad = loss_and_grad(system, lo, h, coef.detach().cpu().numpy(), w, q)
+        ctx.save_for_backward(torch.tensor(grad, dtype=coef.dtype))
+        return torch.tensor(loss, dtype=coef.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return grad_out * grad, None, None, None, None, None

@@ -153,3 +153,73 @@ def test_torch_gradcheck_on_some_coefficients(system, model):
 
     v0 = torch.tensor([c[f, k] for f, k in picks], dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(energy, (v0,), eps=1e-6, atol=1e-5, rtol=1e-5)
+
+
+def _charges(system, seed=5):
+    q = np.random.default_rng(seed).normal(size=system.n) * 0.2
+    return q - q.mean()
+
+
+def test_force_vjp_matches_finite_differences(system, model):
+    """d/dcoef of S = sum_i u_i . F_i(EAM) for a random u against central
+    differences of S over GPU forces (fixed charges: the other force terms do
+    not depend on the coefficients)."""
+    lo, h, c = model
+    q = _charges(system)
+    u = np.random.default_rng(7).normal(size=(system.n, 3))
+
+    def S(cc):
+        g = Simox(system.n, system.box, system.fft, system.rcut, mode=MODE_FP64)
+        g.set_state(system.ids, system.x, system.v, None, q)
+        g.set_eam_splines(lo, h, cc)
+        g.force()
+        val = float((u * g.get_state()[2]).sum())
+        return val, g
+
+    s0, g = S(c)
+    vjp = g.eam_spline_force_vjp(u)
+    g.close()
+    rng = np.random.default_rng(1)
+    step = 1e-6
+    for f in range(es.NSPL):
+        nz = np.flatnonzero(np.abs(vjp[f]) > 1e-3 * np.abs(vjp[f]).max())
+        for k in rng.choice(nz, size=min(3, len(nz)), replace=False):
+            vals = []
+            for sgn in (1, -1):
+                cc = c.copy()
+                cc[f, k] += sgn * step
+                v, gg = S(cc)
+                gg.close()
+                vals.append(v)
+            fd = (vals[0] - vals[1]) / (2 * step)
+            floor = 64 * np.finfo(float).eps * np.abs(u).sum() * 10.0 / step
+            assert abs(fd - vjp[f, k]) <= 1e-6 * abs(vjp[f, k]) + floor, (f, k, fd, vjp[f, k])
+
+
+def test_force_loss_gradient_and_gradcheck(system, model):
+    """L = Epair + Egroup + w sum |F|^2 (total forces, fixed charges): the
+    library gradient against central differences, and torch gradcheck."""
+    lo, h, c = model
+    q = _charges(system)
+    w = 0.01
+    L0, grad = es.loss_and_grad(system, lo, h, c, w, q)
+    step = 1e-6
+    for f, k in ((3, 20), (4, 15), (0, 22), (1, 18), (6, 24), (7, 8)):
+        cp, cm = c.copy(), c.copy()
+        cp[f, k] += step
+        cm[f, k] -= step
+        fd = (es.loss_and_grad(system, lo, h, cp, w, q)[0] -
+              es.loss_and_grad(system, lo, h, cm, w, q)[0]) / (2 * step)
+        floor = np.finfo(float).eps * abs(L0) / step
+        assert abs(fd - grad[f, k]) <= 1e-6 * abs(grad[f, k]) + floor, (f, k, fd, grad[f, k])
+    picks = [(3, 10), (1, 12), (6, 20)]
+    base = torch.tensor(c, dtype=torch.float64)
+
+    def loss(v):
+        full = base.clone()
+        for t, (f, k) in enumerate(picks):
+            full[f, k] = v[t]
+        return es.EamSplineLoss.apply(full, system, lo, h, w, q)
+
+    v0 = torch.tensor([c[f, k] for f, k in picks], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(loss, (v0,), eps=1e-6, atol=1e-5, rtol=1e-5)
