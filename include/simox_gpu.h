@@ -139,6 +139,13 @@ int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters
    following it: kc dEctip/dxi at fixed charges minus lambda . dG/dxi of the
    solver's matrix, by forward-mode dual numbers through the same formulas. */
 int simox_param_grad_all(simox_ctx* ctx, double grad[15], double* lambda, int* iters);
+/* Force term of a fitting loss: grad[s], grad[5+s], grad[10+s] = d(sum_i
+   u_i . F_i)/dchi, /dJ, /dxi of species s+1, F the forces of FORCE + PME at
+   the converged charges (which follow the parameters: adjoint solve with the
+   right-hand side d(sum u.F)/dq).  u: caller order, 3 per atom; NULL uses the
+   current forces, so that grad of simox_param_grad_all + 2 w this is the
+   gradient of E + w sum |F|^2. */
+int simox_param_force_grad(simox_ctx* ctx, const double* u, double grad[15], int* iters);
 int simox_remove_rigid_motion(simox_ctx* ctx);
 int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
 /* FFT launch shapes.  simox_create picks them by timing the candidate x

@@ -24,6 +24,7 @@ EXPORTS = [
     "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
     "simox_eam_spline_grad", "simox_param_grad_all", "simox_eam_spline_force_vjp",
+    "simox_param_force_grad",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
 ]
@@ -95,6 +96,7 @@ def load_library(path=LIB_PATH):
     lib.simox_set_reflect.argtypes = [vp, i]
     lib.simox_param_grad.argtypes = [vp, P(d), vp, P(i)]
     lib.simox_param_grad_all.argtypes = [vp, P(d), vp, P(i)]
+    lib.simox_param_force_grad.argtypes = [vp, vp, P(d), P(i)]
     lib.simox_batch_step.argtypes = [P(vp), i, i, d, i, i, vp]
     lib.simox_remove_rigid_motion.argtypes = [vp]
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
@@ -209,6 +211,16 @@ class Simox:
         if want_lambda:
             out["lambda"] = lam
         return out
+
+    def param_force_grad(self, u=None):
+        """d(sum_i u_i . F_i)/d(chi, J, xi) through the converged charges (F the
+        forces of FORCE + PME; u None: the current forces)."""
+        g = (C.c_double * 15)()
+        it = C.c_int()
+        ua = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
+        self._check(self.lib.simox_param_force_grad(self.h, _ptr(ua), g, C.byref(it)))
+        return dict(dchi=np.array(g[0:5]), dJ=np.array(g[5:10]), dxi=np.array(g[10:15]),
+                    iters=it.value)
 
     def set_reflect(self, on=True):
         """OXIDATION mode first half: VV_NVE1_REFLECT instead of VV_NVE1."""

@@ -76,6 +76,35 @@ class CtipEnergy(torch.autograd.Function):
         return grad_out * grad, None, None, None
 
 
+def solve_loss(system, theta, w, mode=MODE_FP64, tol=1e-12):
+    """(L, dL/dtheta) for L = E + w sum_i |F_i|^2 at the charges of a
+    converged solve, theta = (chi, J, xi) of the five species (15 entries)."""
+    E, g = solve_energy(system, theta, mode, tol)
+    f = g.get_state()[2]
+    L = E + w * float((f ** 2).sum())
+    ge = g.param_grad(with_xi=True)
+    gf = g.param_force_grad()
+    g.close()
+    grad = np.concatenate([ge["dchi"], ge["dJ"], ge["dxi"]]) + 2.0 * w * np.concatenate(
+        [gf["dchi"], gf["dJ"], gf["dxi"]])
+    return L, grad
+
+
+class CtipLoss(torch.autograd.Function):
+    """theta (float64 tensor [15]) -> L = E + w sum |F|^2 for a fixed system."""
+
+    @staticmethod
+    def forward(ctx, theta, system, w, mode=MODE_FP64, tol=1e-12):
+        L, grad = solve_loss(system, theta.detach().cpu().numpy(), w, mode, tol)
+        ctx.save_for_backward(torch.tensor(grad, dtype=theta.dtype))
+        return torch.tensor(L, dtype=theta.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return grad_out * grad, None, None, None, None
+
+
 def gradcheck(system, theta=None, eps=1e-5, atol=1e-6, rtol=1e-4):
     """torch.autograd.gradcheck of CtipEnergy (central differences of E)."""
     t0 = theta_of(potential_setup()) if theta is None else np.asarray(theta)

@@ -227,13 +227,16 @@ __device__ inline void fafb_core(bool same, S a, S b, S c[4]) {
 }
 
 // part[10k + s] = half the sum over the block's list entries of dEctip/dxi_s
-// / kc (FORCE's Q2 form), part[10k + 5 + s] = sum of lambda_i dG_i/dxi_s
-// (the solver's Q3 matrix and its b_i = sum_j Z_j asym_ij), s = species - 1.
+// / kc (FORCE's Q2 form) or, with u != null, of d/dxi_s of the CTIP pair
+// force term dotted with (u_i - u_j) (half of d(sum u.F)/dxi_s at fixed
+// charges); part[10k + 5 + s] = sum of lambda_i dG_i/dxi_s (the solver's Q3
+// matrix and its b_i = sum_j Z_j asym_ij), s = species - 1.
 __global__ void __launch_bounds__(BLOCK) k_xi_grad(Geom g, const Tables* __restrict__ tb,
                                                    const double* __restrict__ x,
                                                    const int* __restrict__ id,
                                                    const double* __restrict__ q,
                                                    const double* __restrict__ lam,
+                                                   const double* __restrict__ u,
                                                    const int* __restrict__ cnt,
                                                    const int* __restrict__ col,
                                                    double* __restrict__ part) {
@@ -264,13 +267,178 @@ __global__ void __launch_bounds__(BLOCK) k_xi_grad(Geom g, const Tables* __restr
                               Dual(qi), Dual(qj), df, e);
         ctip_matrix_core<Dual>(same, xa, xb, c, Dual(r), Dual(g.alpha), sym, asym);
         const int sp = (pass == 0 ? si : sj) - 1;
+        double first = 0.5 * e.d;
+        if (u)
+          first = 0.5 * df.d * ((u[3 * i] - u[3 * j]) * rx[0] + (u[3 * i + 1] - u[3 * j + 1]) * rx[1] +
+                                (u[3 * i + 2] - u[3 * j + 2]) * rx[2]);
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
           if (t != sp) continue;
-          acc[t] += 0.5 * e.d;
+          acc[t] += first;
           acc[5 + t] += li * (sym.d * qj + pc.Z_j * asym.d);
         }
       }
+    }
+  }
+  block_sum<10>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int t = 0; t < 10; ++t) part[10 * blockIdx.x + t] = acc[t];
+}
+
+// ---------------------------------------------------------------------------
+// Charge derivative of S = sum_i u_i . F_i (vector-Jacobian product of the
+// forces with respect to the charges), for the force-loss gradient in the
+// CTIP parameters.
+// ---------------------------------------------------------------------------
+// Pair terms (real-space Ewald, force.f90:94-98, and CTIP, force.f90:101-140):
+// F_i = sum_j s_ij r_ij with s symmetric, so S = sum over pairs of
+// s_ij (u_i - u_j).r_ij and b_i = sum_j ds_ij/dq_i (u_i - u_j).r_ij; the CTIP
+// derivative by a dual number in q_i through ctip_force_core.
+__global__ void k_qvjp_pair(Geom g, const Tables* __restrict__ tb, const double* __restrict__ x,
+                            const int* __restrict__ id, const double* __restrict__ q,
+                            const double* __restrict__ u, const int* __restrict__ cnt,
+                            const int* __restrict__ col, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int si = id[i];
+  const double qi = q[i], a = g.alpha, a2 = a * a;
+  const size_t n = size_t(g.n);
+  double acc = 0.0;
+  for (int k = 0; k < cnt[i]; ++k) {
+    const int j = col[size_t(k) * n + i];
+    double rx[3], r2;
+    pair_sep(g, x, i, j, rx, r2);
+    const double r = sqrt(r2), qj = q[j];
+    const PairCoef& pc = tb->pc[pidx(si, id[j])];
+    const double wij = (u[3 * i] - u[3 * j]) * rx[0] + (u[3 * i + 1] - u[3 * j + 1]) * rx[1] +
+                       (u[3 * i + 2] - u[3 * j + 2]) * rx[2];
+    // d/dq_i of kc q_i q_j (2 e^{-a^2 r^2} a r / sqrt(pi) + erfc(a r)) / r^3
+    const double dcoul = KC * qj * (2.0 * exp(-a2 * r2) * a * r / SQRTPI + ::erfc(a * r)) / r2 / r;
+    Dual c[4] = {Dual(pc.c[0]), Dual(pc.c[1]), Dual(pc.c[2]), Dual(pc.c[3])};
+    Dual df, e;
+    ctip_force_core<Dual>(pc.same != 0, Dual(pc.xi_i), Dual(pc.xi_j), Dual(pc.Z_i), Dual(pc.Z_j), c,
+                          Dual(r), Dual(r2), Dual(qi, 1.0), Dual(qj), df, e);
+    acc += (dcoul + df.d) * wij;
+  }
+  b[i] = acc;
+}
+
+// PME part, step 1: t1_i = sum_p (u'_i . grad w_i(p)) Phi(p), Phi the
+// convolved grid of the charges; grad w as k_gather_force takes it.
+template <class Rg>
+__global__ void k_qvjp_rec1(Geom g, const double* __restrict__ x, const double* __restrict__ up,
+                            const Rg* __restrict__ grid, double* __restrict__ t1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  double uu[3], AX[3][4], AXX[3][4];
+  int b[3];
+  grid_coord(g, x + 3 * i, uu, b);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline4(uu[d], AX[d], AXX[d]);
+  const double kx[3] = {double(g.K[0]) / g.box[0], double(g.K[1]) / g.box[1],
+                        double(g.K[2]) / g.box[2]};
+  double acc = 0.0;
+  for (int c = 0; c < 4; ++c)
+    for (int bb = 0; bb < 4; ++bb) {
+      const size_t row = size_t(g.K[0]) * (size_t(wrapk(b[1], bb, g.K[1])) +
+                                           size_t(g.K[1]) * size_t(wrapk(b[2], c, g.K[2])));
+      for (int a = 0; a < 4; ++a) {
+        const double Qr = double(grid[row + wrapk(b[0], a, g.K[0])]);
+        const double d0 = AXX[0][a] * AX[1][bb] * AX[2][c] * kx[0];
+        const double d1 = AX[0][a] * AXX[1][bb] * AX[2][c] * kx[1];
+        const double d2 = AX[0][a] * AX[1][bb] * AXX[2][c] * kx[2];
+        acc += (up[3 * i] * d0 + up[3 * i + 1] * d1 + up[3 * i + 2] * d2) * Qr;
+      }
+    }
+  t1[i] = acc;
+}
+
+// step 2: the dipole grid D(p) = sum_i q_i u'_i . grad w_i(p), accumulated in
+// 64-bit fixed point at 2^40 with global integer atomics (order-independent)
+constexpr double DIP_SCALE = 1099511627776.0;
+__global__ void k_qvjp_dipole(Geom g, const double* __restrict__ x, const double* __restrict__ q,
+                              const double* __restrict__ up, unsigned long long* __restrict__ acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  double uu[3], AX[3][4], AXX[3][4];
+  int b[3];
+  grid_coord(g, x + 3 * i, uu, b);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline4(uu[d], AX[d], AXX[d]);
+  const double kx[3] = {double(g.K[0]) / g.box[0], double(g.K[1]) / g.box[1],
+                        double(g.K[2]) / g.box[2]};
+  const double qi = q[i];
+  for (int c = 0; c < 4; ++c)
+    for (int bb = 0; bb < 4; ++bb) {
+      const size_t row = size_t(g.K[0]) * (size_t(wrapk(b[1], bb, g.K[1])) +
+                                           size_t(g.K[1]) * size_t(wrapk(b[2], c, g.K[2])));
+      for (int a = 0; a < 4; ++a) {
+        const double d0 = AXX[0][a] * AX[1][bb] * AX[2][c] * kx[0];
+        const double d1 = AX[0][a] * AXX[1][bb] * AX[2][c] * kx[1];
+        const double d2 = AX[0][a] * AX[1][bb] * AXX[2][c] * kx[2];
+        const double v = qi * (up[3 * i] * d0 + up[3 * i + 1] * d1 + up[3 * i + 2] * d2);
+        atomicAdd(&acc[row + wrapk(b[0], a, g.K[0])], (unsigned long long)llrint(v * DIP_SCALE));
+      }
+    }
+}
+
+__global__ void k_fixed_to_grid(size_t ng, const unsigned long long* __restrict__ acc,
+                                double* __restrict__ grid) {
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < ng) grid[t] = double((long long)acc[t]) / DIP_SCALE;
+}
+
+__global__ void k_fill_u64(size_t ng, unsigned long long* __restrict__ a) {
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < ng) a[t] = 0ull;
+}
+
+// step 3: b_i += -cst (t1_i + sum_p w_i(p) (G * D)(p)) (scale applied by the caller)
+template <class Rg>
+__global__ void k_qvjp_rec2(Geom g, const double* __restrict__ x, const Rg* __restrict__ grid,
+                            const double* __restrict__ t1, double cst, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  b[i] += -cst * (t1[i] + gather_phi(g, x + 3 * i, grid));
+}
+
+// u' = u - mean(u) (the reciprocal forces have their mean removed, Q13);
+// sums[0..2] holds sum u
+__global__ void k_center3(int n, const double* __restrict__ u, const double* __restrict__ sums,
+                          double* __restrict__ up) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) up[3 * i + d] = u[3 * i + d] - sums[d] / double(n);
+}
+
+// part[3k + d] = block sums of u_d
+__global__ void __launch_bounds__(BLOCK) k_sum3(int n, const double* __restrict__ u,
+                                                double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < n)
+    for (int d = 0; d < 3; ++d) acc[d] = u[3 * i + d];
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) part[3 * blockIdx.x + d] = acc[d];
+}
+
+// part[10k + s] = sum over species s+1 of mu_i, part[10k + 5 + s] = of mu_i q_i
+__global__ void __launch_bounds__(BLOCK) k_mu_sums(int n, const int* __restrict__ id,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ mu,
+                                                   double* __restrict__ part) {
+  __shared__ double lds[10 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const int s = id[i] - 1;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      acc[t] = (t == s) ? mu[i] : 0.0;
+      acc[5 + t] = (t == s) ? mu[i] * q[i] : 0.0;
     }
   }
   block_sum<10>(acc, lds);

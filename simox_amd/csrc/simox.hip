@@ -121,6 +121,8 @@ struct simox_ctx {
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
   double* vjp_u = nullptr;     // 3N weights of the force vector-Jacobian product
   double* vjp_t = nullptr;     // N per-atom sums T_i of it
+  double* vjp_up = nullptr;    // 3N centred weights (charge part)
+  unsigned long long* dip_acc = nullptr;  // fixed-point dipole grid (charge part)
   double* sv = nullptr;
   double* cv = nullptr;  // 1/sqrt(J) per atom, the preconditioner's fixed scaling
   // CTIP
@@ -1699,7 +1701,7 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
@@ -1906,7 +1908,7 @@ int param_grad_impl(simox_ctx* ctx, double* grad, bool with_xi, double* lambda, 
   if (with_xi) {
     // dE/dxi_s = kc * dEctip/dxi_s (at fixed q) - lambda . dG/dxi_s
     hipLaunchKernelGGL(k_xi_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
-                       ctx->id, ctx->q, ctx->adj + n, ctx->nb_cnt, ctx->nb_col, ctx->part);
+                       ctx->id, ctx->q, ctx->adj + n, nullptr, ctx->nb_cnt, ctx->nb_col, ctx->part);
     HIPCHK(hipGetLastError());
     CHK(reduce(ctx, nb, 10, 0, 50));
     double t[10];
@@ -1923,6 +1925,77 @@ int simox_param_grad(simox_ctx* ctx, double grad[10], double* lambda, int* iters
 
 int simox_param_grad_all(simox_ctx* ctx, double grad[15], double* lambda, int* iters) {
   return param_grad_impl(ctx, grad, true, lambda, iters);
+}
+
+// d(sum_i u_i . F_i)/dtheta, theta = (chi, J, xi) of the five species, F the
+// forces of FORCE + PME, through the charge equilibration: b = dS/dq (pair
+// terms and PME, k_qvjp_*), H mu = P b (the adjoint solve), then
+// -mu . dG/dtheta plus, for xi, the direct CTIP force term.
+int simox_param_force_grad(simox_ctx* ctx, const double* u, double grad[15], int* iters) {
+  if (!ctx || !need_state(ctx) || !grad) return SIMOX_ERR_ARG;
+  if (ctx->mode == SIMOX_MODE_FAITHFUL) {
+    ctx->err = "parameter gradients need a converged charge solve (modes FP64, MIXED)";
+    return SIMOX_ERR_ARG;
+  }
+  const int n = ctx->n, nb = grid_blocks(n);
+  const size_t ng = size_t(ctx->g.K[0]) * ctx->g.K[1] * ctx->g.K[2];
+  if (!ctx->adj) CHK(dalloc(ctx, &ctx->adj, 6 * size_t(n)));
+  if (!ctx->vjp_t) CHK(dalloc(ctx, &ctx->vjp_t, size_t(n)));
+  if (!ctx->vjp_up) CHK(dalloc(ctx, &ctx->vjp_up, 3 * size_t(n)));
+  if (!ctx->dip_acc) CHK(dalloc(ctx, &ctx->dip_acc, ng));
+  CHK(ensure_nlist(ctx));
+  if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  const double* ud = ctx->f;  // the forces of the last FORCE + PME
+  if (u) {
+    if (!ctx->vjp_u) CHK(dalloc(ctx, &ctx->vjp_u, 3 * size_t(n)));
+    CHK(upload_caller(ctx, u, 3, ctx->vjp_u));
+    ud = ctx->vjp_u;
+  }
+  CHK(ensure_part(ctx, size_t(nb) * 10));
+  double* b = ctx->adj;
+  // pair terms
+  hipLaunchKernelGGL(k_qvjp_pair, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                     ctx->id, ctx->q, ud, ctx->nb_cnt, ctx->nb_col, b);
+  // PME: u' = u - mean(u); t1 with the grid of the charges; the dipole grid
+  hipLaunchKernelGGL(k_sum3, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ud, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 3, 0, 35));
+  hipLaunchKernelGGL(k_center3, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ud, ctx->dscal + 35,
+                     ctx->vjp_up);
+  HIPCHK(hipGetLastError());
+  CHK(pme_eval(ctx, ctx->q, nullptr, 0.0, false));
+  hipLaunchKernelGGL(k_qvjp_rec1<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                     ctx->vjp_up, ctx->pm.greal64, ctx->vjp_t);
+  const int ngb = int((ng + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_fill_u64, dim3(ngb), dim3(BLOCK), 0, ctx->st, ng, ctx->dip_acc);
+  hipLaunchKernelGGL(k_qvjp_dipole, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x, ctx->q,
+                     ctx->vjp_up, ctx->dip_acc);
+  hipLaunchKernelGGL(k_fixed_to_grid, dim3(ngb), dim3(BLOCK), 0, ctx->st, ng, ctx->dip_acc,
+                     ctx->pm.greal64);
+  HIPCHK(hipGetLastError());
+  CHK(pme_chain<double2>(ctx, ctx->pm, ctx->pm.spec64, ctx->pm.tw64, ctx->pm.fc64, ctx->pm.G64,
+                         ctx->pm.greal64));
+  const double cst = KC * double(ng) / ctx->V / PI;  // force.f90:436
+  hipLaunchKernelGGL(k_qvjp_rec2<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->x,
+                     ctx->pm.greal64, ctx->vjp_t, cst, b);
+  HIPCHK(hipGetLastError());
+  // adjoint: H mu = P b
+  CHK(adjoint_solve(ctx, 1e-12, 20000, iters));
+  CHK(ensure_part(ctx, size_t(nb) * 10));
+  hipLaunchKernelGGL(k_mu_sums, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->id, ctx->q, b + n,
+                     ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 10, 0, 50));
+  double t[10];
+  CHK(fetch(ctx, 50, 10, t));
+  for (int sp = 0; sp < 10; ++sp) grad[sp] = -t[sp];
+  hipLaunchKernelGGL(k_xi_grad, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab, ctx->x,
+                     ctx->id, ctx->q, b + n, ud, ctx->nb_cnt, ctx->nb_col, ctx->part);
+  HIPCHK(hipGetLastError());
+  CHK(reduce(ctx, nb, 10, 0, 50));
+  CHK(fetch(ctx, 50, 10, t));
+  for (int sp = 0; sp < 5; ++sp) grad[10 + sp] = t[sp] - t[5 + sp];
+  return SIMOX_OK;
 }
 
 int simox_set_reflect(simox_ctx* ctx, int on) {

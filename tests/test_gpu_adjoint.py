@@ -92,3 +92,45 @@ def test_torch_gradcheck_with_xi(system):
 
     v0 = torch.tensor(t0[idx], dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(energy, (v0,), eps=1e-6, atol=1e-4, rtol=1e-4)
+
+
+def test_force_loss_gradient_matches_central_differences(system):
+    """L = E + w sum |F|^2 (F of FORCE + PME at the converged charges): the
+    adjoint gradient in chi, J, xi against central differences of L over
+    re-converged solves; a random-u vector-Jacobian product the same way."""
+    from simox_amd.autograd import potential_setup, solve_energy, solve_loss, theta_of
+    t0 = theta_of(potential_setup(), with_xi=True)
+    w = 0.01
+    L0, grad = solve_loss(system, t0, w)
+    for k in (0, 8, 10, 13):  # chi_Al, J_O, xi_Al, xi_O
+        h = 1e-5
+        tp, tm = t0.copy(), t0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (solve_loss(system, tp, w)[0] - solve_loss(system, tm, w)[0]) / (2 * h)
+        floor = 1e-12 * abs(L0) / h
+        assert abs(fd - grad[k]) <= 1e-5 * abs(grad[k]) + floor, (k, grad[k], fd)
+    # vector-Jacobian product with a random u: d(sum u.F)/dtheta
+    u = np.random.default_rng(3).normal(size=(system.n, 3))
+
+    def S(t):
+        E, g = solve_energy(system, t)
+        val = float((u * g.get_state()[2]).sum())
+        return val, g
+
+    _, g = S(t0)
+    gv = g.param_force_grad(u)
+    g.close()
+    vjp = np.concatenate([gv["dchi"], gv["dJ"], gv["dxi"]])
+    for k in (3, 5, 13):
+        h = 1e-5
+        tp, tm = t0.copy(), t0.copy()
+        tp[k] += h
+        tm[k] -= h
+        sp, gp = S(tp)
+        gp.close()
+        sm, gm = S(tm)
+        gm.close()
+        fd = (sp - sm) / (2 * h)
+        floor = 1e-12 * np.abs(u).sum() * 10.0 / h
+        assert abs(fd - vjp[k]) <= 1e-5 * abs(vjp[k]) + floor, (k, vjp[k], fd)
