@@ -54,6 +54,9 @@ WORKLOADS = {
     "c3": "C3: Al slab 64x64x60 cells (983,040) + 65,536 O, PME 256x256x512, Rcut 10 A, 900 K",
     "c4": "C4: 256 independent Al/O slabs, 4x4x6 to 10x10x12 cells (410-4,904 atoms), "
           "0.1-1.0 ML O, 300-1200 K, per-replica PME grid, batched lockstep steps",
+    "c5": "C5: 64 snapshots of an Al slab 10x10x12 cells (4,800) + 400 O, 300-1200 K, "
+          "PME 40x40x72, Rcut 10 A; gradients of E + w sum|F|^2 in chi, J, xi (adjoint) "
+          "and in 8 x 50 EAM spline coefficients, FP64",
 }
 
 
@@ -274,6 +277,60 @@ def nve_drift(args):
     return out
 
 
+def bench_gradients(args):
+    """Parameter gradients of the fitting loss L = E + w sum |F|^2 over the 64
+    C5 snapshots (FP64): per snapshot a converged charge solve (from q = 0),
+    FORCE + PME, the adjoint gradients in chi, J, xi (energy and force terms)
+    and the spline-EAM coefficient gradients (energy and force terms)."""
+    from simox_amd import MODE_FP64, Simox
+    from simox_amd import eam_spline as es
+    from simox_amd.systems import snapshots
+    snaps = snapshots(args.snapshots)
+    lo, h, coef = es.fit(r_end=10.0, m=50)
+    w = 0.01
+    total = np.zeros(15)
+    total_c = np.zeros_like(coef)
+    loss = 0.0
+    iters = []
+    print("bench: c5 gradients over %d snapshots" % len(snaps), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for k, s in enumerate(snaps):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
+        g.set_charge_tol(1e-10, 20000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        g.set_eam_splines(lo, h, coef)
+        r = g.ctip_solve()
+        e = g.force()
+        erec = g.pme()
+        f = g.get_state()[2]
+        loss += r["Eself"] + e["Edir"] + erec + e["Ectip"] + e["Epair"] + e["Egroup"] + \
+            w * float((f ** 2).sum())
+        ge = g.param_grad(with_xi=True)
+        gf = g.param_force_grad()
+        total += np.concatenate([ge["dchi"], ge["dJ"], ge["dxi"]]) + 2 * w * np.concatenate(
+            [gf["dchi"], gf["dJ"], gf["dxi"]])
+        total_c += g.eam_spline_grad() + 2 * w * g.eam_spline_force_vjp()
+        iters.append((r["iters"], ge["iters"], gf["iters"]))
+        g.close()
+        if (k + 1) % 16 == 0:
+            print("bench: %d snapshots" % (k + 1), file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    n_atoms = sum(s.n for s in snaps)
+    return {
+        "metric": "snapshot gradients/s", "value": len(snaps) / el, "unit": "snapshots/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS["c5"], "snapshots": len(snaps), "atoms_per_snapshot":
+                   snaps[0].n, "loss_weight_w": w, "charge_tol": 1e-10, "adjoint_tol": 1e-12},
+        "seconds": el, "atom_snapshots_per_s": n_atoms / el,
+        "loss": loss, "grad_ctip_norm": float(np.linalg.norm(total)),
+        "grad_spline_norm": float(np.linalg.norm(total_c)),
+        "iters_mean": {"charge_solve": float(np.mean([t[0] for t in iters])),
+                       "adjoint_energy": float(np.mean([t[1] for t in iters])),
+                       "adjoint_force": float(np.mean([t[2] for t in iters]))},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def accuracy_vs_fp64(s, g_mixed, args):
     """Force / charge / energy error of the mixed path against the FP64 path on
     the same state: positions and velocities of the mixed run, FP64 charges
@@ -319,6 +376,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="converged charge solve: RMS projected gradient (eV/e)")
     ap.add_argument("--replicas", type=int, default=256, help="replica count of the c4 workload")
+    ap.add_argument("--snapshots", type=int, default=64, help="snapshot count of the c5 workload")
     ap.add_argument("--drift", type=int, default=0,
                     help="instead of the throughput line: NVE drift of E_total over this many "
                          "C1 steps, MIXED and FP64")
@@ -340,6 +398,9 @@ def main():
         torch.cuda.empty_cache()
     if args.drift > 0:
         print(json.dumps(nve_drift(args)))
+        return 0
+    if args.config == "c5":
+        print(json.dumps(bench_gradients(args)))
         return 0
     if args.config == "c4":
         out = bench_replicas(args)

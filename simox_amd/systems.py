@@ -204,3 +204,16 @@ def tiny_alloy(seed=1, nx=5, ny=5, nz=4, n_o=16, frac_ni=0.15, frac_fe=0.10, fft
     return System(ids=ids, x=s.x, v=v, q=s.q, box=s.box, fft=s.fft, rcut=s.rcut, dt=s.dt,
                   name="tiny_alloy", meta=dict(s.meta, frac_ni=frac_ni, frac_fe=frac_fe))
 
+
+def snapshots(n_snap=64, seed=5):
+    """Seeded snapshots for parameter fitting: Al slab 10x10x12 cells (4,800 Al)
+    + 400 O in a 2-10 A layer, temperatures 300-1200 K, Rcut 10 A (4 x 4 x 6
+    cells), PME grid 40 x 40 x 72."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_snap):
+        T = float(rng.uniform(300.0, 1200.0))
+        out.append(al_o_slab(10, 10, 12, 400, seed * 1000 + k, T, (40, 40, 72), 10.0,
+                             vacuum=20.0, o_layer=(2.0, 10.0), o_dmin=2.0, name="c5_%d" % k))
+    return out
+
