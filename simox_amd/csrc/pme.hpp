@@ -466,6 +466,97 @@ __global__ void __launch_bounds__(BLOCK) k_fft_z_fused(Geom g, FftPlan pl,
   }
 }
 
+// Register-resident transform of one 512-point line by one wave: lane j
+// holds elements j + 64 r (r < 8) on entry and on return.  The three radix-8
+// Stockham stages of fft_stage_ct for L = 512 (Ns = 1, 8, 64), with the same
+// twiddles and butterflies, so the result is that of the LDS transform; the
+// data cross lanes twice, through the wave's LDS line x (padded by one
+// element in nine against bank conflicts).
+__device__ inline int pad9(int e) { return e + (e >> 3); }
+template <class C>
+__device__ inline void reg_fft512(C (&v)[8], C* __restrict__ x, const C* __restrict__ twl,
+                                  int j, int sign) {
+  Dft<C, 8>::run(v, sign);  // Ns = 1: no twiddles; output 8 j + r
+#pragma unroll
+  for (int r = 0; r < 8; ++r) x[pad9(8 * j + r)] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = x[pad9(j + 64 * r)];
+  __syncthreads();
+  {  // Ns = 8: twiddle W^{8 r k}, k = j mod 8; output (j/8) 64 + k + 8 r
+    const int k = j & 7;
+#pragma unroll
+    for (int r = 1; r < 8; ++r)
+      if (k > 0) {
+        C w = twl[r * k * 8];
+        if (sign > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    Dft<C, 8>::run(v, sign);
+    const int o = (j >> 3) * 64 + k;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) x[pad9(o + 8 * r)] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = x[pad9(j + 64 * r)];
+  __syncthreads();
+  // Ns = 64: twiddle W^{r j}; output j + 64 r, the lane's own elements
+#pragma unroll
+  for (int r = 1; r < 8; ++r)
+    if (j > 0) {
+      C w = twl[r * j];
+      if (sign > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  Dft<C, 8>::run(v, sign);
+}
+
+// K7c/K8 for Kz = 512: k_fft_z_fused with one line per wave held in
+// registers (two LDS exchanges per transform instead of the three stage
+// passes through LDS of the tiled kernel).  Four lines per workgroup; the
+// energy partial of each workgroup goes to epart[block].
+template <class C>
+__global__ void __launch_bounds__(BLOCK) k_fft_z512_reg(Geom g, const C* __restrict__ tw,
+                                                        C* __restrict__ spec,
+                                                        const decltype(C{}.x)* __restrict__ G,
+                                                        double* __restrict__ epart) {
+  using R = decltype(C{}.x);
+  constexpr int L = 512, NW = BLOCK / WAVE;
+  __shared__ C twl[L];
+  __shared__ C xb[NW][L + L / 8];
+  __shared__ double lds_red[NW];
+  load_twiddles(twl, tw, L);
+  const int j = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int Ky = g.K[1], nlines = g.H * Ky;
+  const int line = blockIdx.x * NW + w;
+  const bool live = line < nlines;
+  const size_t base = size_t(live ? line : 0) * L;
+  C v[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    v[r].x = R(0); v[r].y = R(0);
+    if (live) v[r] = spec[base + j + 64 * r];
+  }
+  __syncthreads();  // twiddles in LDS
+  reg_fft512(v, xb[w], twl, j, -1);
+  double en[1] = {0.0};
+  const int kx = live ? line / Ky : 0;
+  const double wx = (kx == 0 || 2 * kx == g.K[0]) ? 1.0 : 2.0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const R gv = live ? G[base + j + 64 * r] : R(0);
+    if (live) en[0] += wx * double(gv) * (double(v[r].x) * double(v[r].x) + double(v[r].y) * double(v[r].y));
+    v[r].x *= gv; v[r].y *= gv;
+  }
+  reg_fft512(v, xb[w], twl, j, +1);
+  block_sum<1>(en, lds_red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = en[0];
+  if (live)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) spec[base + j + 64 * r] = v[r];
+}
+
 // K7d: C2R along x (inverse, unnormalised).  Block (bz, y) gathers the half
 // lines (y, z = bz*M ..), stages them in LDS, forms Z = A + i B over the full
 // Hermitian line (A[L-k] = conj(A[k])), inverse-transforms, and writes Re Z

@@ -66,6 +66,7 @@ struct NcgRun {
 // workgroup, and their dynamic LDS bytes.
 struct FftCfg {
   int mx = 0, my = 0, mz = 0, zblocks = 0;
+  bool zreg = false;  // Kz = 512: the register-resident z pass (4 lines per workgroup)
   size_t shx = 0, shy = 0, shz = 0;
 };
 
@@ -554,8 +555,12 @@ int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& 
                      spec, fc.my, Ky + 1, -1);
   if (timed) tstop(ctx, 2, ev);
   if (timed) tstart(ctx, 3, &ev);
-  hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, pm.pl[2], tw[2],
-                     spec, fc.mz, G, ctx->part);
+  if (fc.zreg)
+    hipLaunchKernelGGL(k_fft_z512_reg<C>, dim3(fc.zblocks), dim3(BLOCK), 0, ctx->st, g, tw[2],
+                       spec, G, ctx->part);
+  else
+    hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, pm.pl[2], tw[2],
+                       spec, fc.mz, G, ctx->part);
   if (timed) tstop(ctx, 3, ev);
   if (timed) tstart(ctx, 2, &ev);
   hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, pm.pl[1], tw[1],
@@ -607,6 +612,13 @@ int choose_fft(simox_ctx* ctx, const Geom& g, FftCfg* c) {
   if (const char* e = std::getenv("SIMOX_FFT_MZ")) {
     const int m = std::atoi(e);
     if (m >= 1 && m <= 64 && fft_shz<C>(g, m) <= FFT_LDS_MAX) c->mz = m;
+  }
+  // Kz = 512 outside the FAITHFUL mode: the register-resident z pass, four
+  // lines per workgroup (its energy partials group four lines, like mz = 4)
+  const char* zr = std::getenv("SIMOX_FFT_ZREG");
+  if (Kz == 512 && ctx->mode != SIMOX_MODE_FAITHFUL && !(zr && std::atoi(zr) == 0)) {
+    c->zreg = true;
+    c->mz = 4;
   }
   if (!c->mx || !c->my || !c->mz) return SIMOX_ERR_ARG;
   fft_finish<C>(g, c);

@@ -210,3 +210,62 @@ def test_mixed_preconditioned_matches_jacobi_c2_shape():
     assert e_F(fa, fb) <= 1e-5
     assert np.abs(qa - qb).max() <= 1e-6
     assert ia < ib
+
+
+def test_z512_register_pass(oracle_mod, monkeypatch):
+    """Kz = 512 takes the register-resident z pass (k_fft_z512_reg) outside
+    FAITHFUL: FP64 forces and reciprocal energy against the oracle (exact
+    weights, 1e-10), and against the tiled LDS z pass (SIMOX_FFT_ZREG=0) in
+    FP64 (1e-13); MIXED with either pass within 1e-5 of FP64."""
+    s = tiny(seed=4, nx=3, ny=3, nz=3, n_o=8, fft=(12, 12, 512), rcut=5.0)
+    q = np.random.default_rng(5).normal(size=s.n) * 0.3
+    print("z512 register pass, n", s.n, flush=True)
+
+    def run(mode, zreg):
+        monkeypatch.setenv("SIMOX_FFT_ZREG", "1" if zreg else "0")
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_state(s.ids, s.x, s.v, None, q)
+        g.force()
+        erec = g.pme()
+        f = g.get_state()[2]
+        g.close()
+        return erec, f
+
+    e_reg, f_reg = run(MODE_FP64, True)
+    e_lds, f_lds = run(MODE_FP64, False)
+    assert e_reg == pytest.approx(e_lds, rel=1e-13)
+    assert np.abs(f_reg - f_lds).max() <= 1e-13 * np.abs(f_lds).max()
+    o = oracle_mod.Oracle(s.n, s.box, s.fft, s.rcut)
+    o.cell_sort(s.x)
+    oracle_mod.set_spread_rounding(False)
+    try:
+        _, psi = o.pme_self(s.ids, q)
+        ref = o.force(s.ids, s.x, q)
+        e_o, f_o = o.pme(s.ids, s.x, q, psi, ref["f"])
+    finally:
+        oracle_mod.set_spread_rounding(True)
+    assert np.abs(f_reg - f_o).max() <= 1e-10 * np.abs(f_o).max()
+    assert e_reg == pytest.approx(e_o, rel=1e-10)
+    m_reg, fm_reg = run(MODE_MIXED, True)
+    m_lds, fm_lds = run(MODE_MIXED, False)
+    # MIXED: FP32 grids, the bar of MIXED against FP64 (1e-5 of the RMS force)
+    assert m_reg == pytest.approx(m_lds, rel=1e-6)
+    assert e_F(fm_reg, f_reg) <= 1e-5 and e_F(fm_lds, f_reg) <= 1e-5
+    assert m_reg == pytest.approx(e_reg, rel=1e-5)
+
+
+def test_z512_register_pass_mixed_solve(monkeypatch):
+    """The MIXED charge solve's FP32 PME passes with Kz = 512: the same
+    converged charges (1e-9) with the register z pass as with the tiled one."""
+    s = tiny(seed=4, nx=3, ny=3, nz=3, n_o=8, fft=(12, 12, 512), rcut=5.0)
+    out = {}
+    for zreg in (True, False):
+        monkeypatch.setenv("SIMOX_FFT_ZREG", "1" if zreg else "0")
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_MIXED)
+        g.set_charge_tol(1e-10, 2000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        r = g.ctip_solve()
+        out[zreg] = (g.get_state()[3], r["Eself"])
+        g.close()
+    assert np.abs(out[True][0] - out[False][0]).max() <= 1e-9
+    assert out[True][1] == pytest.approx(out[False][1], rel=1e-9)
