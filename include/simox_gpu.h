@@ -266,6 +266,10 @@ int simox_batch_step(simox_ctx** ctxs, int nrep, int nsteps, double dt, int sort
    accumulated ms and launch count since the last simox_timing_enable. */
 int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launches);
 int simox_timing_enable(simox_ctx* ctx, int on);
+/* Categories timed while timing is enabled: bit `which` of mask (0: all, the
+   default).  Each timed launch costs two event records on the stream, so a
+   throughput measurement times only the kernel it needs. */
+int simox_timing_select(simox_ctx* ctx, unsigned int mask);
 
 #ifdef __cplusplus
 }

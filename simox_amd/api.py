@@ -21,7 +21,7 @@ EXPORTS = [
     "simox_get_state", "simox_cell_sort", "simox_get_cells", "simox_force", "simox_get_rho",
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
-    "simox_timing_enable", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
+    "simox_timing_enable", "simox_timing_select", "simox_set_charge_precond", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
     "simox_eam_spline_grad", "simox_param_grad_all", "simox_eam_spline_force_vjp",
     "simox_param_force_grad",
@@ -118,6 +118,7 @@ def load_library(path=LIB_PATH):
     lib.simox_step.argtypes = [vp, i, d, i, i, vp]
     lib.simox_kernel_timing.argtypes = [vp, i, P(d), P(C.c_longlong)]
     lib.simox_timing_enable.argtypes = [vp, i]
+    lib.simox_timing_select.argtypes = [vp, C.c_uint]
     lib.simox_set_charge_precond.argtypes = [vp, i]
     lib.simox_charge_stats.argtypes = [vp, P(i), P(i)]
     lib.simox_potential_setup.argtypes = [vp]
@@ -393,6 +394,11 @@ class Simox:
 
     def timing(self, on=True):
         self._check(self.lib.simox_timing_enable(self.h, 1 if on else 0))
+
+    def timing_select(self, which=None):
+        """Time only the categories in `which` (None: all)."""
+        mask = 0 if which is None else sum(1 << int(w) for w in which)
+        self._check(self.lib.simox_timing_select(self.h, mask))
 
     def kernel_time(self, which):
         ms = C.c_double()

@@ -181,6 +181,7 @@ struct simox_ctx {
   std::string err;
   // timing
   bool timing = false;
+  unsigned timing_mask = ~0u;  // categories timed while timing is on
   std::vector<TimedLaunch> tl;
   double tms[16] = {0};
   long long tcnt[16] = {0};
@@ -211,13 +212,13 @@ int dalloc(simox_ctx* ctx, T** p, size_t count) {
 }
 
 void tstart(simox_ctx* ctx, int which, hipEvent_t* a) {
-  if (!ctx->timing) return;
+  if (!ctx->timing || !((ctx->timing_mask >> which) & 1u)) return;
   (void)hipEventCreate(a);
   (void)hipEventRecord(*a, ctx->st);
   (void)which;
 }
 void tstop(simox_ctx* ctx, int which, hipEvent_t a) {
-  if (!ctx->timing) return;
+  if (!ctx->timing || !((ctx->timing_mask >> which) & 1u)) return;
   hipEvent_t b;
   (void)hipEventCreate(&b);
   (void)hipEventRecord(b, ctx->st);
@@ -2383,6 +2384,14 @@ int simox_timing_enable(simox_ctx* ctx, int on) {
   tresolve(ctx);
   ctx->timing = on != 0;
   for (int k = 0; k < 16; ++k) { ctx->tms[k] = 0; ctx->tcnt[k] = 0; }
+  return SIMOX_OK;
+}
+
+int simox_timing_select(simox_ctx* ctx, unsigned int mask) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  tresolve(ctx);
+  ctx->timing_mask = mask ? mask : ~0u;
   return SIMOX_OK;
 }
 
