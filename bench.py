@@ -341,6 +341,16 @@ def bench_gradients(args):
     }
 
 
+def inner_operator(s, mode):
+    """The MIXED inner solves' operator (simox_set_inner_grid's default rule)."""
+    if mode != "mixed":
+        return "the reference's operator, FP64"
+    if np.prod([float(k) for k in s.fft]) >= 2 ** 23:
+        return "FP32; reciprocal part on the %dx%dx%d grid (half resolution, order 4)" % tuple(
+            k // 2 for k in s.fft)
+    return "FP32 on the reference's grid"
+
+
 def accuracy_vs_fp64(s, g_mixed, args):
     """Force / charge / energy error of the mixed path against the FP64 path on
     the same state: positions and velocities of the mixed run, FP64 charges
@@ -450,6 +460,7 @@ def main():
                    "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
                    "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
+                   "inner_operator": inner_operator(s, args.mode),
                    "fft_tiles": fft_tiles},
         "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -472,7 +483,8 @@ def main():
                             "charge_iters_last_step": it2, "precond": "jacobi"}
         out["speedup_vs_fp64"] = value / v2
         # the same FP64 kernels with the mixed path's preconditioner: the part of
-        # the speed-up that comes from precision alone
+        # the speed-up that comes from the FP32 inner solves (and their coarse
+        # grid where the mixed path uses one)
         g3, _, el3, _ = run_mode(s, "fp64", args, precond=1)
         it3, _ = g3.charge_stats()
         g3.close()

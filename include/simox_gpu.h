@@ -168,6 +168,16 @@ int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
    pair and gather on the coarse grid per iteration; about half the iterations
    for 1M-atom slabs (the Jacobi count grows with the box, this one does not). */
 int simox_set_charge_precond(simox_ctx* ctx, int on);
+/* Reciprocal part of the MIXED mode's inner operator on a coarse grid: half
+   the resolution of the context's grid in every dimension, order-4
+   B-splines, its own influence table (on = 1), or on the context's grid
+   (on = 0).  The inner solves then stop at 3e-2 (else 1e-4) of their
+   restart's residual; every restart evaluates the true gradient with the
+   context's grid in FP64, so the converged charges meet the tolerance for the
+   reference's operator either way.  Default: on in MIXED from 2^23 grid
+   points (C3, 256 x 256 x 512: 361 -> 287 ms per step), off below (C2, 96 x
+   96 x 192, is slower with it).  SIMOX_ERR_ARG outside MIXED. */
+int simox_set_inner_grid(simox_ctx* ctx, int on);
 /* Iterations and FP64 restarts of the last converged charge solve. */
 int simox_charge_stats(simox_ctx* ctx, int* iters, int* restarts);
 

@@ -116,6 +116,14 @@ struct simox_ctx {
   // PME: the reference's grid (pm) and the half-resolution grid of the charge
   // solver's long-range preconditioner (pc, set up on first use)
   PmeGrid pm, pc;
+  // MIXED inner operator's reciprocal part on a coarser grid (pi, half the
+  // resolution, order 4) when inner_coarse; the restarts keep the reference's
+  // grid, so the converged charges are those of the reference's operator.
+  // Inner solves stop at inner_rtol (inner_rtol_c with the coarse grid) x
+  // the true residual of their restart.
+  PmeGrid pi;
+  bool inner_coarse = false;
+  double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
   bool prec_on = false;
   bool small_path = false;  // single-workgroup vector passes (n <= LS_SMALL_MAX)
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
@@ -426,7 +434,7 @@ int do_cell_sort(simox_ctx* ctx) {
   std::swap(ctx->cell, ctx->cell2);
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = ctx->pi.bricks_valid = false;
   return 0;
 }
 
@@ -1041,6 +1049,30 @@ int prec_grid(simox_ctx* ctx, PmeGrid** out) {
   return 0;
 }
 
+// The MIXED inner operator's coarse grid: half the resolution of the
+// reference's grid, order-4 B-splines, its own influence table; set up on
+// first use where those sizes are valid grids (else the inner operator keeps
+// the reference's grid).
+int inner_grid(simox_ctx* ctx) {
+  if (ctx->pi.ready) return 0;
+  int K[3];
+  FftPlan tmp;
+  for (int d = 0; d < 3; ++d) {
+    K[d] = ctx->pm.K[d] / 2;
+    if (ctx->pm.K[d] % 2 || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) {
+      ctx->inner_coarse = false;
+      return 0;
+    }
+  }
+  const int rc = init_grid(ctx, ctx->pi, K, true, 4);
+  if (rc) {
+    free_grid(ctx->pi);
+    ctx->inner_coarse = false;
+    return rc == SIMOX_ERR_ARG ? 0 : rc;
+  }
+  return 0;
+}
+
 // Long-range preconditioner set-up for the current positions (once per
 // solve) on its grid: rho = 1.03 x the maximum over the grid of the
 // low-passed density of 1/J (a spread of w = 1/J through the FFT chain with
@@ -1140,17 +1172,16 @@ int apply_prec(simox_ctx* ctx, const double* g_old, bool f32, double* ls = nullp
 // phi = c_pme W^T (reference grid) with the LDS-tiled gather (the brick
 // lists are those of the spread that filled the grid).
 template <int GX, int GY, int GZ>
-void phi_gather_t(simox_ctx* ctx, bool f32) {
+void phi_gather_t(simox_ctx* ctx, bool f32, const PmeGrid& pm, double c_pme) {
   using T = GatherTile<GX, GY, GZ>;
-  const int nt = T::tiles(ctx->g);
+  const Geom g = geom_of(ctx, pm);
+  const int nt = T::tiles(g);
   if (f32)
     hipLaunchKernelGGL((k_gather_tile<float, GX, GY, GZ>), dim3(nt), dim3(BLOCK), T::lds(4),
-                       ctx->st, ctx->g, ctx->x, ctx->pm.bstart, ctx->pm.border, ctx->pm.greal32,
-                       ctx->c_pme, ctx->phi);
+                       ctx->st, g, ctx->x, pm.bstart, pm.border, pm.greal32, c_pme, ctx->phi);
   else
     hipLaunchKernelGGL((k_gather_tile<double, GX, GY, GZ>), dim3(nt), dim3(BLOCK), T::lds(8),
-                       ctx->st, ctx->g, ctx->x, ctx->pm.bstart, ctx->pm.border, ctx->pm.greal64,
-                       ctx->c_pme, ctx->phi);
+                       ctx->st, g, ctx->x, pm.bstart, pm.border, pm.greal64, c_pme, ctx->phi);
 }
 
 template <int GX, int GY, int GZ>
@@ -1168,8 +1199,10 @@ int gather_attr(simox_ctx* ctx) {
 // 143 us, 64 x 16 x 8: 154 us, 32 x 32 x 8: 192 us; the per-atom gather: 203 us).
 constexpr int PGX = 64, PGY = 8, PGZ = 8;
 
-int phi_gather(simox_ctx* ctx, bool f32) {
-  phi_gather_t<PGX, PGY, PGZ>(ctx, f32);
+int phi_gather(simox_ctx* ctx, bool f32, const PmeGrid* grid = nullptr) {
+  const PmeGrid& pm = grid ? *grid : ctx->pm;
+  const double c = grid ? KC * double(pm.K[0]) * pm.K[1] * pm.K[2] / ctx->V / PI : ctx->c_pme;
+  phi_gather_t<PGX, PGY, PGZ>(ctx, f32, pm, c);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1181,10 +1214,11 @@ int ncg_iter_body(simox_ctx* ctx) {
   float* p32 = f32 ? ctx->pv32 : nullptr;
   double* ls = ctx->dscal + 40;
   // v = H0 p with p.v, p.g, sum p^2 pen''(q)
-  CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32));
+  const PmeGrid* ig = (f32 && ctx->inner_coarse && ctx->pi.ready) ? &ctx->pi : nullptr;
+  CHK(pme_eval(ctx, ctx->pv, nullptr, 0.0, f32, false, ig ? &ctx->pi : nullptr));
   hipEvent_t ev{};
   tstart(ctx, 10, &ev);
-  CHK(phi_gather(ctx, f32));
+  CHK(phi_gather(ctx, f32, ig));
   tstop(ctx, 10, ev);
   tstart(ctx, 6, &ev);
   if (f32) {
@@ -1323,6 +1357,7 @@ int ncg_begin(simox_ctx* ctx) {
   // in an earlier solve may name the other buffer of each pair, so capture anew
   ctx->iter_graph_gen = -1;
   if (ctx->prec_on) CHK(build_prec(ctx));
+  if (use32(ctx) && ctx->inner_coarse) CHK(inner_grid(ctx));
   return ncg_issue_restart(ctx);
 }
 
@@ -1352,7 +1387,8 @@ int ncg_check(simox_ctx* ctx) {
     }
     if (r.restarts >= 60) { r.phase = NCG_DONE; return 0; }
     ++r.restarts;
-    r.target = use32(ctx) ? std::max(0.3 * ctx->tol, 1e-4 * rms_true) : 0.3 * ctx->tol;
+    const double rt = (ctx->inner_coarse && ctx->pi.ready) ? ctx->inner_rtol_c : ctx->inner_rtol;
+    r.target = use32(ctx) ? std::max(0.3 * ctx->tol, rt * rms_true) : 0.3 * ctx->tol;
     float* p32 = use32(ctx) ? ctx->pv32 : nullptr;
     hipLaunchKernelGGL(k_set_dir, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->zv, ctx->pv, p32);
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, ctx->hscal[19],
@@ -1426,7 +1462,7 @@ int do_vv1(simox_ctx* ctx, double dt) {
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = ctx->pi.bricks_valid = false;
   return 0;
 }
 
@@ -1649,6 +1685,15 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->mv, n));
   for (int h = 0; h < 3; ++h) CHK(dalloc(ctx, &ctx->qhist[h], n));
   CHK(dalloc(ctx, &ctx->qtmp, n));
+  // coarse inner grid by default in MIXED from 2^23 grid points on (measured:
+  // C3, 256 x 256 x 512, 361 -> 287 ms per step; C2, 96 x 96 x 192, slower)
+  ctx->inner_coarse = mode == SIMOX_MODE_MIXED &&
+                      double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
+  if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_INNER_RTOL")) {
+    const double v = std::atof(e);
+    if (v > 0.0 && v < 1.0) ctx->inner_rtol = ctx->inner_rtol_c = v;
+  }
   if (const char* e = std::getenv("SIMOX_PREDICTOR_ORDER")) {
     const int o = std::atoi(e);
     if (o >= 0 && o <= 3) ctx->pred_order = o;
@@ -1722,6 +1767,7 @@ int simox_destroy(simox_ctx* ctx) {
     if (b) (void)hipFree(b);
   free_grid(ctx->pm);
   free_grid(ctx->pc);
+  free_grid(ctx->pi);
   if (ctx->hscal) (void)hipHostFree(ctx->hscal);
   if (ctx->herr) (void)hipHostFree(ctx->herr);
   if (ctx->hstage) (void)hipHostFree(ctx->hstage);
@@ -1756,6 +1802,16 @@ int simox_set_mode(simox_ctx* ctx, int mode) {
 int simox_set_charge_precond(simox_ctx* ctx, int on) {
   if (!ctx) return SIMOX_ERR_ARG;
   ctx->prec_on = on != 0;
+  return SIMOX_OK;
+}
+
+int simox_set_inner_grid(simox_ctx* ctx, int on) {
+  if (!ctx) return SIMOX_ERR_ARG;
+  if (on && ctx->mode != SIMOX_MODE_MIXED) {
+    ctx->err = "the coarse inner grid belongs to the MIXED mode's inner solves";
+    return SIMOX_ERR_ARG;
+  }
+  ctx->inner_coarse = on != 0;
   return SIMOX_OK;
 }
 
@@ -1812,7 +1868,7 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
   if (v) CHK(upload_caller(ctx, v, 3, ctx->v));
   if (f) CHK(upload_caller(ctx, f, 3, ctx->f));
   if (q) { CHK(upload_caller(ctx, q, 1, ctx->q)); ctx->nhist = 0; }
-  if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->pm.bricks_valid = ctx->pc.bricks_valid = false; }
+  if (x) { ctx->matrix_valid = false; ctx->nlist_valid = false; ctx->pm.bricks_valid = ctx->pc.bricks_valid = ctx->pi.bricks_valid = false; }
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }
@@ -2028,7 +2084,7 @@ int simox_remove_rigid_motion(simox_ctx* ctx) {
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = false;
   ctx->nlist_valid = false;
-  ctx->pm.bricks_valid = ctx->pc.bricks_valid = false;
+  ctx->pm.bricks_valid = ctx->pc.bricks_valid = ctx->pi.bricks_valid = false;
   HIPCHK(hipStreamSynchronize(ctx->st));
   return SIMOX_OK;
 }

@@ -269,3 +269,36 @@ def test_z512_register_pass_mixed_solve(monkeypatch):
         g.close()
     assert np.abs(out[True][0] - out[False][0]).max() <= 1e-9
     assert out[True][1] == pytest.approx(out[False][1], rel=1e-9)
+
+
+def test_mixed_coarse_inner_grid_matches_fp64():
+    """MIXED with the inner operator's reciprocal part on the half-resolution
+    grid (simox_set_inner_grid): the restarts use the reference's grid in
+    FP64, so the converged charges equal FP64's to 1e-9 and the forces to
+    1e-5 of the RMS force, with the coarse grid on and off."""
+    from simox_amd.systems import al_o_slab
+    s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (96, 96, 64), 10.0, vacuum=20.0,
+                  o_layer=(2.0, 10.0), o_dmin=2.0, name="inner")
+    print("coarse inner grid, n", s.n, flush=True)
+    out = {}
+    for tag in ("fp64", "coarse", "fine"):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64 if tag == "fp64" else MODE_MIXED)
+        if tag != "fp64":
+            g.set_inner_grid(tag == "coarse")
+        g.set_charge_tol(1e-10, 5000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        r = g.ctip_solve()
+        g.force()
+        g.pme()
+        out[tag] = (g.get_state()[3], g.get_state()[2], r["iters"], g.charge_stats()[1])
+        g.close()
+    for tag in ("coarse", "fine"):
+        assert np.abs(out[tag][0] - out["fp64"][0]).max() <= 1e-9, tag
+        assert e_F(out[tag][1], out["fp64"][1]) <= 1e-5, tag
+    assert out["coarse"][3] >= out["fine"][3]  # more restarts with the coarse operator
+    with pytest.raises(Exception):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64)
+        try:
+            g.set_inner_grid(True)
+        finally:
+            g.close()
