@@ -30,8 +30,18 @@ KERNELS = {0: "k_spread_tile", 1: "k_fft_x_r2c", 2: "k_fft_y (fwd + inv)",
 # timing category 6 is k_dudq in the faithful solve, k_h0matvec in the converged ones
 
 
-def kernel_name(w, mode):
-    return "k_dudq" if (w == 6 and mode == "faithful") else KERNELS[w]
+def coarse_inner(s, mode):
+    """simox_set_inner_grid's default: MIXED from 2^23 grid points on (the
+    inner matvec then reads FP16 matrix values, k_h0matvec16)."""
+    return mode == "mixed" and float(np.prod([float(k) for k in s.fft])) >= 2 ** 23
+
+
+def kernel_name(w, mode, s=None):
+    if w == 6 and mode == "faithful":
+        return "k_dudq"
+    if w == 6 and s is not None and coarse_inner(s, mode):
+        return "k_h0matvec16"
+    return KERNELS[w]
 
 
 def measured_traffic(kernel, cfg, mode):
@@ -84,8 +94,10 @@ def algorithmic_bytes(which, s, mode, nb_entries):
     if which == 6:
         if mode == "faithful":  # dUdq: grid read, x id q b p in, g out, ELL rows
             return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
-        # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows
-        return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + r)
+        # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows; FP16
+        # values in the coarse-grid inner operator
+        rv = 2 if coarse_inner(s, mode) else r
+        return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + rv)
     if which == 10:  # tiled potential gather: grid read once, x in, phi out
         return ng * r + n * (24 + 8)
     if which == 7:   # pair force: x q id dF in, f out
@@ -345,9 +357,9 @@ def inner_operator(s, mode):
     """The MIXED inner solves' operator (simox_set_inner_grid's default rule)."""
     if mode != "mixed":
         return "the reference's operator, FP64"
-    if np.prod([float(k) for k in s.fft]) >= 2 ** 23:
-        return "FP32; reciprocal part on the %dx%dx%d grid (half resolution, order 4)" % tuple(
-            k // 2 for k in s.fft)
+    if coarse_inner(s, mode):
+        return ("FP32 with FP16 matrix values; reciprocal part on the %dx%dx%d grid (half "
+                "resolution, order 4); restarts FP64 on the reference grid" % tuple(k // 2 for k in s.fft))
     return "FP32 on the reference's grid"
 
 
@@ -446,7 +458,7 @@ def main():
     per_launch_s = ms / 1e3 / max(launches, 1)
     bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries)
     achieved = bytes_launch / per_launch_s / 1e9
-    traffic = measured_traffic(kernel_name(dom, args.mode), args.config, args.mode)
+    traffic = measured_traffic(kernel_name(dom, args.mode, s), args.config, args.mode)
     value = s.n * args.steps / elapsed
     out = {
         "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
@@ -462,12 +474,12 @@ def main():
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
                    "inner_operator": inner_operator(s, args.mode),
                    "fft_tiles": fft_tiles},
-        "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode), "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode, s), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": per_launch_s * 1e3, "launches": launches},
-        "kernels_ms": {kernel_name(w, args.mode): round(breakdown[w][0], 3) for w in KERNELS},
-        "kernels_launches": {kernel_name(w, args.mode): breakdown[w][1] for w in KERNELS},
+        "kernels_ms": {kernel_name(w, args.mode, s): round(breakdown[w][0], 3) for w in KERNELS},
+        "kernels_launches": {kernel_name(w, args.mode, s): breakdown[w][1] for w in KERNELS},
         "kernels_note": "kernels_ms / kernels_launches: one untimed step after the warm-up "
                         "with every category timed; the timed steps time the roofline kernel only",
     }

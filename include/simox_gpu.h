@@ -170,12 +170,13 @@ int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
 int simox_set_charge_precond(simox_ctx* ctx, int on);
 /* Reciprocal part of the MIXED mode's inner operator on a coarse grid: half
    the resolution of the context's grid in every dimension, order-4
-   B-splines, its own influence table (on = 1), or on the context's grid
-   (on = 0).  The inner solves then stop at 3e-2 (else 1e-4) of their
-   restart's residual; every restart evaluates the true gradient with the
-   context's grid in FP64, so the converged charges meet the tolerance for the
+   B-splines, its own influence table, and the sparse part from FP16 matrix
+   values (on = 1), or on the context's grid with FP32 values (on = 0).  The
+   inner solves then stop at 3e-2 (else 1e-4) of their restart's residual;
+   every restart evaluates the true gradient with the context's grid and the
+   FP64 matrix, so the converged charges meet the tolerance for the
    reference's operator either way.  Default: on in MIXED from 2^23 grid
-   points (C3, 256 x 256 x 512: 361 -> 287 ms per step), off below (C2, 96 x
+   points (C3, 256 x 256 x 512: 361 -> 269 ms per step), off below (C2, 96 x
    96 x 192, is slower with it).  SIMOX_ERR_ARG outside MIXED. */
 int simox_set_inner_grid(simox_ctx* ctx, int on);
 /* Iterations and FP64 restarts of the last converged charge solve. */

@@ -123,6 +123,8 @@ struct simox_ctx {
   // the true residual of their restart.
   PmeGrid pi;
   bool inner_coarse = false;
+  bool inner_f16 = true;  // with the coarse grid: FP16 matrix values in the inner matvec
+  bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
   bool prec_on = false;
   bool small_path = false;  // single-workgroup vector passes (n <= LS_SMALL_MAX)
@@ -140,6 +142,7 @@ struct simox_ctx {
   int* nb_col = nullptr;
   double* nb_val64 = nullptr;
   float* nb_val32 = nullptr;
+  _Float16* nb_val16 = nullptr;  // FP16 values of the coarse-grid inner operator (MIXED)
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
   double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
@@ -891,6 +894,13 @@ int build_matrix(simox_ctx* ctx) {
   if (use32(ctx))
     hipLaunchKernelGGL(k_ell_to_f32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
                        ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
+  ctx->val16_valid = false;
+  if (use32(ctx) && ctx->inner_coarse && ctx->inner_f16) {
+    if (!ctx->nb_val16) CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
+    hipLaunchKernelGGL(k_ell_to_f16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
+                       ctx->nb_cnt, ctx->nb_val64, ctx->nb_val16);
+    ctx->val16_valid = true;
+  }
   HIPCHK(hipGetLastError());
   ctx->matrix_valid = true;
   return 0;
@@ -1221,7 +1231,11 @@ int ncg_iter_body(simox_ctx* ctx) {
   CHK(phi_gather(ctx, f32, ig));
   tstop(ctx, 10, ev);
   tstart(ctx, 6, &ev);
-  if (f32) {
+  if (ig && ctx->val16_valid) {
+    hipLaunchKernelGGL(k_h0matvec16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt,
+                       ctx->nb_col, ctx->nb_val16, ctx->apv, ctx->part);
+  } else if (f32) {
     hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
                        ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
@@ -1690,6 +1704,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   ctx->inner_coarse = mode == SIMOX_MODE_MIXED &&
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_RTOL")) {
     const double v = std::atof(e);
     if (v > 0.0 && v < 1.0) ctx->inner_rtol = ctx->inner_rtol_c = v;
@@ -1760,7 +1775,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

@@ -653,6 +653,68 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
   }
 }
 
+// FP16 copy of the CTIP matrix values for the MIXED inner operator when its
+// reciprocal part is on the coarse grid (rows up to their counts).  Round to
+// nearest; |H0_ij| stays far below the FP16 range.
+__global__ void k_ell_to_f16(int n, int cap, const int* __restrict__ cnt,
+                             const double* __restrict__ v64, _Float16* __restrict__ v16) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cnt[i] < cap ? cnt[i] : cap;
+  for (int k = 0; k < c; ++k) v16[size_t(k) * n + i] = _Float16(float(v64[size_t(k) * n + i]));
+}
+
+// y = H0 p as k_h0matvec<float, float> with FP16 matrix values (6 bytes per
+// entry instead of 8), FP32 products and sums.
+__global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __restrict__ tb,
+                                                      const int* __restrict__ id,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ p,
+                                                      const float* __restrict__ pt,
+                                                      const double* __restrict__ gr,
+                                                      const double* __restrict__ phi_in,
+                                                      const int* __restrict__ cnt,
+                                                      const int* __restrict__ col,
+                                                      const _Float16* __restrict__ val,
+                                                      double* __restrict__ y,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const size_t n = size_t(g.n);
+    const int c = cnt[i];
+    constexpr int U = 16;
+    float s = 0.f;
+    int k = 0;
+    for (; k + U <= c; k += U) {
+      int cc[U];
+      float vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cc[u] = col[size_t(k + u) * n + i];
+        vv[u] = float(val[size_t(k + u) * n + i]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) s += vv[u] * pt[cc[u]];
+    }
+    for (; k < c; ++k) s += float(val[size_t(k) * n + i]) * pt[col[size_t(k) * n + i]];
+    const double pi = p[i];
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double yi = sc.Jp * pi + phi_in[i] + double(s);
+    y[i] = yi;
+    acc[0] = pi * yi;
+    acc[1] = pi * gr[i];
+    acc[2] = outside(sc, q[i]) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = acc[0];
+    part[3 * blockIdx.x + 1] = acc[1];
+    part[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
 // charge predictor: out = c0 q + c1 h0 + c2 h1 + c3 h2 with the first ord+1
 // terms (polynomial extrapolation in time; the coefficients sum to 1, which
 // preserves the total charge)
