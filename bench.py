@@ -26,7 +26,9 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = {0: "k_spread_tile", 1: "k_fft_x_r2c", 2: "k_fft_y (fwd + inv)",
            3: "k_fft_z_fused", 4: "k_fft_x_c2r", 5: "k_gather_force",
            6: "k_h0matvec", 7: "k_force", 8: "k_nlist_density", 9: "k_ctip_matrix",
-           10: "k_gather_tile"}
+           10: "k_gather_tile", 11: "k_spread_tile (inner grid)",
+           12: "k_fft_x_r2c (inner grid)", 13: "k_fft_y (inner grid, fwd + inv)",
+           14: "k_fft_z_fused (inner grid)", 15: "k_fft_x_c2r (inner grid)"}
 # timing category 6 is k_dudq in the faithful solve, k_h0matvec in the converged ones
 
 
@@ -70,11 +72,15 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(which, s, mode, nb_entries):
+def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None):
     """Unique HBM bytes one launch of kernel `which` must move (each array read or
     written once).  r = bytes of the grid/spectrum real type."""
     n = s.n
     Kx, Ky, Kz = s.fft
+    if which >= 11 or (which == 10 and coarse_inner(s, mode)):
+        # the MIXED inner operator's coarse grid: half the resolution
+        Kx, Ky, Kz = Kx // 2, Ky // 2, Kz // 2
+        which = which - 11 if which >= 11 else which
     ng = Kx * Ky * Kz
     nh = (Kx // 2 + 1) * Ky * Kz
     r = 4 if mode == "mixed" else 8
@@ -96,8 +102,9 @@ def algorithmic_bytes(which, s, mode, nb_entries):
             return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
         # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows; FP16
         # values in the coarse-grid inner operator
-        rv = 2 if coarse_inner(s, mode) else r
-        return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + rv)
+        if coarse_inner(s, mode):  # compacted FP16 matrix of the inner operator
+            return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + inner_entries * (4 + 2)
+        return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + r)
     if which == 10:  # tiled potential gather: grid read once, x in, phi out
         return ng * r + n * (24 + 8)
     if which == 7:   # pair force: x q id dF in, f out
@@ -358,8 +365,9 @@ def inner_operator(s, mode):
     if mode != "mixed":
         return "the reference's operator, FP64"
     if coarse_inner(s, mode):
-        return ("FP32 with FP16 matrix values; reciprocal part on the %dx%dx%d grid (half "
-                "resolution, order 4); restarts FP64 on the reference grid" % tuple(k // 2 for k in s.fft))
+        return ("FP32 with FP16 matrix values, entries below 1e-3 eV/e^2 left out; reciprocal "
+                "part on the %dx%dx%d grid (half resolution, order 4); restarts FP64 with the "
+                "full matrix on the reference grid" % tuple(k // 2 for k in s.fft))
     return "FP32 on the reference's grid"
 
 
@@ -456,7 +464,8 @@ def main():
     # launches in the timed steps are the ones timed there
     ms, launches = times[dom]
     per_launch_s = ms / 1e3 / max(launches, 1)
-    bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries)
+    inner_entries = g.inner_entries() if args.mode != "faithful" else nb_entries
+    bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries, inner_entries)
     achieved = bytes_launch / per_launch_s / 1e9
     traffic = measured_traffic(kernel_name(dom, args.mode, s), args.config, args.mode)
     value = s.n * args.steps / elapsed
@@ -468,7 +477,8 @@ def main():
         "config": {"workload": WORKLOADS[args.config], "mode": args.mode, "n_atoms": s.n,
                    "pme_grid": list(s.fft), "ctip_every": 1, "sort_every": 1,
                    "ns_per_day": args.steps / elapsed * 1.0 * 86400 * 1e-6,
-                   "neighbor_entries": nb_entries, "E_total_last": st["Etotal"],
+                   "neighbor_entries": nb_entries, "inner_matrix_entries": inner_entries,
+                   "E_total_last": st["Etotal"],
                    "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
                    "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),

@@ -171,14 +171,21 @@ int simox_set_charge_precond(simox_ctx* ctx, int on);
 /* Reciprocal part of the MIXED mode's inner operator on a coarse grid: half
    the resolution of the context's grid in every dimension, order-4
    B-splines, its own influence table, and the sparse part from FP16 matrix
-   values (on = 1), or on the context's grid with FP32 values (on = 0).  The
+   values without the entries below 1e-3 eV/e^2 (the far real-space Coulomb
+   tail; about a third of the entries stay) (on = 1), or on the context's
+   grid with all FP32 values (on = 0).  The
    inner solves then stop at 3e-2 (else 1e-4) of their restart's residual;
    every restart evaluates the true gradient with the context's grid and the
    FP64 matrix, so the converged charges meet the tolerance for the
    reference's operator either way.  Default: on in MIXED from 2^23 grid
-   points (C3, 256 x 256 x 512: 361 -> 269 ms per step), off below (C2, 96 x
+   points (C3, 256 x 256 x 512: 361 -> 200 ms per step), off below (C2, 96 x
    96 x 192, is slower with it).  SIMOX_ERR_ARG outside MIXED. */
 int simox_set_inner_grid(simox_ctx* ctx, int on);
+/* Entries of the matrix the inner solves' sparse part reads: the compacted
+   FP16 matrix of the coarse inner operator (entries with |H0_ij| >= 1e-3
+   eV/e^2; SIMOX_INNER_DROP sets the bound, 0 keeps all), else the full list.
+   Valid after a charge solve. */
+int simox_inner_entries(simox_ctx* ctx, long long* entries);
 /* Iterations and FP64 restarts of the last converged charge solve. */
 int simox_charge_stats(simox_ctx* ctx, int* iters, int* restarts);
 
@@ -273,7 +280,9 @@ int simox_batch_step(simox_ctx** ctxs, int nrep, int nsteps, double dt, int sort
    directions), 3 FFT z + influence multiply, 4 FFT x C2R, 5 PME force gather,
    6 dUdq (faithful mode: potential gather + sparse matvec; converged modes:
    the sparse matvec H0 p), 7 pair force, 8 EAM density + embedding, 9 CTIP
-   matrix build, 10 tiled potential gather of the converged solve.  Returns
+   matrix build, 10 tiled potential gather of the converged solve (on the
+   coarse grid when the MIXED inner operator uses it), 11-15 spread, FFT x
+   R2C, y, z and x C2R on that coarse grid.  Returns
    accumulated ms and launch count since the last simox_timing_enable. */
 int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launches);
 int simox_timing_enable(simox_ctx* ctx, int on);

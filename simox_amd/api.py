@@ -21,7 +21,7 @@ EXPORTS = [
     "simox_get_state", "simox_cell_sort", "simox_get_cells", "simox_force", "simox_get_rho",
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
-    "simox_timing_enable", "simox_timing_select", "simox_set_charge_precond", "simox_set_inner_grid", "simox_charge_stats", "simox_set_params",
+    "simox_timing_enable", "simox_timing_select", "simox_set_charge_precond", "simox_set_inner_grid", "simox_inner_entries", "simox_charge_stats", "simox_set_params",
     "simox_autotune", "simox_fft_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
     "simox_eam_spline_grad", "simox_param_grad_all", "simox_eam_spline_force_vjp",
     "simox_param_force_grad",
@@ -121,6 +121,7 @@ def load_library(path=LIB_PATH):
     lib.simox_timing_select.argtypes = [vp, C.c_uint]
     lib.simox_set_charge_precond.argtypes = [vp, i]
     lib.simox_set_inner_grid.argtypes = [vp, i]
+    lib.simox_inner_entries.argtypes = [vp, P(C.c_longlong)]
     lib.simox_charge_stats.argtypes = [vp, P(i), P(i)]
     lib.simox_potential_setup.argtypes = [vp]
     lib.simox_fafb.argtypes = [vp]
@@ -270,6 +271,12 @@ class Simox:
     def set_inner_grid(self, on=True):
         """MIXED: the inner solves' reciprocal part on a half-resolution grid."""
         self._check(self.lib.simox_set_inner_grid(self.h, 1 if on else 0))
+
+    def inner_entries(self):
+        """Entries of the matrix the inner solves' sparse part reads."""
+        e = C.c_longlong()
+        self._check(self.lib.simox_inner_entries(self.h, C.byref(e)))
+        return e.value
 
     def charge_stats(self):
         it, rs = C.c_int(), C.c_int()

@@ -142,7 +142,10 @@ struct simox_ctx {
   int* nb_col = nullptr;
   double* nb_val64 = nullptr;
   float* nb_val32 = nullptr;
-  _Float16* nb_val16 = nullptr;  // FP16 values of the coarse-grid inner operator (MIXED)
+  // compacted FP16 matrix of the coarse-grid inner operator (MIXED)
+  _Float16* nb_val16 = nullptr;
+  int *nb_cnt16 = nullptr, *nb_col16 = nullptr;
+  double inner_drop = 1e-3;  // entries with |H0_ij| below this (eV/e^2) left out
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
   double *rv = nullptr, *zv = nullptr, *apv = nullptr, *mv = nullptr;
@@ -547,8 +550,10 @@ Geom geom_of(const simox_ctx* ctx, const PmeGrid& pm) {
   return g;
 }
 
-// Event timing of the PME passes covers the reference's grid only.
-bool timed_grid(const simox_ctx* ctx, const PmeGrid& pm) { return &pm == &ctx->pm; }
+// Event timing of the PME passes: categories 0-4 on the reference's grid,
+// 11-15 (spread, x R2C, y, z, x C2R) on the MIXED inner operator's coarse grid.
+bool timed_grid(const simox_ctx* ctx, const PmeGrid& pm) { return &pm == &ctx->pm || &pm == &ctx->pi; }
+int tcat(const simox_ctx* ctx, const PmeGrid& pm, int k) { return &pm == &ctx->pi ? 11 + k : k; }
 
 template <class C>
 int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& fc,
@@ -558,30 +563,30 @@ int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& 
   const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
   const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
-  if (timed) tstart(ctx, 1, &ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 1), &ev);
   hipLaunchKernelGGL(fx.r2c, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, pm.pl[0],
                      tw[0], greal, spec, fc.mx);
-  if (timed) tstop(ctx, 1, ev);
-  if (timed) tstart(ctx, 2, &ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 1), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 2), &ev);
   hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, pm.pl[1], tw[1],
                      spec, fc.my, Ky + 1, -1);
-  if (timed) tstop(ctx, 2, ev);
-  if (timed) tstart(ctx, 3, &ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 2), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 3), &ev);
   if (fc.zreg)
     hipLaunchKernelGGL(k_fft_z512_reg<C>, dim3(fc.zblocks), dim3(BLOCK), 0, ctx->st, g, tw[2],
                        spec, G, ctx->part);
   else
     hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, pm.pl[2], tw[2],
                        spec, fc.mz, G, ctx->part);
-  if (timed) tstop(ctx, 3, ev);
-  if (timed) tstart(ctx, 2, &ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 3), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 2), &ev);
   hipLaunchKernelGGL(fy.y, dim3(Kz / fc.my, H), dim3(BLOCK), fc.shy, ctx->st, g, pm.pl[1], tw[1],
                      spec, fc.my, Ky + 1, +1);
-  if (timed) tstop(ctx, 2, ev);
-  if (timed) tstart(ctx, 4, &ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 2), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 4), &ev);
   hipLaunchKernelGGL(fx.c2r, dim3(Kz / fc.mx, Ky), dim3(BLOCK), fc.shx, ctx->st, g, pm.pl[0],
                      tw[0], spec, greal, fc.mx);
-  if (timed) tstop(ctx, 4, ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 4), ev);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -757,7 +762,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   const int ntile = ((g.K[0] + TX - 1) / TX) * ((g.K[1] + TY - 1) / TY) * ((g.K[2] + TZ - 1) / TZ);
   const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
-  if (timed) tstart(ctx, 0, &ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 0), &ev);
   if (pm.order == 2 && f32)
     hipLaunchKernelGGL((k_spread_tile<2, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
@@ -773,7 +778,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   else
     hipLaunchKernelGGL((k_spread_tile<4, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
-  if (timed) tstop(ctx, 0, ev);
+  if (timed) tstop(ctx, tcat(ctx, pm, 0), ev);
   HIPCHK(hipGetLastError());
   if (f32)
     return pme_chain<float2>(ctx, pm, pm.spec32, pm.tw32, pm.fc32, filter ? pm.F32 : pm.G32,
@@ -896,9 +901,14 @@ int build_matrix(simox_ctx* ctx) {
                        ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
   ctx->val16_valid = false;
   if (use32(ctx) && ctx->inner_coarse && ctx->inner_f16) {
-    if (!ctx->nb_val16) CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
+    if (!ctx->nb_val16) {
+      CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
+      CHK(dalloc(ctx, &ctx->nb_col16, size_t(ctx->cap) * ctx->n));
+      CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
+    }
     hipLaunchKernelGGL(k_ell_to_f16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
-                       ctx->nb_cnt, ctx->nb_val64, ctx->nb_val16);
+                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16,
+                       ctx->nb_col16, ctx->nb_val16);
     ctx->val16_valid = true;
   }
   HIPCHK(hipGetLastError());
@@ -1233,8 +1243,8 @@ int ncg_iter_body(simox_ctx* ctx) {
   tstart(ctx, 6, &ev);
   if (ig && ctx->val16_valid) {
     hipLaunchKernelGGL(k_h0matvec16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt,
-                       ctx->nb_col, ctx->nb_val16, ctx->apv, ctx->part);
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
+                       ctx->nb_col16, ctx->nb_val16, ctx->apv, ctx->part);
   } else if (f32) {
     hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
@@ -1705,6 +1715,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
+    const double v = std::atof(e);
+    if (v >= 0.0 && v < 1.0) ctx->inner_drop = v;
+  }
   if (const char* e = std::getenv("SIMOX_INNER_RTOL")) {
     const double v = std::atof(e);
     if (v > 0.0 && v < 1.0) ctx->inner_rtol = ctx->inner_rtol_c = v;
@@ -1775,7 +1789,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
@@ -1827,6 +1841,19 @@ int simox_set_inner_grid(simox_ctx* ctx, int on) {
     return SIMOX_ERR_ARG;
   }
   ctx->inner_coarse = on != 0;
+  return SIMOX_OK;
+}
+
+int simox_inner_entries(simox_ctx* ctx, long long* entries) {
+  if (!ctx || !entries) return SIMOX_ERR_ARG;
+  const int* cnt = ctx->val16_valid ? ctx->nb_cnt16 : ctx->nb_cnt;
+  if (!cnt) return SIMOX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  std::vector<int> h(size_t(ctx->n));
+  HIPCHK(hipMemcpy(h.data(), cnt, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  long long t = 0;
+  for (int c : h) t += std::min(c, ctx->cap);
+  *entries = t;
   return SIMOX_OK;
 }
 

@@ -653,15 +653,27 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec(Geom g, const Tables* __rest
   }
 }
 
-// FP16 copy of the CTIP matrix values for the MIXED inner operator when its
-// reciprocal part is on the coarse grid (rows up to their counts).  Round to
-// nearest; |H0_ij| stays far below the FP16 range.
+// Compacted FP16 copy of the CTIP matrix for the MIXED inner operator when
+// its reciprocal part is on the coarse grid: the entries with |H0_ij| >= tau
+// (the far real-space Coulomb entries fall below it; tau = 0 keeps all), in
+// the same column-major layout with their own row counts.  Round to nearest;
+// |H0_ij| stays far below the FP16 range.
 __global__ void k_ell_to_f16(int n, int cap, const int* __restrict__ cnt,
-                             const double* __restrict__ v64, _Float16* __restrict__ v16) {
+                             const int* __restrict__ col, const double* __restrict__ v64,
+                             double tau, int* __restrict__ cnt16, int* __restrict__ col16,
+                             _Float16* __restrict__ v16) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cnt[i] < cap ? cnt[i] : cap;
-  for (int k = 0; k < c; ++k) v16[size_t(k) * n + i] = _Float16(float(v64[size_t(k) * n + i]));
+  int m = 0;
+  for (int k = 0; k < c; ++k) {
+    const double v = v64[size_t(k) * n + i];
+    if (fabs(v) < tau) continue;
+    col16[size_t(m) * n + i] = col[size_t(k) * n + i];
+    v16[size_t(m) * n + i] = _Float16(float(v));
+    ++m;
+  }
+  cnt16[i] = m;
 }
 
 // y = H0 p as k_h0matvec<float, float> with FP16 matrix values (6 bytes per

@@ -273,9 +273,10 @@ def test_z512_register_pass_mixed_solve(monkeypatch):
 
 def test_mixed_coarse_inner_grid_matches_fp64():
     """MIXED with the inner operator's reciprocal part on the half-resolution
-    grid (simox_set_inner_grid): the restarts use the reference's grid in
-    FP64, so the converged charges equal FP64's to 1e-9 and the forces to
-    1e-5 of the RMS force, with the coarse grid on and off."""
+    grid and its sparse part from the compacted FP16 matrix
+    (simox_set_inner_grid): the restarts use the reference's grid and the
+    FP64 matrix, so the converged charges equal FP64's to 1e-9 and the forces
+    to 1e-5 of the RMS force, with the coarse operator on and off."""
     from simox_amd.systems import al_o_slab
     s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (96, 96, 64), 10.0, vacuum=20.0,
                   o_layer=(2.0, 10.0), o_dmin=2.0, name="inner")
@@ -288,6 +289,11 @@ def test_mixed_coarse_inner_grid_matches_fp64():
         g.set_charge_tol(1e-10, 5000)
         g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
         r = g.ctip_solve()
+        if tag == "coarse":
+            # the inner matrix leaves out the far entries (|H0_ij| < 1e-3)
+            full = int(g.neighbors(stride=0)[0].sum())
+            kept = g.inner_entries()
+            assert 0 < kept < 0.8 * full, (kept, full)
         g.force()
         g.pme()
         out[tag] = (g.get_state()[3], g.get_state()[2], r["iters"], g.charge_stats()[1])
