@@ -123,6 +123,10 @@ struct simox_ctx {
   // the true residual of their restart.
   PmeGrid pi;
   bool inner_coarse = false;
+  // line-search rounds per inner iteration (multi-block path): adapted to the
+  // rounds the previous iteration's search ran when ls_adapt
+  bool ls_adapt = true;
+  int ls_rounds = 8;
   bool inner_f16 = true;  // with the coarse grid: FP16 matrix values in the inner matvec
   bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
@@ -1298,7 +1302,10 @@ int ncg_iter_body(simox_ctx* ctx) {
     HIPCHK(hipGetLastError());
     return 0;
   } else {
-    for (int k = 0; k < 8; ++k) {
+    // rounds launched: one more than the last iteration's line search ran
+    // (8 after one that did not finish, at a solve's start, and in graphs)
+    const int rounds = (ctx->ls_adapt && !ctx->use_graph) ? ctx->ls_rounds : 8;
+    for (int k = 0; k < rounds; ++k) {
       hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                          ctx->qc, ctx->pv, ls, ctx->part);
       hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
@@ -1343,7 +1350,9 @@ int ncg_issue_iter(simox_ctx* ctx) {
   CHK(ncg_launch_iter(ctx));
   // the convergence and descent scalars of this iteration, for ncg_check
   HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-  HIPCHK(hipMemcpyAsync(ctx->hscal + 48, ctx->dscal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  // ls[5..9]: done flag, rz, beta, descent flag, Newton rounds run
+  HIPCHK(hipMemcpyAsync(ctx->hscal + 45, ctx->dscal + 45, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->st));
   ctx->ncg.phase = NCG_WAIT_ITER;
   return 0;
 }
@@ -1377,6 +1386,7 @@ int ncg_begin(simox_ctx* ctx) {
   }
   HIPCHK(hipGetLastError());
   ctx->ncg = NcgRun{};
+  ctx->ls_rounds = 8;
   // the cell sort swaps the atom buffers (x, id, cell, ...): a graph captured
   // in an earlier solve may name the other buffer of each pair, so capture anew
   ctx->iter_graph_gen = -1;
@@ -1425,6 +1435,8 @@ int ncg_check(simox_ctx* ctx) {
   }
   if (r.phase == NCG_WAIT_ITER) {
     ++r.it;
+    const bool ls_done = ctx->hscal[45] != 0.0;
+    ctx->ls_rounds = ls_done ? std::min(8, std::max(2, int(ctx->hscal[49]) + 1)) : 8;
     r.rms = std::sqrt(ctx->hscal[18] / n);
     const bool descent = ctx->hscal[48] == 0.0;  // else restart from the true gradient
     if (descent && r.rms > r.target && r.it < ctx->maxit) return ncg_issue_iter(ctx);
@@ -1715,6 +1727,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
     const double v = std::atof(e);
     if (v >= 0.0 && v < 1.0) ctx->inner_drop = v;

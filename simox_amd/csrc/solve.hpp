@@ -993,7 +993,7 @@ namespace sx {
 
 // Device-side scalar logic of one nonlinear-CG iteration (no host round trip).
 // ls layout: [0] g.p  [1] p.H0p  [2] a  [3] lo  [4] hi  [5] done  [6] rz
-//            [7] beta  [8] descent-failure flag
+//            [7] beta  [8] descent-failure flag  [9] Newton rounds run
 // red: reduction outputs; red[0] = p.H0p, red[1] = p.g, red[2] = sum p^2 pen''(q)
 __device__ inline void ls_init(const double* red, double* ls) {
   const double pHp = red[0], gp = red[1], s2 = red[2];
@@ -1005,6 +1005,7 @@ __device__ inline void ls_init(const double* red, double* ls) {
   ls[8] = bad ? 1.0 : 0.0;
   ls[5] = bad ? 1.0 : 0.0;
   ls[2] = bad ? 0.0 : gp / (pHp + s2);
+  ls[9] = 0.0;
 }
 
 __global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ ls) { ls_init(red, ls); }
@@ -1012,6 +1013,7 @@ __global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ l
 // Newton step on phi'(a) = -gp + a pHp + D1(a), bracketed; red = {D1(a), S2(a)}
 __device__ inline void ls_update(const double* red, double* ls) {
   if (ls[5] != 0.0) return;
+  ls[9] += 1.0;
   const double gp = ls[0], pHp = ls[1], a = ls[2];
   const double fprime = -gp + a * pHp + red[0];
   if (fabs(fprime) <= 1e-13 * (gp + fabs(a * pHp))) { ls[5] = 1.0; return; }
@@ -1073,9 +1075,9 @@ __global__ void __launch_bounds__(1024) k_ncg_ls_small(int n, const Tables* __re
                                                        double* __restrict__ ls,
                                                        double* __restrict__ red) {
   __shared__ double lds[2 * 1024 / WAVE];
-  __shared__ double st[9];
+  __shared__ double st[10];  // ls[0..9]
   __shared__ double sums[2];
-  if (threadIdx.x < 9) st[threadIdx.x] = ls[threadIdx.x];
+  if (threadIdx.x < 10) st[threadIdx.x] = ls[threadIdx.x];
   if (threadIdx.x < 2) sums[threadIdx.x] = red[threadIdx.x];
   __syncthreads();
   for (int round = 0; round < 8; ++round) {
@@ -1104,7 +1106,7 @@ __global__ void __launch_bounds__(1024) k_ncg_ls_small(int n, const Tables* __re
     }
     __syncthreads();
   }
-  if (threadIdx.x < 9) ls[threadIdx.x] = st[threadIdx.x];
+  if (threadIdx.x < 10) ls[threadIdx.x] = st[threadIdx.x];
   if (threadIdx.x < 2) red[threadIdx.x] = sums[threadIdx.x];
 }
 
