@@ -986,6 +986,147 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   }
 }
 
+// K6 for the MIXED inner operator's coarse grid: the same tiles, bricks and
+// order-4 B-splines with FP32 weights and 32-bit fixed point.  Each tile
+// owns its points and writes each once, so it takes its own scale: 2^S with
+// 8 max|v| 2^S < 2^31 over the values it visits (a grid point's sum is at
+// most 16 atoms x 8/27 of max|v|), resolution 2^-S, about 4e-9 of max|v|.
+// Integer sums keep the grid independent of the order of the atomics; the
+// LDS tile is half the size of the 64-bit one (16 KB).
+__global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double* __restrict__ x,
+                                                           const double* __restrict__ q,
+                                                           const int* __restrict__ bstart,
+                                                           const int* __restrict__ border,
+                                                           float* __restrict__ grid) {
+  __shared__ int acc[TILE];
+  __shared__ int bofs[MAXB + 1];
+  __shared__ int bbeg[MAXB];
+  __shared__ int wsum[BLOCK / WAVE];
+  __shared__ double wmax[BLOCK / WAVE];
+  const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
+  const int t = blockIdx.x;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int lo[3] = {tx * TX, ty * TY, tz * TZ};
+  const int ext[3] = {min(TX, g.K[0] - lo[0]), min(TY, g.K[1] - lo[1]), min(TZ, g.K[2] - lo[2])};
+  const int nb[3] = {g.K[0] / BRK, g.K[1] / BRK, g.K[2] / BRK};
+  int fb[3], cb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int a = lo[d] - 2, b = lo[d] + ext[d];
+    int f0 = (a >= 0) ? a / BRK : -((-a + BRK - 1) / BRK);
+    int l0 = b / BRK;
+    int c = l0 - f0 + 1;
+    if (c >= nb[d]) { f0 = 0; c = nb[d]; }
+    fb[d] = f0;
+    cb[d] = c;
+  }
+  const int ncand = cb[0] * cb[1] * cb[2];
+  for (int k = threadIdx.x; k < TILE; k += blockDim.x) acc[k] = 0;
+  int cntk = 0;
+  if (threadIdx.x < ncand) {
+    const int k = threadIdx.x;
+    const int ix = k % cb[0], iy = (k / cb[0]) % cb[1], iz = k / (cb[0] * cb[1]);
+    int bx = (fb[0] + ix) % nb[0]; if (bx < 0) bx += nb[0];
+    int by = (fb[1] + iy) % nb[1]; if (by < 0) by += nb[1];
+    int bz = (fb[2] + iz) % nb[2]; if (bz < 0) bz += nb[2];
+    const int bid = bx + nb[0] * (by + nb[1] * bz);
+    const int b0 = bstart[bid];
+    bbeg[k] = b0;
+    cntk = bstart[bid + 1] - b0;
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int v = __shfl_up(cntk, o, WAVE);
+    if (lane >= o) cntk += v;
+  }
+  if (lane == WAVE - 1) wsum[wid] = cntk;
+  if (threadIdx.x == 0) bofs[0] = 0;
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) cntk += wsum[w];
+  if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
+  __syncthreads();
+  const int total = bofs[ncand];
+  auto atom_of = [&](int f) {
+    int l = 0, h = ncand - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (bofs[mid] <= f) l = mid; else h = mid - 1;
+    }
+    return border[bbeg[l] + (f - bofs[l])];
+  };
+  // the tile's scale from the largest |q| it visits
+  double m = 0.0;
+  for (int f = threadIdx.x; f < total; f += blockDim.x) m = fmax(m, fabs(q[atom_of(f)]));
+  m = wave_max(m);
+  if (lane == 0) wmax[wid] = m;
+  __syncthreads();
+  m = wmax[0];
+  for (int w = 1; w < BLOCK / WAVE; ++w) m = fmax(m, wmax[w]);
+  int e = 0;
+  (void)frexp(8.0 * m, &e);              // 8 m < 2^e
+  const int S = m > 0.0 ? 31 - e : 0;    // 8 m 2^S < 2^31
+  const double scale = ldexp(1.0, S);
+  for (int f = threadIdx.x; f < total; f += blockDim.x) {
+    const int ai = atom_of(f);
+    const float qs = float(q[ai] * scale);
+    double u[3];
+    int b[3];
+    grid_coord(g, x + 3 * ai, u, b);
+    float AX[3][4];
+    int o[3][4];
+    bool in[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // order-4 B-spline weights of the fractional offset (bspline4_w in FP32)
+      const float z0 = float(trunc(u[d]) - u[d] + 1.0);
+      const float z1 = z0 + 1.f, z2 = z0 + 2.f, z3 = z0 + 3.f;
+      constexpr float s6 = 1.f / 6.f;
+      AX[d][0] = z0 * z0 * z0 * s6;
+      AX[d][1] = (((-3.f * z1 + 12.f) * z1 - 12.f) * z1 + 4.f) * s6;
+      AX[d][2] = (((3.f * z2 - 24.f) * z2 + 60.f) * z2 - 44.f) * s6;
+      AX[d][3] = (((-z3 + 12.f) * z3 - 48.f) * z3 + 64.f) * s6;
+      const int K = g.K[d];
+      int bm = b[d] % K;
+      if (bm < 0) bm += K;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        int idx = bm + a - 1;
+        idx += idx < 0 ? K : (idx >= K ? -K : 0);
+        int p = idx - lo[d];
+        if (p < 0) p += K;
+        o[d][a] = p;
+        in[d][a] = p < ext[d];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!in[2][c]) continue;
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb) {
+        if (!in[1][bb]) continue;
+        const int row = TX * (o[1][bb] + TY * o[2][c]);
+        const float w12 = qs * AX[1][bb] * AX[2][c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (!in[0][a]) continue;
+          atomicAdd(&acc[row + o[0][a]], __float2int_rn(w12 * AX[0][a]));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const float inv = float(ldexp(1.0, -S));
+  for (int k = threadIdx.x; k < TILE; k += blockDim.x) {
+    const int px = k % TX, py = (k / TX) % TY, pz = k / (TX * TY);
+    if (px < ext[0] && py < ext[1] && pz < ext[2]) {
+      const size_t gi = size_t(lo[0] + px) +
+                        size_t(g.K[0]) * (size_t(lo[1] + py) + size_t(g.K[1]) * size_t(lo[2] + pz));
+      grid[gi] = float(acc[k]) * inv;
+    }
+  }
+}
+
 // K9 (tiled): out_i = scale * sum w Qfin over the order-4 stencil of atom i
 // (the gather of PHI_MESH, ctip.f90:381-393).  One workgroup per tile of
 // GX x GY x GZ grid points: the tile and its stencil halo (one point below,

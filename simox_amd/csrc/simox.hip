@@ -94,6 +94,7 @@ struct PmeGrid {
   int *bunord = nullptr, *border = nullptr, *bsum = nullptr, *btotal = nullptr;
   bool bricks_valid = false;
   bool ready = false;
+  bool spread32 = false;  // FP32 spreads in 32-bit fixed point (the MIXED inner grid)
 };
 
 struct simox_ctx {
@@ -127,7 +128,8 @@ struct simox_ctx {
   // rounds the previous iteration's search ran when ls_adapt
   bool ls_adapt = true;
   int ls_rounds = 8;
-  bool inner_f16 = true;  // with the coarse grid: FP16 matrix values in the inner matvec
+  bool inner_f16 = true;
+  bool inner_spread32 = true;  // the coarse grid's spread in 32-bit fixed point  // with the coarse grid: FP16 matrix values in the inner matvec
   bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
   bool prec_on = false;
@@ -773,6 +775,9 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   else if (pm.order == 2)
     hipLaunchKernelGGL((k_spread_tile<2, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
+  else if (f32 && pm.spread32)
+    hipLaunchKernelGGL(k_spread_tile_i32, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
+                       pm.bstart, pm.border, pm.greal32);
   else if (f32)
     hipLaunchKernelGGL((k_spread_tile<4, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
@@ -1094,6 +1099,7 @@ int inner_grid(simox_ctx* ctx) {
     ctx->inner_coarse = false;
     return rc == SIMOX_ERR_ARG ? 0 : rc;
   }
+  ctx->pi.spread32 = ctx->inner_spread32;
   return 0;
 }
 
@@ -1727,6 +1733,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
     const double v = std::atof(e);
