@@ -203,6 +203,7 @@ struct simox_ctx {
   bool timing = false;
   unsigned timing_mask = ~0u;  // categories timed while timing is on
   std::vector<TimedLaunch> tl;
+  std::vector<hipEvent_t> evfree;  // recorded-and-resolved events for reuse
   double tms[16] = {0};
   long long tcnt[16] = {0};
 };
@@ -231,16 +232,28 @@ int dalloc(simox_ctx* ctx, T** p, size_t count) {
   return 0;
 }
 
+// Events come from a per-context pool (created once, reused after each
+// resolve), so a timed launch costs two event records and no event creation:
+// the host then keeps ahead of the device and the pair measures the kernel,
+// not a wait for its launch.
+hipEvent_t tevent(simox_ctx* ctx) {
+  if (!ctx->evfree.empty()) {
+    hipEvent_t e = ctx->evfree.back();
+    ctx->evfree.pop_back();
+    return e;
+  }
+  hipEvent_t e{};
+  (void)hipEventCreate(&e);
+  return e;
+}
 void tstart(simox_ctx* ctx, int which, hipEvent_t* a) {
   if (!ctx->timing || !((ctx->timing_mask >> which) & 1u)) return;
-  (void)hipEventCreate(a);
+  *a = tevent(ctx);
   (void)hipEventRecord(*a, ctx->st);
-  (void)which;
 }
 void tstop(simox_ctx* ctx, int which, hipEvent_t a) {
   if (!ctx->timing || !((ctx->timing_mask >> which) & 1u)) return;
-  hipEvent_t b;
-  (void)hipEventCreate(&b);
+  hipEvent_t b = tevent(ctx);
   (void)hipEventRecord(b, ctx->st);
   ctx->tl.push_back({which, a, b});
 }
@@ -251,8 +264,8 @@ void tresolve(simox_ctx* ctx) {
     (void)hipEventElapsedTime(&ms, t.a, t.b);
     ctx->tms[t.which] += ms;
     ctx->tcnt[t.which] += 1;
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    ctx->evfree.push_back(t.a);
+    ctx->evfree.push_back(t.b);
   }
   ctx->tl.clear();
 }
@@ -1805,6 +1818,7 @@ int simox_destroy(simox_ctx* ctx) {
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   tresolve(ctx);
   if (ctx->iter_graph) (void)hipGraphExecDestroy(ctx->iter_graph);
+  for (hipEvent_t e : ctx->evfree) (void)hipEventDestroy(e);
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
