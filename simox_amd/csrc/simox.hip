@@ -691,8 +691,9 @@ int time_launch(simox_ctx* ctx, F launch, float* ms) {
 // line is transformed by the same arithmetic); the z tile, which groups the
 // energy partial sums, keeps its default so energies stay reproducible.
 template <class C>
-int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* greal) {
-  const Geom& g = ctx->g;
+int tune_fft(simox_ctx* ctx, PmeGrid& pm, FftCfg* c, C* spec, C* const* tw,
+             decltype(C{}.x)* greal) {
+  const Geom g = geom_of(ctx, pm);
   const int Ky = g.K[1], Kz = g.K[2], H = g.H;
   const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky);
   float best = 1e30f;
@@ -702,9 +703,9 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
     const size_t sh = fft_shx<C>(g, m);
     float ms = 0.f;
     CHK(time_launch(ctx, [&] {
-      hipLaunchKernelGGL(fx.r2c, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[0], tw[0],
+      hipLaunchKernelGGL(fx.r2c, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, pm.pl[0], tw[0],
                          greal, spec, m);
-      hipLaunchKernelGGL(fx.c2r, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[0], tw[0],
+      hipLaunchKernelGGL(fx.c2r, dim3(Kz / m, Ky), dim3(BLOCK), sh, ctx->st, g, pm.pl[0], tw[0],
                          spec, greal, m);
     }, &ms));
     if (ms < best) { best = ms; pick = m; }
@@ -717,7 +718,7 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
     const size_t sh = fft_shy<C>(g, m);
     float ms = 0.f;
     CHK(time_launch(ctx, [&] {
-      hipLaunchKernelGGL(fy.y, dim3(Kz / m, H), dim3(BLOCK), sh, ctx->st, g, ctx->pm.pl[1], tw[1], spec,
+      hipLaunchKernelGGL(fy.y, dim3(Kz / m, H), dim3(BLOCK), sh, ctx->st, g, pm.pl[1], tw[1], spec,
                          m, Ky + 1, -1);
     }, &ms));
     if (ms < best) { best = ms; pick = m; }
@@ -727,11 +728,18 @@ int tune_fft(simox_ctx* ctx, FftCfg* c, C* spec, C* const* tw, decltype(C{}.x)* 
   return 0;
 }
 
-int autotune(simox_ctx* ctx) {
-  CHK(tune_fft<double2>(ctx, &ctx->pm.fc64, ctx->pm.spec64, ctx->pm.tw64, ctx->pm.greal64));
+int autotune_grid(simox_ctx* ctx, PmeGrid& pm) {
+  CHK(tune_fft<double2>(ctx, pm, &pm.fc64, pm.spec64, pm.tw64, pm.greal64));
   if (ctx->mode == SIMOX_MODE_MIXED)
-    CHK(tune_fft<float2>(ctx, &ctx->pm.fc32, ctx->pm.spec32, ctx->pm.tw32, ctx->pm.greal32));
+    CHK(tune_fft<float2>(ctx, pm, &pm.fc32, pm.spec32, pm.tw32, pm.greal32));
   return 0;
+}
+
+int autotune(simox_ctx* ctx) { return autotune_grid(ctx, ctx->pm); }
+
+bool autotune_on() {
+  const char* e = std::getenv("SIMOX_AUTOTUNE");
+  return !e || std::strcmp(e, "0") != 0;
 }
 
 int zpass_blocks(const simox_ctx* ctx, bool f32) {
@@ -860,6 +868,9 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   if (f32 && with_G)
     hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G32, order);
   HIPCHK(hipGetLastError());
+  // the coarse grids (preconditioner, MIXED inner operator) are set up on
+  // first use and tuned then; the reference's grid at context creation
+  if (&pm != &ctx->pm && autotune_on()) CHK(autotune_grid(ctx, pm));
   pm.bricks_valid = false;
   pm.ready = true;
   return 0;
