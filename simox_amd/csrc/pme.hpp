@@ -512,6 +512,86 @@ __device__ inline void reg_fft512(C (&v)[8], C* __restrict__ x, const C* __restr
   Dft<C, 8>::run(v, sign);
 }
 
+// Register-resident transform of one 256-point line by one wave: lane j
+// holds elements j + 64 r (r < 4); four radix-4 Stockham stages (Ns = 1, 4,
+// 16, 64) with three exchanges through the wave's padded LDS line.  A
+// different factorisation from the tiled pass (8 8 4), so FP32 results
+// differ from it by rounding; used on FP32 chains only.
+template <class C>
+__device__ inline void reg_fft256(C (&v)[4], C* __restrict__ x, const C* __restrict__ twl,
+                                  int j, int sign) {
+  Dft<C, 4>::run(v, sign);  // Ns = 1: output 4 j + r
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x[pad9(4 * j + r)] = v[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = x[pad9(j + 64 * r)];
+  __syncthreads();
+#pragma unroll
+  for (int st = 1; st < 4; ++st) {
+    const int Ns = 1 << (2 * st);       // 4, 16, 64
+    const int k = j & (Ns - 1), tws = 64 / Ns;
+#pragma unroll
+    for (int r = 1; r < 4; ++r)
+      if (k > 0) {
+        C w = twl[r * k * tws];
+        if (sign > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    Dft<C, 4>::run(v, sign);
+    if (st == 3) break;  // Ns = 64: output j + 64 r, the lane's own elements
+    const int o = (j / Ns) * Ns * 4 + k;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[pad9(o + r * Ns)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = x[pad9(j + 64 * r)];
+    __syncthreads();
+  }
+}
+
+// K7c/K8 for Kz = 256 on FP32 chains: as k_fft_z512_reg with 256-point lines.
+template <class C>
+__global__ void __launch_bounds__(BLOCK) k_fft_z256_reg(Geom g, const C* __restrict__ tw,
+                                                        C* __restrict__ spec,
+                                                        const decltype(C{}.x)* __restrict__ G,
+                                                        double* __restrict__ epart) {
+  using R = decltype(C{}.x);
+  constexpr int L = 256, NW = BLOCK / WAVE;
+  __shared__ C twl[L];
+  __shared__ C xb[NW][L + L / 8];
+  __shared__ double lds_red[NW];
+  load_twiddles(twl, tw, L);
+  const int j = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int Ky = g.K[1], nlines = g.H * Ky;
+  const int line = blockIdx.x * NW + w;
+  const bool live = line < nlines;
+  const size_t base = size_t(live ? line : 0) * L;
+  C v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    v[r].x = R(0); v[r].y = R(0);
+    if (live) v[r] = spec[base + j + 64 * r];
+  }
+  __syncthreads();  // twiddles in LDS
+  reg_fft256(v, xb[w], twl, j, -1);
+  double en[1] = {0.0};
+  const int kx = live ? line / Ky : 0;
+  const double wx = (kx == 0 || 2 * kx == g.K[0]) ? 1.0 : 2.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const R gv = live ? G[base + j + 64 * r] : R(0);
+    if (live) en[0] += wx * double(gv) * (double(v[r].x) * double(v[r].x) + double(v[r].y) * double(v[r].y));
+    v[r].x *= gv; v[r].y *= gv;
+  }
+  reg_fft256(v, xb[w], twl, j, +1);
+  block_sum<1>(en, lds_red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = en[0];
+  if (live)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) spec[base + j + 64 * r] = v[r];
+}
+
 // K7c/K8 for Kz = 512: k_fft_z_fused with one line per wave held in
 // registers (two LDS exchanges per transform instead of the three stage
 // passes through LDS of the tiled kernel).  Four lines per workgroup; the

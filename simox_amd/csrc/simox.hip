@@ -66,7 +66,7 @@ struct NcgRun {
 // workgroup, and their dynamic LDS bytes.
 struct FftCfg {
   int mx = 0, my = 0, mz = 0, zblocks = 0;
-  bool zreg = false;  // Kz = 512: the register-resident z pass (4 lines per workgroup)
+  int zreg = 0;  // 512 or 256: the register-resident z pass for that Kz (4 lines per workgroup)
   size_t shx = 0, shy = 0, shz = 0;
 };
 
@@ -591,8 +591,11 @@ int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& 
                      spec, fc.my, Ky + 1, -1);
   if (timed) tstop(ctx, tcat(ctx, pm, 2), ev);
   if (timed) tstart(ctx, tcat(ctx, pm, 3), &ev);
-  if (fc.zreg)
+  if (fc.zreg == 512)
     hipLaunchKernelGGL(k_fft_z512_reg<C>, dim3(fc.zblocks), dim3(BLOCK), 0, ctx->st, g, tw[2],
+                       spec, G, ctx->part);
+  else if (fc.zreg == 256)
+    hipLaunchKernelGGL(k_fft_z256_reg<C>, dim3(fc.zblocks), dim3(BLOCK), 0, ctx->st, g, tw[2],
                        spec, G, ctx->part);
   else
     hipLaunchKernelGGL(fz.fus, dim3(fc.zblocks), dim3(BLOCK), fc.shz, ctx->st, g, pm.pl[2], tw[2],
@@ -653,7 +656,13 @@ int choose_fft(simox_ctx* ctx, const Geom& g, FftCfg* c) {
   // lines per workgroup (its energy partials group four lines, like mz = 4)
   const char* zr = std::getenv("SIMOX_FFT_ZREG");
   if (Kz == 512 && ctx->mode != SIMOX_MODE_FAITHFUL && !(zr && std::atoi(zr) == 0)) {
-    c->zreg = true;
+    c->zreg = 512;
+    c->mz = 4;
+  }
+  // Kz = 256 on the FP32 chain: a radix-4 register pass (its rounding differs
+  // from the tiled 8 8 4 plan, so the FP64 chains keep the tiled one)
+  if (Kz == 256 && sizeof(C) == sizeof(float2) && !(zr && std::atoi(zr) == 0)) {
+    c->zreg = 256;
     c->mz = 4;
   }
   if (!c->mx || !c->my || !c->mz) return SIMOX_ERR_ARG;

@@ -308,3 +308,30 @@ def test_mixed_coarse_inner_grid_matches_fp64():
             g.set_inner_grid(True)
         finally:
             g.close()
+
+
+def test_z256_register_pass_fp32(monkeypatch):
+    """Kz = 256 on the FP32 chain takes the radix-4 register z pass
+    (k_fft_z256_reg): the MIXED reciprocal energy with it and with the tiled
+    pass (SIMOX_FFT_ZREG=0) agree to 1e-6, and the forces of both stay within
+    1e-5 of FP64 (the bar of MIXED; the two FP32 passes round differently)."""
+    s = tiny(seed=4, nx=3, ny=3, nz=3, n_o=8, fft=(12, 12, 256), rcut=5.0)
+    q = np.random.default_rng(6).normal(size=s.n) * 0.3
+    print("z256 register pass, n", s.n, flush=True)
+
+    def run(mode, zreg):
+        monkeypatch.setenv("SIMOX_FFT_ZREG", "1" if zreg else "0")
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_state(s.ids, s.x, s.v, None, q)
+        g.force()
+        erec = g.pme()
+        f = g.get_state()[2]
+        g.close()
+        return erec, f
+
+    e64, f64 = run(MODE_FP64, True)
+    e_reg, f_reg = run(MODE_MIXED, True)
+    e_lds, f_lds = run(MODE_MIXED, False)
+    assert e_reg == pytest.approx(e_lds, rel=1e-6)
+    assert e_F(f_reg, f64) <= 1e-5 and e_F(f_lds, f64) <= 1e-5
+    assert e_reg == pytest.approx(e64, rel=1e-5)
