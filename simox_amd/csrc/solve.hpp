@@ -24,7 +24,10 @@ __device__ inline void beta_update(const double* red, double* ls);
 template <class Tv, class Tp>
 __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
                              const Tv* __restrict__ val, const Tp* __restrict__ vec) {
-  constexpr int U = 16;
+  // batches of U entries with their loads in flight together, then the
+  // rest in masked batches of UT (value 0 at column i past the row's end):
+  // exact zeros added in the order an entry-by-entry tail would sum
+  constexpr int U = 16, UT = 4;
   Tv s = Tv(0);
   int k = 0;
   for (; k + U <= c; k += U) {
@@ -40,7 +43,19 @@ __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col
 #pragma unroll
     for (int u = 0; u < U; ++u) s += vv[u] * pp[u];
   }
-  for (; k < c; ++k) s += val[size_t(k) * n + i] * Tv(vec[col[size_t(k) * n + i]]);
+  for (; k < c; k += UT) {
+    int cc[UT];
+    Tv vv[UT];
+#pragma unroll
+    for (int u = 0; u < UT; ++u) {
+      const bool ok = k + u < c;
+      const size_t e = size_t(ok ? k + u : k) * n + i;  // in bounds either way
+      cc[u] = ok ? col[e] : i;
+      vv[u] = ok ? val[e] : Tv(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UT; ++u) s += vv[u] * Tv(vec[cc[u]]);
+  }
   return s;
 }
 
@@ -710,7 +725,20 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
 #pragma unroll
       for (int u = 0; u < U; ++u) s += vv[u] * pt[cc[u]];
     }
-    for (; k < c; ++k) s += float(val[size_t(k) * n + i]) * pt[col[size_t(k) * n + i]];
+    constexpr int UT = 4;
+    for (; k < c; k += UT) {  // masked batches of the rest, as ell_dot
+      int cc[UT];
+      float vv[UT];
+#pragma unroll
+      for (int u = 0; u < UT; ++u) {
+        const bool ok = k + u < c;
+        const size_t e = size_t(ok ? k + u : k) * n + i;
+        cc[u] = ok ? col[e] : i;
+        vv[u] = ok ? float(val[e]) : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < UT; ++u) s += vv[u] * pt[cc[u]];
+    }
     const double pi = p[i];
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double yi = sc.Jp * pi + phi_in[i] + double(s);
