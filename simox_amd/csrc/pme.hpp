@@ -177,7 +177,12 @@ __host__ __device__ constexpr int ct_ns_at(int L, int s) {
   return ns;
 }
 
-template <class C, int L, int R, int Ns>
+// Padded line layout between two stages: element i at i + i/8, so the
+// strided writes of the first stages (stride R elements at Ns = 1) fall in
+// different LDS banks.  PI / PO: the stage reads / writes padded lines.
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 3); }
+
+template <class C, int L, int R, int Ns, bool PI = false, bool PO = false>
 __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ dst, int M,
                                     int stride, const C* __restrict__ tw, int sign) {
   constexpr int nb = L / R;
@@ -192,7 +197,7 @@ __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ d
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      C a = s[j + r * nb];
+      C a = s[PI ? lpad(j + r * nb) : j + r * nb];
       if (r > 0 && k > 0) {
         C w = tw[r * k * tws];
         if (sign > 0) w.y = -w.y;
@@ -201,11 +206,32 @@ __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ d
       v[r] = a;
     }
     Dft<C, R>::run(v, sign);
-    C* d = dst + line * stride + (j / Ns) * Ns * R + k;
+    C* d = dst + line * stride;
+    const int o = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
+    for (int r = 0; r < R; ++r) d[PO ? lpad(o + r * Ns) : o + r * Ns] = v[r];
   }
 }
+
+// All stages on padded intermediate lines: the first stage reads and the last
+// writes the caller's unpadded lines.  Line stride >= L + L/8.
+template <class C, int L, int S>
+__device__ inline void fft_stages_pad(C* a, C* b, int M, int stride, const C* __restrict__ tw,
+                                      int sign) {
+  if constexpr (S < ct_nstages(L)) {
+    fft_stage_ct<C, L, ct_radix_at(L, S), ct_ns_at(L, S), (S > 0), (S + 1 < ct_nstages(L))>(
+        a, b, M, stride, tw, sign);
+    __syncthreads();
+    fft_stages_pad<C, L, S + 1>(b, a, M, stride, tw, sign);
+  }
+}
+template <class C, int L>
+__device__ inline C* lds_fft_pad(C* b0, C* b1, int M, int stride, const C* __restrict__ tw,
+                                 int sign) {
+  fft_stages_pad<C, L, 0>(b0, b1, M, stride, tw, sign);
+  return (ct_nstages(L) % 2 == 0) ? b0 : b1;
+}
+__host__ __device__ constexpr int pad_stride(int L) { return L + L / 8 + 1; }
 
 template <class C, int L, int S>
 __device__ inline void fft_stages_ct(C* a, C* b, int M, int stride, const C* __restrict__ tw,
@@ -688,6 +714,213 @@ __global__ void __launch_bounds__(BLOCK) k_fft_x_c2r(Geom g, FftPlan pl, const C
     out[row + xk] = v.x;
     out[row + size_t(Ky) * L + xk] = v.y;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Plane-resident FFT chain of the FP32 coarse grids (Kx = Ky = L <= 128): one
+// workgroup holds a whole z plane in LDS, so x and y are transformed in one
+// pass in each direction, and the z pass reads the plane-major half spectrum
+//   spec2[(kz * Ky + ky) * H + kx]
+// in tiles of NL consecutive (ky, kx) lines.  Three launches per convolution
+// instead of five, each reading and writing the grid once.  The same
+// transforms as the five-pass chain; the real x lines are paired along y
+// (rows 2p, 2p + 1) instead of along z, so FP32 results differ from it by
+// rounding.
+// ---------------------------------------------------------------------------
+// LDS: two buffers of H S lines (S = pad_stride(L); the x lines need P S <=
+// H S), then the L twiddles (the x and y tables are equal for Kx = Ky).
+__host__ __device__ constexpr int plane_buf(int L) { return (L / 2 + 1) * pad_stride(L); }
+
+struct F2x2 { float4 a, b; };
+
+// Forward: real plane z -> plane of the half spectrum (x R2C, then y FFT).
+template <int L>
+__global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__ tw,
+                                                      const float* __restrict__ qgrid,
+                                                      float2* __restrict__ spec2) {
+  using C = float2;
+  constexpr int P = L / 2, H = L / 2 + 1, S = pad_stride(L), L4 = L / 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* b0 = reinterpret_cast<C*>(smem);
+  C* b1 = b0 + plane_buf(L);
+  C* twl = b1 + plane_buf(L);
+  load_twiddles(twl, tw, L);
+  const size_t plane = size_t(blockIdx.x) * L * L;
+  const float4* q4 = reinterpret_cast<const float4*>(qgrid + plane);
+  // rows 2p (real part) and 2p + 1 (imaginary part) form complex line p
+  staged<8, F2x2>(
+      P * L4,
+      [&](int e) {
+        const int p = e / L4, x4 = e - p * L4;
+        F2x2 v;
+        v.a = q4[(2 * p) * L4 + x4];
+        v.b = q4[(2 * p + 1) * L4 + x4];
+        return v;
+      },
+      [&](int e, F2x2 v) {
+        const int p = e / L4, x = 4 * (e - p * L4);
+        C* d = b0 + p * S + x;
+        d[0] = make_float2(v.a.x, v.b.x);
+        d[1] = make_float2(v.a.y, v.b.y);
+        d[2] = make_float2(v.a.z, v.b.z);
+        d[3] = make_float2(v.a.w, v.b.w);
+      });
+  __syncthreads();
+  C* res = lds_fft_pad<C, L>(b0, b1, P, S, twl, -1);
+  C* col = (res == b0) ? b1 : b0;
+  // split the pair spectra: column kx of row y at col[kx S + y]
+  //   A[k] = (Z[k] + conj(Z[-k])) / 2,  B[k] = (Z[k] - conj(Z[-k])) / (2i)
+  for (int e = threadIdx.x; e < H * L; e += BLOCK) {
+    const int kx = e / L, y = e - kx * L;
+    const int p = y >> 1;
+    const C z = res[p * S + kx];
+    const C w = res[p * S + (kx == 0 ? 0 : L - kx)];
+    C v;
+    if ((y & 1) == 0) {
+      v.x = 0.5f * (z.x + w.x);
+      v.y = 0.5f * (z.y - w.y);
+    } else {
+      v.x = 0.5f * (z.y + w.y);
+      v.y = 0.5f * (w.x - z.x);
+    }
+    col[kx * S + y] = v;
+  }
+  __syncthreads();
+  const C* out = lds_fft_pad<C, L>(col, res, H, S, twl, -1);
+  float2* dst = spec2 + size_t(blockIdx.x) * L * H;
+  for (int e = threadIdx.x; e < L * H; e += BLOCK) {
+    const int ky = e / H, kx = e - ky * H;
+    dst[e] = out[kx * S + ky];
+  }
+}
+
+// Inverse: plane of the half spectrum -> real plane (y inverse FFT, then x
+// C2R), unnormalised.
+template <int L>
+__global__ void __launch_bounds__(BLOCK) k_fft_xy_c2r(const float2* __restrict__ tw,
+                                                      const float2* __restrict__ spec2,
+                                                      float* __restrict__ out) {
+  using C = float2;
+  constexpr int P = L / 2, H = L / 2 + 1, S = pad_stride(L), L4 = L / 4;
+  constexpr int NP = L * H;  // complex values per plane (even: L is a multiple of 4)
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* b0 = reinterpret_cast<C*>(smem);
+  C* b1 = b0 + plane_buf(L);
+  C* twl = b1 + plane_buf(L);
+  load_twiddles(twl, tw, L);
+  const float4* s4 = reinterpret_cast<const float4*>(spec2 + size_t(blockIdx.x) * NP);
+  staged<8, float4>(
+      NP / 2, [&](int e) { return s4[e]; },
+      [&](int e, float4 v) {
+        int ky = (2 * e) / H, kx = 2 * e - ky * H;
+        b0[kx * S + ky] = make_float2(v.x, v.y);
+        if (++kx == H) { kx = 0; ++ky; }
+        b0[kx * S + ky] = make_float2(v.z, v.w);
+      });
+  __syncthreads();
+  C* res = lds_fft_pad<C, L>(b0, b1, H, S, twl, +1);
+  C* lin = (res == b0) ? b1 : b0;
+  // Z = A + i B over the full Hermitian x line, A = row 2p, B = row 2p + 1
+  for (int e = threadIdx.x; e < P * L; e += BLOCK) {
+    const int p = e / L, k = e - p * L;
+    C a, b;
+    if (k < H) {
+      a = res[k * S + 2 * p];
+      b = res[k * S + 2 * p + 1];
+    } else {
+      a = res[(L - k) * S + 2 * p]; a.y = -a.y;
+      b = res[(L - k) * S + 2 * p + 1]; b.y = -b.y;
+    }
+    C z;
+    z.x = a.x - b.y;
+    z.y = a.y + b.x;
+    lin[p * S + k] = z;
+  }
+  __syncthreads();
+  const C* r = lds_fft_pad<C, L>(lin, res, P, S, twl, +1);
+  float4* o4 = reinterpret_cast<float4*>(out + size_t(blockIdx.x) * L * L);
+  for (int e = threadIdx.x; e < P * L4; e += BLOCK) {
+    const int p = e / L4, x = 4 * (e - p * L4);
+    const C* s = r + p * S + x;
+    o4[(2 * p) * L4 + x / 4] = make_float4(s[0].x, s[1].x, s[2].x, s[3].x);
+    o4[(2 * p + 1) * L4 + x / 4] = make_float4(s[0].y, s[1].y, s[2].y, s[3].y);
+  }
+}
+
+// z pass on the plane-major spectrum: workgroup b takes lines
+// NL b .. NL b + NL - 1 of the Ky H (ky, kx) lines, forward FFT along z,
+// multiply by the influence or filter table (same layout), sum
+// w_kx G |X|^2 into epart[b], inverse FFT.  LDS: two buffers of NL lines of
+// stride pad_stride(KZ), then the KZ twiddles.
+template <int KZ, int NL>
+__global__ void __launch_bounds__(BLOCK) k_fft_zp(Geom g, const float2* __restrict__ tw,
+                                                  float2* __restrict__ spec2,
+                                                  const float* __restrict__ G,
+                                                  double* __restrict__ epart) {
+  using C = float2;
+  constexpr int S = pad_stride(KZ);
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double lds_red[BLOCK / WAVE];
+  C* b0 = reinterpret_cast<C*>(smem);
+  C* b1 = b0 + NL * S;
+  C* twl = b1 + NL * S;
+  load_twiddles(twl, tw, KZ);
+  const int H = g.H, nlines = g.K[1] * H;
+  const int line0 = blockIdx.x * NL;
+  staged<8, C>(
+      NL * KZ,
+      [&](int e) {
+        const int kz = e / NL, t = e - kz * NL;
+        C v = make_float2(0.f, 0.f);
+        if (line0 + t < nlines) v = spec2[size_t(kz) * nlines + line0 + t];
+        return v;
+      },
+      [&](int e, C v) {
+        const int kz = e / NL, t = e - kz * NL;
+        b0[t * S + kz] = v;
+      });
+  __syncthreads();
+  C* res = lds_fft_pad<C, KZ>(b0, b1, NL, S, twl, -1);
+  double en[1] = {0.0};
+  staged<8, float>(
+      NL * KZ,
+      [&](int e) {
+        const int kz = e / NL, t = e - kz * NL;
+        return line0 + t < nlines ? G[size_t(kz) * nlines + line0 + t] : 0.f;
+      },
+      [&](int e, float gv) {
+        const int kz = e / NL, t = e - kz * NL;
+        C v = res[t * S + kz];
+        if (line0 + t < nlines) {
+          const int kx = (line0 + t) % H;
+          const double wx = (kx == 0 || 2 * kx == g.K[0]) ? 1.0 : 2.0;
+          en[0] += wx * double(gv) * (double(v.x) * double(v.x) + double(v.y) * double(v.y));
+        }
+        v.x *= gv; v.y *= gv;
+        res[t * S + kz] = v;
+      });
+  __syncthreads();
+  C* other = (res == b0) ? b1 : b0;
+  res = lds_fft_pad<C, KZ>(res, other, NL, S, twl, +1);
+  block_sum<1>(en, lds_red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = en[0];
+  for (int e = threadIdx.x; e < NL * KZ; e += BLOCK) {
+    const int kz = e / NL, t = e - kz * NL;
+    if (line0 + t < nlines) spec2[size_t(kz) * nlines + line0 + t] = res[t * S + kz];
+  }
+}
+
+// Table in the half-spectrum layout [kx][ky][kz] -> plane-major [kz][ky][kx].
+template <class R>
+__global__ void k_to_plane_major(Geom g, const R* __restrict__ src, R* __restrict__ dst) {
+  const int H = g.H, Ky = g.K[1], Kz = g.K[2];
+  const size_t total = size_t(H) * Ky * Kz;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int kx = int(t % H);
+  const int ky = int((t / H) % Ky);
+  const int kz = int(t / (size_t(H) * Ky));
+  dst[t] = src[(size_t(kx) * Ky + ky) * Kz + kz];
 }
 
 // K14: influence table on the half spectrum, G(k)/Ng with

@@ -68,6 +68,10 @@ struct FftCfg {
   int mx = 0, my = 0, mz = 0, zblocks = 0;
   int zreg = 0;  // 512 or 256: the register-resident z pass for that Kz (4 lines per workgroup)
   size_t shx = 0, shy = 0, shz = 0;
+  // plane-resident chain (FP32 coarse grids with Kx = Ky <= 128): x and y in
+  // one pass per direction, z pass over nl lines per workgroup
+  int plane = 0, nl = 8, zpblocks = 0;
+  size_t shp = 0, shzp = 0;
 };
 
 // One PME grid: sizes, FFT plans and launch shapes, twiddles, the real grid,
@@ -574,9 +578,71 @@ Geom geom_of(const simox_ctx* ctx, const PmeGrid& pm) {
 bool timed_grid(const simox_ctx* ctx, const PmeGrid& pm) { return &pm == &ctx->pm || &pm == &ctx->pi; }
 int tcat(const simox_ctx* ctx, const PmeGrid& pm, int k) { return &pm == &ctx->pi ? 11 + k : k; }
 
+// Kernels of the plane-resident chain for plane size L (Kx = Ky) and Kz.
+using XyFwd = decltype(&k_fft_xy_r2c<128>);
+using XyInv = decltype(&k_fft_xy_c2r<128>);
+using ZpFn = decltype(&k_fft_zp<256, 16>);
+template <int L>
+void xy_of(XyFwd* f, XyInv* b) { *f = &k_fft_xy_r2c<L>; *b = &k_fft_xy_c2r<L>; }
+bool xy_kernels(int L, XyFwd* f, XyInv* b) {
+  switch (L) {
+    case 32: xy_of<32>(f, b); return true;
+    case 48: xy_of<48>(f, b); return true;
+    case 64: xy_of<64>(f, b); return true;
+    case 96: xy_of<96>(f, b); return true;
+    case 128: xy_of<128>(f, b); return true;
+    default: return false;
+  }
+}
+template <int NL>
+ZpFn zp_of(int KZ) {
+  switch (KZ) {
+    case 32: return &k_fft_zp<32, NL>;
+    case 48: return &k_fft_zp<48, NL>;
+    case 64: return &k_fft_zp<64, NL>;
+    case 96: return &k_fft_zp<96, NL>;
+    case 128: return &k_fft_zp<128, NL>;
+    case 192: return &k_fft_zp<192, NL>;
+    case 256: return &k_fft_zp<256, NL>;
+    default: return nullptr;
+  }
+}
+ZpFn zp_kernel(int KZ, int nl) {
+  return nl == 4 ? zp_of<4>(KZ) : nl == 8 ? zp_of<8>(KZ) : nl == 16 ? zp_of<16>(KZ) : nullptr;
+}
+
+int plane_chain(simox_ctx* ctx, PmeGrid& pm, float2* spec, float2* const* tw, const FftCfg& fc,
+                const float* G, float* greal) {
+  const Geom g = geom_of(ctx, pm);
+  XyFwd fwd{};
+  XyInv inv{};
+  const ZpFn zp = zp_kernel(g.K[2], fc.nl);
+  if (!xy_kernels(g.K[0], &fwd, &inv) || !zp || g.K[1] != g.K[0]) {
+    ctx->err = "plane FFT chain: no kernels for these grid sizes";
+    return SIMOX_ERR_ARG;
+  }
+  const bool timed = timed_grid(ctx, pm);
+  hipEvent_t ev{};
+  if (timed) tstart(ctx, tcat(ctx, pm, 1), &ev);
+  hipLaunchKernelGGL(fwd, dim3(g.K[2]), dim3(BLOCK), fc.shp, ctx->st, tw[0], greal, spec);
+  if (timed) tstop(ctx, tcat(ctx, pm, 1), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 3), &ev);
+  hipLaunchKernelGGL(zp, dim3(fc.zpblocks), dim3(BLOCK), fc.shzp, ctx->st, g, tw[2], spec, G,
+                     ctx->part);
+  if (timed) tstop(ctx, tcat(ctx, pm, 3), ev);
+  if (timed) tstart(ctx, tcat(ctx, pm, 4), &ev);
+  hipLaunchKernelGGL(inv, dim3(g.K[2]), dim3(BLOCK), fc.shp, ctx->st, tw[0], spec, greal);
+  if (timed) tstop(ctx, tcat(ctx, pm, 4), ev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 template <class C>
 int pme_chain(simox_ctx* ctx, PmeGrid& pm, C* spec, C* const* tw, const FftCfg& fc,
               const decltype(C{}.x)* G, decltype(C{}.x)* greal) {
+  if constexpr (sizeof(C) == sizeof(float2)) {
+    if (fc.plane) return plane_chain(ctx, pm, spec, tw, fc, G, greal);
+  }
   const Geom g = geom_of(ctx, pm);
   const int H = g.H, Ky = g.K[1], Kz = g.K[2];
   const FftSet<C> fx = fft_set<C>(g.K[0]), fy = fft_set<C>(Ky), fz = fft_set<C>(Kz);
@@ -739,7 +805,7 @@ int tune_fft(simox_ctx* ctx, PmeGrid& pm, FftCfg* c, C* spec, C* const* tw,
 
 int autotune_grid(simox_ctx* ctx, PmeGrid& pm) {
   CHK(tune_fft<double2>(ctx, pm, &pm.fc64, pm.spec64, pm.tw64, pm.greal64));
-  if (ctx->mode == SIMOX_MODE_MIXED)
+  if (ctx->mode == SIMOX_MODE_MIXED && !pm.fc32.plane)
     CHK(tune_fft<float2>(ctx, pm, &pm.fc32, pm.spec32, pm.tw32, pm.greal32));
   return 0;
 }
@@ -752,6 +818,7 @@ bool autotune_on() {
 }
 
 int zpass_blocks(const simox_ctx* ctx, bool f32) {
+  if (f32 && ctx->pm.fc32.plane) return ctx->pm.fc32.zpblocks;
   return f32 ? ctx->pm.fc32.zblocks : ctx->pm.fc64.zblocks;
 }
 
@@ -826,6 +893,56 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
                             pm.greal64);
 }
 
+// The plane-resident FP32 chain on a coarse grid when Kx = Ky and both
+// kernels exist for its sizes (SIMOX_FFT_PLANE=0 keeps the five-pass chain;
+// SIMOX_FFT_PLANE=2 also takes it on the reference's grid, which tests use to
+// compare the two chains on the same energies and forces; SIMOX_FFT_NL = 4, 8
+// or 16 sets the z-pass lines per workgroup, default 8).
+bool plane_everywhere() {
+  const char* e = std::getenv("SIMOX_FFT_PLANE");
+  return e && std::atoi(e) == 2;
+}
+int plane_setup(simox_ctx* ctx, PmeGrid& pm) {
+  FftCfg& c = pm.fc32;
+  c.plane = 0;
+  const char* e = std::getenv("SIMOX_FFT_PLANE");
+  if (e && std::atoi(e) == 0) return 0;
+  const int L = pm.K[0], KZ = pm.K[2];
+  XyFwd fwd{};
+  XyInv inv{};
+  if (pm.K[1] != L || !xy_kernels(L, &fwd, &inv) || !zp_kernel(KZ, 8)) return 0;
+  c.nl = 8;
+  if (const char* s = std::getenv("SIMOX_FFT_NL")) {
+    const int m = std::atoi(s);
+    if (m == 4 || m == 8 || m == 16) c.nl = m;
+  }
+  c.shp = (2 * size_t(plane_buf(L)) + L) * sizeof(float2);
+  c.shzp = (2 * size_t(c.nl) * pad_stride(KZ) + KZ) * sizeof(float2);
+  if (c.shp > FFT_LDS_MAX || c.shzp > FFT_LDS_MAX) return 0;
+  c.zpblocks = (pm.K[1] * pm.H + c.nl - 1) / c.nl;
+  const void* fns[4] = {reinterpret_cast<const void*>(fwd), reinterpret_cast<const void*>(inv),
+                        reinterpret_cast<const void*>(zp_kernel(KZ, 8)),
+                        reinterpret_cast<const void*>(zp_kernel(KZ, 16))};
+  for (const void* f : fns)
+    HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(FFT_LDS_MAX)));
+  CHK(ensure_part(ctx, size_t(c.zpblocks)));
+  c.plane = 1;
+  return 0;
+}
+
+// An FP32 table of grid pm from the half-spectrum layout to the plane-major
+// layout of the plane chain (through the grid's FP32 spectrum buffer).
+int to_plane_major(simox_ctx* ctx, PmeGrid& pm, float* tab) {
+  const Geom g = geom_of(ctx, pm);
+  const size_t nh = size_t(g.H) * g.K[1] * g.K[2];
+  float* tmp = reinterpret_cast<float*>(pm.spec32);
+  hipLaunchKernelGGL(k_to_plane_major<float>, dim3(unsigned((nh + BLOCK - 1) / BLOCK)), dim3(BLOCK),
+                     0, ctx->st, g, tab, tmp);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tab, tmp, nh * sizeof(float), hipMemcpyDeviceToDevice, ctx->st));
+  return 0;
+}
+
 // Buffers, twiddles, plans and launch shapes of PME grid pm with sizes K (each
 // a product of 2, 3, 5 and a multiple of BRK), and its influence table when
 // with_G (the reference's grid; the preconditioner's grid has filter tables
@@ -870,6 +987,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   }
   CHK(choose_fft<double2>(ctx, g, &pm.fc64));
   if (f32) CHK(choose_fft<float2>(ctx, g, &pm.fc32));
+  if (f32 && (&pm != &ctx->pm || plane_everywhere())) CHK(plane_setup(ctx, pm));
   // G64 is kept for every grid: the preconditioner's filter is built from it
   const double a2 = g.alpha * g.alpha;
   const int nbk = int((nh + BLOCK - 1) / BLOCK);
@@ -877,6 +995,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   if (f32 && with_G)
     hipLaunchKernelGGL(k_influence<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, a2, pm.G32, order);
   HIPCHK(hipGetLastError());
+  if (f32 && with_G && pm.fc32.plane) CHK(to_plane_major(ctx, pm, pm.G32));
   // the coarse grids (preconditioner, MIXED inner operator) are set up on
   // first use and tuned then; the reference's grid at context creation
   if (&pm != &ctx->pm && autotune_on()) CHK(autotune_grid(ctx, pm));
@@ -1176,6 +1295,7 @@ int build_prec(simox_ctx* ctx) {
     hipLaunchKernelGGL(k_filter_table<float>, dim3(nbk), dim3(BLOCK), 0, ctx->st, g, pc.G64, 0.0,
                        c_grid, rho, 1, pc.F32);
   HIPCHK(hipGetLastError());
+  if (use32(ctx) && pc.fc32.plane) CHK(to_plane_major(ctx, pc, pc.F32));
   ctx->prec_rho = rho;
   return 0;
 }

@@ -335,3 +335,38 @@ def test_z256_register_pass_fp32(monkeypatch):
     assert e_reg == pytest.approx(e_lds, rel=1e-6)
     assert e_F(f_reg, f64) <= 1e-5 and e_F(f_lds, f64) <= 1e-5
     assert e_reg == pytest.approx(e64, rel=1e-5)
+
+
+@pytest.mark.parametrize("fft,nl", [((32, 32, 64), "16"), ((48, 48, 96), "8"),
+                                    ((128, 128, 32), "16"), ((64, 64, 256), "16"),
+                                    ((96, 96, 128), "4")])
+def test_plane_fft_chain_fp32(monkeypatch, fft, nl):
+    """The plane-resident FP32 chain (x and y transformed in one LDS pass per
+    direction, z pass on the plane-major spectrum) that the coarse grids of
+    the MIXED solve take, switched onto the reference's grid with
+    SIMOX_FFT_PLANE=2: the reciprocal energy against the five-pass FP32 chain
+    (1e-6 relative; the two pair the real x lines differently, so they round
+    differently) and the forces of both against FP64 (1e-5, the bar of
+    MIXED)."""
+    s = tiny(seed=4, nx=3, ny=3, nz=3, n_o=8, fft=fft, rcut=5.0)
+    q = np.random.default_rng(6).normal(size=s.n) * 0.3
+    print("plane chain", fft, "nl", nl, "n", s.n, flush=True)
+    monkeypatch.setenv("SIMOX_FFT_NL", nl)
+
+    def run(mode, plane):
+        monkeypatch.setenv("SIMOX_FFT_PLANE", plane)
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_state(s.ids, s.x, s.v, None, q)
+        g.force()
+        erec = g.pme()
+        f = g.get_state()[2]
+        g.close()
+        return erec, f
+
+    e64, f64 = run(MODE_FP64, "0")
+    e_pl, f_pl = run(MODE_MIXED, "2")
+    e_5, f_5 = run(MODE_MIXED, "0")
+    assert not np.array_equal(f_pl, f_5)  # the plane chain ran
+    assert e_pl == pytest.approx(e_5, rel=1e-6)
+    assert e_F(f_pl, f64) <= 1e-5 and e_F(f_5, f64) <= 1e-5
+    assert e_pl == pytest.approx(e64, rel=1e-5)
