@@ -182,14 +182,14 @@ __host__ __device__ constexpr int ct_ns_at(int L, int s) {
 // different LDS banks.  PI / PO: the stage reads / writes padded lines.
 __host__ __device__ constexpr int lpad(int i) { return i + (i >> 3); }
 
-template <class C, int L, int R, int Ns, bool PI = false, bool PO = false>
+template <class C, int L, int R, int Ns, bool PI = false, bool PO = false, int NT = BLOCK>
 __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ dst, int M,
                                     int stride, const C* __restrict__ tw, int sign) {
   constexpr int nb = L / R;
   constexpr int tws = L / (Ns * R);
   const int total = M * nb;
 #pragma unroll 4
-  for (int b = threadIdx.x; b < total; b += BLOCK) {
+  for (int b = threadIdx.x; b < total; b += NT) {
     const int line = b / nb;
     const int j = b - line * nb;
     const int k = j % Ns;
@@ -215,20 +215,20 @@ __device__ inline void fft_stage_ct(const C* __restrict__ src, C* __restrict__ d
 
 // All stages on padded intermediate lines: the first stage reads and the last
 // writes the caller's unpadded lines.  Line stride >= L + L/8.
-template <class C, int L, int S>
+template <class C, int L, int S, int NT>
 __device__ inline void fft_stages_pad(C* a, C* b, int M, int stride, const C* __restrict__ tw,
                                       int sign) {
   if constexpr (S < ct_nstages(L)) {
-    fft_stage_ct<C, L, ct_radix_at(L, S), ct_ns_at(L, S), (S > 0), (S + 1 < ct_nstages(L))>(
+    fft_stage_ct<C, L, ct_radix_at(L, S), ct_ns_at(L, S), (S > 0), (S + 1 < ct_nstages(L)), NT>(
         a, b, M, stride, tw, sign);
     __syncthreads();
-    fft_stages_pad<C, L, S + 1>(b, a, M, stride, tw, sign);
+    fft_stages_pad<C, L, S + 1, NT>(b, a, M, stride, tw, sign);
   }
 }
-template <class C, int L>
+template <class C, int L, int NT = BLOCK>
 __device__ inline C* lds_fft_pad(C* b0, C* b1, int M, int stride, const C* __restrict__ tw,
                                  int sign) {
-  fft_stages_pad<C, L, 0>(b0, b1, M, stride, tw, sign);
+  fft_stages_pad<C, L, 0, NT>(b0, b1, M, stride, tw, sign);
   return (ct_nstages(L) % 2 == 0) ? b0 : b1;
 }
 __host__ __device__ constexpr int pad_stride(int L) { return L + L / 8 + 1; }
@@ -321,18 +321,18 @@ __device__ inline size_t spec_index(const Geom& g, int kx, int ky, int kz) {
 // Element loop e = threadIdx.x, +BLOCK, ... < total in chunks of U: the U
 // loads of a chunk are issued before any of them is consumed, so a thread
 // keeps U global loads in flight instead of waiting on each in turn.
-template <int U, class T, class Ld, class St>
+template <int U, class T, int NT = BLOCK, class Ld, class St>
 __device__ inline void staged(int total, Ld ld, St st) {
-  for (int e0 = threadIdx.x; e0 < total; e0 += U * BLOCK) {
+  for (int e0 = threadIdx.x; e0 < total; e0 += U * NT) {
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * BLOCK;
+      const int e = e0 + u * NT;
       if (e < total) v[u] = ld(e);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * BLOCK;
+      const int e = e0 + u * NT;
       if (e < total) st(e, v[u]);
     }
   }
@@ -734,8 +734,8 @@ __host__ __device__ constexpr int plane_buf(int L) { return (L / 2 + 1) * pad_st
 struct F2x2 { float4 a, b; };
 
 // Forward: real plane z -> plane of the half spectrum (x R2C, then y FFT).
-template <int L>
-__global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__ tw,
+template <int L, int NT>
+__global__ void __launch_bounds__(NT) k_fft_xy_r2c(const float2* __restrict__ tw,
                                                       const float* __restrict__ qgrid,
                                                       float2* __restrict__ spec2) {
   using C = float2;
@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__
   const size_t plane = size_t(blockIdx.x) * L * L;
   const float4* q4 = reinterpret_cast<const float4*>(qgrid + plane);
   // rows 2p (real part) and 2p + 1 (imaginary part) form complex line p
-  staged<8, F2x2>(
+  staged<8, F2x2, NT>(
       P * L4,
       [&](int e) {
         const int p = e / L4, x4 = e - p * L4;
@@ -766,11 +766,11 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__
         d[3] = make_float2(v.a.w, v.b.w);
       });
   __syncthreads();
-  C* res = lds_fft_pad<C, L>(b0, b1, P, S, twl, -1);
+  C* res = lds_fft_pad<C, L, NT>(b0, b1, P, S, twl, -1);
   C* col = (res == b0) ? b1 : b0;
   // split the pair spectra: column kx of row y at col[kx S + y]
   //   A[k] = (Z[k] + conj(Z[-k])) / 2,  B[k] = (Z[k] - conj(Z[-k])) / (2i)
-  for (int e = threadIdx.x; e < H * L; e += BLOCK) {
+  for (int e = threadIdx.x; e < H * L; e += NT) {
     const int kx = e / L, y = e - kx * L;
     const int p = y >> 1;
     const C z = res[p * S + kx];
@@ -786,9 +786,9 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__
     col[kx * S + y] = v;
   }
   __syncthreads();
-  const C* out = lds_fft_pad<C, L>(col, res, H, S, twl, -1);
+  const C* out = lds_fft_pad<C, L, NT>(col, res, H, S, twl, -1);
   float2* dst = spec2 + size_t(blockIdx.x) * L * H;
-  for (int e = threadIdx.x; e < L * H; e += BLOCK) {
+  for (int e = threadIdx.x; e < L * H; e += NT) {
     const int ky = e / H, kx = e - ky * H;
     dst[e] = out[kx * S + ky];
   }
@@ -796,8 +796,8 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_r2c(const float2* __restrict__
 
 // Inverse: plane of the half spectrum -> real plane (y inverse FFT, then x
 // C2R), unnormalised.
-template <int L>
-__global__ void __launch_bounds__(BLOCK) k_fft_xy_c2r(const float2* __restrict__ tw,
+template <int L, int NT>
+__global__ void __launch_bounds__(NT) k_fft_xy_c2r(const float2* __restrict__ tw,
                                                       const float2* __restrict__ spec2,
                                                       float* __restrict__ out) {
   using C = float2;
@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_c2r(const float2* __restrict__
   C* twl = b1 + plane_buf(L);
   load_twiddles(twl, tw, L);
   const float4* s4 = reinterpret_cast<const float4*>(spec2 + size_t(blockIdx.x) * NP);
-  staged<8, float4>(
+  staged<8, float4, NT>(
       NP / 2, [&](int e) { return s4[e]; },
       [&](int e, float4 v) {
         int ky = (2 * e) / H, kx = 2 * e - ky * H;
@@ -818,10 +818,10 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_c2r(const float2* __restrict__
         b0[kx * S + ky] = make_float2(v.z, v.w);
       });
   __syncthreads();
-  C* res = lds_fft_pad<C, L>(b0, b1, H, S, twl, +1);
+  C* res = lds_fft_pad<C, L, NT>(b0, b1, H, S, twl, +1);
   C* lin = (res == b0) ? b1 : b0;
   // Z = A + i B over the full Hermitian x line, A = row 2p, B = row 2p + 1
-  for (int e = threadIdx.x; e < P * L; e += BLOCK) {
+  for (int e = threadIdx.x; e < P * L; e += NT) {
     const int p = e / L, k = e - p * L;
     C a, b;
     if (k < H) {
@@ -837,9 +837,9 @@ __global__ void __launch_bounds__(BLOCK) k_fft_xy_c2r(const float2* __restrict__
     lin[p * S + k] = z;
   }
   __syncthreads();
-  const C* r = lds_fft_pad<C, L>(lin, res, P, S, twl, +1);
+  const C* r = lds_fft_pad<C, L, NT>(lin, res, P, S, twl, +1);
   float4* o4 = reinterpret_cast<float4*>(out + size_t(blockIdx.x) * L * L);
-  for (int e = threadIdx.x; e < P * L4; e += BLOCK) {
+  for (int e = threadIdx.x; e < P * L4; e += NT) {
     const int p = e / L4, x = 4 * (e - p * L4);
     const C* s = r + p * S + x;
     o4[(2 * p) * L4 + x / 4] = make_float4(s[0].x, s[1].x, s[2].x, s[3].x);

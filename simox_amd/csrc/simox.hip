@@ -70,7 +70,7 @@ struct FftCfg {
   size_t shx = 0, shy = 0, shz = 0;
   // plane-resident chain (FP32 coarse grids with Kx = Ky <= 128): x and y in
   // one pass per direction, z pass over nl lines per workgroup
-  int plane = 0, nl = 8, zpblocks = 0;
+  int plane = 0, nl = 8, zpblocks = 0, xynt = 1024;
   size_t shp = 0, shzp = 0;
 };
 
@@ -579,18 +579,22 @@ bool timed_grid(const simox_ctx* ctx, const PmeGrid& pm) { return &pm == &ctx->p
 int tcat(const simox_ctx* ctx, const PmeGrid& pm, int k) { return &pm == &ctx->pi ? 11 + k : k; }
 
 // Kernels of the plane-resident chain for plane size L (Kx = Ky) and Kz.
-using XyFwd = decltype(&k_fft_xy_r2c<128>);
-using XyInv = decltype(&k_fft_xy_c2r<128>);
+using XyFwd = decltype(&k_fft_xy_r2c<128, 256>);
+using XyInv = decltype(&k_fft_xy_c2r<128, 256>);
 using ZpFn = decltype(&k_fft_zp<256, 16>);
 template <int L>
-void xy_of(XyFwd* f, XyInv* b) { *f = &k_fft_xy_r2c<L>; *b = &k_fft_xy_c2r<L>; }
-bool xy_kernels(int L, XyFwd* f, XyInv* b) {
+void xy_of(int nt, XyFwd* f, XyInv* b) {
+  if (nt == 1024) { *f = &k_fft_xy_r2c<L, 1024>; *b = &k_fft_xy_c2r<L, 1024>; }
+  else if (nt == 512) { *f = &k_fft_xy_r2c<L, 512>; *b = &k_fft_xy_c2r<L, 512>; }
+  else { *f = &k_fft_xy_r2c<L, 256>; *b = &k_fft_xy_c2r<L, 256>; }
+}
+bool xy_kernels(int L, int nt, XyFwd* f, XyInv* b) {
   switch (L) {
-    case 32: xy_of<32>(f, b); return true;
-    case 48: xy_of<48>(f, b); return true;
-    case 64: xy_of<64>(f, b); return true;
-    case 96: xy_of<96>(f, b); return true;
-    case 128: xy_of<128>(f, b); return true;
+    case 32: xy_of<32>(nt, f, b); return true;
+    case 48: xy_of<48>(nt, f, b); return true;
+    case 64: xy_of<64>(nt, f, b); return true;
+    case 96: xy_of<96>(nt, f, b); return true;
+    case 128: xy_of<128>(nt, f, b); return true;
     default: return false;
   }
 }
@@ -608,7 +612,13 @@ ZpFn zp_of(int KZ) {
   }
 }
 ZpFn zp_kernel(int KZ, int nl) {
-  return nl == 4 ? zp_of<4>(KZ) : nl == 8 ? zp_of<8>(KZ) : nl == 16 ? zp_of<16>(KZ) : nullptr;
+  switch (nl) {
+    case 4: return zp_of<4>(KZ);
+    case 8: return zp_of<8>(KZ);
+    case 11: return zp_of<11>(KZ);
+    case 16: return zp_of<16>(KZ);
+    default: return nullptr;
+  }
 }
 
 int plane_chain(simox_ctx* ctx, PmeGrid& pm, float2* spec, float2* const* tw, const FftCfg& fc,
@@ -617,21 +627,21 @@ int plane_chain(simox_ctx* ctx, PmeGrid& pm, float2* spec, float2* const* tw, co
   XyFwd fwd{};
   XyInv inv{};
   const ZpFn zp = zp_kernel(g.K[2], fc.nl);
-  if (!xy_kernels(g.K[0], &fwd, &inv) || !zp || g.K[1] != g.K[0]) {
+  if (!xy_kernels(g.K[0], fc.xynt, &fwd, &inv) || !zp || g.K[1] != g.K[0]) {
     ctx->err = "plane FFT chain: no kernels for these grid sizes";
     return SIMOX_ERR_ARG;
   }
   const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
   if (timed) tstart(ctx, tcat(ctx, pm, 1), &ev);
-  hipLaunchKernelGGL(fwd, dim3(g.K[2]), dim3(BLOCK), fc.shp, ctx->st, tw[0], greal, spec);
+  hipLaunchKernelGGL(fwd, dim3(g.K[2]), dim3(fc.xynt), fc.shp, ctx->st, tw[0], greal, spec);
   if (timed) tstop(ctx, tcat(ctx, pm, 1), ev);
   if (timed) tstart(ctx, tcat(ctx, pm, 3), &ev);
   hipLaunchKernelGGL(zp, dim3(fc.zpblocks), dim3(BLOCK), fc.shzp, ctx->st, g, tw[2], spec, G,
                      ctx->part);
   if (timed) tstop(ctx, tcat(ctx, pm, 3), ev);
   if (timed) tstart(ctx, tcat(ctx, pm, 4), &ev);
-  hipLaunchKernelGGL(inv, dim3(g.K[2]), dim3(BLOCK), fc.shp, ctx->st, tw[0], spec, greal);
+  hipLaunchKernelGGL(inv, dim3(g.K[2]), dim3(fc.xynt), fc.shp, ctx->st, tw[0], spec, greal);
   if (timed) tstop(ctx, tcat(ctx, pm, 4), ev);
   HIPCHK(hipGetLastError());
   return 0;
@@ -896,8 +906,9 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
 // The plane-resident FP32 chain on a coarse grid when Kx = Ky and both
 // kernels exist for its sizes (SIMOX_FFT_PLANE=0 keeps the five-pass chain;
 // SIMOX_FFT_PLANE=2 also takes it on the reference's grid, which tests use to
-// compare the two chains on the same energies and forces; SIMOX_FFT_NL = 4, 8
-// or 16 sets the z-pass lines per workgroup, default 8).
+// compare the two chains on the same energies and forces; SIMOX_FFT_NL = 4, 8,
+// 11 or 16 sets the z-pass lines per workgroup, default 8; SIMOX_FFT_XYNT =
+// 256, 512 or 1024 the threads per workgroup of the x-y passes, default 1024).
 bool plane_everywhere() {
   const char* e = std::getenv("SIMOX_FFT_PLANE");
   return e && std::atoi(e) == 2;
@@ -910,19 +921,23 @@ int plane_setup(simox_ctx* ctx, PmeGrid& pm) {
   const int L = pm.K[0], KZ = pm.K[2];
   XyFwd fwd{};
   XyInv inv{};
-  if (pm.K[1] != L || !xy_kernels(L, &fwd, &inv) || !zp_kernel(KZ, 8)) return 0;
+  c.xynt = 1024;
+  if (const char* s = std::getenv("SIMOX_FFT_XYNT")) {
+    const int t = std::atoi(s);
+    if (t == 256 || t == 512 || t == 1024) c.xynt = t;
+  }
+  if (pm.K[1] != L || !xy_kernels(L, c.xynt, &fwd, &inv) || !zp_kernel(KZ, 8)) return 0;
   c.nl = 8;
   if (const char* s = std::getenv("SIMOX_FFT_NL")) {
     const int m = std::atoi(s);
-    if (m == 4 || m == 8 || m == 16) c.nl = m;
+    if (zp_kernel(KZ, m)) c.nl = m;
   }
   c.shp = (2 * size_t(plane_buf(L)) + L) * sizeof(float2);
   c.shzp = (2 * size_t(c.nl) * pad_stride(KZ) + KZ) * sizeof(float2);
   if (c.shp > FFT_LDS_MAX || c.shzp > FFT_LDS_MAX) return 0;
   c.zpblocks = (pm.K[1] * pm.H + c.nl - 1) / c.nl;
-  const void* fns[4] = {reinterpret_cast<const void*>(fwd), reinterpret_cast<const void*>(inv),
-                        reinterpret_cast<const void*>(zp_kernel(KZ, 8)),
-                        reinterpret_cast<const void*>(zp_kernel(KZ, 16))};
+  const void* fns[3] = {reinterpret_cast<const void*>(fwd), reinterpret_cast<const void*>(inv),
+                        reinterpret_cast<const void*>(zp_kernel(KZ, c.nl))};
   for (const void* f : fns)
     HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(FFT_LDS_MAX)));
   CHK(ensure_part(ctx, size_t(c.zpblocks)));

@@ -337,10 +337,11 @@ def test_z256_register_pass_fp32(monkeypatch):
     assert e_reg == pytest.approx(e64, rel=1e-5)
 
 
-@pytest.mark.parametrize("fft,nl", [((32, 32, 64), "16"), ((48, 48, 96), "8"),
-                                    ((128, 128, 32), "16"), ((64, 64, 256), "16"),
-                                    ((96, 96, 128), "4")])
-def test_plane_fft_chain_fp32(monkeypatch, fft, nl):
+@pytest.mark.parametrize("fft,nl,nt", [((32, 32, 64), "16", "256"), ((48, 48, 96), "8", "512"),
+                                       ((128, 128, 32), "16", "512"), ((64, 64, 256), "16", "256"),
+                                       ((96, 96, 128), "4", "1024"), ((128, 128, 64), "11", "256"),
+                                       ((128, 128, 256), "8", "1024")])
+def test_plane_fft_chain_fp32(monkeypatch, fft, nl, nt):
     """The plane-resident FP32 chain (x and y transformed in one LDS pass per
     direction, z pass on the plane-major spectrum) that the coarse grids of
     the MIXED solve take, switched onto the reference's grid with
@@ -350,8 +351,9 @@ def test_plane_fft_chain_fp32(monkeypatch, fft, nl):
     MIXED)."""
     s = tiny(seed=4, nx=3, ny=3, nz=3, n_o=8, fft=fft, rcut=5.0)
     q = np.random.default_rng(6).normal(size=s.n) * 0.3
-    print("plane chain", fft, "nl", nl, "n", s.n, flush=True)
+    print("plane chain", fft, "nl", nl, "threads", nt, "n", s.n, flush=True)
     monkeypatch.setenv("SIMOX_FFT_NL", nl)
+    monkeypatch.setenv("SIMOX_FFT_XYNT", nt)
 
     def run(mode, plane):
         monkeypatch.setenv("SIMOX_FFT_PLANE", plane)
