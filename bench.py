@@ -38,9 +38,36 @@ def coarse_inner(s, mode):
     return mode == "mixed" and float(np.prod([float(k) for k in s.fft])) >= 2 ** 23
 
 
+def packed_inner(s, mode):
+    """The coarse inner operator's matrix as packed 32-bit entries (column and
+    12-bit value; simox's default up to 2^20 atoms, SIMOX_INNER_PACK=0 off)."""
+    return (coarse_inner(s, mode) and s.n <= 2 ** 20
+            and os.environ.get("SIMOX_INNER_PACK", "1") != "0")
+
+
+PLANE_XY = (32, 48, 64, 96, 128)
+PLANE_Z = (32, 48, 64, 96, 128, 192, 256)
+
+
+def plane_inner(s, mode):
+    """The coarse inner grid's FP32 FFTs on the plane-resident chain (x-y pass,
+    z pass on the plane-major spectrum, inverse x-y pass; Kx = Ky in PLANE_XY,
+    Kz in PLANE_Z; SIMOX_FFT_PLANE=0 off)."""
+    if s is None or not coarse_inner(s, mode) or os.environ.get("SIMOX_FFT_PLANE", "1") == "0":
+        return False
+    kx, ky, kz = (k // 2 for k in s.fft)
+    return kx == ky and kx in PLANE_XY and kz in PLANE_Z
+
+
+PLANE_KERNELS = {12: "k_fft_xy_r2c (inner grid)", 13: "-", 14: "k_fft_zp (inner grid)",
+                 15: "k_fft_xy_c2r (inner grid)"}
+
+
 def kernel_name(w, mode, s=None):
     if w == 6 and mode == "faithful":
         return "k_dudq"
+    if w in PLANE_KERNELS and plane_inner(s, mode):
+        return PLANE_KERNELS[w]
     if w == 6 and s is not None and coarse_inner(s, mode):
         return "k_h0matvec16"
     return KERNELS[w]
@@ -85,10 +112,10 @@ def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None):
     nh = (Kx // 2 + 1) * Ky * Kz
     r = 4 if mode == "mixed" else 8
     c = 2 * r
-    if which == 0:   # spread: x, q (+p, ql) in; fixed-point grid accumulated once
-        return n * (24 + 8) + ng * 8
-    if which == 1:   # R2C: fixed-point grid read + zeroed, half spectrum written
-        return ng * 16 + nh * c
+    if which == 0:   # tiled spread: x, q in; each grid point written once
+        return n * (24 + 8) + ng * r
+    if which == 1:   # R2C (or the plane x-y pass): real grid read, half spectrum written
+        return ng * r + nh * c
     if which == 2:   # y pass: spectrum read + written
         return nh * 2 * c
     if which == 3:   # z pass: spectrum read + written, influence table read
@@ -102,6 +129,8 @@ def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None):
             return ng * r + n * (24 + 4 + 8 + 8 + 8 + 8 + 4) + nb_entries * (4 + r)
         # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows; FP16
         # values in the coarse-grid inner operator
+        if packed_inner(s, mode):  # compacted matrix, packed 32-bit entries
+            return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + inner_entries * 4
         if coarse_inner(s, mode):  # compacted FP16 matrix of the inner operator
             return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + inner_entries * (4 + 2)
         return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + r)
@@ -364,6 +393,11 @@ def inner_operator(s, mode):
     """The MIXED inner solves' operator (simox_set_inner_grid's default rule)."""
     if mode != "mixed":
         return "the reference's operator, FP64"
+    if packed_inner(s, mode):
+        return ("FP32 with 12-bit matrix values packed with their 20-bit columns, entries "
+                "below 1e-3 eV/e^2 left out; reciprocal part on the %dx%dx%d grid (half "
+                "resolution, order 4, plane-resident FFT passes); restarts FP64 with the full "
+                "matrix on the reference grid" % tuple(k // 2 for k in s.fft))
     if coarse_inner(s, mode):
         return ("FP32 with FP16 matrix values, entries below 1e-3 eV/e^2 left out; reciprocal "
                 "part on the %dx%dx%d grid (half resolution, order 4); restarts FP64 with the "
@@ -488,8 +522,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": per_launch_s * 1e3, "launches": launches},
-        "kernels_ms": {kernel_name(w, args.mode, s): round(breakdown[w][0], 3) for w in KERNELS},
-        "kernels_launches": {kernel_name(w, args.mode, s): breakdown[w][1] for w in KERNELS},
+        "kernels_ms": {kernel_name(w, args.mode, s): round(breakdown[w][0], 3) for w in KERNELS
+                       if kernel_name(w, args.mode, s) != "-"},
+        "kernels_launches": {kernel_name(w, args.mode, s): breakdown[w][1] for w in KERNELS
+                             if kernel_name(w, args.mode, s) != "-"},
         "kernels_note": "kernels_ms / kernels_launches: one untimed step after the warm-up "
                         "with every category timed; the timed steps time the roofline kernel only",
     }

@@ -155,6 +155,8 @@ struct simox_ctx {
   // compacted FP16 matrix of the coarse-grid inner operator (MIXED)
   _Float16* nb_val16 = nullptr;
   int *nb_cnt16 = nullptr, *nb_col16 = nullptr;
+  unsigned* nb_pk = nullptr;  // packed 32-bit entries (column + 12-bit value) when n <= 2^20
+  bool inner_pack = true, pk_valid = false;
   double inner_drop = 1e-3;  // entries with |H0_ij| below this (eV/e^2) left out
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
@@ -1076,15 +1078,25 @@ int build_matrix(simox_ctx* ctx) {
     hipLaunchKernelGGL(k_ell_to_f32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
                        ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
   ctx->val16_valid = false;
+  ctx->pk_valid = false;
   if (use32(ctx) && ctx->inner_coarse && ctx->inner_f16) {
-    if (!ctx->nb_val16) {
-      CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
-      CHK(dalloc(ctx, &ctx->nb_col16, size_t(ctx->cap) * ctx->n));
-      CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
+    const bool pack = ctx->inner_pack && ctx->n <= (1 << PK_COL_BITS);
+    if (!ctx->nb_cnt16) CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
+    if (pack) {
+      if (!ctx->nb_pk) CHK(dalloc(ctx, &ctx->nb_pk, size_t(ctx->cap) * ctx->n));
+      hipLaunchKernelGGL(k_ell_to_p32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
+                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16,
+                         ctx->nb_pk);
+      ctx->pk_valid = true;
+    } else {
+      if (!ctx->nb_val16) {
+        CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
+        CHK(dalloc(ctx, &ctx->nb_col16, size_t(ctx->cap) * ctx->n));
+      }
+      hipLaunchKernelGGL(k_ell_to_f16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
+                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16,
+                         ctx->nb_col16, ctx->nb_val16);
     }
-    hipLaunchKernelGGL(k_ell_to_f16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
-                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16,
-                       ctx->nb_col16, ctx->nb_val16);
     ctx->val16_valid = true;
   }
   HIPCHK(hipGetLastError());
@@ -1419,10 +1431,14 @@ int ncg_iter_body(simox_ctx* ctx) {
   CHK(phi_gather(ctx, f32, ig));
   tstop(ctx, 10, ev);
   tstart(ctx, 6, &ev);
-  if (ig && ctx->val16_valid) {
-    hipLaunchKernelGGL(k_h0matvec16, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+  if (ig && ctx->val16_valid && ctx->pk_valid) {
+    hipLaunchKernelGGL(k_h0matvec16<true>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
-                       ctx->nb_col16, ctx->nb_val16, ctx->apv, ctx->part);
+                       ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
+  } else if (ig && ctx->val16_valid) {
+    hipLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
+                       ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
   } else if (f32) {
     hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
@@ -1902,6 +1918,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_INNER_PACK")) ctx->inner_pack = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
     const double v = std::atof(e);
@@ -1978,7 +1995,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

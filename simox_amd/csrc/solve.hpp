@@ -691,8 +691,38 @@ __global__ void k_ell_to_f16(int n, int cap, const int* __restrict__ cnt,
   cnt16[i] = m;
 }
 
+// The compacted matrix as one 32-bit word per entry when n <= 2^20: the
+// column in the low 20 bits, the value in the high 12 as an FP16 number with
+// its mantissa rounded to 6 bits (relative error <= 2^-7), so the word's
+// bits 31..20 are those of the FP16 value (entries |H0_ij| >= tau stay far
+// inside the normal FP16 range).  4 bytes per entry instead of 6.
+constexpr int PK_COL_BITS = 20;
+__global__ void k_ell_to_p32(int n, int cap, const int* __restrict__ cnt,
+                             const int* __restrict__ col, const double* __restrict__ v64,
+                             double tau, int* __restrict__ cnt16, unsigned* __restrict__ pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cnt[i] < cap ? cnt[i] : cap;
+  int m = 0;
+  for (int k = 0; k < c; ++k) {
+    const double v = v64[size_t(k) * n + i];
+    if (fabs(v) < tau) continue;
+    const unsigned hb = __builtin_bit_cast(unsigned short, _Float16(float(v)));
+    const unsigned code = (hb & 0x8000u) | (((hb & 0x7FFFu) + 8u) & 0x7FF0u);
+    pk[size_t(m) * n + i] = (code << 16) | unsigned(col[size_t(k) * n + i]);
+    ++m;
+  }
+  cnt16[i] = m;
+}
+__device__ inline float pk_value(unsigned w) {
+  return float(__builtin_bit_cast(_Float16, (unsigned short)((w >> 16) & 0xFFF0u)));
+}
+__device__ inline int pk_col(unsigned w) { return int(w & ((1u << PK_COL_BITS) - 1u)); }
+
 // y = H0 p as k_h0matvec<float, float> with FP16 matrix values (6 bytes per
-// entry instead of 8), FP32 products and sums.
+// entry instead of 8), FP32 products and sums; PK: the packed 32-bit entries
+// of k_ell_to_p32 (pk) instead of col and val.
+template <bool PK>
 __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __restrict__ tb,
                                                       const int* __restrict__ id,
                                                       const double* __restrict__ q,
@@ -703,6 +733,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
                                                       const int* __restrict__ cnt,
                                                       const int* __restrict__ col,
                                                       const _Float16* __restrict__ val,
+                                                      const unsigned* __restrict__ pk,
                                                       double* __restrict__ y,
                                                       double* __restrict__ part) {
   __shared__ double lds[3 * BLOCK / WAVE];
@@ -719,8 +750,14 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
       float vv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        cc[u] = col[size_t(k + u) * n + i];
-        vv[u] = float(val[size_t(k + u) * n + i]);
+        if constexpr (PK) {
+          const unsigned w = pk[size_t(k + u) * n + i];
+          cc[u] = pk_col(w);
+          vv[u] = pk_value(w);
+        } else {
+          cc[u] = col[size_t(k + u) * n + i];
+          vv[u] = float(val[size_t(k + u) * n + i]);
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) s += vv[u] * pt[cc[u]];
@@ -733,8 +770,14 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
       for (int u = 0; u < UT; ++u) {
         const bool ok = k + u < c;
         const size_t e = size_t(ok ? k + u : k) * n + i;
-        cc[u] = ok ? col[e] : i;
-        vv[u] = ok ? float(val[e]) : 0.f;
+        if constexpr (PK) {
+          const unsigned w = pk[e];
+          cc[u] = ok ? pk_col(w) : i;
+          vv[u] = ok ? pk_value(w) : 0.f;
+        } else {
+          cc[u] = ok ? col[e] : i;
+          vv[u] = ok ? float(val[e]) : 0.f;
+        }
       }
 #pragma unroll
       for (int u = 0; u < UT; ++u) s += vv[u] * pt[cc[u]];

@@ -271,16 +271,19 @@ def test_z512_register_pass_mixed_solve(monkeypatch):
     assert out[True][1] == pytest.approx(out[False][1], rel=1e-9)
 
 
-def test_mixed_coarse_inner_grid_matches_fp64():
+@pytest.mark.parametrize("pack", ["1", "0"])
+def test_mixed_coarse_inner_grid_matches_fp64(monkeypatch, pack):
     """MIXED with the inner operator's reciprocal part on the half-resolution
-    grid and its sparse part from the compacted FP16 matrix
-    (simox_set_inner_grid): the restarts use the reference's grid and the
-    FP64 matrix, so the converged charges equal FP64's to 1e-9 and the forces
-    to 1e-5 of the RMS force, with the coarse operator on and off."""
+    grid and its sparse part from the compacted matrix (simox_set_inner_grid;
+    pack 1: column and 12-bit value in one 32-bit word, 0: FP16 values): the
+    restarts use the reference's grid and the FP64 matrix, so the converged
+    charges equal FP64's to 1e-9 and the forces to 1e-5 of the RMS force,
+    with the coarse operator on and off."""
     from simox_amd.systems import al_o_slab
+    monkeypatch.setenv("SIMOX_INNER_PACK", pack)
     s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (96, 96, 64), 10.0, vacuum=20.0,
                   o_layer=(2.0, 10.0), o_dmin=2.0, name="inner")
-    print("coarse inner grid, n", s.n, flush=True)
+    print("coarse inner grid, n", s.n, "pack", pack, flush=True)
     out = {}
     for tag in ("fp64", "coarse", "fine"):
         g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_FP64 if tag == "fp64" else MODE_MIXED)
