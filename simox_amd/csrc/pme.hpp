@@ -1299,6 +1299,30 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   }
 }
 
+// Stencil records of the 32-bit spread in brick order, once per set of
+// positions: for atom border[f], z0_d = 1 - frac(u_d) in FP32 (the B-spline
+// argument of k_spread_tile_i32) and the base index INT(u_d) mod K_d in 10
+// bits per dimension (K_d <= 1024), so the spread reads 16 contiguous bytes
+// per candidate instead of gathering x and redoing the FP64 grid coordinates.
+__global__ void k_brick_records(Geom g, int n, const double* __restrict__ x,
+                                const int* __restrict__ border, float4* __restrict__ rec) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  double u[3];
+  int b[3];
+  grid_coord(g, x + 3 * border[f], u, b);
+  float z[3];
+  unsigned pk = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    z[d] = float(trunc(u[d]) - u[d] + 1.0);
+    int bm = b[d] % g.K[d];
+    if (bm < 0) bm += g.K[d];
+    pk |= unsigned(bm) << (10 * d);
+  }
+  rec[f] = make_float4(z[0], z[1], z[2], __uint_as_float(pk));
+}
+
 // K6 for the MIXED inner operator's coarse grid: the same tiles, bricks and
 // order-4 B-splines with FP32 weights and 32-bit fixed point.  Each tile
 // owns its points and writes each once, so it takes its own scale: 2^S with
@@ -1310,8 +1334,12 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
                                                            const double* __restrict__ q,
                                                            const int* __restrict__ bstart,
                                                            const int* __restrict__ border,
+                                                           const float4* __restrict__ rec,
                                                            float* __restrict__ grid) {
+  constexpr int MAXC = 6144;  // candidates per tile with a slot table
   __shared__ int acc[TILE];
+  __shared__ unsigned short ctab[MAXC];  // candidate f: (candidate brick << 8) | offset
+  __shared__ int tab_bad;
   __shared__ int bofs[MAXB + 1];
   __shared__ int bbeg[MAXB];
   __shared__ int wsum[BLOCK / WAVE];
@@ -1360,14 +1388,30 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
   __syncthreads();
   const int total = bofs[ncand];
-  auto atom_of = [&](int f) {
+  // the slot table when the tile's candidates fit it (bricks of < 256 atoms)
+  if (threadIdx.x == 0) tab_bad = total > MAXC;
+  __syncthreads();
+  if (!tab_bad && threadIdx.x < ncand) {
+    const int k = threadIdx.x, o = bofs[k], c = bofs[k + 1] - o;
+    if (c > 256) tab_bad = 1;
+    else
+      for (int j = 0; j < c; ++j) ctab[o + j] = (unsigned short)((k << 8) | j);
+  }
+  __syncthreads();
+  const bool table = !tab_bad;
+  auto slot_of = [&](int f) {  // brick-order index of the tile's f-th candidate
+    if (table) {
+      const int e = ctab[f];
+      return bbeg[e >> 8] + (e & 255);
+    }
     int l = 0, h = ncand - 1;
     while (l < h) {
       const int mid = (l + h + 1) >> 1;
       if (bofs[mid] <= f) l = mid; else h = mid - 1;
     }
-    return border[bbeg[l] + (f - bofs[l])];
+    return bbeg[l] + (f - bofs[l]);
   };
+  auto atom_of = [&](int f) { return border[slot_of(f)]; };
   // the tile's scale from the largest |q| it visits
   double m = 0.0;
   for (int f = threadIdx.x; f < total; f += blockDim.x) m = fmax(m, fabs(q[atom_of(f)]));
@@ -1381,18 +1425,18 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   const int S = m > 0.0 ? 31 - e : 0;    // 8 m 2^S < 2^31
   const double scale = ldexp(1.0, S);
   for (int f = threadIdx.x; f < total; f += blockDim.x) {
-    const int ai = atom_of(f);
-    const float qs = float(q[ai] * scale);
-    double u[3];
-    int b[3];
-    grid_coord(g, x + 3 * ai, u, b);
+    const int sl = slot_of(f);
+    const float qs = float(q[border[sl]] * scale);
+    const float4 rc = rec[sl];
+    const float zr[3] = {rc.x, rc.y, rc.z};
+    const unsigned pk = __float_as_uint(rc.w);
     float AX[3][4];
     int o[3][4];
     bool in[3][4];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       // order-4 B-spline weights of the fractional offset (bspline4_w in FP32)
-      const float z0 = float(trunc(u[d]) - u[d] + 1.0);
+      const float z0 = zr[d];
       const float z1 = z0 + 1.f, z2 = z0 + 2.f, z3 = z0 + 3.f;
       constexpr float s6 = 1.f / 6.f;
       AX[d][0] = z0 * z0 * z0 * s6;
@@ -1400,8 +1444,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
       AX[d][2] = (((3.f * z2 - 24.f) * z2 + 60.f) * z2 - 44.f) * s6;
       AX[d][3] = (((-z3 + 12.f) * z3 - 48.f) * z3 + 64.f) * s6;
       const int K = g.K[d];
-      int bm = b[d] % K;
-      if (bm < 0) bm += K;
+      const int bm = int((pk >> (10 * d)) & 1023u);
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         int idx = bm + a - 1;

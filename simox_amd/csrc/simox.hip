@@ -99,6 +99,7 @@ struct PmeGrid {
   bool bricks_valid = false;
   bool ready = false;
   bool spread32 = false;  // FP32 spreads in 32-bit fixed point (the MIXED inner grid)
+  float4* brec = nullptr;  // its stencil records in brick order (k_brick_records)
 };
 
 struct simox_ctx {
@@ -852,6 +853,9 @@ int ensure_bricks(simox_ctx* ctx, PmeGrid& pm) {
                      pm.bfill, pm.bunord);
   hipLaunchKernelGGL(k_brick_place, dim3(nb), dim3(BLOCK), 0, ctx->st, n, pm.brick, pm.bstart,
                      pm.bfill, pm.bunord, pm.border);
+  if (pm.spread32)
+    hipLaunchKernelGGL(k_brick_records, dim3(nb), dim3(BLOCK), 0, ctx->st, geom_of(ctx, pm), n,
+                       ctx->x, pm.border, pm.brec);
   HIPCHK(hipGetLastError());
   pm.bricks_valid = true;
   return 0;
@@ -886,7 +890,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
   else if (f32 && pm.spread32)
     hipLaunchKernelGGL(k_spread_tile_i32, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
-                       pm.bstart, pm.border, pm.greal32);
+                       pm.bstart, pm.border, pm.brec, pm.greal32);
   else if (f32)
     hipLaunchKernelGGL((k_spread_tile<4, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
@@ -986,6 +990,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   CHK(dalloc(ctx, &pm.F64, nh));
   if (f32) {
     CHK(dalloc(ctx, &pm.greal32, ng)); CHK(dalloc(ctx, &pm.spec32, nh));
+    CHK(dalloc(ctx, &pm.brec, n));
     if (with_G) CHK(dalloc(ctx, &pm.G32, nh));
     CHK(dalloc(ctx, &pm.F32, nh));
   }
@@ -1024,7 +1029,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
 void free_grid(PmeGrid& pm) {
   void* bufs[] = {pm.brick, pm.bunord, pm.border, pm.bcount, pm.bfill, pm.bstart, pm.bsum,
                   pm.btotal, pm.greal64, pm.greal32, pm.spec64, pm.spec32, pm.G64, pm.G32,
-                  pm.F64, pm.F32, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
+                  pm.F64, pm.F32, pm.brec, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
                   pm.tw32[2]};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1278,7 +1283,9 @@ int inner_grid(simox_ctx* ctx) {
     ctx->inner_coarse = false;
     return rc == SIMOX_ERR_ARG ? 0 : rc;
   }
-  ctx->pi.spread32 = ctx->inner_spread32;
+  // the stencil records hold 10-bit base indices
+  ctx->pi.spread32 = ctx->inner_spread32 && ctx->pi.brec && K[0] <= 1024 && K[1] <= 1024 &&
+                     K[2] <= 1024;
   return 0;
 }
 
