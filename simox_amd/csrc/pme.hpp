@@ -1154,6 +1154,40 @@ __global__ void k_brick_scatter(int n, const int* __restrict__ brick, const int*
   unordered[start[b] + atomicAdd(&fill[b], 1)] = i;
 }
 
+// Candidate slot table of a spread tile: entry f = (candidate brick k << 8)
+// | offset of the tile's f-th candidate atom within brick k, when the tile has
+// at most SPREAD_MAXC candidates and no brick holds more than 256 atoms (else
+// the slot is found by a binary search over the prefix array bofs).
+constexpr int SPREAD_MAXC = 6144;
+static_assert(MAXB <= 256, "candidate brick index in 8 bits");
+__device__ inline bool spread_slot_table(int ncand, int total, const int* bofs,
+                                         unsigned short* ctab, int* bad) {
+  if (threadIdx.x == 0) *bad = total > SPREAD_MAXC;
+  __syncthreads();
+  if (!*bad && int(threadIdx.x) < ncand) {
+    const int k = threadIdx.x, o = bofs[k], c = bofs[k + 1] - o;
+    if (c > 256) *bad = 1;
+    else
+      for (int j = 0; j < c; ++j) ctab[o + j] = (unsigned short)((k << 8) | j);
+  }
+  __syncthreads();
+  return !*bad;
+}
+// Brick-order index (into border) of the tile's f-th candidate.
+__device__ inline int spread_slot(int f, bool table, int ncand, const int* bofs, const int* bbeg,
+                                  const unsigned short* ctab) {
+  if (table) {
+    const int e = ctab[f];
+    return bbeg[e >> 8] + (e & 255);
+  }
+  int l = 0, h = ncand - 1;
+  while (l < h) {
+    const int mid = (l + h + 1) >> 1;
+    if (bofs[mid] <= f) l = mid; else h = mid - 1;
+  }
+  return bbeg[l] + (f - bofs[l]);
+}
+
 // K6 (tiled): grid[tile] = sum over atoms of q w, written as R (double/float).
 // ORD 4: the reference's order-4 B-splines (points base-1 .. base+2); ORD 2:
 // linear weights (points base, base+1) for the preconditioner grid.
@@ -1164,6 +1198,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
                                                        const int* __restrict__ border,
                                                        R* __restrict__ grid) {
   __shared__ unsigned long long acc[TILE];
+  __shared__ unsigned short ctab[SPREAD_MAXC];  // candidate f: (candidate brick << 8) | offset
+  __shared__ int tab_bad;
   __shared__ int bofs[MAXB + 1];
   __shared__ int bbeg[MAXB];
   __shared__ int wsum[BLOCK / WAVE];
@@ -1215,14 +1251,9 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
   __syncthreads();
   const int total = bofs[ncand];
+  const bool table = spread_slot_table(ncand, total, bofs, ctab, &tab_bad);
   for (int f = threadIdx.x; f < total; f += blockDim.x) {
-    // brick of flattened candidate f (binary search in the prefix array)
-    int l = 0, h = ncand - 1;
-    while (l < h) {
-      const int mid = (l + h + 1) >> 1;
-      if (bofs[mid] <= f) l = mid; else h = mid - 1;
-    }
-    const int ai = border[bbeg[l] + (f - bofs[l])];
+    const int ai = border[spread_slot(f, table, ncand, bofs, bbeg, ctab)];
     const double qi = q[ai];
     if constexpr (ORD == 2) {
       double W[3][2];
@@ -1336,9 +1367,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
                                                            const int* __restrict__ border,
                                                            const float4* __restrict__ rec,
                                                            float* __restrict__ grid) {
-  constexpr int MAXC = 6144;  // candidates per tile with a slot table
   __shared__ int acc[TILE];
-  __shared__ unsigned short ctab[MAXC];  // candidate f: (candidate brick << 8) | offset
+  __shared__ unsigned short ctab[SPREAD_MAXC];  // candidate f: (candidate brick << 8) | offset
   __shared__ int tab_bad;
   __shared__ int bofs[MAXB + 1];
   __shared__ int bbeg[MAXB];
@@ -1388,29 +1418,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
   __syncthreads();
   const int total = bofs[ncand];
-  // the slot table when the tile's candidates fit it (bricks of < 256 atoms)
-  if (threadIdx.x == 0) tab_bad = total > MAXC;
-  __syncthreads();
-  if (!tab_bad && threadIdx.x < ncand) {
-    const int k = threadIdx.x, o = bofs[k], c = bofs[k + 1] - o;
-    if (c > 256) tab_bad = 1;
-    else
-      for (int j = 0; j < c; ++j) ctab[o + j] = (unsigned short)((k << 8) | j);
-  }
-  __syncthreads();
-  const bool table = !tab_bad;
-  auto slot_of = [&](int f) {  // brick-order index of the tile's f-th candidate
-    if (table) {
-      const int e = ctab[f];
-      return bbeg[e >> 8] + (e & 255);
-    }
-    int l = 0, h = ncand - 1;
-    while (l < h) {
-      const int mid = (l + h + 1) >> 1;
-      if (bofs[mid] <= f) l = mid; else h = mid - 1;
-    }
-    return bbeg[l] + (f - bofs[l]);
-  };
+  const bool table = spread_slot_table(ncand, total, bofs, ctab, &tab_bad);
+  auto slot_of = [&](int f) { return spread_slot(f, table, ncand, bofs, bbeg, ctab); };
   auto atom_of = [&](int f) { return border[slot_of(f)]; };
   // the tile's scale from the largest |q| it visits
   double m = 0.0;
