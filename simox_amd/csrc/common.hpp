@@ -17,6 +17,17 @@ constexpr int NSPEC = 4;                        // Al, Ni, Fe, O handled on the 
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): workgroup b runs on XCD b mod 8.  xcd_tile(b, nb) maps it to tile
+// (b mod 8) (nb / 8) + b / 8, so consecutive tiles, which share cache lines
+// or halo data, run on one XCD; the last nb mod 8 workgroups keep their index.
+constexpr int NXCD = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+  const int per = nb / NXCD;
+  if (b >= per * NXCD) return b;
+  return (b % NXCD) * per + b / NXCD;
+}
+
 // Fixed-point scale of the PME charge grid (2^50, quantum 8.9e-16): integer
 // atomics make the spread order-independent, hence deterministic.  Grid values
 // must stay below 2^13 = 8192 in magnitude (a grid point collects a few atoms'

@@ -866,7 +866,7 @@ __global__ void __launch_bounds__(BLOCK) k_fft_zp(Geom g, const float2* __restri
   C* twl = b1 + NL * S;
   load_twiddles(twl, tw, KZ);
   const int H = g.H, nlines = g.K[1] * H;
-  const int line0 = blockIdx.x * NL;
+  const int line0 = xcd_tile(blockIdx.x, gridDim.x) * NL;
   staged<8, C>(
       NL * KZ,
       [&](int e) {
@@ -1204,7 +1204,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
   __shared__ int bbeg[MAXB];
   __shared__ int wsum[BLOCK / WAVE];
   const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
-  const int t = blockIdx.x;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   const int lo[3] = {tx * TX, ty * TY, tz * TZ};
   const int ext[3] = {min(TX, g.K[0] - lo[0]), min(TY, g.K[1] - lo[1]), min(TZ, g.K[2] - lo[2])};
@@ -1375,7 +1375,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   __shared__ int wsum[BLOCK / WAVE];
   __shared__ double wmax[BLOCK / WAVE];
   const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
-  const int t = blockIdx.x;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   const int lo[3] = {tx * TX, ty * TY, tz * TZ};
   const int ext[3] = {min(TX, g.K[0] - lo[0]), min(TY, g.K[1] - lo[1]), min(TZ, g.K[2] - lo[2])};
@@ -1514,7 +1514,7 @@ __global__ void __launch_bounds__(BLOCK) k_gather_tile(Geom g, const double* __r
   __shared__ int bbeg[GBRICKS];
   __shared__ int wsum[BLOCK / WAVE];
   const int ntx = (g.K[0] + GX - 1) / GX, nty = (g.K[1] + GY - 1) / GY;
-  const int t = blockIdx.x;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   const int lo[3] = {tx * GX, ty * GY, tz * GZ};
   const int ext[3] = {min(GX, g.K[0] - lo[0]), min(GY, g.K[1] - lo[1]), min(GZ, g.K[2] - lo[2])};
