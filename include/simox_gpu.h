@@ -170,19 +170,21 @@ int simox_set_charge_tol(simox_ctx* ctx, double tol, int maxit);
 int simox_set_charge_precond(simox_ctx* ctx, int on);
 /* Reciprocal part of the MIXED mode's inner operator on a coarse grid: half
    the resolution of the context's grid in every dimension, order-4
-   B-splines, its own influence table, and the sparse part from FP16 matrix
-   values without the entries below 1e-3 eV/e^2 (the far real-space Coulomb
-   tail; about a third of the entries stay) (on = 1), or on the context's
+   B-splines, its own influence table, and the sparse part from 12-bit
+   matrix values packed with their 20-bit columns in one 32-bit word (up to
+   2^20 atoms; FP16 values with 4-byte columns above) without the entries
+   below 1e-3 eV/e^2 (the far real-space Coulomb tail; about a quarter of
+   the entries stay) (on = 1), or on the context's
    grid with all FP32 values (on = 0).  The
    inner solves then stop at 3e-2 (else 1e-4) of their restart's residual;
    every restart evaluates the true gradient with the context's grid and the
    FP64 matrix, so the converged charges meet the tolerance for the
    reference's operator either way.  Default: on in MIXED from 2^23 grid
-   points (C3, 256 x 256 x 512: 361 -> 200 ms per step), off below (C2, 96 x
+   points (C3, 256 x 256 x 512: 361 -> 148 ms per step), off below (C2, 96 x
    96 x 192, is slower with it).  SIMOX_ERR_ARG outside MIXED. */
 int simox_set_inner_grid(simox_ctx* ctx, int on);
 /* Entries of the matrix the inner solves' sparse part reads: the compacted
-   FP16 matrix of the coarse inner operator (entries with |H0_ij| >= 1e-3
+   matrix of the coarse inner operator (entries with |H0_ij| >= 1e-3
    eV/e^2; SIMOX_INNER_DROP sets the bound, 0 keeps all), else the full list.
    Valid after a charge solve. */
 int simox_inner_entries(simox_ctx* ctx, long long* entries);
