@@ -253,6 +253,43 @@ class Oracle:
                     Edir=out["Edir"], Ectip=out["Ectip"], Egroup=out["Egroup"])
 
 
+LINE_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+GRAD_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def brent(f):
+    """BRENT_SOLVER's scalar search (orc_brent) on a Python f(alpha):
+    returns (alpha, evaluations); raises where the reference STOPs."""
+    cb = LINE_FN(lambda ctx, a: float(f(a)))
+    alpha = C.c_double()
+    nev = C.c_int()
+    err = lib().orc_brent(cb, None, C.byref(alpha), C.byref(nev))
+    if err:
+        raise RuntimeError("Brent solver limit breached (%d evaluations)" % nev.value)
+    return alpha.value, nev.value
+
+
+def ncg(grad, q0, maxit=0):
+    """CTIP_SOLVER's nonlinear CG loop (orc_ncg) on a Python grad(q) -> g =
+    -dU/dq: returns (q, iterations, gradient evaluations)."""
+    n = len(q0)
+
+    def _g(ctx, qp, gp):
+        q = np.ctypeslib.as_array(qp, shape=(n,)).copy()
+        g = np.asarray(grad(q), dtype=np.float64)
+        np.ctypeslib.as_array(gp, shape=(n,))[:] = g
+
+    cb = GRAD_FN(_g)
+    q = np.array(q0, dtype=np.float64)
+    it = C.c_int()
+    nev = C.c_int()
+    err = lib().orc_ncg(C.c_int(n), cb, None, _d(q), C.c_int(int(maxit)), C.byref(it),
+                        C.byref(nev))
+    if err:
+        raise RuntimeError("Brent solver limit breached in the CG loop")
+    return q, it.value, nev.value
+
+
 def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None,
                   t_given=600.0, alpha=0.1, zshift=0.0):
     """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90.

@@ -849,28 +849,28 @@ static void grad(ctip_ws* w, const double* q, double* g) {
   w->n_eval++;
 }
 
-/* BRENT_SOLVER, ctip.f90:216-304 */
-static int brent(ctip_ws* w, const double* q, const double* pdir, double* alpha) {
-  const int n = w->s->n;
+/* BRENT_SOLVER, ctip.f90:216-304: the scalar root search of
+   f(alpha) = -g(q + alpha p) . p on [-1, 1], each f a callback.  Its bracket
+   bookkeeping, secant / inverse-quadratic steps and its two tolerances
+   (E_crit = 1e-8 on the half bracket, E_end = 1e-11 on |f|) follow the
+   reference line for line; *n_eval counts the callbacks. */
+int orc_brent(orc_line_fn f, void* ctx, double* alpha, int* n_eval) {
   const double E_crit = 1.0e-8, E_end = 1.0e-11;
-  double* ql = malloc(sizeof(double) * n);
-  double* g = malloc(sizeof(double) * n);
   double fa, fb, fc, x, y, z, a, b, c, d = 0.0, e = 0.0, xm, sgn;
+  int ne = 0;
   a = -1.0;
-  for (int i = 0; i < n; ++i) ql[i] = q[i] + a * pdir[i];
-  grad(w, ql, g);
-  fa = 0.0; for (int i = 0; i < n; ++i) fa = fa - g[i] * pdir[i];
+  fa = f(ctx, a); ne++;
   b = 1.0;
-  for (int i = 0; i < n; ++i) ql[i] = q[i] + b * pdir[i];
-  grad(w, ql, g);
-  fb = 0.0; for (int i = 0; i < n; ++i) fb = fb - g[i] * pdir[i];
+  fb = f(ctx, b); ne++;
   c = b; fc = fb;
   for (int it = 1; it <= 100; ++it) {
     if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) { c = a; fc = fa; d = b - a; e = d; }
     if (fabs(fc) < fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
     xm = 0.5 * (c - b);
     if (fabs(xm) < E_crit || fabs(fb) < E_end) {
-      *alpha = b; free(ql); free(g); return ORC_OK;
+      *alpha = b;
+      if (n_eval) *n_eval = ne;
+      return ORC_OK;
     }
     if (fabs(e) >= E_crit && fabs(fa) > fabs(fb)) {
       double s = fb / fa;
@@ -889,13 +889,72 @@ static int brent(ctip_ws* w, const double* q, const double* pdir, double* alpha)
     a = b; fa = fb;
     sgn = copysign(E_crit, xm);
     if (fabs(d) > E_crit) b = b + d; else b = b + sgn;
-    for (int i = 0; i < n; ++i) ql[i] = q[i] + b * pdir[i];
-    grad(w, ql, g);
-    fb = 0.0; for (int i = 0; i < n; ++i) fb = fb - g[i] * pdir[i];
+    fb = f(ctx, b); ne++;
   }
-  free(ql); free(g);
+  if (n_eval) *n_eval = ne;
   return ORC_ERR_BRENT;
 }
+
+/* The nonlinear CG loop of CTIP_SOLVER, ctip.f90:167-190, on a gradient
+   callback grad(ctx, q, g) that returns g = -dU/dq (dUdq's sign):
+   beta = sum(g) / sum(g_old) (plain sums, not squared norms; quirk Q6),
+   d = g + beta p, p = d - mean(d), alpha from BRENT_SOLVER on
+   f(alpha) = -g(q + alpha p) . p, q += alpha p, stop when
+   max|p| |alpha| <= 1e-3 (E_crit, ctip.f90:26).  maxit > 0 stops after that
+   many iterations as well (0: no cap).  *n_eval counts gradient calls. */
+typedef struct {
+  int n; orc_grad_fn grad; void* ctx; const double* q; const double* p; double* ql; double* g;
+  int n_eval;
+} line_ws;
+
+static double line_f(void* vw, double alpha) {
+  line_ws* w = (line_ws*)vw;
+  for (int i = 0; i < w->n; ++i) w->ql[i] = w->q[i] + alpha * w->p[i];
+  w->grad(w->ctx, w->ql, w->g);
+  w->n_eval++;
+  double fb = 0.0;
+  for (int i = 0; i < w->n; ++i) fb = fb - w->g[i] * w->p[i];
+  return fb;
+}
+
+int orc_ncg(int n, orc_grad_fn grad, void* ctx, double* qi, int maxit, int* iters, int* n_eval) {
+  double* g = malloc(sizeof(double) * n);
+  double* d = malloc(sizeof(double) * n);
+  double* pdir = calloc(n, sizeof(double));
+  double* ql = malloc(sizeof(double) * n);
+  double* gl = malloc(sizeof(double) * n);
+  line_ws lw = {n, grad, ctx, qi, pdir, ql, gl, 0};
+  int k = 0, err = ORC_OK, ne = 0;
+  double Error = 1.0, g_sum = 1.0, g_sum_old, d_sum, alpha = 0.0;
+  while (Error > 1.0e-3 && (maxit <= 0 || k < maxit)) {
+    grad(ctx, qi, g); ne++;
+    g_sum_old = g_sum; g_sum = 0.0;
+    for (int i = 0; i < n; ++i) g_sum += g[i];
+    for (int i = 0; i < n; ++i) d[i] = g[i] + pdir[i] * g_sum / g_sum_old;
+    d_sum = 0.0;
+    for (int i = 0; i < n; ++i) d_sum += d[i];
+    d_sum = d_sum / (double)n;
+    for (int i = 0; i < n; ++i) pdir[i] = d[i] - d_sum;
+    int nb = 0;
+    lw.n_eval = 0;
+    err = orc_brent(line_f, &lw, &alpha, &nb);
+    ne += lw.n_eval;
+    if (err) break;
+    for (int i = 0; i < n; ++i) qi[i] = qi[i] + alpha * pdir[i];
+    double pm = 0.0;
+    for (int i = 0; i < n; ++i) if (fabs(pdir[i]) > pm) pm = fabs(pdir[i]);
+    Error = pm * fabs(alpha);
+    k++;
+  }
+  *iters = k;
+  if (n_eval) *n_eval = ne;
+  free(g); free(d); free(pdir); free(ql); free(gl);
+  return err;
+}
+
+/* PHI_MESH x c_pme + dUdq: the gradient CTIP_SOLVER's loop and BRENT_SOLVER
+   evaluate (ctip.f90:170-171, 232-233) */
+static void ctip_grad_cb(void* vw, const double* q, double* g) { grad((ctip_ws*)vw, q, g); }
 
 static void xjz_fill(const orc_sys* s, const orc_params* p, const int* id, double* xjz,
                      double* qmx) {
@@ -921,9 +980,6 @@ int orc_ctip_solve(const orc_sys* s, const orc_params* p, const int* id, const d
   double* asym = malloc(sizeof(double) * (size_t)n * ORC_NNB_MAX);
   double* sym = malloc(sizeof(double) * (size_t)n * ORC_NNB_MAX);
   double* phi = malloc(sizeof(double) * n);
-  double* g = malloc(sizeof(double) * n);
-  double* d = malloc(sizeof(double) * n);
-  double* pdir = calloc(n, sizeof(double));
   double* qi = malloc(sizeof(double) * n);
   xjz_fill(s, p, id, xjz, qmx);
   int err = orc_ctip_matrix(s, p, id, x, npt, link, pair, nlist, list, asym, sym);
@@ -931,27 +987,8 @@ int orc_ctip_solve(const orc_sys* s, const orc_params* p, const int* id, const d
   w.list = list; w.nlist = nlist; w.phi = phi;
   w.c_pme = KC * (double)s->fft[0] * (double)s->fft[1] * (double)s->fft[2] / V / PI_;
   for (int i = 0; i < n; ++i) qi[i] = q[i];
-  int k = 0;
-  if (!err) {
-    double Error = 1.0, g_sum = 1.0, g_sum_old, d_sum, alpha = 0.0;
-    while (Error > 1.0e-3) { /* E_crit, ctip.f90:26 */
-      grad(&w, qi, g);
-      g_sum_old = g_sum; g_sum = 0.0;
-      for (int i = 0; i < n; ++i) g_sum += g[i];
-      for (int i = 0; i < n; ++i) d[i] = g[i] + pdir[i] * g_sum / g_sum_old;
-      d_sum = 0.0;
-      for (int i = 0; i < n; ++i) d_sum += d[i];
-      d_sum = d_sum / (double)n;
-      for (int i = 0; i < n; ++i) pdir[i] = d[i] - d_sum;
-      err = brent(&w, qi, pdir, &alpha);
-      if (err) break;
-      for (int i = 0; i < n; ++i) qi[i] = qi[i] + alpha * pdir[i];
-      double pm = 0.0;
-      for (int i = 0; i < n; ++i) if (fabs(pdir[i]) > pm) pm = fabs(pdir[i]);
-      Error = pm * fabs(alpha);
-      k++;
-    }
-  }
+  int k = 0, ne = 0;
+  if (!err) err = orc_ncg(n, ctip_grad_cb, &w, qi, 0, &k, &ne);
   double Eself = 0.0, Esum = 0.0;
   for (int i = 0; i < n; ++i) {
     q[i] = qi[i];
@@ -965,8 +1002,7 @@ int orc_ctip_solve(const orc_sys* s, const orc_params* p, const int* id, const d
   *eself = Eself * KC * s->a / SQRTPI + Esum;
   *iters = k;
   *n_eval = w.n_eval;
-  free(xjz); free(qmx); free(nlist); free(list); free(asym); free(sym); free(phi); free(g);
-  free(d); free(pdir); free(qi);
+  free(xjz); free(qmx); free(nlist); free(list); free(asym); free(sym); free(phi); free(qi);
   return err;
 }
 

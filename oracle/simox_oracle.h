@@ -22,10 +22,15 @@
  *     unity, central finite differences of E_rec.
  *   - orc_ctip_solve: KKT conditions of the reference's own charge energy
  *     (dense numpy solve on small systems), finite differences of U(q) vs dUdq,
- *     charge conservation.  The exact nonlinear-CG/Brent iterate sequence of the
- *     reference (ctip.f90:167-304) cannot be pinned by the reference build,
- *     because the reference needs the absent FFTE library: parity unpinned for
- *     the iterate sequence itself, pinned for its fixed point.
+ *     charge conservation.  The reference's own CG/Brent cannot be built here
+ *     (PHI_MESH needs the absent FFTE library), so the iterate sequence is
+ *     pinned piecewise by closed forms (tests/test_oracle_ctip_pin.py):
+ *     orc_brent on functions where one secant or inverse-quadratic step is
+ *     exact, on a step function (bisection count fixes E_crit) and at the
+ *     E_end exit; orc_ncg on 2- and 3-atom quadratic energies whose first
+ *     iterates are known in exact arithmetic (beta = sum g / sum g_old, mean
+ *     projection, the 1e-3 stop); and orc_ctip_solve equals orc_ncg on the
+ *     pinned PHI_MESH + dUdq gradient bit for bit.
  */
 #ifndef SIMOX_ORACLE_H
 #define SIMOX_ORACLE_H
@@ -117,6 +122,15 @@ void orc_phi_mesh(const orc_sys* s, const double* x, const double* q, const doub
 int orc_ctip_solve(const orc_sys* s, const orc_params* p, const int* id, const double* x,
                    const int* npt, const int* link, const int* pair, const double* psi,
                    double* q, double* eself, int* iters, int* n_eval);
+/* The two pieces of that solve on callbacks, so that they can be driven by
+   closed-form functions: BRENT_SOLVER's scalar root search (ctip.f90:216-304)
+   of f on [-1, 1], and the nonlinear CG loop (ctip.f90:167-190) on a
+   gradient g(q) = -dU/dq; maxit > 0 caps the iterations (0: the reference's
+   1e-3 stop only). */
+typedef double (*orc_line_fn)(void* ctx, double alpha);
+typedef void (*orc_grad_fn)(void* ctx, const double* q, double* g);
+int orc_brent(orc_line_fn f, void* ctx, double* alpha, int* n_eval);
+int orc_ncg(int n, orc_grad_fn grad, void* ctx, double* q, int maxit, int* iters, int* n_eval);
 /* CTIP sparse matrices as CTIP_SOLVER builds them (ctip.f90:61-165). */
 int orc_ctip_matrix(const orc_sys* s, const orc_params* p, const int* id, const double* x,
                     const int* npt, const int* link, const int* pair,
