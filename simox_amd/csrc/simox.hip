@@ -119,6 +119,9 @@ struct simox_ctx {
   int *count = nullptr, *start = nullptr, *fill = nullptr, *slot = nullptr, *order = nullptr;
   double *rho = nullptr, *dF64 = nullptr, *frec = nullptr;
   float* dF32 = nullptr;
+  // MIXED: pairs closer than R_NEAR per atom (count, NEAR_CAP columns
+  // column-major), evaluated in FP64 by k_near_density / k_force_near
+  int *near_cnt = nullptr, *near_col = nullptr;
   // PME: the reference's grid (pm) and the half-resolution grid of the charge
   // solver's long-range preconditioner (pc, set up on first use)
   PmeGrid pm, pc;
@@ -479,14 +482,20 @@ int ensure_nlist(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
   hipEvent_t ev{};
   tstart(ctx, 8, &ev);
-  if (use32(ctx))
+  if (use32(ctx)) {
     hipLaunchKernelGGL(k_nlist_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
-                       ctx->nb_col, ctx->rho, ctx->dF32, ctx->part_eg, ctx->derr);
-  else
+                       ctx->nb_col, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF32,
+                       ctx->part_eg, ctx->derr);
+    hipLaunchKernelGGL(k_near_density, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF32,
+                       ctx->part_eg);
+  } else {
     hipLaunchKernelGGL(k_nlist_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
-                       ctx->nb_col, ctx->rho, ctx->dF64, ctx->part_eg, ctx->derr);
+                       ctx->nb_col, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF64,
+                       ctx->part_eg, ctx->derr);
+  }
   tstop(ctx, 8, ev);
   HIPCHK(hipGetLastError());
   ctx->nlist_valid = true;
@@ -498,23 +507,30 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
   CHK(ensure_nlist(ctx));
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part_eg, nb, 1, 0,
                      ctx->dscal + 0);
+  // block partials of k_force in rows [0, nb), of k_force_near in [nb, 2 nb)
+  CHK(ensure_part(ctx, size_t(nb) * 7));
+  double* ep = ctx->part + nb;
   hipEvent_t ev{};
   tstart(ctx, 7, &ev);
   unsigned long long* rdf = ctx->rdf_on ? ctx->rdf : nullptr;
   const size_t sh = rdf ? RDF_BINS * sizeof(unsigned) : 0;
   const double dr = ctx->rdf_dr;
-  if (use32(ctx))
+  if (use32(ctx)) {
     hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
                        ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF32, ctx->f,
-                       ctx->part + nb, rdf, dr);
-  else
+                       ep, rdf, dr);
+    hipLaunchKernelGGL(k_force_near, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->q, ctx->id, ctx->near_cnt, ctx->near_col, ctx->dF32, ctx->f,
+                       ep + size_t(3) * nb);
+  } else {
     hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
                        ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF64, ctx->f,
-                       ctx->part + nb, rdf, dr);
+                       ep, rdf, dr);
+  }
   tstop(ctx, 7, ev);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part + nb, nb, 3, 0,
-                     ctx->dscal + 1);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ep,
+                     use32(ctx) ? 2 * nb : nb, 3, 0, ctx->dscal + 1);
   HIPCHK(hipGetLastError());
   if (fetch_energies) {
     double e[4];
@@ -1911,6 +1927,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->count, g.ncall)); CHK(dalloc(ctx, &ctx->fill, g.ncall));
   CHK(dalloc(ctx, &ctx->start, g.ncall + 1));
   CHK(dalloc(ctx, &ctx->rho, n)); CHK(dalloc(ctx, &ctx->dF64, n)); CHK(dalloc(ctx, &ctx->dF32, n));
+  CHK(dalloc(ctx, &ctx->near_cnt, n)); CHK(dalloc(ctx, &ctx->near_col, size_t(NEAR_CAP) * n));
   CHK(dalloc(ctx, &ctx->frec, n3));
   CHK(dalloc(ctx, &ctx->qc, n)); CHK(dalloc(ctx, &ctx->ql, n)); CHK(dalloc(ctx, &ctx->gv, n));
   CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
@@ -2001,7 +2018,7 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->near_cnt, ctx->near_col, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
