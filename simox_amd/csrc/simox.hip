@@ -252,8 +252,10 @@ hipEvent_t tevent(simox_ctx* ctx) {
     ctx->evfree.pop_back();
     return e;
   }
+  // no system-scope fence at the records: the default event's cache writeback
+  // and invalidation would slow the timed kernel itself (cold L2)
   hipEvent_t e{};
-  (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 void tstart(simox_ctx* ctx, int which, hipEvent_t* a) {
@@ -774,8 +776,8 @@ int choose_fft(simox_ctx* ctx, const Geom& g, FftCfg* c) {
 template <class F>
 int time_launch(simox_ctx* ctx, F launch, float* ms) {
   hipEvent_t a{}, b{};
-  HIPCHK(hipEventCreate(&a));
-  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+  HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
   launch();
   HIPCHK(hipEventRecord(a, ctx->st));
   for (int r = 0; r < 3; ++r) launch();
