@@ -204,6 +204,20 @@ int simox_cell_sort(simox_ctx* ctx);
 /* Cell occupancy (Npt per cell, 0-based cell id = ix + iy*Nx + iz*Nx*Ny) and
    members in link order (caller indices, 0-based); link may be NULL. */
 int simox_get_cells(simox_ctx* ctx, int* npt, int* link, int link_stride);
+/* No counterpart in the reference: CELL_SORT for callers that cannot allocate
+   device memory (code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc).  Same binning and link order as simox_cell_sort +
+   simox_get_cells, without a context: x = device pointer to 3n doubles in
+   caller order; box, rcut on the host (Ncell = INT(box/rcut) >= 2 per axis);
+   outputs are caller-owned device buffers: npt[ncell] atoms per cell (cell id
+   ix + iy*Nx + iz*Nx*Ny), start[ncell+1] their exclusive prefix sums, link[n]
+   the caller indices (0-based) in cell order, ascending within a cell.
+   Allocates nothing: its scratch is a static device array of 16.5 MiB per GPU
+   (held until the process ends), so n <= 2^21 and ncell <= 2^17, else
+   SIMOX_ERR_ARG.  Synchronous (null stream).  Calls on one GPU share that
+   scratch: calls that overlap on one GPU race. */
+int simox_cell_sort_noalloc(int n, const double box[3], double rcut, const double* x, int* npt,
+                            int* start, int* link);
 
 /* FORCE (force.f90:3-361): overwrites forces with EAM + real-space Ewald +
    CTIP short-range forces.  energies[4] = Epair, Edir, Ectip, Egroup (eV),

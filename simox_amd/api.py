@@ -27,6 +27,7 @@ EXPORTS = [
     "simox_param_force_grad",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
+    "simox_cell_sort_noalloc",
 ]
 
 # simox_kernel_timing categories
@@ -125,6 +126,7 @@ def load_library(path=LIB_PATH):
     lib.simox_charge_stats.argtypes = [vp, P(i), P(i)]
     lib.simox_potential_setup.argtypes = [vp]
     lib.simox_fafb.argtypes = [vp]
+    lib.simox_cell_sort_noalloc.argtypes = [i, P(d), d, vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -417,6 +419,20 @@ class Simox:
         cnt = C.c_longlong()
         self._check(self.lib.simox_kernel_timing(self.h, int(which), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+def cell_sort_noalloc(x, box, rcut, npt, start, link):
+    """simox_cell_sort_noalloc on device buffers given as addresses (int):
+    x (3n float64), npt (ncell int32), start (ncell + 1 int32), link (n
+    int32).  No allocation and no context; returns nothing, raises on a
+    nonzero status."""
+    lib = load_library()
+    n = int(x[1])
+    b = (C.c_double * 3)(*[float(v) for v in box])
+    rc = lib.simox_cell_sort_noalloc(n, b, float(rcut), C.c_void_p(x[0]), C.c_void_p(npt),
+                                     C.c_void_p(start), C.c_void_p(link))
+    if rc:
+        raise SimoxError(rc, "simox_cell_sort_noalloc failed with status %d" % rc)
 
 
 def batch_step(sims, nsteps, dt, sort_every=1, ctip_every=1, stats=True):

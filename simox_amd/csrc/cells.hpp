@@ -70,7 +70,8 @@ __global__ void k_cell_scatter(int n, const int* __restrict__ cell, const int* _
 }
 
 // Order each cell's members by caller index (rank counting in LDS; a cell holds
-// at most NPT_MAX atoms), giving order[new_pos] = old_pos.
+// at most NPT_MAX atoms), giving order[new_pos] = old_pos.  perm = null: the
+// positions are in caller order already (the key is the position itself).
 __global__ void __launch_bounds__(BLOCK) k_cell_order(const int* __restrict__ start,
                                                       const int* __restrict__ slot_src,
                                                       const int* __restrict__ perm,
@@ -83,7 +84,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_order(const int* __restrict__ st
   if (m > NPT_MAX) return;  // reported by k_cell_scan
   for (int t = threadIdx.x; t < m; t += blockDim.x) {
     src[t] = slot_src[s0 + t];
-    key[t] = perm[src[t]];
+    key[t] = perm ? perm[src[t]] : src[t];
   }
   __syncthreads();
   for (int t = threadIdx.x; t < m; t += blockDim.x) {

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "cells.hpp"
@@ -1779,6 +1780,40 @@ bool need_state(simox_ctx* ctx) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of simox_cell_sort_noalloc: per-atom cell index and slot, per-cell
+// fill counter, error flag.  One copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address comes from hipGetSymbolAddress (noalloc_scratch) and only
+// that address is passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_ATOMS = 1 << 21;  // 2,097,152 atoms
+constexpr int NOALLOC_MAX_CELLS = 1 << 17;  // 131,072 cells
+constexpr size_t NOALLOC_WORDS = 2 * size_t(NOALLOC_MAX_ATOMS) + NOALLOC_MAX_CELLS + 64;
+static __device__ __align__(256) int scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex noalloc_mu;
+int* noalloc_ptr[NOALLOC_MAX_DEV] = {};
+
+// device address of `scratch` on the current device (looked up once per
+// device, with that device current)
+int noalloc_scratch(int** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) return SIMOX_ERR_HIP;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    int* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return SIMOX_ERR_HIP;
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return SIMOX_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -2732,6 +2767,49 @@ int simox_kernel_timing(simox_ctx* ctx, int which, double* ms, long long* launch
   if (ms) *ms = ctx->tms[which];
   if (launches) *launches = ctx->tcnt[which];
   return SIMOX_OK;
+}
+
+// CELL_SORT (cellsort.f90:53-80) without a context and without allocating:
+// the kernels of simox_cell_sort on caller-owned device buffers plus the
+// static scratch above.  Calls on one GPU share that scratch and race if
+// they overlap.
+int simox_cell_sort_noalloc(int n, const double box[3], double rcut, const double* x, int* npt,
+                            int* start, int* link) {
+  if (n < 1 || n > NOALLOC_MAX_ATOMS || !box || !x || !npt || !start || !link || !(rcut > 0.0))
+    return SIMOX_ERR_ARG;
+  Geom g{};
+  g.n = n;
+  for (int d = 0; d < 3; ++d) {
+    if (!(box[d] > 0.0)) return SIMOX_ERR_ARG;
+    g.nc[d] = int(box[d] / rcut);  // main.f90:189
+    if (g.nc[d] < 2) return SIMOX_ERR_ARG;  // main.f90:191-193
+    g.lcell[d] = box[d] / double(g.nc[d]);  // main.f90:190
+    g.box[d] = box[d];
+    g.hbox[d] = 0.5 * box[d];
+  }
+  const long long ncall = (long long)g.nc[0] * g.nc[1] * g.nc[2];
+  if (ncall > NOALLOC_MAX_CELLS) return SIMOX_ERR_ARG;
+  g.ncall = int(ncall);
+  int* sc = nullptr;
+  if (noalloc_scratch(&sc)) return SIMOX_ERR_HIP;
+  int* cell = sc;
+  int* slot = sc + NOALLOC_MAX_ATOMS;
+  int* fill = sc + 2 * size_t(NOALLOC_MAX_ATOMS);
+  int* err = fill + NOALLOC_MAX_CELLS;
+  const int nb = grid_blocks(n);
+  // counters start at zero on every call (fill kernels, no memset nodes)
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(g.ncall)), dim3(BLOCK), 0, nullptr, g.ncall, 0,
+                     npt);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, nullptr, 1, 0, err);
+  hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(BLOCK), 0, nullptr, g, x, cell, npt);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, nullptr, g.ncall, npt, start, fill, err);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(BLOCK), 0, nullptr, n, cell, start, fill, slot);
+  hipLaunchKernelGGL(k_cell_order, dim3(g.ncall), dim3(BLOCK), 0, nullptr, start, slot, nullptr,
+                     link);
+  if (hipGetLastError() != hipSuccess) return SIMOX_ERR_HIP;
+  int flag = 0;
+  if (hipMemcpy(&flag, err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return SIMOX_ERR_HIP;
+  return flag ? flag : SIMOX_OK;
 }
 
 }  // extern "C"
