@@ -64,16 +64,20 @@ def build_dropin_demo(ref="/root/reference"):
     fdir = os.path.join(HERE, "fortran")
     bdir = os.path.join(fdir, "_build", "dropin")
     os.makedirs(bdir, exist_ok=True)
-    objs = []
+    objs = {}
     for src in [os.path.join(ref, f + ".f90") for f in ("datafmt", "eam", "vverlet", "util")] + \
             [os.path.join(fdir, "simox_gpu.f90"), os.path.join(fdir, "simox_dropin.f90"),
              os.path.join(ref, "main.f90")]:
         o = os.path.join(bdir, os.path.basename(src).replace(".f90", ".o"))
         _run([FC, "-O2", "-c", src, "-o", o, "-module-dir", bdir], cwd=bdir)
-        objs.append(o)
+        objs[os.path.basename(src)[:-4]] = o
+    # simox_dropin.o ahead of vverlet.o: its VV_NVE1 / VV_NVE2 are the
+    # definitions the link keeps (first definition wins), the rest of
+    # vverlet.f90 (VINIT, thermostats, the reflecting half) stays the reference's
+    order = ["datafmt", "eam", "simox_gpu", "simox_dropin", "vverlet", "util", "main"]
     exe = os.path.join(fdir, "_build", "simox_dropin_demo")
-    _run([FC, "-O2", "-o", exe, *objs, "-L" + HERE, "-lsimox_gpu", "-Wl,-rpath," + HERE,
-          "-Wl,-rpath,$ORIGIN/../.."], cwd=bdir)
+    _run([FC, "-O2", "-o", exe, *[objs[k] for k in order], "-Wl,--allow-multiple-definition",
+          "-L" + HERE, "-lsimox_gpu", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../.."], cwd=bdir)
     return exe
 
 

@@ -1354,19 +1354,26 @@ __global__ void k_brick_records(Geom g, int n, const double* __restrict__ x,
   rec[f] = make_float4(z[0], z[1], z[2], __uint_as_float(pk));
 }
 
-// K6 for the MIXED inner operator's coarse grid: the same tiles, bricks and
-// order-4 B-splines with FP32 weights and 32-bit fixed point.  Each tile
-// owns its points and writes each once, so it takes its own scale: 2^S with
-// 8 max|v| 2^S < 2^31 over the values it visits (a grid point's sum is at
-// most 16 atoms x 8/27 of max|v|), resolution 2^-S, about 4e-9 of max|v|.
-// Integer sums keep the grid independent of the order of the atomics; the
-// LDS tile is half the size of the 64-bit one (16 KB).
+// K6 in FP32 for the MIXED mode's coarse grids (the inner operator's,
+// order-4 B-splines, and the preconditioner's, ORD 2: linear weights w0 = z0,
+// w1 = 1 - z0 at b, b + 1): the same tiles and bricks with FP32 weights and
+// 32-bit fixed point.  Each tile owns its points and writes each once, so it
+// takes its own scale 2^S, from a bound that no grid point's sum can exceed:
+// a point's stencil atoms have their bases in at most two bricks per axis,
+// so at most 8 bricks of the tile's candidates reach it, and one atom adds at
+// most wmax |v| (wmax = (2/3)^3 for the B-splines, 1 for linear weights):
+// 8 maxcnt wmax max|v| 2^S < 2^30 (maxcnt: the fullest candidate brick; the
+// factor 2 left to 2^31 covers the rounding of each product).  Resolution
+// 2^-S, a few 1e-8 of max|v|.  Integer sums keep the grid independent of the
+// order of the atomics; the LDS tile is half the size of the 64-bit one.
+template <int ORD>
 __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double* __restrict__ x,
                                                            const double* __restrict__ q,
                                                            const int* __restrict__ bstart,
                                                            const int* __restrict__ border,
                                                            const float4* __restrict__ rec,
                                                            float* __restrict__ grid) {
+  static_assert(ORD == 2 || ORD == 4, "linear or cubic B-spline assignment");
   __shared__ int acc[TILE];
   __shared__ unsigned short ctab[SPREAD_MAXC];  // candidate f: (candidate brick << 8) | offset
   __shared__ int tab_bad;
@@ -1374,6 +1381,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   __shared__ int bbeg[MAXB];
   __shared__ int wsum[BLOCK / WAVE];
   __shared__ double wmax[BLOCK / WAVE];
+  __shared__ int wcnt[BLOCK / WAVE];
   const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
@@ -1383,7 +1391,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   int fb[3], cb[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int a = lo[d] - 2, b = lo[d] + ext[d];
+    // bases whose stencil meets the tile: lo-2 .. hi+1 (ORD 4), lo-1 .. hi (ORD 2)
+    const int a = lo[d] - (ORD == 4 ? 2 : 1), b = lo[d] + ext[d] - (ORD == 4 ? 0 : 1);
     int f0 = (a >= 0) ? a / BRK : -((-a + BRK - 1) / BRK);
     int l0 = b / BRK;
     int c = l0 - f0 + 1;
@@ -1406,12 +1415,15 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
     cntk = bstart[bid + 1] - b0;
   }
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  int cmax = cntk;
 #pragma unroll
   for (int o = 1; o < WAVE; o <<= 1) {
     const int v = __shfl_up(cntk, o, WAVE);
     if (lane >= o) cntk += v;
+    cmax = max(cmax, __shfl_xor(cmax, o, WAVE));
   }
   if (lane == WAVE - 1) wsum[wid] = cntk;
+  if (lane == 0) wcnt[wid] = cmax;
   if (threadIdx.x == 0) bofs[0] = 0;
   __syncthreads();
   for (int w = 0; w < wid; ++w) cntk += wsum[w];
@@ -1421,17 +1433,23 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   const bool table = spread_slot_table(ncand, total, bofs, ctab, &tab_bad);
   auto slot_of = [&](int f) { return spread_slot(f, table, ncand, bofs, bbeg, ctab); };
   auto atom_of = [&](int f) { return border[slot_of(f)]; };
-  // the tile's scale from the largest |q| it visits
+  // the tile's scale from the largest |q| it visits and its fullest brick
   double m = 0.0;
   for (int f = threadIdx.x; f < total; f += blockDim.x) m = fmax(m, fabs(q[atom_of(f)]));
   m = wave_max(m);
   if (lane == 0) wmax[wid] = m;
   __syncthreads();
   m = wmax[0];
-  for (int w = 1; w < BLOCK / WAVE; ++w) m = fmax(m, wmax[w]);
+  int maxcnt = wcnt[0];
+  for (int w = 1; w < BLOCK / WAVE; ++w) {
+    m = fmax(m, wmax[w]);
+    maxcnt = max(maxcnt, wcnt[w]);
+  }
+  constexpr double WMAX = ORD == 4 ? 0.3 : 1.0;  // (2/3)^3 = 0.296, rounded up
+  const double bound = 8.0 * double(maxcnt) * WMAX * m;
   int e = 0;
-  (void)frexp(8.0 * m, &e);              // 8 m < 2^e
-  const int S = m > 0.0 ? 31 - e : 0;    // 8 m 2^S < 2^31
+  (void)frexp(bound, &e);                 // bound < 2^e
+  const int S = bound > 0.0 ? 30 - e : 0;  // bound 2^S < 2^30
   const double scale = ldexp(1.0, S);
   for (int f = threadIdx.x; f < total; f += blockDim.x) {
     const int sl = slot_of(f);
@@ -1439,24 +1457,29 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
     const float4 rc = rec[sl];
     const float zr[3] = {rc.x, rc.y, rc.z};
     const unsigned pk = __float_as_uint(rc.w);
-    float AX[3][4];
-    int o[3][4];
-    bool in[3][4];
+    float AX[3][ORD];
+    int o[3][ORD];
+    bool in[3][ORD];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      // order-4 B-spline weights of the fractional offset (bspline4_w in FP32)
       const float z0 = zr[d];
-      const float z1 = z0 + 1.f, z2 = z0 + 2.f, z3 = z0 + 3.f;
-      constexpr float s6 = 1.f / 6.f;
-      AX[d][0] = z0 * z0 * z0 * s6;
-      AX[d][1] = (((-3.f * z1 + 12.f) * z1 - 12.f) * z1 + 4.f) * s6;
-      AX[d][2] = (((3.f * z2 - 24.f) * z2 + 60.f) * z2 - 44.f) * s6;
-      AX[d][3] = (((-z3 + 12.f) * z3 - 48.f) * z3 + 64.f) * s6;
+      if constexpr (ORD == 4) {
+        // order-4 B-spline weights of the fractional offset (bspline4_w in FP32)
+        const float z1 = z0 + 1.f, z2 = z0 + 2.f, z3 = z0 + 3.f;
+        constexpr float s6 = 1.f / 6.f;
+        AX[d][0] = z0 * z0 * z0 * s6;
+        AX[d][1] = (((-3.f * z1 + 12.f) * z1 - 12.f) * z1 + 4.f) * s6;
+        AX[d][2] = (((3.f * z2 - 24.f) * z2 + 60.f) * z2 - 44.f) * s6;
+        AX[d][3] = (((-z3 + 12.f) * z3 - 48.f) * z3 + 64.f) * s6;
+      } else {
+        AX[d][0] = z0;        // 1 - f at the base point (cic_weights)
+        AX[d][1] = 1.f - z0;  // f at the next one
+      }
       const int K = g.K[d];
       const int bm = int((pk >> (10 * d)) & 1023u);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        int idx = bm + a - 1;
+      for (int a = 0; a < ORD; ++a) {
+        int idx = bm + a - (ORD == 4 ? 1 : 0);
         idx += idx < 0 ? K : (idx >= K ? -K : 0);
         int p = idx - lo[d];
         if (p < 0) p += K;
@@ -1465,15 +1488,15 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
       }
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < ORD; ++c) {
       if (!in[2][c]) continue;
 #pragma unroll
-      for (int bb = 0; bb < 4; ++bb) {
+      for (int bb = 0; bb < ORD; ++bb) {
         if (!in[1][bb]) continue;
         const int row = TX * (o[1][bb] + TY * o[2][c]);
         const float w12 = qs * AX[1][bb] * AX[2][c];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < ORD; ++a) {
           if (!in[0][a]) continue;
           atomicAdd(&acc[row + o[0][a]], __float2int_rn(w12 * AX[0][a]));
         }

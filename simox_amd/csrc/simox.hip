@@ -58,7 +58,9 @@ enum { NCG_IDLE = 0, NCG_WAIT_RESTART = 1, NCG_WAIT_ITER = 2, NCG_DONE = 3 };
 struct NcgRun {
   int phase = NCG_IDLE;
   int total_it = 0, nev = 0, restarts = 0, it = 0;
+  int batch = 1;        // inner iterations issued before the next host check
   double rms = 0.0, target = 0.0;
+  double rms0 = 0.0;    // residual at the restart that began this inner solve
   bool ok = false;
 };
 
@@ -138,11 +140,16 @@ struct simox_ctx {
   bool ls_adapt = true;
   int ls_rounds = 8;
   bool inner_f16 = true;
+  bool prec_spread32 = true;   // the preconditioner grid's FP32 spread in 32-bit fixed point
   bool inner_spread32 = true;  // the coarse grid's spread in 32-bit fixed point  // with the coarse grid: FP16 matrix values in the inner matvec
   bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
   bool prec_on = false;
   bool small_path = false;  // single-workgroup vector passes (n <= LS_SMALL_MAX)
+  // inner iterations per host check: predicted from the inner solve's rate,
+  // at most max_batch (SIMOX_ITER_BATCH, 1 = a check after every iteration)
+  int max_batch = 16;
+  double last_rate = 0.0;  // residual reduction per iteration of the last inner solve
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
   double* vjp_u = nullptr;     // 3N weights of the force vector-Jacobian product
@@ -161,6 +168,8 @@ struct simox_ctx {
   _Float16* nb_val16 = nullptr;
   int *nb_cnt16 = nullptr, *nb_col16 = nullptr;
   unsigned* nb_pk = nullptr;  // packed 32-bit entries (column + 12-bit value) when n <= 2^20
+  // their SELL-4 slices of 64 rows: groups of 4 entries per row, offsets
+  int *pk_slen = nullptr, *pk_soff = nullptr, *pk_bsum = nullptr, *pk_total = nullptr;
   bool inner_pack = true, pk_valid = false;
   double inner_drop = 1e-3;  // entries with |H0_ij| below this (eV/e^2) left out
   double* bvec = nullptr;
@@ -371,6 +380,10 @@ int ensure_stage(simox_ctx* ctx, size_t doubles) {
 
 // Reduce nv interleaved partial values into dscal[off .. off+nv).
 int reduce(simox_ctx* ctx, int nparts, int nv, int maxmask, int off) {
+  if (nv > RED_MAXV) {
+    ctx->err = "internal: reduction of more than RED_MAXV values";
+    return SIMOX_ERR_ARG;
+  }
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nparts, nv,
                      maxmask, ctx->dscal + off);
   HIPCHK(hipGetLastError());
@@ -901,14 +914,17 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   const bool timed = timed_grid(ctx, pm);
   hipEvent_t ev{};
   if (timed) tstart(ctx, tcat(ctx, pm, 0), &ev);
-  if (pm.order == 2 && f32)
+  if (pm.order == 2 && f32 && pm.spread32)
+    hipLaunchKernelGGL(k_spread_tile_i32<2>, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
+                       pm.bstart, pm.border, pm.brec, pm.greal32);
+  else if (pm.order == 2 && f32)
     hipLaunchKernelGGL((k_spread_tile<2, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
   else if (pm.order == 2)
     hipLaunchKernelGGL((k_spread_tile<2, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
   else if (f32 && pm.spread32)
-    hipLaunchKernelGGL(k_spread_tile_i32, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
+    hipLaunchKernelGGL(k_spread_tile_i32<4>, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
                        pm.bstart, pm.border, pm.brec, pm.greal32);
   else if (f32)
     hipLaunchKernelGGL((k_spread_tile<4, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
@@ -1107,10 +1123,30 @@ int build_matrix(simox_ctx* ctx) {
     const bool pack = ctx->inner_pack && ctx->n <= (1 << PK_COL_BITS);
     if (!ctx->nb_cnt16) CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
     if (pack) {
-      if (!ctx->nb_pk) CHK(dalloc(ctx, &ctx->nb_pk, size_t(ctx->cap) * ctx->n));
-      hipLaunchKernelGGL(k_ell_to_p32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
-                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16,
-                         ctx->nb_pk);
+      const int n = ctx->n, nsl = (n + SLICE - 1) / SLICE, nsb = (nsl + 1023) / 1024;
+      if (!ctx->nb_pk) {
+        // slices padded to whole groups of 4: at most (cap + 3) entries per row
+        CHK(dalloc(ctx, &ctx->nb_pk, size_t(ctx->cap + 4) * (size_t(nsl) * SLICE)));
+        CHK(dalloc(ctx, &ctx->pk_slen, size_t(nsl)));
+        CHK(dalloc(ctx, &ctx->pk_soff, size_t(nsl) + 1));
+        CHK(dalloc(ctx, &ctx->pk_bsum, size_t(nsb) + 1));
+        CHK(dalloc(ctx, &ctx->pk_total, 1));
+      }
+      hipLaunchKernelGGL(k_pk_count, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->cap, ctx->nb_cnt,
+                         ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16);
+      hipLaunchKernelGGL(k_pk_slices, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->nb_cnt16,
+                         ctx->pk_slen);
+      hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(1024), 0, ctx->st, nsl, ctx->pk_slen,
+                         ctx->pk_soff, ctx->pk_bsum);
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, ctx->st, nsb, ctx->pk_bsum,
+                         ctx->pk_total);
+      hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(1024), 0, ctx->st, nsl, ctx->pk_soff,
+                         ctx->pk_bsum, ctx->pk_total);
+      hipLaunchKernelGGL(k_pk_scale, dim3(grid_blocks(nsl + 1)), dim3(BLOCK), 0, ctx->st, nsl,
+                         ctx->pk_soff);
+      hipLaunchKernelGGL(k_ell_to_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->cap,
+                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->pk_slen,
+                         ctx->pk_soff, ctx->nb_pk);
       ctx->pk_valid = true;
     } else {
       if (!ctx->nb_val16) {
@@ -1277,6 +1313,10 @@ int prec_grid(simox_ctx* ctx, PmeGrid** out) {
     if (rc == SIMOX_ERR_ARG) return 0;  // no launch shape for these sizes: fine grid
     return rc;
   }
+  // MIXED: the FP32 spread of the preconditioner's pass in 32-bit fixed point
+  // (stencil records with 10-bit base indices)
+  ctx->pc.spread32 = ctx->prec_spread32 && ctx->pc.brec && K[0] <= 1024 && K[1] <= 1024 &&
+                     K[2] <= 1024;
   *out = &ctx->pc;
   return 0;
 }
@@ -1443,6 +1483,10 @@ int phi_gather(simox_ctx* ctx, bool f32, const PmeGrid* grid = nullptr) {
   return 0;
 }
 
+// The multi-block path with the long-range preconditioner runs the fused
+// iteration tail (k_center_scale ... k_pcg_dir_proj).
+bool fused_tail(const simox_ctx* ctx) { return !ctx->small_path && ctx->prec_on; }
+
 // One inner iteration: H0 p, exact line search, step, new direction.
 int ncg_iter_body(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
@@ -1458,9 +1502,10 @@ int ncg_iter_body(simox_ctx* ctx) {
   tstop(ctx, 10, ev);
   tstart(ctx, 6, &ev);
   if (ig && ctx->val16_valid && ctx->pk_valid) {
-    hipLaunchKernelGGL(k_h0matvec16<true>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
-                       ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
+    hipLaunchKernelGGL(k_h0matvec_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
+                       ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk), ctx->apv,
+                       ctx->part);
   } else if (ig && ctx->val16_valid) {
     hipLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
@@ -1531,6 +1576,37 @@ int ncg_iter_body(simox_ctx* ctx) {
   hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
   CHK(reduce(ctx, nb, 1, 0, 0));
+  if (fused_tail(ctx)) {
+    // centre + scale, the preconditioner's grid pass, z with its six sums,
+    // projected beta, direction (ncg_issue_iter reads sum g^2 from ls[10])
+    PmeGrid& pc = ctx->pc.ready ? ctx->pc : ctx->pm;  // chosen by build_prec
+    const Geom gp = geom_of(ctx, pc);
+    hipLaunchKernelGGL(k_center_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                       ctx->qc, ctx->dscal + 0, ctx->gv, ctx->cv, ctx->sv);
+    CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
+    if (f32 && pc.order == 2)
+      hipLaunchKernelGGL((k_prec_finish6<2, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal32, ctx->zv, ctx->part);
+    else if (pc.order == 2)
+      hipLaunchKernelGGL((k_prec_finish6<2, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal64, ctx->zv, ctx->part);
+    else if (f32)
+      hipLaunchKernelGGL((k_prec_finish6<4, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal32, ctx->zv, ctx->part);
+    else
+      hipLaunchKernelGGL((k_prec_finish6<4, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+                         ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
+                         pc.greal64, ctx->zv, ctx->part);
+    hipLaunchKernelGGL(k_reduce_op<4>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 6,
+                       ctx->dscal + 56, ls);
+    hipLaunchKernelGGL(k_pcg_dir_proj, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
+                       p32);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_ncg_center, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ctx->qc, ctx->dscal + 0, ctx->gv, ctx->mv, ctx->part);
   CHK(reduce(ctx, nb, 3, 0, 16));
@@ -1562,15 +1638,39 @@ int ncg_launch_iter(simox_ctx* ctx) {
   return 0;
 }
 
+// Issues r.batch inner iterations, then the scalars of the last one for
+// ncg_check (one host round trip per batch).
 int ncg_issue_iter(simox_ctx* ctx) {
-  CHK(ncg_launch_iter(ctx));
-  // the convergence and descent scalars of this iteration, for ncg_check
-  HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-  // ls[5..9]: done flag, rz, beta, descent flag, Newton rounds run
-  HIPCHK(hipMemcpyAsync(ctx->hscal + 45, ctx->dscal + 45, 5 * sizeof(double), hipMemcpyDeviceToHost,
-                        ctx->st));
+  for (int k = 0; k < ctx->ncg.batch; ++k) CHK(ncg_launch_iter(ctx));
+  if (fused_tail(ctx)) {
+    // ls[5..10]: done flag, rz, beta, descent flag, Newton rounds run, sum g^2
+    HIPCHK(hipMemcpyAsync(ctx->hscal + 45, ctx->dscal + 45, 6 * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->st));
+  } else {
+    // the convergence and descent scalars of this iteration, for ncg_check
+    HIPCHK(hipMemcpyAsync(ctx->hscal + 18, ctx->dscal + 18, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->st));
+    // ls[5..9]: done flag, rz, beta, descent flag, Newton rounds run
+    HIPCHK(hipMemcpyAsync(ctx->hscal + 45, ctx->dscal + 45, 5 * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->st));
+  }
   ctx->ncg.phase = NCG_WAIT_ITER;
   return 0;
+}
+
+// Inner iterations to issue before the next check: the iterations the
+// solve's own rate (or the last solve's, at its start) predicts to reach the
+// target, less 15 %, within [1, max_batch].  Issuing more iterations than
+// needed only continues the CG past its target; the host checks the last.
+int ncg_batch(const simox_ctx* ctx) {
+  const NcgRun& r = ctx->ncg;
+  if (ctx->max_batch <= 1) return 1;
+  double rate = ctx->last_rate;
+  if (r.it > 0 && r.rms < r.rms0 && r.rms0 > 0.0) rate = std::pow(r.rms / r.rms0, 1.0 / r.it);
+  if (!(rate > 0.0 && rate < 1.0) || !(r.rms > r.target)) return r.it > 0 ? 1 : 4;
+  const double need = std::log(r.target / r.rms) / std::log(rate);
+  const int b = int(0.85 * need);
+  return std::max(1, std::min(std::min(ctx->max_batch, b), ctx->maxit - r.it));
 }
 
 // true gradient (FP64 operator) and its projected RMS
@@ -1603,11 +1703,18 @@ int ncg_begin(simox_ctx* ctx) {
   HIPCHK(hipGetLastError());
   ctx->ncg = NcgRun{};
   ctx->ls_rounds = 8;
+  // the fused tail's six sums per block, and ls[12] = n for its mean (set
+  // before any capture of the iteration body)
+  CHK(ensure_part(ctx, size_t(nb) * 6));
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, ctx->st, 1, double(n), ctx->dscal + 52);
   // the cell sort swaps the atom buffers (x, id, cell, ...): a graph captured
   // in an earlier solve may name the other buffer of each pair, so capture anew
   ctx->iter_graph_gen = -1;
   if (ctx->prec_on) CHK(build_prec(ctx));
   if (use32(ctx) && ctx->inner_coarse) CHK(inner_grid(ctx));
+  // brick lists of the inner grid for these positions, here rather than in
+  // the first iteration (which a batched solve captures into its graph)
+  if (use32(ctx) && ctx->inner_coarse && ctx->pi.ready) CHK(ensure_bricks(ctx, ctx->pi));
   return ncg_issue_restart(ctx);
 }
 
@@ -1646,16 +1753,25 @@ int ncg_check(simox_ctx* ctx) {
     HIPCHK(hipGetLastError());
     r.it = 0;
     r.rms = rms_true;
-    if (r.rms > r.target && r.it < ctx->maxit) return ncg_issue_iter(ctx);
+    r.rms0 = rms_true;
+    if (r.rms > r.target && r.it < ctx->maxit) {
+      r.batch = ncg_batch(ctx);
+      return ncg_issue_iter(ctx);
+    }
     return ncg_issue_restart(ctx);
   }
   if (r.phase == NCG_WAIT_ITER) {
-    ++r.it;
+    r.it += r.batch;
     const bool ls_done = ctx->hscal[45] != 0.0;
     ctx->ls_rounds = ls_done ? std::min(8, std::max(2, int(ctx->hscal[49]) + 1)) : 8;
-    r.rms = std::sqrt(ctx->hscal[18] / n);
+    r.rms = std::sqrt((fused_tail(ctx) ? ctx->hscal[50] : ctx->hscal[18]) / n);
     const bool descent = ctx->hscal[48] == 0.0;  // else restart from the true gradient
-    if (descent && r.rms > r.target && r.it < ctx->maxit) return ncg_issue_iter(ctx);
+    if (descent && r.rms > r.target && r.it < ctx->maxit) {
+      r.batch = ncg_batch(ctx);
+      return ncg_issue_iter(ctx);
+    }
+    if (r.it > 0 && r.rms < r.rms0 && r.rms0 > 0.0)
+      ctx->last_rate = std::pow(r.rms / r.rms0, 1.0 / r.it);
     r.total_it += r.it;
     r.nev += r.it;
     return ncg_issue_restart(ctx);
@@ -1915,6 +2031,8 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   {
     const char* e = std::getenv("SIMOX_SMALL_PATH");
     ctx->small_path = n <= LS_SMALL_MAX && !(e && std::strcmp(e, "0") == 0);
+    const char* eb = std::getenv("SIMOX_ITER_BATCH");
+    if (eb) ctx->max_batch = std::max(1, std::min(64, std::atoi(eb)));
   }
   Geom& g = ctx->g;
   g.n = n;
@@ -1979,6 +2097,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_PREC_SPREAD32")) ctx->prec_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_PACK")) ctx->inner_pack = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
@@ -2056,7 +2175,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->near_cnt, ctx->near_col, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

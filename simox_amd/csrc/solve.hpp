@@ -104,27 +104,41 @@ __global__ void __launch_bounds__(BLOCK) k_dudq(Geom g, const Tables* __restrict
 }
 
 // Final pass of a deterministic reduction: out[v] = sum over nparts of
-// part[nv*k + v] (single block, fixed order); MAXV marks values reduced by max.
+// part[nv*k + v] (single block, fixed order); maxmask marks values reduced by
+// max.  All nv <= RED_MAXV values of a row are read in one pass.
+constexpr int RED_MAXV = 10;
 __global__ void k_reduce_final(const double* __restrict__ part, int nparts, int nv, int maxmask,
                                double* __restrict__ out) {
-  __shared__ double lds[1024 / WAVE];
-  for (int v = 0; v < nv; ++v) {
-    const bool mx = (maxmask >> v) & 1;
-    double s = mx ? 0.0 : 0.0;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+  constexpr int NW = 1024 / WAVE;
+  __shared__ double lds[RED_MAXV * NW];
+  if (nv > RED_MAXV) return;  // callers reduce at most RED_MAXV values
+  double s[RED_MAXV];
+#pragma unroll
+  for (int v = 0; v < RED_MAXV; ++v) s[v] = 0.0;
+  // every value of a partial row in one pass (fixed order per thread)
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < RED_MAXV; ++v) {
+      if (v >= nv) break;
       const double e = part[size_t(nv) * k + v];
-      s = mx ? fmax(s, e) : s + e;
+      s[v] = ((maxmask >> v) & 1) ? fmax(s[v], e) : s[v] + e;
     }
-    s = mx ? wave_max(s) : wave_sum(s);
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    if (lane == 0) lds[w] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = lds[0];
-      for (int q = 1; q < int(blockDim.x / WAVE); ++q) t = mx ? fmax(t, lds[q]) : t + lds[q];
-      out[v] = t;
-    }
-    __syncthreads();
+  }
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int v = 0; v < RED_MAXV; ++v) {
+    if (v >= nv) break;
+    const double t = ((maxmask >> v) & 1) ? wave_max(s[v]) : wave_sum(s[v]);
+    if (lane == 0) lds[v * NW + w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const int v = threadIdx.x;
+    const bool mx = (maxmask >> v) & 1;
+    const int nw = int(blockDim.x / WAVE);
+    double t = lds[v * NW];
+    for (int q = 1; q < nw; ++q) t = mx ? fmax(t, lds[v * NW + q]) : t + lds[v * NW + q];
+    out[v] = t;
   }
 }
 
@@ -697,6 +711,7 @@ __global__ void k_ell_to_f16(int n, int cap, const int* __restrict__ cnt,
 // bits 31..20 are those of the FP16 value (entries |H0_ij| >= tau stay far
 // inside the normal FP16 range).  4 bytes per entry instead of 6.
 constexpr int PK_COL_BITS = 20;
+constexpr int SLICE = 64;  // rows per slice of the SELL-4 layout (one wave)
 __global__ void k_ell_to_p32(int n, int cap, const int* __restrict__ cnt,
                              const int* __restrict__ col, const double* __restrict__ v64,
                              double tau, int* __restrict__ cnt16, unsigned* __restrict__ pk) {
@@ -798,6 +813,65 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
   }
 }
 
+// k_h0matvec16 over the SELL-4 packed matrix: each lane streams its row as
+// 16-byte groups of 4 entries, 4 groups (16 entries) in flight.
+__global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* __restrict__ tb,
+                                                          const int* __restrict__ id,
+                                                          const double* __restrict__ q,
+                                                          const double* __restrict__ p,
+                                                          const float* __restrict__ pt,
+                                                          const double* __restrict__ gr,
+                                                          const double* __restrict__ phi_in,
+                                                          const int* __restrict__ slen,
+                                                          const int* __restrict__ soff,
+                                                          const uint4* __restrict__ pk,
+                                                          double* __restrict__ y,
+                                                          double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const int s = i / SLICE, lane = i % SLICE;
+    const int L = slen[s];
+    const uint4* row = pk + size_t(soff[s]) + lane;
+    float sum = 0.f;
+    int k = 0;
+    constexpr int U = 4;
+    for (; k + U <= L; k += U) {
+      uint4 w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = row[size_t(k + u) * SLICE];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        sum += pk_value(w[u].x) * pt[pk_col(w[u].x)];
+        sum += pk_value(w[u].y) * pt[pk_col(w[u].y)];
+        sum += pk_value(w[u].z) * pt[pk_col(w[u].z)];
+        sum += pk_value(w[u].w) * pt[pk_col(w[u].w)];
+      }
+    }
+    for (; k < L; ++k) {
+      const uint4 w = row[size_t(k) * SLICE];
+      sum += pk_value(w.x) * pt[pk_col(w.x)];
+      sum += pk_value(w.y) * pt[pk_col(w.y)];
+      sum += pk_value(w.z) * pt[pk_col(w.z)];
+      sum += pk_value(w.w) * pt[pk_col(w.w)];
+    }
+    const double pi = p[i];
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double yi = sc.Jp * pi + phi_in[i] + double(sum);
+    y[i] = yi;
+    acc[0] = pi * yi;
+    acc[1] = pi * gr[i];
+    acc[2] = outside(sc, q[i]) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = acc[0];
+    part[3 * blockIdx.x + 1] = acc[1];
+    part[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
 // charge predictor: out = c0 q + c1 h0 + c2 h1 + c3 h2 with the first ord+1
 // terms (polynomial extrapolation in time; the coefficients sum to 1, which
 // preserves the total charge)
@@ -812,6 +886,59 @@ __global__ void k_extrapolate(int n, int ord, double c0, double c1, double c2, d
   if (ord >= 2) v += c2 * h1[i];
   if (ord >= 3) v += c3 * h2[i];
   out[i] = v;
+}
+
+// The packed inner matrix in slices of 64 rows (one wave, SELL-4 layout):
+// slice s holds 4 * slen[s] entries per row, slen[s] = ceil(max kept entries
+// of its rows / 4), group g of lane l at uint4 index soff[s] + 64 g + l, so a
+// wave reads 1 KiB per 16-byte load and each lane 4 entries.  Rows shorter
+// than the slice are padded with zero values on their own column.
+__global__ void k_pk_count(int n, int cap, const int* __restrict__ cnt,
+                           const double* __restrict__ v64, double tau, int* __restrict__ cnt16) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cnt[i] < cap ? cnt[i] : cap;
+  int m = 0;
+  for (int k = 0; k < c; ++k) m += fabs(v64[size_t(k) * n + i]) >= tau;
+  cnt16[i] = m;
+}
+
+// slen[s] = ceil(max over the slice's rows of cnt16 / 4)
+__global__ void k_pk_slices(int n, const int* __restrict__ cnt16, int* __restrict__ slen) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = i < n ? cnt16[i] : 0;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) c = max(c, __shfl_xor(c, o, WAVE));
+  const int nsl = (n + SLICE - 1) / SLICE;
+  if ((threadIdx.x & (WAVE - 1)) == 0 && i / SLICE < nsl) slen[i / SLICE] = (c + 3) / 4;
+}
+
+// soff = 64 x exclusive prefix sums of slen (k_scan_block etc. on slen give
+// the group offsets; this scales them to uint4 units of 64 lanes)
+__global__ void k_pk_scale(int nsl, int* __restrict__ soff) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= nsl) soff[s] *= SLICE;
+}
+
+__global__ void k_ell_to_sell4(int n, int cap, const int* __restrict__ cnt,
+                               const int* __restrict__ col, const double* __restrict__ v64,
+                               double tau, const int* __restrict__ slen,
+                               const int* __restrict__ soff, unsigned* __restrict__ pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = i / SLICE, lane = i % SLICE;
+  unsigned* base = pk + 4 * (size_t(soff[s]) + lane);
+  const int c = cnt[i] < cap ? cnt[i] : cap;
+  int m = 0;
+  for (int k = 0; k < c; ++k) {
+    const double v = v64[size_t(k) * n + i];
+    if (fabs(v) < tau) continue;
+    const unsigned hb = __builtin_bit_cast(unsigned short, _Float16(float(v)));
+    const unsigned code = (hb & 0x8000u) | (((hb & 0x7FFFu) + 8u) & 0x7FF0u);
+    base[size_t(m / 4) * 4 * SLICE + (m % 4)] = (code << 16) | unsigned(col[size_t(k) * n + i]);
+    ++m;
+  }
+  for (; m < 4 * slen[s]; ++m) base[size_t(m / 4) * 4 * SLICE + (m % 4)] = unsigned(i);
 }
 
 // Line-search terms at step a: part[2k] = sum p (pen'(q + a p) - pen'(q)),
@@ -1033,6 +1160,71 @@ __global__ void __launch_bounds__(BLOCK) k_prec_finish(Geom g, const Tables* __r
   }
   block_sum<1>(acc, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Fused tail of an inner iteration with the long-range preconditioner
+// (multi-block path): k_center_scale centres g (its mean, gsum / n, is the
+// neutrality multiplier's part) and scales it for the preconditioner's
+// spread, s = c g / sqrt(D); k_prec_finish6 forms z and the six sums
+// {z, z.g, z.g_old, g, g_old, g^2} from which k_reduce_op<4> derives the
+// projected z' = z - mean(z): z'.g = z.g - mean(z) sum g, likewise z'.g_old,
+// then beta and rz; k_pcg_dir_proj applies the projection with the direction
+// update.  The same arithmetic as centre, scale, finish, project and beta in
+// five launches fewer.
+__global__ void k_center_scale(int n, const Tables* __restrict__ tb, const int* __restrict__ id,
+                               const double* __restrict__ q, const double* __restrict__ gsum,
+                               double* __restrict__ g, const double* __restrict__ c,
+                               double* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double gi = g[i] - gsum[0] / double(n);
+    g[i] = gi;
+    s[i] = c[i] * gi / sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+  }
+}
+
+// z = (g/sqrt(D) - c W^T grid)/sqrt(D); part[6k..] = sums of z, z g, z g_old,
+// g, g_old, g^2
+template <int ORD, class Rg>
+__global__ void __launch_bounds__(BLOCK) k_prec_finish6(Geom g, const Tables* __restrict__ tb,
+                                                        const double* __restrict__ x,
+                                                        const int* __restrict__ id,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ gr,
+                                                        const double* __restrict__ g_old,
+                                                        const double* __restrict__ c,
+                                                        const Rg* __restrict__ grid,
+                                                        double* __restrict__ z,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[6 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+    const double gi = gr[i], go = g_old[i];
+    const double zi = (gi / sd - c[i] * gather_ord<ORD>(g, x + 3 * i, grid)) / sd;
+    z[i] = zi;
+    acc[0] = zi;
+    acc[1] = zi * gi;
+    acc[2] = zi * go;
+    acc[3] = gi;
+    acc[4] = go;
+    acc[5] = gi * gi;
+  }
+  block_sum<6>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
+}
+
+// p = (z - mean z) + beta p (mean z in ls[11], beta in ls[7])
+__global__ void k_pcg_dir_proj(int n, const double* __restrict__ ls, const double* __restrict__ z,
+                               double* __restrict__ p, float* __restrict__ p32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double v = (z[i] - ls[11]) + ls[7] * p[i];
+    p[i] = v;
+    if (p32) p32[i] = float(v);
+  }
 }
 
 // Neutrality: z -= mean(z) (P z with P the projector on sum = 0; P M P is
@@ -1333,30 +1525,52 @@ __global__ void __launch_bounds__(1024) k_small_prec_tail(Geom gp, const Tables*
 // scalar step of the device-side CG on the sums, in the same launch:
 // OP 1 = line-search start (ls_init), 2 = Newton update of the line search
 // (ls_update; nothing once ls[5] marks it done), 3 = PR+ beta (beta_update,
-// when ls != null), 0 = none.
+// when ls != null), 4 = projected beta from k_prec_finish6's sums (mean z to
+// ls[11], sum g^2 to ls[10]; ls[12] holds the atom count), 0 = none.
 template <int OP>
 __global__ void k_reduce_op(const double* __restrict__ part, int nparts, int nv,
                             double* __restrict__ out, double* __restrict__ ls) {
-  __shared__ double lds[1024 / WAVE];
+  constexpr int NW = 1024 / WAVE;
+  __shared__ double lds[RED_MAXV * NW];
   if (OP == 2 && ls[5] != 0.0) return;
-  for (int v = 0; v < nv; ++v) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(nv) * k + v];
-    s = wave_sum(s);
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    if (lane == 0) lds[w] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = lds[0];
-      for (int q = 1; q < int(blockDim.x / WAVE); ++q) t += lds[q];
-      out[v] = t;
+  if (nv > RED_MAXV) return;
+  double s[RED_MAXV];
+#pragma unroll
+  for (int v = 0; v < RED_MAXV; ++v) s[v] = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < RED_MAXV; ++v) {
+      if (v >= nv) break;
+      s[v] += part[size_t(nv) * k + v];
     }
-    __syncthreads();
   }
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int v = 0; v < RED_MAXV; ++v) {
+    if (v >= nv) break;
+    const double t = wave_sum(s[v]);
+    if (lane == 0) lds[v * NW + w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const int v = threadIdx.x;
+    const int nw = int(blockDim.x / WAVE);
+    double t = lds[v * NW];
+    for (int q = 1; q < nw; ++q) t += lds[v * NW + q];
+    out[v] = t;
+  }
+  __syncthreads();  // out[] complete before the scalar step below reads it
   if (threadIdx.x == 0 && ls) {
     if (OP == 1) ls_init(out, ls);
     if (OP == 2) ls_update(out, ls);
     if (OP == 3) beta_update(out, ls);
+    if (OP == 4) {  // out = the six sums of k_prec_finish6 (n = nparts' atom count in ls[12])
+      const double zbar = out[0] / ls[12];
+      const double red[2] = {out[1] - zbar * out[3], out[2] - zbar * out[4]};
+      beta_update(red, ls);
+      ls[10] = out[5];
+      ls[11] = zbar;
+    }
   }
 }
 

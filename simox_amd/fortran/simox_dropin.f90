@@ -1,9 +1,13 @@
 ! Drop-in replacement of SIMOX's force.f90 (FORCE, PME, PME_SELF), ctip.f90
-! (CTIP_SOLVER) and cellsort.f90 (CELL_HIERARCHY, ID_CELL, CELL_SORT): the
-! same external names and dummy-argument lists, computed by the MI355X
-! library through iso_c_binding.  Link this unit in place of force.o, ctip.o
-! and cellsort.o (FFTE is then no longer needed); it uses the caller's DATAFMT
-! module, so it is compiled against the reference's datafmt.f90.
+! (CTIP_SOLVER), cellsort.f90 (CELL_HIERARCHY, ID_CELL, CELL_SORT) and the
+! NVE integrator of vverlet.f90 (VV_NVE1, VV_NVE2): the same external names
+! and dummy-argument lists, computed by the MI355X library through
+! iso_c_binding.  Link this unit in place of force.o, ctip.o and cellsort.o
+! (FFTE is then no longer needed), and ahead of vverlet.o with the linker's
+! first-definition rule for the two integrator halves (vverlet.f90 keeps
+! VINIT, the thermostats and OXIDATION's reflecting half); it uses the
+! caller's DATAFMT module, so it is compiled against the reference's
+! datafmt.f90.
 !
 ! Device state: one context, created on the first call with the published
 ! parameters and re-created when Npt_all changes (OXIDATION mode appends
@@ -24,7 +28,7 @@ module simox_dropin_state
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: n_ctx = -1
   type(c_params), target, save :: cur
-  real(c_double), allocatable, target, save :: xb(:,:), fb(:,:), qb(:)
+  real(c_double), allocatable, target, save :: xb(:,:), fb(:,:), qb(:), vb(:,:)
   ! positions at a CELL_SORT that ran before the context existed
   real(c_double), allocatable, target, save :: xsort(:,:)
   integer(c_int), allocatable, target, save :: idb(:), npt_b(:), link_b(:,:)
@@ -65,8 +69,8 @@ contains
     call simox_check(ctx, rc)
     rc = simox_set_rdf(ctx, 1)   ! FORCE also fills pair%rdf
     call simox_check(ctx, rc)
-    if (allocated(xb)) deallocate(xb, fb, qb, idb)
-    allocate(xb(3, n), fb(3, n), qb(n), idb(n))
+    if (allocated(xb)) deallocate(xb, fb, qb, idb, vb)
+    allocate(xb(3, n), fb(3, n), qb(n), idb(n), vb(3, n))
     n_ctx = n
     do i = 1, n
       xb(:, i) = q(i)%xx(:)
@@ -359,3 +363,75 @@ subroutine ctip_solver(nset, param, sys, q, cell, psi)
   end do
   sys%eself = e
 end subroutine ctip_solver
+
+! VV_NVE1 (vverlet.f90:74-96): v += dt/2 F/m, x += dt v, wrapped into the box,
+! on the device; q%xx and q%xv are updated.  The context exists by then
+! (PME_SELF runs before VINIT and the loop, main.f90:33-45).
+subroutine vv_nve1(nset, param, q, dt)
+  use datafmt
+  use simox_dropin_state
+  implicit none
+  type(amnt) :: nset
+  type(potn) :: param(natom)
+  type(ptcl) :: q(nset%npt_all)
+  real(dp) :: dt
+  type(c_params), target, save :: pc
+  integer(c_int) :: rc
+  integer :: i
+  if (.not. c_associated(ctx) .or. nset%npt_all /= n_ctx) then
+    write(*, *) "=== VV_NVE1 drop-in: no device context for these atoms ==="
+    stop
+  end if
+  pc = cur
+  call simox_dropin_potn(param, pc)
+  call simox_dropin_params_update(pc)
+  do i = 1, nset%npt_all
+    xb(:, i) = q(i)%xx(:)
+    vb(:, i) = q(i)%xv(:)
+    fb(:, i) = q(i)%ff(:)
+  end do
+  rc = simox_set_state(ctx, c_null_ptr, c_loc(xb), c_loc(vb), c_loc(fb), c_null_ptr)
+  call simox_check(ctx, rc)
+  rc = simox_vv1(ctx, real(dt, c_double))
+  call simox_check(ctx, rc)
+  rc = simox_get_state(ctx, c_loc(xb), c_loc(vb), c_null_ptr, c_null_ptr)
+  call simox_check(ctx, rc)
+  do i = 1, nset%npt_all
+    q(i)%xx(:) = xb(:, i)
+    q(i)%xv(:) = vb(:, i)
+  end do
+end subroutine vv_nve1
+
+! VV_NVE2 (vverlet.f90:132-157): v += dt/2 F/m on the device, sys%mv2 = sum m v^2.
+subroutine vv_nve2(nset, param, sys, q, dt)
+  use datafmt
+  use simox_dropin_state
+  implicit none
+  type(amnt) :: nset
+  type(potn) :: param(natom)
+  type(stat) :: sys
+  type(ptcl) :: q(nset%npt_all)
+  real(dp) :: dt
+  type(c_params), target, save :: pc
+  real(c_double) :: mv2
+  integer(c_int) :: rc
+  integer :: i
+  call simox_dropin_ensure(nset, sys, q)
+  pc = cur
+  call simox_dropin_potn(param, pc)
+  call simox_dropin_params_update(pc)
+  do i = 1, nset%npt_all
+    vb(:, i) = q(i)%xv(:)
+    fb(:, i) = q(i)%ff(:)
+  end do
+  rc = simox_set_state(ctx, c_null_ptr, c_null_ptr, c_loc(vb), c_loc(fb), c_null_ptr)
+  call simox_check(ctx, rc)
+  rc = simox_vv2(ctx, real(dt, c_double), mv2)
+  call simox_check(ctx, rc)
+  rc = simox_get_state(ctx, c_null_ptr, c_loc(vb), c_null_ptr, c_null_ptr)
+  call simox_check(ctx, rc)
+  do i = 1, nset%npt_all
+    q(i)%xv(:) = vb(:, i)
+  end do
+  sys%mv2 = mv2
+end subroutine vv_nve2
