@@ -8,6 +8,7 @@
 // which equals the reference's Qfin = ZFFT3D(-1)(G . ZFFT3D(+1)(Qbas))
 // (force.f90:411-426) because G is real and even: Qfin = IFFT(G . FFT(Q)) / Ng.
 #pragma once
+#include <type_traits>
 #include "common.hpp"
 
 namespace sx {
@@ -1336,7 +1337,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile(Geom g, const double* __r
 // bits per dimension (K_d <= 1024), so the spread reads 16 contiguous bytes
 // per candidate instead of gathering x and redoing the FP64 grid coordinates.
 __global__ void k_brick_records(Geom g, int n, const double* __restrict__ x,
-                                const int* __restrict__ border, float4* __restrict__ rec) {
+                                const int* __restrict__ border, float4* __restrict__ rec,
+                                float4* __restrict__ rec_atom) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
   double u[3];
@@ -1352,6 +1354,55 @@ __global__ void k_brick_records(Geom g, int n, const double* __restrict__ x,
     pk |= unsigned(bm) << (10 * d);
   }
   rec[f] = make_float4(z[0], z[1], z[2], __uint_as_float(pk));
+  if (rec_atom) rec_atom[border[f]] = rec[f];  // the same record in atom order
+}
+
+// FP32 stencil weights of a record (the ones the 32-bit spread uses), ORD 4:
+// the B-splines of z0, ORD 2: z0, 1 - z0; base index of axis d in bm[d].
+template <int ORD>
+__device__ inline void rec_weights(const float4& rc, float AX[3][ORD], int bm[3]) {
+  const float zr[3] = {rc.x, rc.y, rc.z};
+  const unsigned pk = __float_as_uint(rc.w);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float z0 = zr[d];
+    if constexpr (ORD == 4) {
+      const float z1 = z0 + 1.f, z2 = z0 + 2.f, z3 = z0 + 3.f;
+      constexpr float s6 = 1.f / 6.f;
+      AX[d][0] = z0 * z0 * z0 * s6;
+      AX[d][1] = (((-3.f * z1 + 12.f) * z1 - 12.f) * z1 + 4.f) * s6;
+      AX[d][2] = (((3.f * z2 - 24.f) * z2 + 60.f) * z2 - 44.f) * s6;
+      AX[d][3] = (((-z3 + 12.f) * z3 - 48.f) * z3 + 64.f) * s6;
+    } else {
+      AX[d][0] = z0;
+      AX[d][1] = 1.f - z0;
+    }
+    bm[d] = int((pk >> (10 * d)) & 1023u);
+  }
+}
+
+// The linear (ORD 2) gather of the preconditioner's grid at atom i from its
+// atom-order record: the transpose of k_spread_tile_i32<2> (same FP32 weights).
+__device__ inline double gather_cic_rec(const Geom& g, const float4& rc,
+                                        const float* __restrict__ grid) {
+  float AX[3][2];
+  int bm[3];
+  rec_weights<2>(rc, AX, bm);
+  int ix[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ix[d][0] = bm[d];
+    ix[d][1] = bm[d] + 1 == g.K[d] ? 0 : bm[d] + 1;
+  }
+  float e = 0.f;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      const size_t row = size_t(g.K[0]) * (size_t(ix[1][bb]) + size_t(g.K[1]) * size_t(ix[2][c]));
+      e += AX[1][bb] * AX[2][c] * (AX[0][0] * grid[row + ix[0][0]] + AX[0][1] * grid[row + ix[0][1]]);
+    }
+  return double(e);
 }
 
 // K6 in FP32 for the MIXED mode's coarse grids (the inner operator's,
@@ -1366,27 +1417,52 @@ __global__ void k_brick_records(Geom g, int n, const double* __restrict__ x,
 // factor 2 left to 2^31 covers the rounding of each product).  Resolution
 // 2^-S, a few 1e-8 of max|v|.  Integer sums keep the grid independent of the
 // order of the atomics; the LDS tile is half the size of the 64-bit one.
-template <int ORD>
-__global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double* __restrict__ x,
-                                                           const double* __restrict__ q,
-                                                           const int* __restrict__ bstart,
-                                                           const int* __restrict__ border,
-                                                           const float4* __restrict__ rec,
-                                                           float* __restrict__ grid) {
+// Tile shape (TXg x TYg x TZg points, NT threads): 32 x 16 x 8 with 256
+// threads, or 32 x 32 x 16 with 1024 threads (one workgroup per CU, 2.3 halo
+// bricks per owned brick instead of 3.75 for the order-4 stencil).
+template <int ORD, int TXg, int TYg, int TZg, int NT>
+struct SpreadI32 {
+  static constexpr int TILEg = TXg * TYg * TZg;
+  static constexpr int MAXBg = (TXg / BRK + 2) * (TYg / BRK + 2) * (TZg / BRK + 2);
+  static constexpr int MAXCg = NT == 1024 ? 16384 : 6144;
+  // slot table entries: 32 bits (brick << 12 | offset) for the big tile,
+  // 16 bits (brick << 8 | offset) for the small one
+  static constexpr bool WIDE = NT == 1024;
+  using Entry = typename std::conditional<WIDE, unsigned, unsigned short>::type;
+  static constexpr int OFS_BITS = WIDE ? 12 : 8;
+  static_assert(MAXBg <= NT, "one thread per candidate brick");
+  static_assert(MAXBg < (1 << (WIDE ? 20 : 8)), "candidate brick index fits the entry");
+  static int tiles(const Geom& g) {
+    return ((g.K[0] + TXg - 1) / TXg) * ((g.K[1] + TYg - 1) / TYg) * ((g.K[2] + TZg - 1) / TZg);
+  }
+};
+
+template <int ORD, int TXg, int TYg, int TZg, int NT>
+__global__ void __launch_bounds__(NT) k_spread_tile_i32(Geom g, const double* __restrict__ x,
+                                                        const double* __restrict__ q,
+                                                        const int* __restrict__ bstart,
+                                                        const int* __restrict__ border,
+                                                        const float4* __restrict__ rec,
+                                                        float* __restrict__ grid) {
   static_assert(ORD == 2 || ORD == 4, "linear or cubic B-spline assignment");
-  __shared__ int acc[TILE];
-  __shared__ unsigned short ctab[SPREAD_MAXC];  // candidate f: (candidate brick << 8) | offset
+  using P = SpreadI32<ORD, TXg, TYg, TZg, NT>;
+  constexpr int TILEg = P::TILEg, MAXBg = P::MAXBg, MAXCg = P::MAXCg;
+  __shared__ int acc[TILEg];
+  using Entry = typename P::Entry;
+  constexpr int OB = P::OFS_BITS;
+  __shared__ Entry ctab[MAXCg];  // candidate f: (candidate brick << OB) | offset
   __shared__ int tab_bad;
-  __shared__ int bofs[MAXB + 1];
-  __shared__ int bbeg[MAXB];
-  __shared__ int wsum[BLOCK / WAVE];
-  __shared__ double wmax[BLOCK / WAVE];
-  __shared__ int wcnt[BLOCK / WAVE];
-  const int ntx = (g.K[0] + TX - 1) / TX, nty = (g.K[1] + TY - 1) / TY;
+  __shared__ int bofs[MAXBg + 1];
+  __shared__ int bbeg[MAXBg];
+  __shared__ int wsum[NT / WAVE];
+  __shared__ double wmax[NT / WAVE];
+  __shared__ int wcnt[NT / WAVE];
+  const int ntx = (g.K[0] + TXg - 1) / TXg, nty = (g.K[1] + TYg - 1) / TYg;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
-  const int lo[3] = {tx * TX, ty * TY, tz * TZ};
-  const int ext[3] = {min(TX, g.K[0] - lo[0]), min(TY, g.K[1] - lo[1]), min(TZ, g.K[2] - lo[2])};
+  const int lo[3] = {tx * TXg, ty * TYg, tz * TZg};
+  const int ext[3] = {min(TXg, g.K[0] - lo[0]), min(TYg, g.K[1] - lo[1]),
+                      min(TZg, g.K[2] - lo[2])};
   const int nb[3] = {g.K[0] / BRK, g.K[1] / BRK, g.K[2] / BRK};
   int fb[3], cb[3];
 #pragma unroll
@@ -1401,7 +1477,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
     cb[d] = c;
   }
   const int ncand = cb[0] * cb[1] * cb[2];
-  for (int k = threadIdx.x; k < TILE; k += blockDim.x) acc[k] = 0;
+  for (int k = threadIdx.x; k < TILEg; k += blockDim.x) acc[k] = 0;
   int cntk = 0;
   if (threadIdx.x < ncand) {
     const int k = threadIdx.x;
@@ -1428,10 +1504,31 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   __syncthreads();
   for (int w = 0; w < wid; ++w) cntk += wsum[w];
   if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
+  if (threadIdx.x == 0) tab_bad = 0;
   __syncthreads();
   const int total = bofs[ncand];
-  const bool table = spread_slot_table(ncand, total, bofs, ctab, &tab_bad);
-  auto slot_of = [&](int f) { return spread_slot(f, table, ncand, bofs, bbeg, ctab); };
+  // slot table (else a binary search over bofs per candidate)
+  const bool table = total <= MAXCg;
+  if (table && int(threadIdx.x) < ncand) {
+    const int k = threadIdx.x, o = bofs[k], c = bofs[k + 1] - o;
+    if (c > (1 << OB)) tab_bad = 1;
+    else
+      for (int j = 0; j < c; ++j) ctab[o + j] = Entry((unsigned(k) << OB) | unsigned(j));
+  }
+  __syncthreads();
+  const bool use_tab = table && !tab_bad;
+  auto slot_of = [&](int f) {
+    if (use_tab) {
+      const unsigned e = ctab[f];
+      return bbeg[e >> OB] + int(e & ((1u << OB) - 1u));
+    }
+    int l = 0, h = ncand - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (bofs[mid] <= f) l = mid; else h = mid - 1;
+    }
+    return bbeg[l] + (f - bofs[l]);
+  };
   auto atom_of = [&](int f) { return border[slot_of(f)]; };
   // the tile's scale from the largest |q| it visits and its fullest brick
   double m = 0.0;
@@ -1441,7 +1538,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   __syncthreads();
   m = wmax[0];
   int maxcnt = wcnt[0];
-  for (int w = 1; w < BLOCK / WAVE; ++w) {
+  for (int w = 1; w < NT / WAVE; ++w) {
     m = fmax(m, wmax[w]);
     maxcnt = max(maxcnt, wcnt[w]);
   }
@@ -1493,7 +1590,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
 #pragma unroll
       for (int bb = 0; bb < ORD; ++bb) {
         if (!in[1][bb]) continue;
-        const int row = TX * (o[1][bb] + TY * o[2][c]);
+        const int row = TXg * (o[1][bb] + TYg * o[2][c]);
         const float w12 = qs * AX[1][bb] * AX[2][c];
 #pragma unroll
         for (int a = 0; a < ORD; ++a) {
@@ -1505,8 +1602,8 @@ __global__ void __launch_bounds__(BLOCK) k_spread_tile_i32(Geom g, const double*
   }
   __syncthreads();
   const float inv = float(ldexp(1.0, -S));
-  for (int k = threadIdx.x; k < TILE; k += blockDim.x) {
-    const int px = k % TX, py = (k / TX) % TY, pz = k / (TX * TY);
+  for (int k = threadIdx.x; k < TILEg; k += blockDim.x) {
+    const int px = k % TXg, py = (k / TXg) % TYg, pz = k / (TXg * TYg);
     if (px < ext[0] && py < ext[1] && pz < ext[2]) {
       const size_t gi = size_t(lo[0] + px) +
                         size_t(g.K[0]) * (size_t(lo[1] + py) + size_t(g.K[1]) * size_t(lo[2] + pz));
@@ -1623,6 +1720,109 @@ __global__ void __launch_bounds__(BLOCK) k_gather_tile(Geom g, const double* __r
           e += AX[0][a] * AX[1][bb] * AX[2][c] *
                double(tile[(o[0] + a) + GLX * ((o[1] + bb) + GLY * (o[2] + c))]);
     out[ai] = scale * e;
+  }
+}
+
+// k_gather_tile on an FP32 grid with stencil records (the MIXED inner grid):
+// each candidate's FP32 weights come from its brick-order record, i.e. the
+// weights k_spread_tile_i32<4> spread with (the gather is the spread's exact
+// transpose), instead of FP64 coordinates from x; FP32 products, FP64 sum.
+template <int GX, int GY, int GZ>
+__global__ void __launch_bounds__(BLOCK) k_gather_tile_rec(Geom g, const int* __restrict__ bstart,
+                                                           const int* __restrict__ border,
+                                                           const float4* __restrict__ rec,
+                                                           const float* __restrict__ grid,
+                                                           double scale,
+                                                           double* __restrict__ out) {
+  constexpr int GLX = GX + 3, GLY = GY + 3, GLZ = GZ + 3;
+  constexpr int GBRICKS = (GX / BRK) * (GY / BRK) * (GZ / BRK);
+  static_assert(GBRICKS <= BLOCK, "one thread per brick of the tile");
+  extern __shared__ unsigned char gather_lds[];
+  float* tile = reinterpret_cast<float*>(gather_lds);
+  __shared__ int bofs[GBRICKS + 1];
+  __shared__ int bbeg[GBRICKS];
+  __shared__ int wsum[BLOCK / WAVE];
+  const int ntx = (g.K[0] + GX - 1) / GX, nty = (g.K[1] + GY - 1) / GY;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int lo[3] = {tx * GX, ty * GY, tz * GZ};
+  const int ext[3] = {min(GX, g.K[0] - lo[0]), min(GY, g.K[1] - lo[1]), min(GZ, g.K[2] - lo[2])};
+  const int lx = ext[0] + 3, ly = ext[1] + 3, lz = ext[2] + 3;
+  constexpr int GL = GLX * GLY * GLZ;
+  constexpr int U = 8;
+  for (int k0 = threadIdx.x; k0 < GL; k0 += U * BLOCK) {
+    float v[U];
+#pragma unroll
+    for (int w = 0; w < U; ++w) {
+      const int k = k0 + w * BLOCK;
+      v[w] = 0.f;
+      const int px = k % GLX, py = (k / GLX) % GLY, pz = k / (GLX * GLY);
+      if (k < GL && px < lx && py < ly && pz < lz) {
+        int gx = lo[0] - 1 + px, gy = lo[1] - 1 + py, gz = lo[2] - 1 + pz;
+        gx += gx < 0 ? g.K[0] : (gx >= g.K[0] ? -g.K[0] : 0);
+        gy += gy < 0 ? g.K[1] : (gy >= g.K[1] ? -g.K[1] : 0);
+        gz += gz < 0 ? g.K[2] : (gz >= g.K[2] ? -g.K[2] : 0);
+        v[w] = grid[size_t(gx) + size_t(g.K[0]) * (size_t(gy) + size_t(g.K[1]) * size_t(gz))];
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < U; ++w) {
+      const int k = k0 + w * BLOCK;
+      if (k < GL) tile[k] = v[w];
+    }
+  }
+  const int nbx = g.K[0] / BRK, nby = g.K[1] / BRK;
+  const int cbx = ext[0] / BRK, cby = ext[1] / BRK, cbz = ext[2] / BRK;
+  const int ncand = cbx * cby * cbz;
+  int cntk = 0;
+  if (threadIdx.x < ncand) {
+    const int k = threadIdx.x;
+    const int bx = lo[0] / BRK + k % cbx, by = lo[1] / BRK + (k / cbx) % cby,
+              bz = lo[2] / BRK + k / (cbx * cby);
+    const int bid = bx + nbx * (by + nby * bz);
+    const int b0 = bstart[bid];
+    bbeg[k] = b0;
+    cntk = bstart[bid + 1] - b0;
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int v = __shfl_up(cntk, o, WAVE);
+    if (lane >= o) cntk += v;
+  }
+  if (lane == WAVE - 1) wsum[wid] = cntk;
+  if (threadIdx.x == 0) bofs[0] = 0;
+  __syncthreads();
+  for (int w = 0; w < wid; ++w) cntk += wsum[w];
+  if (threadIdx.x < ncand) bofs[threadIdx.x + 1] = cntk;
+  __syncthreads();
+  const int total = bofs[ncand];
+  for (int f = threadIdx.x; f < total; f += blockDim.x) {
+    int l = 0, h = ncand - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (bofs[mid] <= f) l = mid; else h = mid - 1;
+    }
+    const int sl = bbeg[l] + (f - bofs[l]);
+    float AX[3][4];
+    int bm[3];
+    rec_weights<4>(rec[sl], AX, bm);
+    int o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] = bm[d] - lo[d];  // stencil point a at LDS index o + a
+    double e = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float ec = 0.f;
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb) {
+        const float* rowp = tile + GLX * ((o[1] + bb) + GLY * (o[2] + c)) + o[0];
+        ec += AX[1][bb] * (AX[0][0] * rowp[0] + AX[0][1] * rowp[1] + AX[0][2] * rowp[2] +
+                           AX[0][3] * rowp[3]);
+      }
+      e += double(AX[2][c] * ec);
+    }
+    out[border[sl]] = scale * e;
   }
 }
 

@@ -103,6 +103,7 @@ struct PmeGrid {
   bool ready = false;
   bool spread32 = false;  // FP32 spreads in 32-bit fixed point (the MIXED inner grid)
   float4* brec = nullptr;  // its stencil records in brick order (k_brick_records)
+  float4* brec_atom = nullptr;  // the same records in atom order
 };
 
 struct simox_ctx {
@@ -141,6 +142,7 @@ struct simox_ctx {
   int ls_rounds = 8;
   bool inner_f16 = true;
   bool prec_spread32 = true;   // the preconditioner grid's FP32 spread in 32-bit fixed point
+  bool spread_big = true;      // 32 x 32 x 16 tiles for the 32-bit spreads
   bool inner_spread32 = true;  // the coarse grid's spread in 32-bit fixed point  // with the coarse grid: FP16 matrix values in the inner matvec
   bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
@@ -149,6 +151,8 @@ struct simox_ctx {
   // inner iterations per host check: predicted from the inner solve's rate,
   // at most max_batch (SIMOX_ITER_BATCH, 1 = a check after every iteration)
   int max_batch = 16;
+  // Newton rounds launched beyond the last search's count (SIMOX_LS_SPARE)
+  int ls_spare = 1;
   double last_rate = 0.0;  // residual reduction per iteration of the last inner solve
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
@@ -887,10 +891,26 @@ int ensure_bricks(simox_ctx* ctx, PmeGrid& pm) {
                      pm.bfill, pm.bunord, pm.border);
   if (pm.spread32)
     hipLaunchKernelGGL(k_brick_records, dim3(nb), dim3(BLOCK), 0, ctx->st, geom_of(ctx, pm), n,
-                       ctx->x, pm.border, pm.brec);
+                       ctx->x, pm.border, pm.brec, pm.brec_atom);
   HIPCHK(hipGetLastError());
   pm.bricks_valid = true;
   return 0;
+}
+
+// The 32-bit spread of a coarse FP32 grid: 32 x 32 x 16 tiles with 1024
+// threads where the grid holds them (SIMOX_SPREAD_BIG=0: 32 x 16 x 8 tiles
+// with 256 threads).
+template <int ORD>
+void spread_i32(simox_ctx* ctx, const Geom& g, PmeGrid& pm, const double* src) {
+  if (ctx->spread_big && g.K[0] >= 32 && g.K[1] >= 32 && g.K[2] >= 16) {
+    using P = SpreadI32<ORD, 32, 32, 16, 1024>;
+    hipLaunchKernelGGL((k_spread_tile_i32<ORD, 32, 32, 16, 1024>), dim3(P::tiles(g)), dim3(1024),
+                       0, ctx->st, g, ctx->x, src, pm.bstart, pm.border, pm.brec, pm.greal32);
+  } else {
+    using P = SpreadI32<ORD, 32, 16, 8, 256>;
+    hipLaunchKernelGGL((k_spread_tile_i32<ORD, 32, 16, 8, 256>), dim3(P::tiles(g)), dim3(256), 0,
+                       ctx->st, g, ctx->x, src, pm.bstart, pm.border, pm.brec, pm.greal32);
+  }
 }
 
 // Spread charges q (or ql = q + step p when p != null) and convolve, with the
@@ -915,8 +935,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
   hipEvent_t ev{};
   if (timed) tstart(ctx, tcat(ctx, pm, 0), &ev);
   if (pm.order == 2 && f32 && pm.spread32)
-    hipLaunchKernelGGL(k_spread_tile_i32<2>, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
-                       pm.bstart, pm.border, pm.brec, pm.greal32);
+    spread_i32<2>(ctx, g, pm, src);
   else if (pm.order == 2 && f32)
     hipLaunchKernelGGL((k_spread_tile<2, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
@@ -924,8 +943,7 @@ int pme_eval(simox_ctx* ctx, const double* q, const double* p, double step, bool
     hipLaunchKernelGGL((k_spread_tile<2, false, double>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal64);
   else if (f32 && pm.spread32)
-    hipLaunchKernelGGL(k_spread_tile_i32<4>, dim3(ntile), dim3(BLOCK), 0, ctx->st, g, ctx->x, src,
-                       pm.bstart, pm.border, pm.brec, pm.greal32);
+    spread_i32<4>(ctx, g, pm, src);
   else if (f32)
     hipLaunchKernelGGL((k_spread_tile<4, false, float>), dim3(ntile), dim3(BLOCK), 0, ctx->st, g,
                        ctx->x, src, pm.bstart, pm.border, pm.greal32);
@@ -1026,6 +1044,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   if (f32) {
     CHK(dalloc(ctx, &pm.greal32, ng)); CHK(dalloc(ctx, &pm.spec32, nh));
     CHK(dalloc(ctx, &pm.brec, n));
+    CHK(dalloc(ctx, &pm.brec_atom, n));
     if (with_G) CHK(dalloc(ctx, &pm.G32, nh));
     CHK(dalloc(ctx, &pm.F32, nh));
   }
@@ -1064,7 +1083,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
 void free_grid(PmeGrid& pm) {
   void* bufs[] = {pm.brick, pm.bunord, pm.border, pm.bcount, pm.bfill, pm.bstart, pm.bsum,
                   pm.btotal, pm.greal64, pm.greal32, pm.spec64, pm.spec32, pm.G64, pm.G32,
-                  pm.F64, pm.F32, pm.brec, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
+                  pm.F64, pm.F32, pm.brec, pm.brec_atom, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
                   pm.tw32[2]};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1452,7 +1471,10 @@ void phi_gather_t(simox_ctx* ctx, bool f32, const PmeGrid& pm, double c_pme) {
   using T = GatherTile<GX, GY, GZ>;
   const Geom g = geom_of(ctx, pm);
   const int nt = T::tiles(g);
-  if (f32)
+  if (f32 && pm.spread32)  // the inner grid: weights from the spread's records
+    hipLaunchKernelGGL((k_gather_tile_rec<GX, GY, GZ>), dim3(nt), dim3(BLOCK), T::lds(4), ctx->st,
+                       g, pm.bstart, pm.border, pm.brec, pm.greal32, c_pme, ctx->phi);
+  else if (f32)
     hipLaunchKernelGGL((k_gather_tile<float, GX, GY, GZ>), dim3(nt), dim3(BLOCK), T::lds(4),
                        ctx->st, g, ctx->x, pm.bstart, pm.border, pm.greal32, c_pme, ctx->phi);
   else
@@ -1466,6 +1488,8 @@ int gather_attr(simox_ctx* ctx) {
   HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gather_tile<double, GX, GY, GZ>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gather_tile<float, GX, GY, GZ>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gather_tile_rec<GX, GY, GZ>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   return 0;
 }
@@ -1584,7 +1608,11 @@ int ncg_iter_body(simox_ctx* ctx) {
     hipLaunchKernelGGL(k_center_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                        ctx->qc, ctx->dscal + 0, ctx->gv, ctx->cv, ctx->sv);
     CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
-    if (f32 && pc.order == 2)
+    if (f32 && pc.order == 2 && pc.spread32)
+      hipLaunchKernelGGL(k_prec_finish6_rec, dim3(nb), dim3(BLOCK), 0, ctx->st, gp, ctx->dtab,
+                         pc.brec_atom, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv, pc.greal32,
+                         ctx->zv, ctx->part);
+    else if (f32 && pc.order == 2)
       hipLaunchKernelGGL((k_prec_finish6<2, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
                          ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
                          pc.greal32, ctx->zv, ctx->part);
@@ -1763,7 +1791,9 @@ int ncg_check(simox_ctx* ctx) {
   if (r.phase == NCG_WAIT_ITER) {
     r.it += r.batch;
     const bool ls_done = ctx->hscal[45] != 0.0;
-    ctx->ls_rounds = ls_done ? std::min(8, std::max(2, int(ctx->hscal[49]) + 1)) : 8;
+    ctx->ls_rounds = ls_done ? std::min(8, std::max(1 + ctx->ls_spare,
+                                                     int(ctx->hscal[49]) + ctx->ls_spare))
+                             : 8;
     r.rms = std::sqrt((fused_tail(ctx) ? ctx->hscal[50] : ctx->hscal[18]) / n);
     const bool descent = ctx->hscal[48] == 0.0;  // else restart from the true gradient
     if (descent && r.rms > r.target && r.it < ctx->maxit) {
@@ -2033,6 +2063,8 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     ctx->small_path = n <= LS_SMALL_MAX && !(e && std::strcmp(e, "0") == 0);
     const char* eb = std::getenv("SIMOX_ITER_BATCH");
     if (eb) ctx->max_batch = std::max(1, std::min(64, std::atoi(eb)));
+    const char* es = std::getenv("SIMOX_LS_SPARE");
+    if (es) ctx->ls_spare = std::max(0, std::min(4, std::atoi(es)));
   }
   Geom& g = ctx->g;
   g.n = n;
@@ -2098,6 +2130,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_PREC_SPREAD32")) ctx->prec_spread32 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_SPREAD_BIG")) ctx->spread_big = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_PACK")) ctx->inner_pack = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {

@@ -1216,6 +1216,39 @@ __global__ void __launch_bounds__(BLOCK) k_prec_finish6(Geom g, const Tables* __
     for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
 }
 
+// k_prec_finish6<2, float> with the preconditioner grid's atom-order stencil
+// records: the same FP32 weights as its 32-bit spread (the gather is the
+// spread's transpose), no FP64 grid coordinates per atom.
+__global__ void __launch_bounds__(BLOCK) k_prec_finish6_rec(Geom g, const Tables* __restrict__ tb,
+                                                            const float4* __restrict__ rec_atom,
+                                                            const int* __restrict__ id,
+                                                            const double* __restrict__ q,
+                                                            const double* __restrict__ gr,
+                                                            const double* __restrict__ g_old,
+                                                            const double* __restrict__ c,
+                                                            const float* __restrict__ grid,
+                                                            double* __restrict__ z,
+                                                            double* __restrict__ part) {
+  __shared__ double lds[6 * BLOCK / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
+    const double gi = gr[i], go = g_old[i];
+    const double zi = (gi / sd - c[i] * gather_cic_rec(g, rec_atom[i], grid)) / sd;
+    z[i] = zi;
+    acc[0] = zi;
+    acc[1] = zi * gi;
+    acc[2] = zi * go;
+    acc[3] = gi;
+    acc[4] = go;
+    acc[5] = gi * gi;
+  }
+  block_sum<6>(acc, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
+}
+
 // p = (z - mean z) + beta p (mean z in ls[11], beta in ls[7])
 __global__ void k_pcg_dir_proj(int n, const double* __restrict__ ls, const double* __restrict__ z,
                                double* __restrict__ p, float* __restrict__ p32) {
