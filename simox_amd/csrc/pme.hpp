@@ -1418,13 +1418,17 @@ __device__ inline double gather_cic_rec(const Geom& g, const float4& rc,
 // 2^-S, a few 1e-8 of max|v|.  Integer sums keep the grid independent of the
 // order of the atomics; the LDS tile is half the size of the 64-bit one.
 // Tile shape (TXg x TYg x TZg points, NT threads): 32 x 16 x 8 with 256
-// threads, or 32 x 32 x 16 with 1024 threads (one workgroup per CU, 2.3 halo
-// bricks per owned brick instead of 3.75 for the order-4 stencil).
+// threads, 32 x 32 x 16 with 1024 threads (one workgroup per CU, 2.3 halo
+// bricks per owned brick instead of 3.75 for the order-4 stencil), or, for
+// grids coarser than the atoms (more atoms than grid points), 16 x 16 x 8
+// with 1024 threads, so that the launch still has a workgroup per CU.
 template <int ORD, int TXg, int TYg, int TZg, int NT>
 struct SpreadI32 {
   static constexpr int TILEg = TXg * TYg * TZg;
   static constexpr int MAXBg = (TXg / BRK + 2) * (TYg / BRK + 2) * (TZg / BRK + 2);
-  static constexpr int MAXCg = NT == 1024 ? 16384 : 6144;
+  // slot-table capacity: the big tile's ~10^4 candidates, the small dense
+  // tile's (grids with more atoms than points: ~250 atoms per brick) ~2 10^4
+  static constexpr int MAXCg = NT != 1024 ? 6144 : (TILEg < 4096 ? 20480 : 16384);
   // slot table entries: 32 bits (brick << 12 | offset) for the big tile,
   // 16 bits (brick << 8 | offset) for the small one
   static constexpr bool WIDE = NT == 1024;

@@ -143,6 +143,8 @@ struct simox_ctx {
   bool inner_f16 = true;
   bool prec_spread32 = true;   // the preconditioner grid's FP32 spread in 32-bit fixed point
   bool spread_big = true;      // 32 x 32 x 16 tiles for the 32-bit spreads
+  int prec_div = 2;            // the preconditioner grid: the context's grid / prec_div per axis
+  int inner_div = 2;           // the coarse inner operator's grid: the context's grid / inner_div
   bool inner_spread32 = true;  // the coarse grid's spread in 32-bit fixed point  // with the coarse grid: FP16 matrix values in the inner matvec
   bool val16_valid = false;  // nb_val16 holds the current matrix
   double inner_rtol = 1e-4, inner_rtol_c = 3e-2;
@@ -898,11 +900,17 @@ int ensure_bricks(simox_ctx* ctx, PmeGrid& pm) {
 }
 
 // The 32-bit spread of a coarse FP32 grid: 32 x 32 x 16 tiles with 1024
-// threads where the grid holds them (SIMOX_SPREAD_BIG=0: 32 x 16 x 8 tiles
+// threads where the grid holds them, 16 x 16 x 8 tiles with 1024 threads on
+// grids with more atoms than points (SIMOX_SPREAD_BIG=0: 32 x 16 x 8 tiles
 // with 256 threads).
 template <int ORD>
 void spread_i32(simox_ctx* ctx, const Geom& g, PmeGrid& pm, const double* src) {
-  if (ctx->spread_big && g.K[0] >= 32 && g.K[1] >= 32 && g.K[2] >= 16) {
+  const double ng = double(g.K[0]) * g.K[1] * g.K[2];
+  if (ctx->spread_big && double(ctx->n) > ng && g.K[0] >= 16 && g.K[1] >= 16 && g.K[2] >= 8) {
+    using P = SpreadI32<ORD, 16, 16, 8, 1024>;
+    hipLaunchKernelGGL((k_spread_tile_i32<ORD, 16, 16, 8, 1024>), dim3(P::tiles(g)), dim3(1024),
+                       0, ctx->st, g, ctx->x, src, pm.bstart, pm.border, pm.brec, pm.greal32);
+  } else if (ctx->spread_big && g.K[0] >= 32 && g.K[1] >= 32 && g.K[2] >= 16) {
     using P = SpreadI32<ORD, 32, 32, 16, 1024>;
     hipLaunchKernelGGL((k_spread_tile_i32<ORD, 32, 32, 16, 1024>), dim3(P::tiles(g)), dim3(1024),
                        0, ctx->st, g, ctx->x, src, pm.bstart, pm.border, pm.brec, pm.greal32);
@@ -1322,9 +1330,10 @@ int prec_grid(simox_ctx* ctx, PmeGrid** out) {
   if (e && std::strcmp(e, "1") == 0) return 0;
   int K[3];
   FftPlan tmp;
+  const int div = ctx->prec_div;
   for (int d = 0; d < 3; ++d) {
-    K[d] = ctx->pm.K[d] / 2;
-    if (ctx->pm.K[d] % 2 || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) return 0;
+    K[d] = ctx->pm.K[d] / div;
+    if (ctx->pm.K[d] % div || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) return 0;
   }
   const int rc = init_grid(ctx, ctx->pc, K, false, prec_order(ctx));
   if (rc) {
@@ -1348,9 +1357,10 @@ int inner_grid(simox_ctx* ctx) {
   if (ctx->pi.ready) return 0;
   int K[3];
   FftPlan tmp;
+  const int div = ctx->inner_div;
   for (int d = 0; d < 3; ++d) {
-    K[d] = ctx->pm.K[d] / 2;
-    if (ctx->pm.K[d] % 2 || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) {
+    K[d] = ctx->pm.K[d] / div;
+    if (ctx->pm.K[d] % div || K[d] < 2 * BRK || K[d] % BRK || !make_plan(K[d], &tmp)) {
       ctx->inner_coarse = false;
       return 0;
     }
@@ -2131,6 +2141,12 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_PREC_SPREAD32")) ctx->prec_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_SPREAD_BIG")) ctx->spread_big = std::atoi(e) != 0;
+  // the preconditioner's grid: a quarter of the resolution from 2^23 points
+  // (C3, 256 x 256 x 512: same iteration count as half, 143 -> 135 ms per
+  // step), half below (C2, 96 x 96 x 192: a quarter needs more iterations)
+  ctx->prec_div = double(fft[0]) * fft[1] * fft[2] >= double(1 << 23) ? 4 : 2;
+  if (const char* e = std::getenv("SIMOX_PREC_DIV")) ctx->prec_div = std::atoi(e) == 4 ? 4 : 2;
+  if (const char* e = std::getenv("SIMOX_INNER_DIV")) ctx->inner_div = std::atoi(e) == 4 ? 4 : 2;
   if (const char* e = std::getenv("SIMOX_INNER_PACK")) ctx->inner_pack = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DROP")) {
