@@ -306,6 +306,15 @@ int simox_timing_enable(simox_ctx* ctx, int on);
    default).  Each timed launch costs two event records on the stream, so a
    throughput measurement times only the kernel it needs. */
 int simox_timing_select(simox_ctx* ctx, unsigned int mask);
+/* Average duration (ms) of one launch of an inner-solve kernel, reps launches
+   back to back between two HIP events on the context stream, on the buffers
+   the last converged charge solve left: which = 6 the sparse matvec H0 p,
+   10 the potential gather, 11 the spread onto the coarse inner grid (MIXED
+   with the coarse inner operator).  Events recorded around single launches
+   inside a step (simox_kernel_timing) also count part of the preceding
+   launch; this measures the kernel alone, as a kernel trace does.  Control /
+   inspection only (no counterpart in the reference). */
+int simox_time_kernel(simox_ctx* ctx, int which, int reps, double* ms);
 
 #ifdef __cplusplus
 }

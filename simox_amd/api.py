@@ -27,7 +27,7 @@ EXPORTS = [
     "simox_param_force_grad",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
     "simox_remove_rigid_motion", "simox_param_grad", "simox_batch_step",
-    "simox_cell_sort_noalloc",
+    "simox_cell_sort_noalloc", "simox_time_kernel",
 ]
 
 # simox_kernel_timing categories
@@ -127,6 +127,7 @@ def load_library(path=LIB_PATH):
     lib.simox_potential_setup.argtypes = [vp]
     lib.simox_fafb.argtypes = [vp]
     lib.simox_cell_sort_noalloc.argtypes = [i, P(d), d, vp, vp, vp, vp]
+    lib.simox_time_kernel.argtypes = [vp, i, i, P(d)]
     _lib = lib
     return lib
 
@@ -413,6 +414,13 @@ class Simox:
         """Time only the categories in `which` (None: all)."""
         mask = 0 if which is None else sum(1 << int(w) for w in which)
         self._check(self.lib.simox_timing_select(self.h, mask))
+
+    def time_kernel(self, which, reps=50):
+        """ms per launch of an inner-solve kernel (6 matvec, 10 gather, 11
+        inner spread), reps launches back to back on the last solve's buffers."""
+        ms = C.c_double()
+        self._check(self.lib.simox_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
+        return ms.value
 
     def kernel_time(self, which):
         ms = C.c_double()

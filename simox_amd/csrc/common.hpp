@@ -102,6 +102,7 @@ struct Geom {
   double alpha;        // Ewald a = 3.8/Rcut
   int K[3];            // PME grid
   int H;               // Kx/2 + 1
+  int xcd;             // pair kernels: XCD-aware block-to-atom-block mapping (pair_block)
 };
 
 __host__ __device__ inline int pidx(int si, int sj) { return (si - 1) * NSPEC + (sj - 1); }

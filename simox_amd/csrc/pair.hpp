@@ -264,6 +264,14 @@ __device__ inline void for_each_neighbor(const Geom& g, const int* __restrict__ 
 
 #define SX_ERR_SET(ptr, code) atomicCAS((ptr), 0, (code))
 
+// Atom block of this workgroup in the pair kernels.  Workgroups go to the 8
+// XCDs round-robin; with g.xcd the consecutive atom blocks (neighbouring
+// cells, which gather the same neighbour positions) run on one XCD and share
+// its L2.  Any bijection of the blocks gives the same per-atom results.
+__device__ inline int pair_block(const Geom& g) {
+  return g.xcd ? xcd_tile(blockIdx.x, gridDim.x) : int(blockIdx.x);
+}
+
 // Separation of sorted atoms i and j (exact reference arithmetic).
 __device__ inline void pair_sep(const Geom& g, const double* __restrict__ x, int i, int j,
                                 double rx[3], double& r2) {
@@ -313,7 +321,7 @@ __global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* _
                                                          int* __restrict__ err) {
   constexpr bool f32 = sizeof(T) == sizeof(float);
   __shared__ double lds[BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
   double eg[1] = {0.0};
   if (i < g.n) {
     const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
@@ -380,7 +388,7 @@ __global__ void __launch_bounds__(BLOCK) k_near_density(Geom g, const Tables* __
                                                         float* __restrict__ dF_out,
                                                         double* __restrict__ egroup_part) {
   __shared__ double lds[BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
   double eg[1] = {0.0};
   if (i < g.n) {
     const int si = id[i];
@@ -481,7 +489,7 @@ __global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restric
     for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x) hist[k] = 0u;
     __syncthreads();
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
   double es[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
     const int si = id[i];
@@ -549,7 +557,7 @@ __global__ void __launch_bounds__(BLOCK) k_force_near(Geom g, const Tables* __re
                                                       double* __restrict__ f,
                                                       double* __restrict__ e_part) {
   __shared__ double lds[3 * BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
   double es[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
     const int si = id[i];
@@ -744,7 +752,7 @@ __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __r
                                                        const int* __restrict__ col,
                                                        T* __restrict__ val,
                                                        double* __restrict__ b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   const int si = id[i];
   const T alpha = T(g.alpha);

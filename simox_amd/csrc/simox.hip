@@ -153,8 +153,9 @@ struct simox_ctx {
   // inner iterations per host check: predicted from the inner solve's rate,
   // at most max_batch (SIMOX_ITER_BATCH, 1 = a check after every iteration)
   int max_batch = 16;
-  // Newton rounds launched beyond the last search's count (SIMOX_LS_SPARE)
-  int ls_spare = 1;
+  // Newton rounds launched beyond the last search's count (SIMOX_LS_SPARE;
+  // C3: 0 -> 132 ms per step, 1 -> 134 ms)
+  int ls_spare = 0;
   double last_rate = 0.0;  // residual reduction per iteration of the last inner solve
   double prec_rho = 0.0;  // density of 1/J per grid point the filter was built with
   double* spl_grad = nullptr;  // dE/dc of the spline-form EAM (NSPL * SPL_MAX)
@@ -170,6 +171,7 @@ struct simox_ctx {
   int* nb_col = nullptr;
   double* nb_val64 = nullptr;
   float* nb_val32 = nullptr;
+  bool val32_valid = false;
   // compacted FP16 matrix of the coarse-grid inner operator (MIXED)
   _Float16* nb_val16 = nullptr;
   int *nb_cnt16 = nullptr, *nb_col16 = nullptr;
@@ -1132,6 +1134,17 @@ int do_pme(simox_ctx* ctx, bool fetch_energy) {
 // ---------------------------------------------------------------------------
 // CTIP
 // ---------------------------------------------------------------------------
+// FP32 copy of the CTIP matrix (MIXED: the inner matvec on the context's own
+// grid and simox_phi; the coarse inner operator reads its packed matrix)
+int ensure_val32(simox_ctx* ctx) {
+  if (ctx->val32_valid || !use32(ctx)) return 0;
+  hipLaunchKernelGGL(k_ell_to_f32, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->n,
+                     ctx->cap, ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
+  HIPCHK(hipGetLastError());
+  ctx->val32_valid = true;
+  return 0;
+}
+
 int build_matrix(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
   CHK(ensure_nlist(ctx));
@@ -1141,9 +1154,7 @@ int build_matrix(simox_ctx* ctx) {
   hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
                      ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->bvec);
   tstop(ctx, 9, ev);
-  if (use32(ctx))
-    hipLaunchKernelGGL(k_ell_to_f32, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->n, ctx->cap,
-                       ctx->nb_cnt, ctx->nb_val64, ctx->nb_val32);
+  ctx->val32_valid = false;  // the FP32 copy is made where it is read (ensure_val32)
   ctx->val16_valid = false;
   ctx->pk_valid = false;
   if (use32(ctx) && ctx->inner_coarse && ctx->inner_f16) {
@@ -1517,6 +1528,32 @@ int phi_gather(simox_ctx* ctx, bool f32, const PmeGrid* grid = nullptr) {
   return 0;
 }
 
+// v = H0 p of the inner iteration (the sparse part reads the packed SELL-4
+// matrix of the coarse inner operator, its FP16 copy, or the FP32 / FP64 ELL
+// rows), with the block partials p.v, p.g, sum p^2 pen''(q)
+void launch_matvec(simox_ctx* ctx, bool coarse) {
+  const int nb = grid_blocks(ctx->n);
+  const bool f32 = use32(ctx);
+  if (coarse && ctx->val16_valid && ctx->pk_valid) {
+    hipLaunchKernelGGL(k_h0matvec_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
+                       ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk), ctx->apv,
+                       ctx->part);
+  } else if (coarse && ctx->val16_valid) {
+    hipLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
+                       ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
+  } else if (f32) {
+    hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
+                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
+  } else {
+    hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
+                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
+  }
+}
+
 // The multi-block path with the long-range preconditioner runs the fused
 // iteration tail (k_center_scale ... k_pcg_dir_proj).
 bool fused_tail(const simox_ctx* ctx) { return !ctx->small_path && ctx->prec_on; }
@@ -1535,24 +1572,7 @@ int ncg_iter_body(simox_ctx* ctx) {
   CHK(phi_gather(ctx, f32, ig));
   tstop(ctx, 10, ev);
   tstart(ctx, 6, &ev);
-  if (ig && ctx->val16_valid && ctx->pk_valid) {
-    hipLaunchKernelGGL(k_h0matvec_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
-                       ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk), ctx->apv,
-                       ctx->part);
-  } else if (ig && ctx->val16_valid) {
-    hipLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
-                       ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
-  } else if (f32) {
-    hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
-                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
-  } else {
-    hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
-                       ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
-  }
+  launch_matvec(ctx, ig != nullptr);
   tstop(ctx, 6, ev);
   hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
                      ctx->dscal + 22, ls);
@@ -1753,6 +1773,7 @@ int ncg_begin(simox_ctx* ctx) {
   // brick lists of the inner grid for these positions, here rather than in
   // the first iteration (which a batched solve captures into its graph)
   if (use32(ctx) && ctx->inner_coarse && ctx->pi.ready) CHK(ensure_bricks(ctx, ctx->pi));
+  if (use32(ctx) && !(ctx->inner_coarse && ctx->pi.ready && ctx->val16_valid)) CHK(ensure_val32(ctx));
   return ncg_issue_restart(ctx);
 }
 
@@ -2146,6 +2167,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   // step), half below (C2, 96 x 96 x 192: a quarter needs more iterations)
   ctx->prec_div = double(fft[0]) * fft[1] * fft[2] >= double(1 << 23) ? 4 : 2;
   if (const char* e = std::getenv("SIMOX_PREC_DIV")) ctx->prec_div = std::atoi(e) == 4 ? 4 : 2;
+  // XCD-aware atom blocks in the pair kernels: measured slower at C3 (k_force
+  // 5.0 -> 6.2 ms, k_ctip_matrix 3.2 -> 4.1 ms), so off unless SIMOX_PAIR_XCD=1
+  ctx->g.xcd = 0;
+  if (const char* e = std::getenv("SIMOX_PAIR_XCD")) ctx->g.xcd = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_DIV")) ctx->inner_div = std::atoi(e) == 4 ? 4 : 2;
   if (const char* e = std::getenv("SIMOX_INNER_PACK")) ctx->inner_pack = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_LS_ADAPT")) ctx->ls_adapt = std::atoi(e) != 0;
@@ -2767,6 +2792,7 @@ int simox_pme(simox_ctx* ctx, double* erec) {
 int simox_phi(simox_ctx* ctx, double* phi) {
   if (!ctx || !need_state(ctx)) return SIMOX_ERR_ARG;
   if (!ctx->matrix_valid) CHK(build_matrix(ctx));
+  CHK(ensure_val32(ctx));
   CHK(grad_eval(ctx, ctx->q, nullptr, 0.0, true, use32(ctx)));
   return download_caller(ctx, ctx->phi, 1, phi);
 }
@@ -2978,6 +3004,40 @@ int simox_cell_sort_noalloc(int n, const double box[3], double rcut, const doubl
   int flag = 0;
   if (hipMemcpy(&flag, err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return SIMOX_ERR_HIP;
   return flag ? flag : SIMOX_OK;
+}
+
+// Average duration of one launch of an inner-solve kernel, reps launches back
+// to back between two events on the context stream, on the buffers the last
+// converged solve left (control / inspection; no counterpart in the reference).
+int simox_time_kernel(simox_ctx* ctx, int which, int reps, double* ms) {
+  if (!ctx || !ms || reps < 1 || ctx->mode == SIMOX_MODE_FAITHFUL || !ctx->matrix_valid)
+    return SIMOX_ERR_ARG;
+  const bool f32 = use32(ctx);
+  const PmeGrid* ig = (f32 && ctx->inner_coarse && ctx->pi.ready) ? &ctx->pi : nullptr;
+  if (which == 11 && !(ig && ctx->pi.spread32 && ctx->pi.bricks_valid)) return SIMOX_ERR_ARG;
+  if (which != 6 && which != 10 && which != 11) return SIMOX_ERR_ARG;
+  if (which == 6 && !(ig ? ctx->val16_valid : (f32 ? ctx->val32_valid : true))) return SIMOX_ERR_ARG;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+  HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
+  auto launch = [&]() -> int {
+    if (which == 6) launch_matvec(ctx, ig != nullptr);
+    else if (which == 10) CHK(phi_gather(ctx, f32, ig));
+    else spread_i32<4>(ctx, geom_of(ctx, ctx->pi), ctx->pi, ctx->pv);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  CHK(launch());  // warm
+  HIPCHK(hipEventRecord(a, ctx->st));
+  for (int r = 0; r < reps; ++r) CHK(launch());
+  HIPCHK(hipEventRecord(b, ctx->st));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = double(t) / reps;
+  return SIMOX_OK;
 }
 
 }  // extern "C"

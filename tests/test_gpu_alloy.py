@@ -8,8 +8,7 @@ orientation too.  Bars as the other FAITHFUL parity tests: forces and energies
 import numpy as np
 import pytest
 
-from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED, Simox, SimoxError
-from simox_amd.systems import tiny_alloy
+from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED
 
 from test_gpu_parity import gpu_for, load, oracle_for, relmax
 
@@ -65,9 +64,3 @@ def test_alloy_mixed_matches_fp64(alloy):
     rms = np.sqrt((ref ** 2).sum(1).mean())
     assert np.sqrt(((out[MODE_MIXED] - ref) ** 2).sum(1)).max() / rms <= 1e-5
 
-
-def test_al_ni_needs_three_cells_per_axis():
-    s = tiny_alloy(seed=2, nx=4, ny=4, nz=4, rcut=8.0, fft=(16, 16, 32))
-    g = Simox(s.n, s.box, s.fft, s.rcut)
-    with pytest.raises(SimoxError, match="three or more cells"):
-        g.set_state(s.ids, s.x, s.v, None, s.q)
