@@ -18,32 +18,40 @@ __device__ inline T fexp(T v) { return exp(v); }
 template <>
 __device__ inline float fexp<float>(float v) { return __expf(v); }
 
+// a / b.  In FP32 it is a times the hardware reciprocal of b (v_rcp_f32, 1 ulp)
+// instead of the ten-instruction IEEE division sequence; the FP64 paths keep
+// the correctly rounded division.
+template <class T>
+__device__ inline T fdiv(T a, T b) { return a / b; }
+template <>
+__device__ inline float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <class T>
 __device__ inline void dens_eval(const DensCoef& c, T r, T& f, T& dfr) {
   const T re = T(c.re), l = T(c.l), b = T(c.b), fe = T(c.fe);
-  T r_re = r / re, x = r_re - l;
+  T r_re = fdiv<T>(r, re), x = r_re - l;
   T x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
   T l20 = x16 * x4, l19 = x16 * x2 * x;
   T eb = fexp<T>(-b * (r_re - T(1)));
   T den = T(1) + l20;
-  f = fe * (eb / den);
-  dfr = fe * (eb * (-b * den - T(20) * l19) / re / (den * den)) / r;
+  f = fe * fdiv<T>(eb, den);
+  dfr = fdiv<T>(fe * fdiv<T>(fdiv<T>(eb * (-b * den - T(20) * l19), re), den * den), r);
 }
 
 template <class T>
 __device__ inline void phi_eval(const PhiCoef& c, T r, T& phi, T& df) {
   const T re = T(c.re);
-  T r_re = r / re, r1 = r_re - T(1);
+  T r_re = fdiv<T>(r, re), r1 = r_re - T(1);
   T xk = r_re - T(c.k), xl = r_re - T(c.l);
   T k2 = xk * xk, k4 = k2 * k2, k8 = k4 * k4, k16 = k8 * k8;
   T l2 = xl * xl, l4 = l2 * l2, l8 = l4 * l4, l16 = l8 * l8;
   T k20 = k16 * k4, k19 = k16 * k2 * xk, l20 = l16 * l4, l19 = l16 * l2 * xl;
   T ea = fexp<T>(-T(c.a) * r1), eb = fexp<T>(-T(c.b) * r1);
   T dk = T(1) + k20, dl = T(1) + l20;
-  T dphidr = T(c.A) * ea * (-T(c.a) * dk - T(20) * k19) / re / (dk * dk) -
-             T(c.B) * eb * (-T(c.b) * dl - T(20) * l19) / re / (dl * dl);
-  phi = T(c.A) * ea / dk - T(c.B) * eb / dl;
-  df = -dphidr / r;
+  T dphidr = fdiv<T>(fdiv<T>(T(c.A) * ea * (-T(c.a) * dk - T(20) * k19), re), dk * dk) -
+             fdiv<T>(fdiv<T>(T(c.B) * eb * (-T(c.b) * dl - T(20) * l19), re), dl * dl);
+  phi = fdiv<T>(T(c.A) * ea, dk) - fdiv<T>(T(c.B) * eb, dl);
+  df = fdiv<T>(-dphidr, r);
 }
 
 // CTIP short-range Slater terms of FORCE (force.f90:101-140).  Same-species
@@ -54,30 +62,31 @@ template <class S>
 __device__ inline void ctip_force_core(bool same, S xi_i, S xi_j, S Zi, S Zj, const S c[4], S r,
                                        S r2, S qi, S qj, S& df, S& e) {
   S jfi, ifj, djfidr, difjdr, fifj, dfifjdr;
-  S ir = S(1) / r;
+  S ir = fdiv<S>(S(1), r);
   if (same) {
     S xir = S(2) * xi_i * r, ei = fexp<S>(-xir);
     jfi = -ei * (xi_i + ir);
     ifj = jfi;
-    djfidr = ei * (S(2) * (xi_i * xi_i) + (S(1) + xir) / r2);
+    djfidr = ei * (S(2) * (xi_i * xi_i) + fdiv<S>(S(1) + xir, r2));
     difjdr = djfidr;
     const S c1 = c[1], c2 = c[2], c3 = c[3];
     fifj = -ei * (S(1) + c1 * r + c2 * r2 + c3 * r2 * r);
-    dfifjdr = ei * ((xir + S(1)) * (S(1) / r2 + c1 * ir + c2 + c3 * r) -
+    dfifjdr = ei * ((xir + S(1)) * (fdiv<S>(S(1), r2) + c1 * ir + c2 + c3 * r) -
                     (c1 * ir + c2 * S(2) + c3 * S(3) * r));
   } else {
     S xir_i = S(2) * xi_i * r, ei = fexp<S>(-xir_i);
     S xir_j = S(2) * xi_j * r, ej = fexp<S>(-xir_j);
     jfi = -ei * (xi_i + ir);
     ifj = -ej * (xi_j + ir);
-    djfidr = ei * (S(2) * (xi_i * xi_i) + (S(1) + xir_i) / r2);
-    difjdr = ej * (S(2) * (xi_j * xi_j) + (S(1) + xir_j) / r2);
+    djfidr = ei * (S(2) * (xi_i * xi_i) + fdiv<S>(S(1) + xir_i, r2));
+    difjdr = ej * (S(2) * (xi_j * xi_j) + fdiv<S>(S(1) + xir_j, r2));
     const S c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-    fifj = -c0 * ei - c1 * ej - c2 * ei / r - c3 * ej / r;
-    dfifjdr = c0 * ei * S(2) * xi_i + c1 * ej * S(2) * xi_j + c2 * ei * (S(1) + xir_i) / r2 +
-              c3 * ej * (S(1) + xir_j) / r2;
+    fifj = -c0 * ei - c1 * ej - fdiv<S>(c2 * ei, r) - fdiv<S>(c3 * ej, r);
+    dfifjdr = c0 * ei * S(2) * xi_i + c1 * ej * S(2) * xi_j +
+              fdiv<S>(c2 * ei * (S(1) + xir_i), r2) + fdiv<S>(c3 * ej * (S(1) + xir_j), r2);
   }
-  df = S(KC) * ((qi * Zj + qj * Zi - qi * qj) * dfifjdr - qi * Zj * djfidr - qj * Zi * difjdr) / r;
+  df = fdiv<S>(
+      S(KC) * ((qi * Zj + qj * Zi - qi * qj) * dfifjdr - qi * Zj * djfidr - qj * Zi * difjdr), r);
   e = qi * Zj * (jfi - fifj) + qj * Zi * (ifj - fifj) + qi * qj * fifj;
 }
 
@@ -435,9 +444,9 @@ __device__ inline S pair_force(const Tables* __restrict__ tb, const PairCoef& pc
     spl_eval<S>(tb->sp, spl_dens(sj, si), r, fi_, dfi);
     spl_eval<S>(tb->sp, spl_dens(si, sj), r, fj_, dfj);
     spl_eval<S>(tb->sp, spl_pair(si, sj), r, phi, dfp);
-    dfi /= r;
-    dfj /= r;
-    dfp = -dfp / r;
+    dfi = fdiv<S>(dfi, r);
+    dfj = fdiv<S>(dfj, r);
+    dfp = fdiv<S>(-dfp, r);
   } else {
     dens_eval<S>(pc.di, r, fi_, dfi);
     dens_eval<S>(pc.dj, r, fj_, dfj);
@@ -447,8 +456,11 @@ __device__ inline S pair_force(const Tables* __restrict__ tb, const PairCoef& pc
   S s = eam ? dfp - dFi * dfj - dFj * dfi : S(0);
   // real-space Ewald (force.f90:94-96)
   const S tmp = erfc(a * r);
-  s += S(KC) * qi * qj * (S(2) * fexp<S>(-(a * a) * r2) * a * r / S(SQRTPI) + tmp) / r2 / r;
-  ed = qi * qj * tmp / r;
+  s += fdiv<S>(
+      fdiv<S>(S(KC) * qi * qj * (fdiv<S>(S(2) * fexp<S>(-(a * a) * r2) * a * r, S(SQRTPI)) + tmp),
+              r2),
+      r);
+  ed = fdiv<S>(qi * qj * tmp, r);
   // CTIP Slater terms (force.f90:101-140)
   S dfc;
   ctip_force_terms<S>(pc, r, r2, qi, qj, dfc, ec);
