@@ -291,10 +291,11 @@ def ncg(grad, q0, maxit=0):
 
 
 def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None,
-                  t_given=600.0, alpha=0.1, zshift=0.0):
+                  t_given=600.0, alpha=0.1, zshift=0.0, reps=0):
     """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90.
     t_given (K) is the Berendsen target; alpha the Langevin friction (run at T = 0);
-    zshift moves z before VV_NVE1_REFLECT."""
+    zshift moves z before VV_NVE1_REFLECT.  reps > 0 also times that many
+    CELL_SORT + FORCE repetitions: out["ref_time"] = (reps, seconds)."""
     import tempfile
     if not os.path.exists(REF_DRIVER):
         raise RuntimeError("reference build absent (oracle/_ref/ref_driver)")
@@ -319,8 +320,14 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
             np.asfortranarray(sym).T.tofile(fh)
             np.asarray(phi, dtype=np.float64).tofile(fh)
     env = dict(os.environ)
-    subprocess.run(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$1" "$2"', REF_DRIVER, fin, fout],
-                   check=True, env=env)
+    r = subprocess.run(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$1" "$2" "$3"', REF_DRIVER,
+                        fin, fout, str(int(reps))], check=True, env=env, stdout=subprocess.PIPE,
+                       text=True)
+    ref_time = None
+    for line in r.stdout.splitlines():
+        if line.startswith("REF_TIME"):
+            _, k, sec = line.split()
+            ref_time = (int(k), float(sec))
     raw = open(fout, "rb").read()
     off = 0
 
@@ -357,4 +364,6 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
         out[k] = take(np.float64, 3 * n).reshape(n, 3)
     if dudq_inputs is not None:
         out["g"] = take(np.float64, n)
+    if reps:
+        out["ref_time"] = ref_time
     return out

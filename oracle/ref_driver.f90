@@ -21,6 +21,10 @@
 ! PME, PHI_MESH, CTIP_SOLVER and BRENT_SOLVER call FFTE's ZFFT3D and are not
 ! linked (link-time optimisation drops them because nothing here calls them).
 !
+! With a third argument R > 0 it then times R repetitions of the per-step pair
+! work the reference can run here, CELL_SORT + FORCE (main.f90:41-61 without
+! the CTIP solve and PME, which need FFTE), and prints "REF_TIME R seconds".
+!
 ! Input  (stream file, native endian):
 !   int32 n, fft(3), nnb; real64 box(3), rcut, dt, t_given (K), alpha, zshift
 !   int32 id(n); real64 x(3,n), v(3,n), f(3,n), q(n)
@@ -53,7 +57,9 @@ real(dp), allocatable :: xjz(:,:), asym(:,:), sym(:,:), phi(:), qmx(:,:), g(:)
 integer(gi), allocatable :: ids(:), list(:,:), nlist(:)
 integer(gi) :: n, fft(3), nnb, i, m
 real(dp) :: box(3), rcut, dt, t_given, alpha, zshift
-character(len=512) :: fin, fout
+character(len=512) :: fin, fout, arg
+integer(8) :: c0, c1, crate
+integer :: reps
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
@@ -157,4 +163,18 @@ if (nnb > 0) then
 end if
 close(10)
 close(11)
+
+if (command_argument_count() >= 3) then
+   call get_command_argument(3, arg)
+   read(arg, *) reps
+   if (reps > 0) then
+      call system_clock(c0, crate)
+      do m = 1, reps
+         call cell_sort(nset, cell, q)
+         call force(nset, eam, oxygn, sys, param, pair, q, cell)
+      end do
+      call system_clock(c1)
+      write(*, '(A,I8,ES16.8)') 'REF_TIME ', reps, dble(c1 - c0)/dble(crate)
+   end if
+end if
 end program ref_driver

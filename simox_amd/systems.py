@@ -128,7 +128,10 @@ def al_o_slab(nx, ny, nz, n_o, seed, T, fft, rcut, vacuum=20.0, o_layer=(2.0, 6.
 def config(name, seed=1):
     """The named workloads.  C1: 6x6x8 Al (1152) + 64 O, 300 K, grid 32x32x96.
     C2: 24x24x32 Al (73,728) + 4,096 O, 600 K, grid 96x96x192.
-    C3: 64x64x60 Al (983,040) + 65,536 O, 900 K, grid 256x256x512."""
+    C3: 64x64x60 Al (983,040) + 65,536 O, 900 K, grid 256x256x512.
+    c3_sample: a sixteenth of C3 by the same recipe (16x16x60 Al + 4,096 O,
+    grid 64x64x512: the same atoms and grid points per unit of surface), for
+    the CPU timings."""
     if name == "c1":
         return al_o_slab(6, 6, 8, 64, seed, 300.0, (32, 32, 96), 8.0, vacuum=20.0,
                          o_layer=(2.0, 6.0), o_dmin=2.0, name="c1")
@@ -141,6 +144,9 @@ def config(name, seed=1):
     if name == "c3":
         return al_o_slab(64, 64, 60, 65536, seed, 900.0, (256, 256, 512), 10.0, vacuum=28.0,
                          o_layer=(2.0, 10.0), o_dmin=1.5, name="c3")
+    if name == "c3_sample":
+        return al_o_slab(16, 16, 60, 4096, seed, 900.0, (64, 64, 512), 10.0, vacuum=28.0,
+                         o_layer=(2.0, 10.0), o_dmin=1.5, name="c3_sample")
     raise KeyError(name)
 
 

@@ -100,3 +100,20 @@ def test_neighbor_overflow_is_reported(oracle_mod):
     o.cell_sort(s.x)
     with pytest.raises(RuntimeError):
         o.force(s.ids, s.x, s.q)
+
+
+def test_reference_timing_run_keeps_its_outputs(oracle_mod):
+    """The CPU baseline's timing run of the reference build (CELL_SORT + FORCE
+    repeated after the normal outputs) reports its repetitions and leaves the
+    golden outputs as they are."""
+    if not os.path.exists(oracle_mod.REF_DRIVER):
+        pytest.skip("reference build absent (oracle/_ref/ref_driver)")
+    d = load(os.path.join(HERE, "golden", "tiny_s1.npz"))
+    args = (d["ids"], d["x"], d["v"], d["f0"], d["q"], d["box"], d["fft"], d["rcut"], d["dt"])
+    kw = dict(t_given=float(d["t_given"]), alpha=float(d["alpha"]), zshift=float(d["zshift"]))
+    r = oracle_mod.run_reference(*args, reps=3, **kw)
+    k, sec = r["ref_time"]
+    assert k == 3 and sec > 0.0
+    np.testing.assert_array_equal(r["f"], d["ref_f"])
+    np.testing.assert_array_equal(r["v2"], d["ref_v2"])
+    assert "ref_time" not in oracle_mod.run_reference(*args, **kw)
