@@ -23,6 +23,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU: 256 CUs x 64 lanes x 2 flop (FMA) x 2.4 GHz, FP64 and non-packed FP32
+VALU_PEAK_TFLOPS = 78.6
+# canonical work of the pair kernels, flop-equivalents per in-cutoff pair
+# counted once (the reference formulas; exp, erfc and sqrt count 4 each)
+PAIR_FLOPS = {7: 200, 8: 40, 9: 80}
 FORCE_ERR_BAR = 1e-5  # the metric's relative force error bar (max_i |dF_i| / RMS |F|)
 KERNELS = {0: "k_spread_tile", 1: "k_fft_x_r2c", 2: "k_fft_y (fwd + inv)",
            3: "k_fft_z_fused", 4: "k_fft_x_c2r", 5: "k_gather_force",
@@ -479,6 +484,28 @@ def inner_operator(s, mode):
     return "FP32 on the reference's grid"
 
 
+def pair_valu(g, s, mode, nb_entries, breakdown):
+    """The pair kernels against the VALU roofline: canonical flops (each
+    in-cutoff pair once) over the kernel's duration, 5 launches back to back on
+    the current positions.  The kernels evaluate every pair from both of its
+    atoms (owner computes, no scatter), so they execute twice that work."""
+    if mode == "faithful":
+        return None
+    pairs = nb_entries / 2
+    out = {"peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "pairs": pairs,
+           "note": "canonical flop-equivalents per pair: force 200, density 40, CTIP matrix "
+                   "80; each pair is evaluated from both atoms, so the executed work is twice "
+                   "the canonical; force and density include their FP64 near-pair passes "
+                   "in MIXED"}
+    for w in (7, 8, 9):
+        ms = g.time_kernel(w, 5)
+        tf = PAIR_FLOPS[w] * pairs / (ms / 1e3) / 1e12
+        out[KERNELS[w]] = {"flop_per_pair": PAIR_FLOPS[w], "avg_launch_ms": ms,
+                           "ms_in_step": breakdown[w][0], "achieved": tf,
+                           "frac": tf / VALU_PEAK_TFLOPS}
+    return out
+
+
 def accuracy_vs_fp64(s, g_mixed, args):
     """Force / charge / energy error of the mixed path against the FP64 path on
     the same state: positions and velocities of the mixed run, FP64 charges
@@ -586,6 +613,7 @@ def main():
         timed_as = ("HIP events around 50 back-to-back launches on the timed region's last "
                     "solve buffers (context stream)")
     achieved = bytes_launch / per_launch_s / 1e9
+    valu = pair_valu(g, s, args.mode, nb_entries, breakdown)
     traffic = measured_traffic(kernel_name(dom, args.mode, s), args.config, args.mode)
     value = s.n * args.steps / elapsed
     out = {
@@ -609,6 +637,7 @@ def main():
                      "avg_launch_ms": per_launch_s * 1e3, "timed_as": timed_as,
                      "avg_launch_ms_in_steps": per_launch_events_s * 1e3,
                      "launches_in_steps": launches},
+        "valu": valu,
         "kernels_ms": {kernel_name(w, args.mode, s): round(breakdown[w][0], 3) for w in KERNELS
                        if kernel_name(w, args.mode, s) != "-"},
         "kernels_launches": {kernel_name(w, args.mode, s): breakdown[w][1] for w in KERNELS

@@ -310,7 +310,10 @@ int simox_timing_select(simox_ctx* ctx, unsigned int mask);
    back to back between two HIP events on the context stream, on the buffers
    the last converged charge solve left: which = 6 the sparse matvec H0 p,
    10 the potential gather, 11 the spread onto the coarse inner grid (MIXED
-   with the coarse inner operator).  Events recorded around single launches
+   with the coarse inner operator); or of a pair kernel on the current
+   positions, which rewrites what it wrote last: 7 the force pass (with its
+   FP64 near-pair pass in MIXED; not while the RDF is on), 8 the neighbour
+   list + density pass, 9 the CTIP matrix.  Events recorded around single launches
    inside a step (simox_kernel_timing) also count part of the preceding
    launch; this measures the kernel alone, as a kernel trace does.  Control /
    inspection only (no counterpart in the reference). */

@@ -417,7 +417,9 @@ class Simox:
 
     def time_kernel(self, which, reps=50):
         """ms per launch of an inner-solve kernel (6 matvec, 10 gather, 11
-        inner spread), reps launches back to back on the last solve's buffers."""
+        inner spread) on the last solve's buffers, or of a pair kernel (7 force,
+        8 neighbour list + density, 9 CTIP matrix) on the current positions:
+        reps launches back to back."""
         ms = C.c_double()
         self._check(self.lib.simox_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
         return ms.value

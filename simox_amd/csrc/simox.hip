@@ -501,11 +501,8 @@ int do_cell_sort(simox_ctx* ctx) {
 // K2-K4: FORCE
 // ---------------------------------------------------------------------------
 // Neighbour list + EAM density/embedding (once per set of positions).
-int ensure_nlist(simox_ctx* ctx) {
-  if (ctx->nlist_valid) return 0;
+void launch_nlist(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
-  hipEvent_t ev{};
-  tstart(ctx, 8, &ev);
   if (use32(ctx)) {
     hipLaunchKernelGGL(k_nlist_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
@@ -520,22 +517,23 @@ int ensure_nlist(simox_ctx* ctx) {
                        ctx->nb_col, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF64,
                        ctx->part_eg, ctx->derr);
   }
+}
+
+int ensure_nlist(simox_ctx* ctx) {
+  if (ctx->nlist_valid) return 0;
+  hipEvent_t ev{};
+  tstart(ctx, 8, &ev);
+  launch_nlist(ctx);
   tstop(ctx, 8, ev);
   HIPCHK(hipGetLastError());
   ctx->nlist_valid = true;
   return 0;
 }
 
-int do_force(simox_ctx* ctx, bool fetch_energies) {
+// K4 over the list (block partials of k_force in ep rows [0, nb), of
+// k_force_near in [nb, 2 nb)).
+void launch_force(simox_ctx* ctx, double* ep) {
   const int nb = grid_blocks(ctx->n);
-  CHK(ensure_nlist(ctx));
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part_eg, nb, 1, 0,
-                     ctx->dscal + 0);
-  // block partials of k_force in rows [0, nb), of k_force_near in [nb, 2 nb)
-  CHK(ensure_part(ctx, size_t(nb) * 7));
-  double* ep = ctx->part + nb;
-  hipEvent_t ev{};
-  tstart(ctx, 7, &ev);
   unsigned long long* rdf = ctx->rdf_on ? ctx->rdf : nullptr;
   const size_t sh = rdf ? RDF_BINS * sizeof(unsigned) : 0;
   const double dr = ctx->rdf_dr;
@@ -551,6 +549,19 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
                        ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF64, ctx->f,
                        ep, rdf, dr);
   }
+}
+
+int do_force(simox_ctx* ctx, bool fetch_energies) {
+  const int nb = grid_blocks(ctx->n);
+  CHK(ensure_nlist(ctx));
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part_eg, nb, 1, 0,
+                     ctx->dscal + 0);
+  // block partials of k_force in rows [0, nb), of k_force_near in [nb, 2 nb)
+  CHK(ensure_part(ctx, size_t(nb) * 7));
+  double* ep = ctx->part + nb;
+  hipEvent_t ev{};
+  tstart(ctx, 7, &ev);
+  launch_force(ctx, ep);
   tstop(ctx, 7, ev);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ep,
@@ -1145,14 +1156,19 @@ int ensure_val32(simox_ctx* ctx) {
   return 0;
 }
 
+void launch_ctip_matrix(simox_ctx* ctx) {
+  hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st,
+                     ctx->g, ctx->dtab, ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                     ctx->bvec);
+}
+
 int build_matrix(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
   CHK(ensure_nlist(ctx));
   CHK(check_err(ctx, 1));
   hipEvent_t ev{};
   tstart(ctx, 9, &ev);
-  hipLaunchKernelGGL(k_ctip_matrix<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                     ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->bvec);
+  launch_ctip_matrix(ctx);
   tstop(ctx, 9, ev);
   ctx->val32_valid = false;  // the FP32 copy is made where it is read (ensure_val32)
   ctx->val16_valid = false;
@@ -3010,18 +3026,25 @@ int simox_cell_sort_noalloc(int n, const double box[3], double rcut, const doubl
 // to back between two events on the context stream, on the buffers the last
 // converged solve left (control / inspection; no counterpart in the reference).
 int simox_time_kernel(simox_ctx* ctx, int which, int reps, double* ms) {
-  if (!ctx || !ms || reps < 1 || ctx->mode == SIMOX_MODE_FAITHFUL || !ctx->matrix_valid)
-    return SIMOX_ERR_ARG;
+  if (!ctx || !ms || reps < 1 || ctx->mode == SIMOX_MODE_FAITHFUL) return SIMOX_ERR_ARG;
+  const bool pair = which == 7 || which == 8 || which == 9;
+  if (pair ? !(ctx->nlist_valid && ctx->matrix_valid) : !ctx->matrix_valid) return SIMOX_ERR_ARG;
+  if (which == 7 && ctx->rdf_on) return SIMOX_ERR_ARG;  // would add to the histogram
+  if (pair) CHK(ensure_part(ctx, size_t(grid_blocks(ctx->n)) * 7));
   const bool f32 = use32(ctx);
   const PmeGrid* ig = (f32 && ctx->inner_coarse && ctx->pi.ready) ? &ctx->pi : nullptr;
   if (which == 11 && !(ig && ctx->pi.spread32 && ctx->pi.bricks_valid)) return SIMOX_ERR_ARG;
-  if (which != 6 && which != 10 && which != 11) return SIMOX_ERR_ARG;
+  if (which != 6 && which != 10 && which != 11 && !pair) return SIMOX_ERR_ARG;
   if (which == 6 && !(ig ? ctx->val16_valid : (f32 ? ctx->val32_valid : true))) return SIMOX_ERR_ARG;
   hipEvent_t a, b;
   HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
   HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
   auto launch = [&]() -> int {
-    if (which == 6) launch_matvec(ctx, ig != nullptr);
+    // the pair kernels rewrite what they wrote last (same list, forces, matrix)
+    if (which == 7) launch_force(ctx, ctx->part + grid_blocks(ctx->n));
+    else if (which == 8) launch_nlist(ctx);
+    else if (which == 9) launch_ctip_matrix(ctx);
+    else if (which == 6) launch_matvec(ctx, ig != nullptr);
     else if (which == 10) CHK(phi_gather(ctx, f32, ig));
     else spread_i32<4>(ctx, geom_of(ctx, ctx->pi), ctx->pi, ctx->pv);
     HIPCHK(hipGetLastError());

@@ -134,6 +134,32 @@ def test_steps_are_deterministic(mode):
     assert out[0][1] == out[1][1]
 
 
+@pytest.mark.parametrize("mode", [MODE_FP64, MODE_MIXED])
+def test_kernel_timing_leaves_the_state(mode):
+    """simox_time_kernel relaunches the pair kernels (7 force, 8 list +
+    density, 9 CTIP matrix) and the solve's matvec and gather (6, 10) on the
+    current buffers: the following steps are bit-identical to those of a
+    context that was not timed."""
+    s = tiny(seed=4)
+    out = []
+    for timed in (False, True):
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        g.set_charge_tol(1e-10, 2000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        g.force()
+        g.pme()
+        g.step(2, s.dt, 1, 1)
+        if timed:
+            for w in (7, 8, 9, 6, 10):
+                assert g.time_kernel(w, 3) > 0.0
+        st = g.step(2, s.dt, 1, 1)
+        out.append((g.get_state(), st))
+        g.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][1] == out[1][1]
+
+
 def test_split_cutoff_matches_oracle(oracle_mod):
     """Build-only split cutoff (EAM 6 A inside Rcut 8 A): FAITHFUL forces and
     energies against the oracle with the same split, 1e-10 relative."""
