@@ -3050,11 +3050,23 @@ int simox_time_kernel(simox_ctx* ctx, int which, int reps, double* ms) {
     HIPCHK(hipGetLastError());
     return 0;
   };
+  // the force pass overwrites f with the pair forces alone: keep the total
+  double* fkeep = nullptr;
+  const size_t fbytes = size_t(3) * ctx->n * sizeof(double);
+  if (which == 7) {
+    HIPCHK(hipMalloc((void**)&fkeep, fbytes));
+    HIPCHK(hipMemcpyAsync(fkeep, ctx->f, fbytes, hipMemcpyDeviceToDevice, ctx->st));
+  }
   CHK(launch());  // warm
   HIPCHK(hipEventRecord(a, ctx->st));
   for (int r = 0; r < reps; ++r) CHK(launch());
   HIPCHK(hipEventRecord(b, ctx->st));
+  if (fkeep) HIPCHK(hipMemcpyAsync(ctx->f, fkeep, fbytes, hipMemcpyDeviceToDevice, ctx->st));
   HIPCHK(hipEventSynchronize(b));
+  if (fkeep) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipFree(fkeep));
+  }
   float t = 0.f;
   HIPCHK(hipEventElapsedTime(&t, a, b));
   (void)hipEventDestroy(a);
