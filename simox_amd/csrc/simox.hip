@@ -3,6 +3,7 @@
 // state (one context = one system), the launch sequences, the charge-solver
 // control logic and the error mapping to the reference's STOP texts.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -286,6 +287,19 @@ void tstop(simox_ctx* ctx, int which, hipEvent_t a) {
   hipEvent_t b = tevent(ctx);
   (void)hipEventRecord(b, ctx->st);
   ctx->tl.push_back({which, a, b});
+}
+// Events for one kernel launched with hipExtLaunchKernel (the runtime records
+// them at the kernel's start and end: the kernel alone, as a kernel trace
+// times it); both null when the category is not timed or the stream is being
+// captured into a graph.
+void tkernel(simox_ctx* ctx, int which, hipEvent_t* a, hipEvent_t* b) {
+  *a = *b = nullptr;
+  if (!ctx->timing || !((ctx->timing_mask >> which) & 1u)) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+  *a = tevent(ctx);
+  *b = tevent(ctx);
+  ctx->tl.push_back({which, *a, *b});
 }
 void tresolve(simox_ctx* ctx) {
   for (auto& t : ctx->tl) {
@@ -1547,24 +1561,39 @@ int phi_gather(simox_ctx* ctx, bool f32, const PmeGrid* grid = nullptr) {
 // v = H0 p of the inner iteration (the sparse part reads the packed SELL-4
 // matrix of the coarse inner operator, its FP16 copy, or the FP32 / FP64 ELL
 // rows), with the block partials p.v, p.g, sum p^2 pen''(q)
-void launch_matvec(simox_ctx* ctx, bool coarse) {
+// ea / eb (may be null): events the runtime records at the kernel's own start
+// and end (hipExtLaunchKernel), so that they time the kernel alone.
+void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
+                   hipEvent_t eb = nullptr) {
   const int nb = grid_blocks(ctx->n);
   const bool f32 = use32(ctx);
   if (coarse && ctx->val16_valid && ctx->pk_valid) {
-    hipLaunchKernelGGL(k_h0matvec_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+    // groups in flight per lane and streaming loads (SIMOX_MV_VARIANT: 1 = 4
+    // groups non-temporal, the default; 0 = 4 groups, 2 = 8, 3 = 8
+    // non-temporal).  At the 1M-atom slab the non-temporal stream is 4 %
+    // faster and the whole step 4 % (the other kernels keep their L2 lines).
+    static const auto kv = [] {
+      const char* e = std::getenv("SIMOX_MV_VARIANT");
+      const int v = e ? std::atoi(e) : 1;
+      return v == 0   ? &k_h0matvec_sell4<4, false>
+             : v == 2 ? &k_h0matvec_sell4<8, false>
+             : v == 3 ? &k_h0matvec_sell4<8, true>
+                      : &k_h0matvec_sell4<4, true>;
+    }();
+    hipExtLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
                        ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk), ctx->apv,
                        ctx->part);
   } else if (coarse && ctx->val16_valid) {
-    hipLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+    hipExtLaunchKernelGGL(k_h0matvec16<false>, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->nb_cnt16,
                        ctx->nb_col16, ctx->nb_val16, ctx->nb_pk, ctx->apv, ctx->part);
   } else if (f32) {
-    hipLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+    hipExtLaunchKernelGGL((k_h0matvec<float, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
                        ctx->nb_cnt, ctx->nb_col, ctx->nb_val32, ctx->apv, ctx->part);
   } else {
-    hipLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
+    hipExtLaunchKernelGGL((k_h0matvec<double, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g,
                        ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->pv, ctx->pv, ctx->gv, ctx->phi,
                        ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->apv, ctx->part);
   }
@@ -1587,9 +1616,11 @@ int ncg_iter_body(simox_ctx* ctx) {
   tstart(ctx, 10, &ev);
   CHK(phi_gather(ctx, f32, ig));
   tstop(ctx, 10, ev);
-  tstart(ctx, 6, &ev);
-  launch_matvec(ctx, ig != nullptr);
-  tstop(ctx, 6, ev);
+  {
+    hipEvent_t ka{}, kb{};
+    tkernel(ctx, 6, &ka, &kb);
+    launch_matvec(ctx, ig != nullptr, ka, kb);
+  }
   hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
                      ctx->dscal + 22, ls);
   // exact line search, Newton with bracketing, on the device: one workgroup
@@ -3059,6 +3090,26 @@ int simox_time_kernel(simox_ctx* ctx, int which, int reps, double* ms) {
   }
   CHK(launch());  // warm
   HIPCHK(hipEventRecord(a, ctx->st));
+  if (which == 6) {
+    // the matvec alone: events recorded at each launch's start and end
+    std::vector<hipEvent_t> ke(2 * size_t(reps));
+    for (auto& e : ke) HIPCHK(hipEventCreate(&e));
+    for (int r = 0; r < reps; ++r) launch_matvec(ctx, ig != nullptr, ke[2 * r], ke[2 * r + 1]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b, ctx->st));
+    HIPCHK(hipEventSynchronize(b));
+    double tot = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      float t1 = 0.f;
+      HIPCHK(hipEventElapsedTime(&t1, ke[2 * r], ke[2 * r + 1]));
+      tot += t1;
+    }
+    for (auto& e : ke) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = tot / reps;
+    return SIMOX_OK;
+  }
   for (int r = 0; r < reps; ++r) CHK(launch());
   HIPCHK(hipEventRecord(b, ctx->st));
   if (fkeep) HIPCHK(hipMemcpyAsync(ctx->f, fkeep, fbytes, hipMemcpyDeviceToDevice, ctx->st));

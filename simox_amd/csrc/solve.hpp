@@ -813,8 +813,19 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec16(Geom g, const Tables* __re
   }
 }
 
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// One 16-byte group of the packed matrix; NT: a non-temporal (streaming) load,
+// so that the matrix stream does not evict the gathered vector from L2.
+template <bool NT>
+__device__ inline u32x4 pk_group(const uint4* __restrict__ a) {
+  if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a));
+  return *reinterpret_cast<const u32x4*>(a);
+}
+
 // k_h0matvec16 over the SELL-4 packed matrix: each lane streams its row as
-// 16-byte groups of 4 entries, 4 groups (16 entries) in flight.
+// 16-byte groups of 4 entries, U groups (4 U entries) in flight.
+template <int U, bool NT>
 __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* __restrict__ tb,
                                                           const int* __restrict__ id,
                                                           const double* __restrict__ q,
@@ -836,11 +847,10 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
     const uint4* row = pk + size_t(soff[s]) + lane;
     float sum = 0.f;
     int k = 0;
-    constexpr int U = 4;
     for (; k + U <= L; k += U) {
-      uint4 w[U];
+      u32x4 w[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) w[u] = row[size_t(k + u) * SLICE];
+      for (int u = 0; u < U; ++u) w[u] = pk_group<NT>(row + size_t(k + u) * SLICE);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         sum += pk_value(w[u].x) * pt[pk_col(w[u].x)];
@@ -850,7 +860,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
       }
     }
     for (; k < L; ++k) {
-      const uint4 w = row[size_t(k) * SLICE];
+      const u32x4 w = pk_group<NT>(row + size_t(k) * SLICE);
       sum += pk_value(w.x) * pt[pk_col(w.x)];
       sum += pk_value(w.y) * pt[pk_col(w.y)];
       sum += pk_value(w.z) * pt[pk_col(w.z)];
