@@ -291,11 +291,14 @@ def ncg(grad, q0, maxit=0):
 
 
 def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir=None,
-                  t_given=600.0, alpha=0.1, zshift=0.0, reps=0):
+                  t_given=600.0, alpha=0.1, zshift=0.0, reps=0, oxid=None):
     """Run the reference build (oracle/_ref/ref_driver) on one record; see ref_driver.f90.
     t_given (K) is the Berendsen target; alpha the Langevin friction (run at T = 0);
     zshift moves z before VV_NVE1_REFLECT.  reps > 0 also times that many
-    CELL_SORT + FORCE repetitions: out["ref_time"] = (reps, seconds)."""
+    CELL_SORT + FORCE repetitions: out["ref_time"] = (reps, seconds).
+    oxid = (n_o2, seed, z_new, q_crits): OXIDE_CHECK per q_crit (out["new_o2"])
+    and NEW_O2 after RANDOM_SEED(put = (seed, 1, ...)) (out["o2_x"], "o2_v",
+    "o2_id", "o2_q")."""
     import tempfile
     if not os.path.exists(REF_DRIVER):
         raise RuntimeError("reference build absent (oracle/_ref/ref_driver)")
@@ -319,6 +322,10 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
             np.asfortranarray(asym).T.tofile(fh)
             np.asfortranarray(sym).T.tofile(fh)
             np.asarray(phi, dtype=np.float64).tofile(fh)
+        if oxid is not None:
+            n_o2, seed, z_new, q_crits = oxid
+            np.array([n_o2, seed, len(q_crits)], dtype=np.int32).tofile(fh)
+            np.array([z_new, *q_crits], dtype=np.float64).tofile(fh)
     env = dict(os.environ)
     r = subprocess.run(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$1" "$2" "$3"', REF_DRIVER,
                         fin, fout, str(int(reps))], check=True, env=env, stdout=subprocess.PIPE,
@@ -364,6 +371,14 @@ def run_reference(ids, x, v, f, q, box, fft, rcut, dt, dudq_inputs=None, workdir
         out[k] = take(np.float64, 3 * n).reshape(n, 3)
     if dudq_inputs is not None:
         out["g"] = take(np.float64, n)
+    if oxid is not None:
+        n_o2, _, _, q_crits = oxid
+        m = 2 * n_o2
+        out["new_o2"] = take(np.int32, len(q_crits))
+        out["o2_x"] = take(np.float64, 3 * m).reshape(m, 3)
+        out["o2_v"] = take(np.float64, 3 * m).reshape(m, 3)
+        out["o2_id"] = take(np.int32, m)
+        out["o2_q"] = take(np.float64, m)
     if reps:
         out["ref_time"] = ref_time
     return out

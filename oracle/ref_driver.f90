@@ -41,6 +41,10 @@
 !   real64 xr(3,n), vr(3,n) after VV_NVE1_REFLECT; xr2(3,n), vr2(3,n) after
 !     REMOVE_RIGID_MOTION
 !   real64 g(n) (if nnb > 0)
+! Optional trailing input: int32 n_o2, seed, ncrit; real64 z_new, q_crit(ncrit)
+!   -> int32 new_o2(ncrit) of OXIDE_CHECK (util.f90:152-188) per q_crit, then
+!   NEW_O2 (datafmt.f90:98-139) after RANDOM_SEED(put = (seed, 1, 1, ...)):
+!   real64 x(3, 2 n_o2), v(3, 2 n_o2); int32 id(2 n_o2); real64 q(2 n_o2)
 program ref_driver
 use datafmt
 implicit none
@@ -59,7 +63,13 @@ integer(gi) :: n, fft(3), nnb, i, m
 real(dp) :: box(3), rcut, dt, t_given, alpha, zshift
 character(len=512) :: fin, fout, arg
 integer(8) :: c0, c1, crate
-integer :: reps
+integer :: reps, ios, nseed
+integer(gi) :: n_o2, seedv, ncrit
+real(dp) :: z_new
+real(dp), allocatable :: q_crit(:)
+integer, allocatable :: seed(:)
+type(tckt) :: tag
+type(ptcl), pointer :: qp(:)
 
 call get_command_argument(1, fin)
 call get_command_argument(2, fout)
@@ -160,6 +170,35 @@ if (nnb > 0) then
    end do
    call dudq(n, xjz, asym, sym, list, nlist, chg, qmx, phi, g)
    write(11) g
+end if
+
+! optional OXIDATION block: OXIDE_CHECK for each q_crit on the input record
+! (N_o2 last atoms), then NEW_O2 after RANDOM_SEED(put = (seed, 1, 1, ...))
+read(10, iostat=ios) n_o2, seedv, ncrit
+if (ios == 0) then
+   allocate(q_crit(ncrit))
+   read(10) z_new, q_crit
+   nset%n_o2 = n_o2
+   nset%z_new = z_new
+   do m = 1, ncrit
+      nset%q_crit = q_crit(m)
+      call oxide_check(nset, tag, q)
+      write(11) merge(1_gi, 0_gi, tag%new_o2)
+   end do
+   call random_seed(size=nseed)
+   allocate(seed(nseed))
+   seed = 1
+   seed(1) = seedv
+   call random_seed(put=seed)
+   sys%t_given = t_given*kb
+   allocate(qp(n))
+   qp(:) = q(:)
+   call new_o2(nset, sys, qp)
+   write(11) (qp(i)%xx(:), i = n + 1, n + 2*n_o2)
+   write(11) (qp(i)%xv(:), i = n + 1, n + 2*n_o2)
+   write(11) (qp(i)%id, i = n + 1, n + 2*n_o2)
+   write(11) (qp(i)%q, i = n + 1, n + 2*n_o2)
+   nset%npt_all = n
 end if
 close(10)
 close(11)

@@ -44,9 +44,12 @@ def build_fortran():
     bdir = os.path.join(fdir, "_build")
     os.makedirs(bdir, exist_ok=True)
     objs = []
-    for src in ("simox_gpu.f90", "simox_md.f90"):
+    for src in ("simox_gpu.f90", "simox_oxid.f90", "simox_md.f90"):
         o = os.path.join(bdir, src.replace(".f90", ".o"))
-        _run([FC, "-O2", "-c", os.path.join(fdir, src), "-o", o, "-module-dir", bdir], cwd=bdir)
+        # simox_oxid: no FMA contraction, as the reference build compiles its own
+        fl = ["-ffp-contract=off"] if src == "simox_oxid.f90" else []
+        _run([FC, "-O2", *fl, "-c", os.path.join(fdir, src), "-o", o, "-module-dir", bdir],
+             cwd=bdir)
         objs.append(o)
     exe = os.path.join(fdir, "simox_md")
     _run([FC, "-O2", "-o", exe, *objs, "-L" + HERE, "-lsimox_gpu", "-Wl,-rpath," + HERE,

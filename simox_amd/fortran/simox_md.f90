@@ -29,6 +29,7 @@
 program simox_md
   use iso_c_binding
   use simox_gpu
+  use simox_oxid, only: oxide_check, o2_dimers
   implicit none
   real(c_double), parameter :: tfm = 10.180505306645389d0, kb = 8.617343d-5
   character(len=256) :: dummy, runtype, tstat, rstart, fname, modearg
@@ -201,7 +202,9 @@ contains
     if (do_ctip) then
       rc = simox_remove_rigid_motion(ctx)
       call simox_check(ctx, rc)
-      call oxide_check()
+      rc = simox_get_state(ctx, c_loc(x), c_null_ptr, c_null_ptr, c_loc(q))
+      call simox_check(ctx, rc)
+      new_o2 = oxide_check(n, id, x, q, n_o2, q_crit)
       if (new_o2) call add_o2()
       rc = simox_ctip_solve(ctx, e_self, it, nev)
       call simox_check(ctx, rc)
@@ -215,57 +218,19 @@ contains
     st(1:7) = [e_self, e4(2), e_rec, e4(3), e4(1), e4(4), mv2]
   end subroutine oxidation_step
 
-  ! OXIDE_CHECK (util.f90:152-188): a new dimer when the highest Al is above
-  ! the last inserted oxygens or their largest charge is below q_crit.
-  subroutine oxide_check()
-    real(c_double) :: m_height, o_height, q_max
-    integer :: k
-    rc = simox_get_state(ctx, c_loc(x), c_null_ptr, c_null_ptr, c_loc(q))
-    call simox_check(ctx, rc)
-    m_height = 0.0d0
-    do k = 1, n
-      if (id(k) == 1) m_height = max(x(3, k), m_height)
-    end do
-    o_height = 0.0d0
-    q_max = -10.0d0
-    do k = n, n - 2 * n_o2 + 1, -1
-      if (id(k) == 4) then
-        o_height = max(x(3, k), o_height)
-        q_max = max(q(k), q_max)
-      end if
-    end do
-    new_o2 = m_height > o_height .or. q_max < q_crit
-  end subroutine oxide_check
-
-  ! NEW_O2 (datafmt.f90:98-139) with RANDOM_POSITION_O2 (util.f90:104-149):
-  ! n_o2 dimers (O-O 2 A) at height z_new, centres at least
-  ! sqrt(Lx Ly / n_o2) / 2 apart, random orientation (the y component uses
-  ! cos(phi) as there), v = (0, 0, -sqrt(3 kB T / 16)), f = 0, q = 0.  The
-  ! context is rebuilt for the new atom count.
+  ! NEW_O2 (datafmt.f90:98-139): n_o2 new dimers (o2_dimers: positions of
+  ! RANDOM_POSITION_O2, v = (0, 0, -sqrt(3 kB T / 16))) with f = 0, q = 0
+  ! appended; the context is rebuilt for the new atom count.
   subroutine add_o2()
-    real(c_double), parameter :: pi = 3.1415926535897931d0, l0 = 2.0d0
     real(c_double), allocatable, target :: xn(:,:), vn(:,:), fn(:,:), qn(:)
     integer(c_int), allocatable, target :: idn(:)
     character(len=3), allocatable :: eln(:)
-    real(c_double) :: cx(n_o2, 3), r, rmin2, phi, theta, d(3)
-    integer :: k, j, m, nn
-    logical :: close_pair
+    integer :: m, nn
     integer(c_long_long), target :: h(5, 5, 201)
+    real(c_double), allocatable :: xo(:,:), vo(:,:)
     if (n_o2 <= 0) return
-    rmin2 = (sqrt(box(1) * box(2) / dble(n_o2)) * 0.5d0)**2
-    if (rmin2 < l0**2) stop "=== Too many new oxygens ==="
-    do k = 1, n_o2
-      close_pair = .true.
-      do while (close_pair)
-        call random_number(r); cx(k, 1) = (r - 0.5d0) * box(1)
-        call random_number(r); cx(k, 2) = (r - 0.5d0) * box(2)
-        close_pair = .false.
-        do j = 1, k - 1
-          if ((cx(k, 1) - cx(j, 1))**2 + (cx(k, 2) - cx(j, 2))**2 < rmin2) close_pair = .true.
-        end do
-      end do
-    end do
-    cx(:, 3) = z_new
+    allocate(xo(3, 2 * n_o2), vo(3, 2 * n_o2))
+    call o2_dimers(n_o2, z_new, box(1), box(2), t_given * kb, xo, vo)
     rc = simox_get_state(ctx, c_loc(x), c_loc(v), c_loc(f), c_loc(q))
     call simox_check(ctx, rc)
     rc = simox_get_rdf(ctx, h, 1)
@@ -274,16 +239,9 @@ contains
     nn = n + 2 * n_o2
     allocate(xn(3, nn), vn(3, nn), fn(3, nn), qn(nn), idn(nn), eln(nn))
     xn(:, 1:n) = x; vn(:, 1:n) = v; fn(:, 1:n) = f; qn(1:n) = q; idn(1:n) = id; eln(1:n) = el
-    do k = 1, n_o2
-      call random_number(r); phi = r * pi * 2.0d0
-      call random_number(r); theta = r * pi
-      d = [l0 * sin(theta) * cos(phi), l0 * sin(theta) * cos(phi), l0 * cos(theta)]
-      m = n + 2 * k - 1
-      xn(:, m) = cx(k, :) + d
-      xn(:, m + 1) = cx(k, :) - d
-    end do
+    xn(:, n + 1:nn) = xo
+    vn(:, n + 1:nn) = vo
     do m = n + 1, nn
-      vn(:, m) = [0.0d0, 0.0d0, -sqrt(t_given * kb * 3.0d0 / 16.0d0)]
       fn(:, m) = 0.0d0
       qn(m) = 0.0d0
       idn(m) = 4

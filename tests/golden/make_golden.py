@@ -62,9 +62,42 @@ def one(name, s, seed, with_dudq=True):
     print(path, os.path.getsize(path))
 
 
+def oxidation(name, s, seed):
+    """OXIDE_CHECK (util.f90:152-188) and NEW_O2 (datafmt.f90:98-139) of the
+    reference build: per record, the new-dimer flag for each q_crit, then the
+    dimers NEW_O2 places after RANDOM_SEED(put = (seed, 1, 1, ...)).  Records:
+    the slab as built (its last O above the top Al), the same with those O
+    moved 3 A below the top Al, and a record whose last 2 n_o2 atoms include
+    Al (OXIDE_CHECK skips them)."""
+    rng = np.random.default_rng(seed)
+    q = -np.abs(rng.normal(size=s.n) * 0.3)
+    q_crits = [-5.0, -0.2, -0.1, 0.0, 5.0]
+    top_al = s.x[s.ids == 1, 2].max()
+    x_low = s.x.copy()
+    x_low[-4:, 2] = top_al - 3.0
+    ids_mix = s.ids.copy()
+    ids_mix[-1] = 1
+    recs = [(s.ids, s.x, 2, 7, 10.0), (s.ids, x_low, 2, 8, 11.5), (ids_mix, s.x, 3, 9, 12.0),
+            (s.ids, s.x, 1, 10, 9.0)]
+    out = dict(box=s.box, q=q, t_given=np.array(T_BEREND), q_crit=np.array(q_crits),
+               nrec=np.array(len(recs)))
+    for k, (ids, x, n_o2, sd, z_new) in enumerate(recs):
+        ref = run_reference(ids, x, s.v, np.zeros((s.n, 3)), q, s.box, s.fft, s.rcut, s.dt,
+                            t_given=T_BEREND, oxid=(n_o2, sd, z_new, q_crits))
+        out.update({"ids%d" % k: ids, "x%d" % k: x, "n_o2_%d" % k: np.array(n_o2),
+                    "seed%d" % k: np.array(sd), "z_new%d" % k: np.array(z_new),
+                    "ref_new_o2_%d" % k: ref["new_o2"], "ref_o2_x%d" % k: ref["o2_x"],
+                    "ref_o2_v%d" % k: ref["o2_v"], "ref_o2_id%d" % k: ref["o2_id"],
+                    "ref_o2_q%d" % k: ref["o2_q"]})
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     one("tiny_s1", tiny(seed=1), 11)
     one("tiny_s2", tiny(seed=2, nx=3, ny=3, nz=4, n_o=8, fft=(12, 12, 24), rcut=5.0), 12)
     one("c1_s1", config("c1", seed=1), 21)
     # Al slab with Ni and Fe substitutions: the AL_NI and METAL_HETERO branches
     one("alloy_s1", tiny_alloy(seed=1), 31)
+    oxidation("oxid_s1", tiny(seed=1), 41)
