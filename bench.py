@@ -639,7 +639,8 @@ def main():
                    "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
                    "inner_operator": inner_operator(s, args.mode),
-                   "fft_tiles": fft_tiles},
+                   "fft_tiles": fft_tiles,
+                   "inner_shapes": g.inner_shapes() if args.mode == "mixed" else None},
         "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode, s), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bytes_launch,

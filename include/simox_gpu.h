@@ -155,6 +155,17 @@ int simox_get_rdf(simox_ctx* ctx, long long* hist, int reset);
    the FP32 chain (zeros outside MIXED mode).  Shapes change no result. */
 int simox_autotune(simox_ctx* ctx);
 int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]);
+/* Launch shapes of the MIXED inner solve with the coarse inner operator,
+   which the context times on its own buffers at its first solve (environment
+   SIMOX_AUTOTUNE=0 keeps the defaults; SIMOX_MV_VARIANT, SIMOX_GATHER_SHAPE,
+   SIMOX_FFT_XYNT and SIMOX_FFT_NL fix one each): {packed matvec variant (0-3:
+   4 or 8 groups of 4 entries in flight per lane, odd: streaming loads),
+   record gather tile (0-5: 64x8x8, 32x8x8, 32x16x8, 64x16x8, 32x16x16,
+   64x16x16), inner grid's x-y pass threads per workgroup and z-pass lines
+   per workgroup, the same two of the preconditioner's grid (0: no plane
+   chain)}.  Every choice gives the same results bit for bit.  Inspection
+   only (no counterpart in the reference). */
+int simox_inner_shapes(const simox_ctx* ctx, int shapes[6]);
 int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
 int simox_set_mode(simox_ctx* ctx, int mode);
 /* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged

@@ -105,7 +105,9 @@ struct PmeGrid {
   bool spread32 = false;  // FP32 spreads in 32-bit fixed point (the MIXED inner grid)
   float4* brec = nullptr;  // its stencil records in brick order (k_brick_records)
   float4* brec_atom = nullptr;  // the same records in atom order
+  int gshape = 0;  // tile shape of the record gather (GATHER_SHAPES), autotuned
 };
+constexpr int NGATHER_SHAPES = 6;
 
 struct simox_ctx {
   int n = 0;
@@ -137,6 +139,10 @@ struct simox_ctx {
   // the true residual of their restart.
   PmeGrid pi;
   bool inner_coarse = false;
+  // launch shapes of the inner solve that the autotuner sets (tune_inner)
+  int mv_variant = 1;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads)
+  bool inner_tuned = false;
+  bool mv_fixed = false, gather_fixed = false, xynt_fixed = false, nl_fixed = false;
   // line-search rounds per inner iteration (multi-block path): adapted to the
   // rounds the previous iteration's search ran when ls_adapt
   bool ls_adapt = true;
@@ -1111,6 +1117,10 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
   // first use and tuned then; the reference's grid at context creation
   if (&pm != &ctx->pm && autotune_on()) CHK(autotune_grid(ctx, pm));
   pm.bricks_valid = false;
+  if (const char* e = std::getenv("SIMOX_GATHER_SHAPE")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v < NGATHER_SHAPES) pm.gshape = v;
+  }
   pm.ready = true;
   return 0;
 }
@@ -1550,10 +1560,47 @@ int gather_attr(simox_ctx* ctx) {
 // 143 us, 64 x 16 x 8: 154 us, 32 x 32 x 8: 192 us; the per-atom gather: 203 us).
 constexpr int PGX = 64, PGY = 8, PGZ = 8;
 
+// Tile shapes of the record gather on the MIXED coarse grids (FP32 tiles),
+// one of which the autotuner keeps per grid (PmeGrid::gshape; the default is
+// shape 0).  Every shape sums each atom's 64 products in the same order.
+constexpr int GATHER_SHAPES[NGATHER_SHAPES][3] = {{64, 8, 8},  {32, 8, 8},   {32, 16, 8},
+                                                  {64, 16, 8}, {32, 16, 16}, {64, 16, 16}};
+
+template <int GX, int GY, int GZ>
+int gather_rec_attr(simox_ctx* ctx) {
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gather_tile_rec<GX, GY, GZ>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             int(GatherTile<GX, GY, GZ>::lds(4))));
+  return 0;
+}
+int gather_rec_attrs(simox_ctx* ctx) {
+  CHK((gather_rec_attr<32, 8, 8>(ctx)));
+  CHK((gather_rec_attr<32, 16, 8>(ctx)));
+  CHK((gather_rec_attr<64, 16, 8>(ctx)));
+  CHK((gather_rec_attr<32, 16, 16>(ctx)));
+  CHK((gather_rec_attr<64, 16, 16>(ctx)));
+  return 0;
+}
+
+template <int GX, int GY, int GZ>
+void gather_rec_t(simox_ctx* ctx, const PmeGrid& pm, double c_pme) {
+  using T = GatherTile<GX, GY, GZ>;
+  const Geom g = geom_of(ctx, pm);
+  hipLaunchKernelGGL((k_gather_tile_rec<GX, GY, GZ>), dim3(T::tiles(g)), dim3(BLOCK), T::lds(4),
+                     ctx->st, g, pm.bstart, pm.border, pm.brec, pm.greal32, c_pme, ctx->phi);
+}
+
 int phi_gather(simox_ctx* ctx, bool f32, const PmeGrid* grid = nullptr) {
   const PmeGrid& pm = grid ? *grid : ctx->pm;
   const double c = grid ? KC * double(pm.K[0]) * pm.K[1] * pm.K[2] / ctx->V / PI : ctx->c_pme;
-  phi_gather_t<PGX, PGY, PGZ>(ctx, f32, pm, c);
+  switch ((f32 && pm.spread32) ? pm.gshape : 0) {
+    case 1: gather_rec_t<32, 8, 8>(ctx, pm, c); break;
+    case 2: gather_rec_t<32, 16, 8>(ctx, pm, c); break;
+    case 3: gather_rec_t<64, 16, 8>(ctx, pm, c); break;
+    case 4: gather_rec_t<32, 16, 16>(ctx, pm, c); break;
+    case 5: gather_rec_t<64, 16, 16>(ctx, pm, c); break;
+    default: phi_gather_t<PGX, PGY, PGZ>(ctx, f32, pm, c); break;
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1568,18 +1615,16 @@ void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
   const int nb = grid_blocks(ctx->n);
   const bool f32 = use32(ctx);
   if (coarse && ctx->val16_valid && ctx->pk_valid) {
-    // groups in flight per lane and streaming loads (SIMOX_MV_VARIANT: 1 = 4
-    // groups non-temporal, the default; 0 = 4 groups, 2 = 8, 3 = 8
-    // non-temporal).  At the 1M-atom slab the non-temporal stream is 4 %
-    // faster and the whole step 4 % (the other kernels keep their L2 lines).
-    static const auto kv = [] {
-      const char* e = std::getenv("SIMOX_MV_VARIANT");
-      const int v = e ? std::atoi(e) : 1;
-      return v == 0   ? &k_h0matvec_sell4<4, false>
-             : v == 2 ? &k_h0matvec_sell4<8, false>
-             : v == 3 ? &k_h0matvec_sell4<8, true>
-                      : &k_h0matvec_sell4<4, true>;
-    }();
+    // groups in flight per lane and streaming loads (mv_variant: 1 = 4 groups
+    // non-temporal, the default; 0 = 4 groups, 2 = 8, 3 = 8 non-temporal; the
+    // autotuner picks one, SIMOX_MV_VARIANT fixes it).  At the 1M-atom slab
+    // the non-temporal stream is 4 % faster and the whole step 4 % (the other
+    // kernels keep their L2 lines).  All four sum in the same order.
+    const int v = ctx->mv_variant;
+    const auto kv = v == 0   ? &k_h0matvec_sell4<4, false>
+                    : v == 2 ? &k_h0matvec_sell4<8, false>
+                    : v == 3 ? &k_h0matvec_sell4<8, true>
+                             : &k_h0matvec_sell4<4, true>;
     hipExtLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
                        ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk), ctx->apv,
@@ -1793,6 +1838,102 @@ int ncg_issue_restart(simox_ctx* ctx) {
   return 0;
 }
 
+// Plane-chain shapes of grid pm: threads per workgroup of the x-y passes
+// (xynt) and z lines per workgroup (nl); false where no kernel exists or the
+// LDS does not hold them.
+bool plane_shape(simox_ctx* ctx, PmeGrid& pm, int xynt, int nl) {
+  FftCfg& c = pm.fc32;
+  XyFwd fwd{};
+  XyInv inv{};
+  const int L = pm.K[0], KZ = pm.K[2];
+  if (!xy_kernels(L, xynt, &fwd, &inv) || !zp_kernel(KZ, nl)) return false;
+  const size_t shzp = (2 * size_t(nl) * pad_stride(KZ) + KZ) * sizeof(float2);
+  if (shzp > FFT_LDS_MAX) return false;
+  const void* fns[3] = {reinterpret_cast<const void*>(fwd), reinterpret_cast<const void*>(inv),
+                        reinterpret_cast<const void*>(zp_kernel(KZ, nl))};
+  for (const void* f : fns)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(FFT_LDS_MAX)) !=
+        hipSuccess)
+      return false;
+  const int zpb = (pm.K[1] * pm.H + nl - 1) / nl;
+  if (ensure_part(ctx, size_t(zpb))) return false;
+  c.xynt = xynt;
+  c.nl = nl;
+  c.shzp = shzp;
+  c.zpblocks = zpb;
+  return true;
+}
+
+// Autotuner of the inner solve's launch shapes, on this context's buffers at
+// its first solve with the coarse inner operator (SIMOX_AUTOTUNE=0 keeps the
+// defaults; SIMOX_MV_VARIANT, SIMOX_GATHER_SHAPE, SIMOX_FFT_XYNT and
+// SIMOX_FFT_NL fix one each): the packed matvec's variant, the record
+// gather's tile (GATHER_SHAPES) and, on both coarse grids, the plane FFT
+// chain's x-y threads per workgroup and z lines per workgroup.  Each
+// candidate is timed over 3 launches after a warm one.  Every candidate
+// computes the same values in the same order (the gather sums each atom's
+// products in one order, the matvec each row's, the FFT passes each line's;
+// the z pass's partial sums, which its lines per workgroup group, are not
+// read on the coarse grids), so the choice changes times only.  It runs
+// before the solve's first restart, so the buffers it writes (coarse grids,
+// potential, matvec output, block partials) are all rewritten before use.
+int tune_inner(simox_ctx* ctx) {
+  ctx->inner_tuned = true;
+  PmeGrid& pi = ctx->pi;
+  if (!autotune_on() || !(use32(ctx) && ctx->inner_coarse && pi.ready && pi.spread32 &&
+                          pi.bricks_valid))
+    return 0;
+  float best = 1e30f, ms = 0.f;
+  if (!ctx->mv_fixed && ctx->val16_valid && ctx->pk_valid) {
+    int pick = ctx->mv_variant;
+    for (int v = 0; v < 4; ++v) {
+      ctx->mv_variant = v;
+      CHK(time_launch(ctx, [&] { launch_matvec(ctx, true); }, &ms));
+      if (ms < best) { best = ms; pick = v; }
+    }
+    ctx->mv_variant = pick;
+  }
+  if (!ctx->gather_fixed) {
+    best = 1e30f;
+    int pick = pi.gshape;
+    for (int sh = 0; sh < NGATHER_SHAPES; ++sh) {
+      pi.gshape = sh;
+      CHK(time_launch(ctx, [&] { (void)phi_gather(ctx, true, &pi); }, &ms));
+      if (ms < best) { best = ms; pick = sh; }
+    }
+    pi.gshape = pick;
+  }
+  for (PmeGrid* pm : {&ctx->pi, &ctx->pc}) {
+    FftCfg& c = pm->fc32;
+    if (!pm->ready || !c.plane) continue;
+    const float* tab = pm == &ctx->pc ? pm->F32 : pm->G32;
+    auto chain = [&] { (void)plane_chain(ctx, *pm, pm->spec32, pm->tw32, c, tab, pm->greal32); };
+    if (!ctx->xynt_fixed) {
+      best = 1e30f;
+      int pick = c.xynt;
+      const int nl0 = c.nl;
+      for (int t : {256, 512, 1024}) {
+        if (!plane_shape(ctx, *pm, t, nl0)) continue;
+        CHK(time_launch(ctx, chain, &ms));
+        if (ms < best) { best = ms; pick = t; }
+      }
+      if (!plane_shape(ctx, *pm, pick, nl0)) return SIMOX_ERR_ARG;
+    }
+    if (!ctx->nl_fixed) {
+      best = 1e30f;
+      int pick = c.nl;
+      const int t0 = c.xynt;
+      for (int m : {4, 8, 11, 16}) {
+        if (!plane_shape(ctx, *pm, t0, m)) continue;
+        CHK(time_launch(ctx, chain, &ms));
+        if (ms < best) { best = ms; pick = m; }
+      }
+      if (!plane_shape(ctx, *pm, t0, pick)) return SIMOX_ERR_ARG;
+    }
+  }
+  return 0;
+}
+
 int ncg_begin(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
   {
@@ -1821,6 +1962,7 @@ int ncg_begin(simox_ctx* ctx) {
   // the first iteration (which a batched solve captures into its graph)
   if (use32(ctx) && ctx->inner_coarse && ctx->pi.ready) CHK(ensure_bricks(ctx, ctx->pi));
   if (use32(ctx) && !(ctx->inner_coarse && ctx->pi.ready && ctx->val16_valid)) CHK(ensure_val32(ctx));
+  if (!ctx->inner_tuned) CHK(tune_inner(ctx));
   return ncg_issue_restart(ctx);
 }
 
@@ -2205,6 +2347,13 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   ctx->inner_coarse = mode == SIMOX_MODE_MIXED &&
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_MV_VARIANT")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v < 4) { ctx->mv_variant = v; ctx->mv_fixed = true; }
+  }
+  ctx->gather_fixed = std::getenv("SIMOX_GATHER_SHAPE") != nullptr;
+  ctx->xynt_fixed = std::getenv("SIMOX_FFT_XYNT") != nullptr;
+  ctx->nl_fixed = std::getenv("SIMOX_FFT_NL") != nullptr;
   if (const char* e = std::getenv("SIMOX_INNER_F16")) ctx->inner_f16 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_PREC_SPREAD32")) ctx->prec_spread32 = std::atoi(e) != 0;
@@ -2246,6 +2395,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     if (rc == SIMOX_ERR_ARG) ctx->err = "PME grid dimension too large for the LDS-resident FFT";
     if (rc) return rc;
     CHK((gather_attr<PGX, PGY, PGZ>(ctx)));
+    CHK(gather_rec_attrs(ctx));
   }
   CHK(dalloc(ctx, &ctx->dscal, 64));
   CHK(dalloc(ctx, &ctx->part_eg, grid_blocks(n)));
@@ -2283,6 +2433,17 @@ int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]) {
     shapes[3 * k + 1] = c[k]->my;
     shapes[3 * k + 2] = c[k]->mz;
   }
+  return SIMOX_OK;
+}
+
+int simox_inner_shapes(const simox_ctx* ctx, int shapes[6]) {
+  if (!ctx || !shapes) return SIMOX_ERR_ARG;
+  shapes[0] = ctx->mv_variant;
+  shapes[1] = ctx->pi.gshape;
+  shapes[2] = ctx->pi.fc32.plane ? ctx->pi.fc32.xynt : 0;
+  shapes[3] = ctx->pi.fc32.plane ? ctx->pi.fc32.nl : 0;
+  shapes[4] = ctx->pc.fc32.plane ? ctx->pc.fc32.xynt : 0;
+  shapes[5] = ctx->pc.fc32.plane ? ctx->pc.fc32.nl : 0;
   return SIMOX_OK;
 }
 

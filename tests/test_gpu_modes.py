@@ -339,6 +339,48 @@ def test_mixed_coarse_inner_grid_matches_fp64(monkeypatch, pack):
             g.close()
 
 
+def test_inner_shapes_change_no_result(monkeypatch):
+    """The inner solve's tuned launch shapes (simox_inner_shapes: packed
+    matvec variant, record gather tile, plane FFT threads and z lines of both
+    coarse grids) give the same charges and forces bit for bit whichever is
+    chosen: each fixed in turn (SIMOX_AUTOTUNE=0 and one switch) against the
+    tuned run."""
+    from simox_amd.systems import al_o_slab
+    s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (96, 96, 64), 10.0, vacuum=20.0,
+                  o_layer=(2.0, 10.0), o_dmin=2.0, name="inner")
+
+    def run(env):
+        for k in ("SIMOX_AUTOTUNE", "SIMOX_MV_VARIANT", "SIMOX_GATHER_SHAPE", "SIMOX_FFT_XYNT",
+                  "SIMOX_FFT_NL"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=MODE_MIXED)
+        g.set_inner_grid(True)
+        g.set_charge_tol(1e-10, 5000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        g.ctip_solve()
+        g.force()
+        g.pme()
+        _, _, f, q = g.get_state()
+        sh = g.inner_shapes()
+        g.close()
+        return q, f, sh
+
+    q0, f0, sh = run({})
+    print("tuned inner shapes", sh, flush=True)
+    assert sh["matvec"] in range(4) and sh["gather"] in range(6)
+    assert sh["inner_grid"][0] in (256, 512, 1024) and sh["inner_grid"][1] in (4, 8, 11, 16)
+    cases = [{"SIMOX_MV_VARIANT": str(v)} for v in range(4)]
+    cases += [{"SIMOX_GATHER_SHAPE": str(v)} for v in range(6)]
+    cases += [{"SIMOX_FFT_XYNT": t} for t in ("256", "512", "1024")]
+    cases += [{"SIMOX_FFT_NL": m} for m in ("4", "8", "16")]
+    for env in cases:
+        env = dict(env, SIMOX_AUTOTUNE="0")
+        q, f, sh = run(env)
+        assert np.array_equal(q, q0) and np.array_equal(f, f0), (env, sh)
+
+
 def test_z256_register_pass_fp32(monkeypatch):
     """Kz = 256 on the FP32 chain takes the radix-4 register z pass
     (k_fft_z256_reg): the MIXED reciprocal energy with it and with the tiled
