@@ -35,6 +35,9 @@ extern "C" {
 #define SIMOX_ERR_ARG 5           /* invalid argument / geometry (main.f90:191-193) */
 #define SIMOX_ERR_HIP 6           /* HIP runtime error */
 #define SIMOX_ERR_NOT_CONVERGED 7 /* linear charge solve hit its iteration limit */
+#define SIMOX_ERR_FORCE_RANGE 8   /* a pair force component of 16384 eV/A or more: beyond the
+                                     range of the 64-bit fixed-point force sums (no reference
+                                     counterpart; the reference has no such limit) */
 
 /* ---- numerical modes ----------------------------------------------------- */
 /* FP64, reproducing the reference's arithmetic including its quirks: FP32-

@@ -81,7 +81,12 @@ __global__ void __launch_bounds__(BLOCK) k_cell_order(const int* __restrict__ st
   const int c = blockIdx.x;
   const int s0 = start[c];
   const int m = start[c + 1] - s0;
-  if (m > NPT_MAX) return;  // reported by k_cell_scan
+  if (m > NPT_MAX) {
+    // reported by k_cell_scan; the slots stay unordered but every order entry
+    // is written, so the kernels after the sort index in bounds
+    for (int t = threadIdx.x; t < m; t += blockDim.x) order[s0 + t] = slot_src[s0 + t];
+    return;
+  }
   for (int t = threadIdx.x; t < m; t += blockDim.x) {
     src[t] = slot_src[s0 + t];
     key[t] = perm ? perm[src[t]] : src[t];

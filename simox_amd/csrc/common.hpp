@@ -36,8 +36,9 @@ constexpr double GRID_SCALE = 1125899906842624.0;
 constexpr double GRID_INV_SCALE = 1.0 / 1125899906842624.0;
 
 // ---------------------------------------------------------------------------
-// Per species-pair coefficient table.  Every EAM branch of eam.f90 used here
-// (METAL_HOMO, METAL_OXYGEN, OXYGEN_METAL, OXYGEN_ONLY) is one radial form
+// Per species-pair coefficient table.  The EAM branches of eam.f90
+// (METAL_HOMO, AL_NI, METAL_OXYGEN, OXYGEN_METAL, OXYGEN_ONLY) are one radial
+// form each, METAL_HETERO a weighted mean of two:
 //   f(r)   = fe e^{-b(r/re-1)} / (1 + (r/re - l)^20)             (densities)
 //   phi(r) = A e^{-a(r/re-1)} / (1 + (r/re - k)^20)
 //          - B e^{-b(r/re-1)} / (1 + (r/re - l)^20)             (pair term)
@@ -50,8 +51,10 @@ struct PhiCoef { double re, a, b, A, B, k, l; };
 struct PairCoef {
   int valid;
   int same;            // id_i == id_j (CTIP branch, force.f90:101)
-  DensCoef di, dj;     // density of species i (reference fj(1)) and of j (fj(2))
-  PhiCoef ph;
+  int kind;            // 0: pair term ph; 1: METAL_HETERO's mean of ph (metal i) and ph2 (metal j)
+  int ctip64;          // Slater terms of a pair of close xi (MIXED evaluates them in FP64)
+  DensCoef di, dj;     // fj(1) and fj(2) of the reference's EAM routine for (id_i, id_j)
+  PhiCoef ph, ph2;
   double xi_i, xi_j, Z_i, Z_j;
   double c[4];         // fafb(id_i, id_j, 1:4)
 };
@@ -115,7 +118,7 @@ __device__ inline T pow20(T x) {
 
 // Minimum-image separation exactly as dx - box*DNINT(dx/box) (force.f90:51-53):
 // no FMA contraction, IEEE division, round half away from zero.
-__device__ inline void min_image(const Geom& g, double xi, double yi, double zi, double xj,
+__host__ __device__ inline void min_image(const Geom& g, double xi, double yi, double zi, double xj,
                                  double yj, double zj, double rx[3], double& r2) {
 #pragma clang fp contract(off)
   double d[3] = {xi - xj, yi - yj, zi - zj};
@@ -128,7 +131,7 @@ __device__ inline void min_image(const Geom& g, double xi, double yi, double zi,
   r2 = rx[0] * rx[0] + rx[1] * rx[1] + rx[2] * rx[2];
 }
 
-__device__ inline int cell_wrap(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+__host__ __device__ inline int cell_wrap(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
 // ---------------------------------------------------------------------------
 // Reductions: deterministic (fixed shuffle tree per wave, fixed order across

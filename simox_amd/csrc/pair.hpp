@@ -1,10 +1,8 @@
-// Short-range kernels: cell binning, EAM density + embedding, pair forces,
-// CTIP sparse matrix build.  One thread per atom in cell-sorted order; each
-// thread walks the full 27-cell shell of its own cell, so every pair is
-// evaluated from both sides and no write conflicts arise (deterministic sums
-// without atomics).  With Ncell = 2 along an axis the shell visits the
-// neighbour cell twice, which reproduces the reference's duplicated pairs
-// (main.f90:191 guard, quirk Q8).
+// Short-range pair forms and kernels over the neighbour list: EAM densities,
+// pair terms and embedding (eam.f90, force.f90:256-348), the real-space Ewald
+// and CTIP terms of FORCE, the CTIP sparse matrix, and the EAM spline
+// gradients.  The list itself, the densities and the half-shell force pass
+// are in tile.hpp.
 #pragma once
 #include "common.hpp"
 
@@ -244,33 +242,6 @@ __device__ inline void spl_eval(const SplineSet& sp, int f, T x, T& v, T& dv) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Full-shell neighbour walk over the 27 cells around atom i's cell.
-// ---------------------------------------------------------------------------
-template <class Fn>
-__device__ inline void for_each_neighbor(const Geom& g, const int* __restrict__ cell_start, int i,
-                                         int ci, Fn&& fn) {
-  const int cx = ci % g.nc[0];
-  const int t = ci / g.nc[0];
-  const int cy = t % g.nc[1];
-  const int cz = t / g.nc[1];
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int zz = cell_wrap(cz + dz, g.nc[2]);
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int yy = cell_wrap(cy + dy, g.nc[1]);
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int xx = cell_wrap(cx + dx, g.nc[0]);
-        const int c = xx + g.nc[0] * (yy + g.nc[1] * zz);
-        const int j0 = cell_start[c], j1 = cell_start[c + 1];
-        for (int j = j0; j < j1; ++j) {
-          if (j == i) continue;
-          fn(j);
-        }
-      }
-    }
-  }
-}
-
 #define SX_ERR_SET(ptr, code) atomicCAS((ptr), 0, (code))
 
 // Atom block of this workgroup in the pair kernels.  Workgroups go to the 8
@@ -292,144 +263,15 @@ __device__ inline void pair_sep(const Geom& g, const double* __restrict__ x, int
 // Al-O and O-O contacts of an oxidising surface) and the FP32 forms lose
 // ~1e-7 of them, i.e. ~1e-5 of an RMS force; beyond 2.4 A every term is below
 // ~1.5 eV/A and FP32 keeps the error under 2e-6 of the RMS force.  Bulk Al
-// (nearest neighbours 2.86 A) has no such pairs.  The FP32 kernels record the
-// near pairs of each atom in a side list (NEAR_CAP per atom, column-major
-// like the main list) and skip them; two small FP64 kernels then add their
-// density and their forces, so the FP32 kernels keep their register budget.
+// (nearest neighbours 2.86 A) has no such pairs.  The density pass evaluates
+// them in FP64 in place; the FP32 force pass queues them for a small FP64
+// kernel, so that it keeps its register budget.
 constexpr double R_NEAR2 = 2.4 * 2.4;
-constexpr int NEAR_CAP = 16;
 
 template <class T>
 __device__ inline bool near_pair(double r2) {
   if constexpr (sizeof(T) == sizeof(float)) return r2 < R_NEAR2;
   return false;
-}
-
-// K1b + K2 + K3: neighbour list, EAM density and embedding in one walk of the
-// 27-cell shell.  The list holds every j with r2 < Rcut^2 (the exact test of
-// force.f90:53-54), in ELL layout stored column-major (entry k of atom i at
-// k*n + i: coalesced across a wave); the force and CTIP-matrix kernels
-// iterate it instead of re-testing ~7x more candidates.  rho is summed in
-// FP64.  T = double: F(rho), F'(rho) here, egroup_part[blockIdx.x] = sum of F
-// over the block.  T = float: rho_out gets the density of the pairs beyond
-// R_NEAR (FP32 terms), near_col / near_cnt the near pairs; k_near_density
-// completes rho and the embedding.
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_nlist_density(Geom g, const Tables* __restrict__ tb,
-                                                         const double* __restrict__ x,
-                                                         const int* __restrict__ id,
-                                                         const int* __restrict__ cell_of,
-                                                         const int* __restrict__ cell_start,
-                                                         int cap, int* __restrict__ cnt,
-                                                         int* __restrict__ col,
-                                                         int* __restrict__ near_cnt,
-                                                         int* __restrict__ near_col,
-                                                         double* __restrict__ rho_out,
-                                                         T* __restrict__ dF_out,
-                                                         double* __restrict__ egroup_part,
-                                                         int* __restrict__ err) {
-  constexpr bool f32 = sizeof(T) == sizeof(float);
-  __shared__ double lds[BLOCK / WAVE];
-  const int i = pair_block(g) * blockDim.x + threadIdx.x;
-  double eg[1] = {0.0};
-  if (i < g.n) {
-    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-    const int si = id[i];
-    const size_t n = size_t(g.n);
-    double rho = 0.0;
-    int k = 0, kn = 0;
-    bool bad = si < 1 || si > NSPEC;
-    for_each_neighbor(g, cell_start, i, cell_of[i], [&](int j) {
-      double rx[3], r2;
-      min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
-      if (r2 < g.rcut2) {
-        if (k < cap) col[size_t(k) * n + i] = j;
-        ++k;
-        const int sj = id[j];
-        if (bad || sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid) { bad = true; return; }
-        if (tb->sp.on && !(spl_species_ok(si) && spl_species_ok(sj))) { bad = true; return; }
-        if (near_pair<T>(r2)) {
-          if (kn < NEAR_CAP) near_col[size_t(kn) * n + i] = j;
-          ++kn;
-        } else if (r2 < g.ream2) {  // split cutoff: EAM only inside r_eam
-          T f, dfr;
-          if (tb->sp.on)
-            spl_eval<T>(tb->sp, spl_dens(si, sj), T(sqrt(r2)), f, dfr);
-          else
-            dens_eval<T>(tb->pc[pidx(si, sj)].dj, T(sqrt(r2)), f, dfr);
-          rho += double(f);
-        }
-      }
-    });
-    if (bad) SX_ERR_SET(err, SIMOX_ERR_UNKNOWN_PAIR);
-    if (k > NNB_MAX || k > cap || kn > NEAR_CAP) SX_ERR_SET(err, SIMOX_ERR_NB_OVERFLOW);
-    cnt[i] = k < cap ? k : cap;
-    rho_out[i] = rho;
-    if constexpr (f32) {
-      near_cnt[i] = kn < NEAR_CAP ? kn : NEAR_CAP;
-    } else {
-      double F = 0.0, dF = 0.0;
-      if (!bad) {
-        if (tb->sp.on)
-          spl_eval<double>(tb->sp, spl_embed(si), rho, F, dF);
-        else
-          embed<double>(tb->ec[si - 1], rho, F, dF);
-      }
-      dF_out[i] = T(dF);
-      eg[0] = F;
-    }
-  }
-  if constexpr (!f32) {
-    block_sum<1>(eg, lds);
-    if (threadIdx.x == 0) egroup_part[blockIdx.x] = eg[0];
-  }
-}
-
-// MIXED: adds the FP64 density of each atom's near pairs to the FP32-term
-// density of k_nlist_density<float>, then F(rho), F'(rho) in FP64
-// (force.f90:256-348); egroup_part[blockIdx.x] = sum of F over the block.
-__global__ void __launch_bounds__(BLOCK) k_near_density(Geom g, const Tables* __restrict__ tb,
-                                                        const double* __restrict__ x,
-                                                        const int* __restrict__ id,
-                                                        const int* __restrict__ near_cnt,
-                                                        const int* __restrict__ near_col,
-                                                        double* __restrict__ rho_io,
-                                                        float* __restrict__ dF_out,
-                                                        double* __restrict__ egroup_part) {
-  __shared__ double lds[BLOCK / WAVE];
-  const int i = pair_block(g) * blockDim.x + threadIdx.x;
-  double eg[1] = {0.0};
-  if (i < g.n) {
-    const int si = id[i];
-    const size_t n = size_t(g.n);
-    double rho = rho_io[i];
-    const int kn = near_cnt[i];
-    double F = 0.0, dF = 0.0;
-    if (si >= 1 && si <= NSPEC) {
-      for (int k = 0; k < kn; ++k) {
-        const int j = near_col[size_t(k) * n + i];
-        double rx[3], r2;
-        pair_sep(g, x, i, j, rx, r2);
-        if (!(r2 < g.ream2)) continue;
-        const int sj = id[j];
-        double f, dfr;
-        if (tb->sp.on)
-          spl_eval<double>(tb->sp, spl_dens(si, sj), sqrt(r2), f, dfr);
-        else
-          dens_eval<double>(tb->pc[pidx(si, sj)].dj, sqrt(r2), f, dfr);
-        rho += f;
-      }
-      if (tb->sp.on)
-        spl_eval<double>(tb->sp, spl_embed(si), rho, F, dF);
-      else
-        embed<double>(tb->ec[si - 1], rho, F, dF);
-    }
-    rho_io[i] = rho;
-    dF_out[i] = float(dF);
-    eg[0] = F;
-  }
-  block_sum<1>(eg, lds);
-  if (threadIdx.x == 0) egroup_part[blockIdx.x] = eg[0];
 }
 
 // One pair of FORCE's loop (force.f90:86-140) in arithmetic type S: the
@@ -451,6 +293,17 @@ __device__ inline S pair_force(const Tables* __restrict__ tb, const PairCoef& pc
     dens_eval<S>(pc.di, r, fi_, dfi);
     dens_eval<S>(pc.dj, r, fj_, dfj);
     phi_eval<S>(pc.ph, r, phi, dfp);
+    if (pc.kind == 1) {
+      // METAL_HETERO (eam.f90:90-96): phi = (phi_i f_j/f_i + phi_j f_i/f_j)/2
+      // and -phi'/r, with d(f_j/f_i)/dr / r = (f_j'/r f_i - f_j f_i'/r)/f_i^2
+      S phj, dpj;
+      phi_eval<S>(pc.ph2, r, phj, dpj);
+      const S phi_i = phi, ri = fdiv<S>(fj_, fi_), rj = fdiv<S>(fi_, fj_);
+      phi = S(0.5) * (phi_i * ri + phj * rj);
+      dfp = S(0.5) * (dfp * ri + dpj * rj) -
+            S(0.5) * (phi_i * fdiv<S>(dfj * fi_ - fj_ * dfi, fi_ * fi_) +
+                      phj * fdiv<S>(dfi * fj_ - fi_ * dfj, fj_ * fj_));
+    }
   }
   // EAM: pair force and embedding force of both atoms (force.f90:86-90,349,356)
   S s = eam ? dfp - dFi * dfj - dFj * dfi : S(0);
@@ -461,151 +314,19 @@ __device__ inline S pair_force(const Tables* __restrict__ tb, const PairCoef& pc
               r2),
       r);
   ed = fdiv<S>(qi * qj * tmp, r);
-  // CTIP Slater terms (force.f90:101-140)
-  S dfc;
-  ctip_force_terms<S>(pc, r, r2, qi, qj, dfc, ec);
+  // CTIP Slater terms (force.f90:101-140); in FP32 not those of pc.ctip64,
+  // which k_force_ctip64 adds in FP64
+  S dfc = S(0);
+  ec = S(0);
+  if (!(sizeof(S) == sizeof(float) && pc.ctip64)) ctip_force_terms<S>(pc, r, r2, qi, qj, dfc, ec);
   ep = eam ? phi : S(0);
   return s + dfc;
 }
 
-// K4: EAM pair + embedding forces, real-space Ewald, CTIP short-range terms,
-// over the neighbour list.  Overwrites f (force.f90:28);
-// e_part[3*blockIdx.x + {0,1,2}] = half-pair sums of phi, q_i q_j erfc(ar)/r
-// and the CTIP energy (FORCE's Epair, Edir/kc, Ectip/kc).  Forces and
-// energies are summed in FP64 from the FP64 separation.  T = float (MIXED):
-// FP32 pair terms, near pairs skipped (k_force_near adds them).
-// rdf != null (the launch then brings RDF_BINS unsigned ints of dynamic LDS):
-// also the pair histogram of FORCE (force.f90:55,76,159,180), bin INT(r/dr),
-// one count per list entry, i.e. per direction of each pair, into
-// rdf[(bin-1)*25 + (sj-1)*5 + (si-1)] for bin >= 1 (the layout of the
-// reference's rdf(5,5,201) from its second bin on).  The reference adds 2 per
-// pair under one orientation (id_i, id_j); its output uses rdf(i,j)+rdf(j,i)
-// (util.f90:270-277), which these counts reproduce.
+// Pair histogram of FORCE (force.f90:55,76,159,180): NRDF bins of width
+// pair%dr per ordered species pair, rdf(5,5,201) from its second bin on.
 constexpr int NRDF = 200;  // datafmt.f90:31
 constexpr int RDF_BINS = 25 * (NRDF + 1);
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_force(Geom g, const Tables* __restrict__ tb,
-                                                 const double* __restrict__ x,
-                                                 const double* __restrict__ q,
-                                                 const int* __restrict__ id,
-                                                 const int* __restrict__ cnt,
-                                                 const int* __restrict__ col,
-                                                 const T* __restrict__ dF,
-                                                 double* __restrict__ f,
-                                                 double* __restrict__ e_part,
-                                                 unsigned long long* __restrict__ rdf,
-                                                 double dr) {
-  __shared__ double lds[3 * BLOCK / WAVE];
-  extern __shared__ unsigned int hist[];
-  if (rdf) {
-    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x) hist[k] = 0u;
-    __syncthreads();
-  }
-  const int i = pair_block(g) * blockDim.x + threadIdx.x;
-  double es[3] = {0.0, 0.0, 0.0};
-  if (i < g.n) {
-    const int si = id[i];
-    const T qi = T(q[i]);
-    const T dFi = dF[i];
-    const T a = T(g.alpha);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double ep = 0.0, ed = 0.0, ec = 0.0;
-    const size_t n = size_t(g.n);
-    const int c = cnt[i];
-    if (si >= 1 && si <= NSPEC) {
-      for (int k = 0; k < c; ++k) {
-        const int j = col[size_t(k) * n + i];
-        double rxd[3], r2d;
-        pair_sep(g, x, i, j, rxd, r2d);
-        const int sj = id[j];
-        if (sj < 1 || sj > NSPEC) continue;
-        const PairCoef& pc = tb->pc[pidx(si, sj)];
-        if (!pc.valid) continue;
-        if (rdf) {
-          const int bin = int(sqrt(r2d) / dr);  // INT(r/pair%dr)
-          if (bin >= 1 && bin <= NRDF + 1)
-            atomicAdd(&hist[(bin - 1) * 25 + (sj - 1) * 5 + (si - 1)], 1u);
-        }
-        if (near_pair<T>(r2d)) continue;
-        T e0, e1, e2;
-        const double s = double(pair_force<T>(tb, pc, si, sj, T(sqrt(r2d)), T(r2d), qi, T(q[j]),
-                                              dFi, dF[j], r2d < g.ream2, a, e0, e1, e2));
-        ep += double(e0);
-        ed += double(e1);
-        ec += double(e2);
-        fx += rxd[0] * s;
-        fy += rxd[1] * s;
-        fz += rxd[2] * s;
-      }
-    }
-    f[3 * i] = fx;
-    f[3 * i + 1] = fy;
-    f[3 * i + 2] = fz;
-    es[0] = 0.5 * ep;
-    es[1] = 0.5 * ed;
-    es[2] = 0.5 * ec;
-  }
-  block_sum<3>(es, lds);
-  if (threadIdx.x == 0) {
-    e_part[3 * blockIdx.x] = es[0];
-    e_part[3 * blockIdx.x + 1] = es[1];
-    e_part[3 * blockIdx.x + 2] = es[2];
-  }
-  if (rdf) {  // block_sum ended with a barrier: the histogram is complete
-    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x)
-      if (hist[k]) atomicAdd(&rdf[k], (unsigned long long)hist[k]);
-  }
-}
-
-// MIXED: the near pairs of each atom (side list of k_nlist_density<float>)
-// in FP64, added to the forces k_force<float> wrote; e_part as k_force's.
-__global__ void __launch_bounds__(BLOCK) k_force_near(Geom g, const Tables* __restrict__ tb,
-                                                      const double* __restrict__ x,
-                                                      const double* __restrict__ q,
-                                                      const int* __restrict__ id,
-                                                      const int* __restrict__ near_cnt,
-                                                      const int* __restrict__ near_col,
-                                                      const float* __restrict__ dF,
-                                                      double* __restrict__ f,
-                                                      double* __restrict__ e_part) {
-  __shared__ double lds[3 * BLOCK / WAVE];
-  const int i = pair_block(g) * blockDim.x + threadIdx.x;
-  double es[3] = {0.0, 0.0, 0.0};
-  if (i < g.n) {
-    const int si = id[i];
-    const int kn = near_cnt[i];
-    if (kn > 0 && si >= 1 && si <= NSPEC) {
-      const size_t n = size_t(g.n);
-      const double qi = q[i], dFi = double(dF[i]);
-      double fx = 0.0, fy = 0.0, fz = 0.0;
-      for (int k = 0; k < kn; ++k) {
-        const int j = near_col[size_t(k) * n + i];
-        double rxd[3], r2d;
-        pair_sep(g, x, i, j, rxd, r2d);
-        const int sj = id[j];
-        const PairCoef& pc = tb->pc[pidx(si, sj)];
-        double e0, e1, e2;
-        const double s = pair_force<double>(tb, pc, si, sj, sqrt(r2d), r2d, qi, q[j], dFi,
-                                            double(dF[j]), r2d < g.ream2, g.alpha, e0, e1, e2);
-        es[0] += 0.5 * e0;
-        es[1] += 0.5 * e1;
-        es[2] += 0.5 * e2;
-        fx += rxd[0] * s;
-        fy += rxd[1] * s;
-        fz += rxd[2] * s;
-      }
-      f[3 * i] += fx;
-      f[3 * i + 1] += fy;
-      f[3 * i + 2] += fz;
-    }
-  }
-  block_sum<3>(es, lds);
-  if (threadIdx.x == 0) {
-    e_part[3 * blockIdx.x] = es[0];
-    e_part[3 * blockIdx.x + 1] = es[1];
-    e_part[3 * blockIdx.x + 2] = es[2];
-  }
-}
 
 // Gradient of the spline-form EAM energy E = sum_pairs phi(r) + sum_i F(rho_i)
 // with respect to the spline coefficients (build-only):

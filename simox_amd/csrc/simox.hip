@@ -17,6 +17,7 @@
 #include "cells.hpp"
 #include "common.hpp"
 #include "pair.hpp"
+#include "tile.hpp"
 #include "pme.hpp"
 #include "solve.hpp"
 #include "adjoint.hpp"
@@ -33,6 +34,7 @@ const char* status_text(int code, int where_ctip) {
                         : "=== Memeory error for EAM density ===";
     case SIMOX_ERR_UNKNOWN_PAIR: return "=== Unknown particle pair ===";
     case SIMOX_ERR_BRENT: return "# **** Brent solver limit breached *** ";
+    case SIMOX_ERR_FORCE_RANGE: return "pair force beyond the range of the fixed-point force sums";
     default: return "unknown error";
   }
 }
@@ -126,9 +128,18 @@ struct simox_ctx {
   int *count = nullptr, *start = nullptr, *fill = nullptr, *slot = nullptr, *order = nullptr;
   double *rho = nullptr, *dF64 = nullptr, *frec = nullptr;
   float* dF32 = nullptr;
-  // MIXED: pairs closer than R_NEAR per atom (count, NEAR_CAP columns
-  // column-major), evaluated in FP64 by k_near_density / k_force_near
-  int *near_cnt = nullptr, *near_col = nullptr;
+  // half-shell force pass: forward entries per atom (the reference's "i"
+  // side, first in each list), fixed-point force sums (3N, zero between
+  // passes), near-pair energies (MIXED, 3) and the queue of MIXED pairs closer
+  // than R_NEAR, evaluated in FP64 by k_force_near_q
+  int* nb_fwd = nullptr;
+  unsigned long long *facc = nullptr, *eacc = nullptr;
+  int2* nq = nullptr;
+  int* nq_n = nullptr;
+  int nq_cap = 0;
+  int tile_xcd = 1;  // k_nlist_tile: neighbouring cells on one XCD
+  bool ctip64_on = false;  // MIXED: species present form a pc.ctip64 pair
+  unsigned spec_present = 0u;  // bit s: species s among the atoms
   // PME: the reference's grid (pm) and the half-resolution grid of the charge
   // solver's long-range preconditioner (pc, set up on first use)
   PmeGrid pm, pc;
@@ -367,12 +378,32 @@ void build_tables(simox_ctx* ctx) {
       pc.xi_i = p.prm[si - 1].xi; pc.xi_j = p.prm[sj - 1].xi;
       pc.Z_i = p.prm[si - 1].Z; pc.Z_j = p.prm[sj - 1].Z;
       for (int k = 0; k < 4; ++k) pc.c[k] = p.prm[si - 1].fafb[sj - 1][k];
-      if (si < 4 && sj < 4) {
-        if (si != sj) { pc.valid = 0; continue; }  // METAL_HETERO / AL_NI: out of scope
-        const simox_metal& m = p.eam[si - 1];
+      auto metal_phi = [&](int s) {
+        const simox_metal& m = p.eam[s - 1];
+        return PhiCoef{m.r_e, m.alpha, m.beta, m.A, m.B, m.kappa, m.lambda};
+      };
+      pc.kind = 0;
+      // [fi|fj] of two exponents within 50 % of their mean: FP64 in MIXED
+      pc.ctip64 = si != sj && std::fabs(pc.xi_i - pc.xi_j) < 0.25 * (pc.xi_i + pc.xi_j);
+      if (si < 4 && sj < 4 && si == sj) {  // METAL_HOMO
         pc.di = metal_dens(si);
         pc.dj = pc.di;
-        pc.ph = PhiCoef{m.r_e, m.alpha, m.beta, m.A, m.B, m.kappa, m.lambda};
+        pc.ph = metal_phi(si);
+      } else if (si < 4 && sj < 4 && si + sj == 3) {
+        // AL_NI (eam.f90:101-168): called as AL_NI(Al, Ni) for both orders of
+        // the pair (force.f90:60-63), so fj(1) is Al's density and fj(2) Ni's
+        // whichever atom is "i"; a fixed Al-Ni pair term (eam.f90:147-153)
+        pc.di = metal_dens(1);
+        pc.dj = metal_dens(2);
+        pc.ph = PhiCoef{2.71579, 8.00443, 4.75970, 0.44254, 0.68349, 0.63279, 0.81777};
+      } else if (si < 4 && sj < 4) {
+        // METAL_HETERO (eam.f90:38-98): each metal's own density and the
+        // density-weighted mean of the two metals' pair terms
+        pc.kind = 1;
+        pc.di = metal_dens(si);
+        pc.dj = metal_dens(sj);
+        pc.ph = metal_phi(si);
+        pc.ph2 = metal_phi(sj);
       } else if (si < 4 && sj == 4) {  // METAL_OXYGEN
         pc.di = metal_dens(si);
         pc.dj = oxy_dens(si);
@@ -387,6 +418,16 @@ void build_tables(simox_ctx* ctx) {
         pc.ph = oxy_phi(4);
       }
     }
+}
+
+// k_force_ctip64 runs when two species present form a pair of close exponents.
+void set_ctip64(simox_ctx* ctx) {
+  ctx->ctip64_on = false;
+  for (int a = 1; a <= NSPEC; ++a)
+    for (int b = 1; b <= NSPEC; ++b)
+      if (((ctx->spec_present >> a) & 1u) && ((ctx->spec_present >> b) & 1u) &&
+          ctx->htab.pc[pidx(a, b)].ctip64)
+        ctx->ctip64_on = true;
 }
 
 int ensure_part(simox_ctx* ctx, size_t count) {
@@ -520,23 +561,22 @@ int do_cell_sort(simox_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // K2-K4: FORCE
 // ---------------------------------------------------------------------------
-// Neighbour list + EAM density/embedding (once per set of positions).
+// Neighbour list (k_nlist_tile, one workgroup per cell) + EAM density and
+// embedding (k_density), once per set of positions.
 void launch_nlist(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
-  if (use32(ctx)) {
-    hipLaunchKernelGGL(k_nlist_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
-                       ctx->nb_col, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF32,
-                       ctx->part_eg, ctx->derr);
-    hipLaunchKernelGGL(k_near_density, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->id, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF32,
-                       ctx->part_eg);
-  } else {
-    hipLaunchKernelGGL(k_nlist_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g,
-                       ctx->dtab, ctx->x, ctx->id, ctx->cell, ctx->start, ctx->cap, ctx->nb_cnt,
-                       ctx->nb_col, ctx->near_cnt, ctx->near_col, ctx->rho, ctx->dF64,
-                       ctx->part_eg, ctx->derr);
-  }
+  TileArgs ta{ctx->g, ctx->x, ctx->start, ctx->cap, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col,
+              ctx->derr};
+  ta.g.xcd = ctx->tile_xcd;
+  hipLaunchKernelGGL(k_nlist_tile, dim3(ctx->g.ncall), dim3(TILE_NT), 0, ctx->st, ta);
+  if (use32(ctx))
+    hipLaunchKernelGGL(k_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col, ctx->rho,
+                       ctx->dF32, ctx->part_eg, ctx->derr);
+  else
+    hipLaunchKernelGGL(k_density<double>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                       ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col, ctx->rho,
+                       ctx->dF64, ctx->part_eg, ctx->derr);
 }
 
 int ensure_nlist(simox_ctx* ctx) {
@@ -550,25 +590,33 @@ int ensure_nlist(simox_ctx* ctx) {
   return 0;
 }
 
-// K4 over the list (block partials of k_force in ep rows [0, nb), of
-// k_force_near in [nb, 2 nb)).
+// K4 over the forward entries: forces into the fixed-point sums, then f.
+// Energy partials of k_force_half in ep rows [0, nb), the near pairs' (MIXED;
+// zero otherwise) in row nb.
 void launch_force(simox_ctx* ctx, double* ep) {
   const int nb = grid_blocks(ctx->n);
   unsigned long long* rdf = ctx->rdf_on ? ctx->rdf : nullptr;
   const size_t sh = rdf ? RDF_BINS * sizeof(unsigned) : 0;
   const double dr = ctx->rdf_dr;
   if (use32(ctx)) {
-    hipLaunchKernelGGL(k_force<float>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF32, ctx->f,
-                       ep, rdf, dr);
-    hipLaunchKernelGGL(k_force_near, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->q, ctx->id, ctx->near_cnt, ctx->near_col, ctx->dF32, ctx->f,
-                       ep + size_t(3) * nb);
+    hipLaunchKernelGGL(k_force_half<float>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->q, ctx->id, ctx->nb_fwd, ctx->nb_col, ctx->dF32,
+                       ctx->facc, ep, rdf, dr, ctx->nq, ctx->nq_n, ctx->nq_cap, ctx->derr);
+    if (ctx->ctip64_on)
+      hipLaunchKernelGGL(k_force_ctip64, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
+                         ctx->x, ctx->q, ctx->id, ctx->nb_fwd, ctx->nb_col, ctx->facc, ctx->eacc,
+                         ctx->derr);
+    hipLaunchKernelGGL(k_force_near_q, dim3(std::max(1, std::min(nb, 1024))), dim3(BLOCK), 0,
+                       ctx->st, ctx->g, ctx->dtab, ctx->x, ctx->q, ctx->id, ctx->dF32, ctx->nq,
+                       ctx->nq_n, ctx->nq_cap, ctx->facc, ctx->eacc, ctx->derr);
   } else {
-    hipLaunchKernelGGL(k_force<double>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g, ctx->dtab,
-                       ctx->x, ctx->q, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->dF64, ctx->f,
-                       ep, rdf, dr);
+    hipLaunchKernelGGL(k_force_half<double>, dim3(nb), dim3(BLOCK), sh, ctx->st, ctx->g,
+                       ctx->dtab, ctx->x, ctx->q, ctx->id, ctx->nb_fwd, ctx->nb_col, ctx->dF64,
+                       ctx->facc, ep, rdf, dr, ctx->nq, ctx->nq_n, ctx->nq_cap, ctx->derr);
   }
+  hipLaunchKernelGGL(k_force_finish, dim3(grid_blocks(3ll * ctx->n)), dim3(BLOCK), 0, ctx->st,
+                     ctx->n, ctx->facc, ctx->f, ctx->eacc, ep + size_t(3) * nb, ctx->nq_n,
+                     ctx->nq_cap, ctx->derr);
 }
 
 int do_force(simox_ctx* ctx, bool fetch_energies) {
@@ -576,7 +624,7 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
   CHK(ensure_nlist(ctx));
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ctx->part_eg, nb, 1, 0,
                      ctx->dscal + 0);
-  // block partials of k_force in rows [0, nb), of k_force_near in [nb, 2 nb)
+  // block partials of k_force_half in rows [0, nb), the near pairs' in row nb
   CHK(ensure_part(ctx, size_t(nb) * 7));
   double* ep = ctx->part + nb;
   hipEvent_t ev{};
@@ -584,8 +632,8 @@ int do_force(simox_ctx* ctx, bool fetch_energies) {
   launch_force(ctx, ep);
   tstop(ctx, 7, ev);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ep,
-                     use32(ctx) ? 2 * nb : nb, 3, 0, ctx->dscal + 1);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, ctx->st, ep, nb + 1, 3, 0,
+                     ctx->dscal + 1);
   HIPCHK(hipGetLastError());
   if (fetch_energies) {
     double e[4];
@@ -2334,7 +2382,14 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->count, g.ncall)); CHK(dalloc(ctx, &ctx->fill, g.ncall));
   CHK(dalloc(ctx, &ctx->start, g.ncall + 1));
   CHK(dalloc(ctx, &ctx->rho, n)); CHK(dalloc(ctx, &ctx->dF64, n)); CHK(dalloc(ctx, &ctx->dF32, n));
-  CHK(dalloc(ctx, &ctx->near_cnt, n)); CHK(dalloc(ctx, &ctx->near_col, size_t(NEAR_CAP) * n));
+  CHK(dalloc(ctx, &ctx->nb_fwd, n));
+  CHK(dalloc(ctx, &ctx->facc, n3)); CHK(dalloc(ctx, &ctx->eacc, 3));
+  HIPCHK(hipMemset(ctx->facc, 0, n3 * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(ctx->eacc, 0, 3 * sizeof(unsigned long long)));
+  ctx->nq_cap = mode == SIMOX_MODE_MIXED ? 8 * n + 1024 : 1;
+  CHK(dalloc(ctx, &ctx->nq, size_t(ctx->nq_cap))); CHK(dalloc(ctx, &ctx->nq_n, 1));
+  HIPCHK(hipMemset(ctx->nq_n, 0, sizeof(int)));
+  if (const char* e = std::getenv("SIMOX_TILE_XCD")) ctx->tile_xcd = std::atoi(e) != 0;
   CHK(dalloc(ctx, &ctx->frec, n3));
   CHK(dalloc(ctx, &ctx->qc, n)); CHK(dalloc(ctx, &ctx->ql, n)); CHK(dalloc(ctx, &ctx->gv, n));
   CHK(dalloc(ctx, &ctx->dv, n)); CHK(dalloc(ctx, &ctx->pv, n)); CHK(dalloc(ctx, &ctx->phi, n));
@@ -2456,7 +2511,7 @@ int simox_destroy(simox_ctx* ctx) {
   void* bufs[] = {ctx->dtab, ctx->x, ctx->v, ctx->f, ctx->q, ctx->id, ctx->perm, ctx->cell,
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
-                  ctx->dF32, ctx->near_cnt, ctx->near_col, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
+                  ctx->dF32, ctx->nb_fwd, ctx->nq, ctx->nq_n, ctx->facc, ctx->eacc, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
@@ -2558,6 +2613,9 @@ int simox_set_state(simox_ctx* ctx, const int* id, const double* x, const double
       ctx->err = "positions are required together with species ids";
       return SIMOX_ERR_ARG;
     }
+    ctx->spec_present = 0u;
+    for (int i = 0; i < n; ++i) ctx->spec_present |= 1u << id[i];
+    set_ctip64(ctx);
     hipLaunchKernelGGL(k_iota, dim3(grid_blocks(n)), dim3(BLOCK), 0, ctx->st, n, ctx->perm);
     HIPCHK(hipMemcpyAsync(ctx->id, id, sizeof(int) * n, hipMemcpyHostToDevice, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->v, 0, sizeof(double) * 3 * n, ctx->st));
@@ -2848,6 +2906,7 @@ int simox_set_params(simox_ctx* ctx, const simox_params* params) {
   if (!ctx || !params) return SIMOX_ERR_ARG;
   ctx->prm = *params;
   build_tables(ctx);
+  set_ctip64(ctx);
   HIPCHK(hipMemcpyAsync(ctx->dtab, &ctx->htab, sizeof(Tables), hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->matrix_valid = false;

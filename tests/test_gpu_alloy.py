@@ -4,7 +4,11 @@ with 60 Ni and 40 Fe substitutions and 16 O, written by the reference's own
 FORCE).  AL_NI hands each atom of an Al-Ni pair the density of the
 reference's loop orientation (force.f90:60-63), so the densities pin that
 orientation too.  Bars as the other FAITHFUL parity tests: forces and energies
-1e-10 relative; MIXED against FP64 e_F <= 1e-5."""
+1e-10 relative; MIXED against FP64 e_F <= 1e-5.
+
+The device evaluates each pair once from its forward list entry, which
+carries FORCE's orientation (simox_amd/csrc/tile.hpp; the list's order and
+indexing are checked on the CPU over alloy_s1 by tests/test_tile_cpu.py)."""
 import numpy as np
 import pytest
 
@@ -12,16 +16,7 @@ from simox_amd import MODE_FAITHFUL, MODE_FP64, MODE_MIXED
 
 from test_gpu_parity import gpu_for, load, oracle_for, relmax
 
-# The device branches for Ni/Fe cross pairs faulted the GPU on their first
-# run (an illegal memory access during FORCE of alloy_s1).  Reading the new
-# indexing (pair table (id_j, id_i), cell ids of both atoms) found nothing out
-# of bounds, so no cause is shown yet: the branches stay off on the device
-# (FORCE returns SIMOX_ERR_UNKNOWN_PAIR for Ni/Fe cross pairs, as before) and
-# these tests are skipped.  The oracle's METAL_HETERO / AL_NI restatement is
-# pinned against the reference build by tests/test_oracle_ref.py (alloy_s1).
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="Ni/Fe cross-pair device path faulted the GPU on its first "
-                                      "run; cause not found, path switched off on the device")]
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
