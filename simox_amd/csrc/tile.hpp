@@ -62,11 +62,11 @@ constexpr double EFIX_MAX = 1048576.0;
 // is 2^-13 Rcut^2 = 1.2e-4 Rcut^2.
 constexpr float TILE_BAND = 1.0f / 8192.0f;
 
-constexpr int TILE_CAP = 2560;  // staged atoms per chunk (30 KB of LDS)
+constexpr int TILE_CAP = 1024;  // staged atoms per chunk (16 KB of LDS: 4+ waves per SIMD)
 constexpr int TILE_NT = 128;    // threads per workgroup (two waves)
 
 struct TileLds {
-  float px[TILE_CAP], py[TILE_CAP], pz[TILE_CAP];
+  float4 p[TILE_CAP];             // x, y, z relative to the home centre (w unused)
   int vcell[NVISIT];        // cell of each visit
   int vl[NVISIT + 1];       // LDS start of each visit of the chunk
   int kcnt[NPT_MAX];        // entries found so far, per home atom
@@ -125,9 +125,9 @@ __host__ __device__ inline void tile_stage(const TileArgs& A, TileLds& L, int v0
     const int s0 = A.start[c], m = A.start[c + 1] - s0;
     for (int a = t; a < m; a += nt) {
       const int j = s0 + a;
-      L.px[lo + a] = rel_coord(A.x[3 * j], h[0], A.g.box[0]);
-      L.py[lo + a] = rel_coord(A.x[3 * j + 1], h[1], A.g.box[1]);
-      L.pz[lo + a] = rel_coord(A.x[3 * j + 2], h[2], A.g.box[2]);
+      L.p[lo + a] = make_float4(rel_coord(A.x[3 * j], h[0], A.g.box[0]),
+                                rel_coord(A.x[3 * j + 1], h[1], A.g.box[1]),
+                                rel_coord(A.x[3 * j + 2], h[2], A.g.box[2]), 0.0f);
     }
     lo += m;
   }
@@ -144,6 +144,10 @@ __host__ __device__ inline void tile_layout(const TileArgs& A, TileLds& L, int v
 }
 
 // Candidates of visits [v0, v1) for home atom a (sorted index i = hs + a).
+// Four candidates per round: their LDS reads and distances are independent,
+// so their latencies overlap.  WRAP: the box has fewer than four cells along
+// some axis, where the staged image need not be the pair's nearest.
+template <bool WRAP>
 __host__ __device__ inline void tile_scan(const TileArgs& A, TileLds& L, int v0, int v1, int hs,
                                           int a, const double h[3]) {
   const Geom& g = A.g;
@@ -159,30 +163,48 @@ __host__ __device__ inline void tile_scan(const TileArgs& A, TileLds& L, int v0,
   const bool wx = g.nc[0] < 4, wy = g.nc[1] < 4, wz = g.nc[2] < 4;
   const float rc2 = float(g.rcut2);
   const float lo2 = rc2 * (1.0f - TILE_BAND), hi2 = rc2 * (1.0f + TILE_BAND);
+  auto dist2 = [&](const float4& c) {
+    float dx = pxi - c.x, dy = pyi - c.y, dz = pzi - c.z;
+    if constexpr (WRAP) {
+      if (wx) dx -= bx * rintf(dx * ibx);
+      if (wy) dy -= by * rintf(dy * iby);
+      if (wz) dz -= bz * rintf(dz * ibz);
+    }
+    return dx * dx + dy * dy + dz * dz;
+  };
   int k = L.kcnt[a];
+  int* out = A.col + size_t(k) * n + i;  // entry k of atom i
+  // candidate j with FP32 r^2 = r2f below hi2: FP64 test in the band
+  auto take = [&](int j, float r2f) {
+    if (!(r2f < lo2)) {
+      double rx[3], r2;
+      min_image(g, xi, yi, zi, A.x[3 * j], A.x[3 * j + 1], A.x[3 * j + 2], rx, r2);
+      if (!(r2 < g.rcut2)) return;
+    }
+    if (k < A.cap) *out = j;
+    out += n;
+    ++k;
+  };
   for (int v = v0; v < v1; ++v) {
     if (v == 14) L.kfwd[a] = k;
     const int c = L.vcell[v];
     const int s0 = L.vl[v], s1 = L.vl[v + 1];
     const int gofs = A.start[c] - s0;  // sorted index of LDS slot s: s + gofs
-    int sb = s0, se = s1;
-    if (v == 0) sb = s0 + a + 1;  // home cell: j after i
-    if (v == 14) se = s0 + a;     // home cell: j before i
-    for (int s = sb; s < se; ++s) {
-      float dx = pxi - L.px[s], dy = pyi - L.py[s], dz = pzi - L.pz[s];
-      if (wx) dx -= bx * rintf(dx * ibx);
-      if (wy) dy -= by * rintf(dy * iby);
-      if (wz) dz -= bz * rintf(dz * ibz);
-      const float r2f = dx * dx + dy * dy + dz * dz;
-      if (!(r2f < hi2)) continue;  // also NaN: the FP64 test fails too
-      const int j = s + gofs;
-      if (!(r2f < lo2)) {
-        double rx[3], r2;
-        min_image(g, xi, yi, zi, A.x[3 * j], A.x[3 * j + 1], A.x[3 * j + 2], rx, r2);
-        if (!(r2 < g.rcut2)) continue;
-      }
-      if (k < A.cap) A.col[size_t(k) * n + i] = j;
-      ++k;
+    int s = s0, se = s1;
+    if (v == 0) s = s0 + a + 1;  // home cell: j after i
+    if (v == 14) se = s0 + a;    // home cell: j before i
+    for (; s + 4 <= se; s += 4) {
+      const float r0 = dist2(L.p[s]), r1 = dist2(L.p[s + 1]);
+      const float r2 = dist2(L.p[s + 2]), r3 = dist2(L.p[s + 3]);
+      // NaN compares false: the FP64 test would fail too
+      if (r0 < hi2) take(s + gofs, r0);
+      if (r1 < hi2) take(s + 1 + gofs, r1);
+      if (r2 < hi2) take(s + 2 + gofs, r2);
+      if (r3 < hi2) take(s + 3 + gofs, r3);
+    }
+    for (; s < se; ++s) {
+      const float r0 = dist2(L.p[s]);
+      if (r0 < hi2) take(s + gofs, r0);
     }
   }
   L.kcnt[a] = k;
@@ -235,13 +257,17 @@ __global__ void __launch_bounds__(TILE_NT) k_nlist_tile(TileArgs A) {
   for (int a = t; a < m; a += TILE_NT) { L.kcnt[a] = 0; L.kfwd[a] = 0; }
   __syncthreads();
   if (!tile_cells_ok(A, L)) return;  // uniform: every thread reads the same counts
+  const bool wrap = A.g.nc[0] < 4 || A.g.nc[1] < 4 || A.g.nc[2] < 4;
   for (int v0 = 0; v0 < NVISIT;) {
     const int v1 = tile_chunk_end(A, L, v0);
     if (v0 > 0) __syncthreads();  // the previous chunk is read by everyone
     if (t == 0) tile_layout(A, L, v0, v1);
     tile_stage(A, L, v0, v1, t, TILE_NT, h);
     __syncthreads();
-    for (int a = t; a < m; a += TILE_NT) tile_scan(A, L, v0, v1, hs, a, h);
+    if (wrap)
+      for (int a = t; a < m; a += TILE_NT) tile_scan<true>(A, L, v0, v1, hs, a, h);
+    else
+      for (int a = t; a < m; a += TILE_NT) tile_scan<false>(A, L, v0, v1, hs, a, h);
     v0 = v1;
   }
   for (int a = t; a < m; a += TILE_NT)
@@ -432,6 +458,106 @@ __global__ void __launch_bounds__(BLOCK) k_force_half(Geom g, const Tables* __re
   }
 }
 
+// K4 (default): FORCE's pair terms over the whole list, each atom summing its
+// own force in FP64 in list order: no atomics, deterministic, every pair
+// evaluated from both of its atoms.  A backward entry is the reference's pair
+// (j, i): its terms are evaluated with j as "i" (the pair table of (s_j, s_i)
+// and the arguments swapped), so that AL_NI's orientation holds, and its
+// separation r_ji = -r_ij gives F_i += r_ij s_ji.  Energies: half of every
+// entry's.  T = float (MIXED): pairs closer than R_NEAR are skipped; their
+// forward entries go to the near queue, which adds both atoms' shares in fixed
+// point (facc), as k_force_ctip64 adds the FP64 Slater terms of pc.ctip64
+// pairs; k_force_finish then adds facc to f.  rdf: as k_force_half, one count
+// per entry.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_force_full(Geom g, const Tables* __restrict__ tb,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ q,
+                                                      const int* __restrict__ id,
+                                                      const int* __restrict__ cnt,
+                                                      const int* __restrict__ cnt_fwd,
+                                                      const int* __restrict__ col,
+                                                      const T* __restrict__ dF,
+                                                      double* __restrict__ f,
+                                                      double* __restrict__ e_part,
+                                                      unsigned long long* __restrict__ rdf,
+                                                      double dr, int2* __restrict__ nq,
+                                                      int* __restrict__ nq_n, int nq_cap) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  extern __shared__ unsigned int hist[];
+  if (rdf) {
+    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x) hist[k] = 0u;
+    __syncthreads();
+  }
+  const int i = pair_block(g) * blockDim.x + threadIdx.x;
+  double es[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const int si = id[i];
+    const T qi = T(q[i]);
+    const T dFi = dF[i];
+    const T a = T(g.alpha);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double ep = 0.0, ed = 0.0, ec = 0.0;
+    const size_t n = size_t(g.n);
+    if (si >= 1 && si <= NSPEC) {
+      const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+      const int c = cnt[i], cf = cnt_fwd[i];
+      for (int k = 0; k < c; ++k) {
+        const int j = col[size_t(k) * n + i];
+        const int sj = id[j];
+        if (sj < 1 || sj > NSPEC) continue;
+        const bool fwd = k < cf;
+        const PairCoef& pc = fwd ? tb->pc[pidx(si, sj)] : tb->pc[pidx(sj, si)];
+        if (!pc.valid) continue;
+        double rxd[3], r2d;
+        min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rxd, r2d);
+        if (rdf) {
+          const int bin = int(sqrt(r2d) / dr);  // INT(r/pair%dr)
+          if (bin >= 1 && bin <= NRDF + 1)
+            atomicAdd(&hist[(bin - 1) * 25 + (sj - 1) * 5 + (si - 1)], 1u);
+        }
+        if (near_pair<T>(r2d)) {
+          if (fwd) {
+            const int p = atomicAdd(nq_n, 1);
+            if (p < nq_cap) nq[p] = make_int2(i, j);
+          }
+          continue;
+        }
+        const T qj = T(q[j]), dFj = dF[j];
+        // one evaluation with the pair's own "i" first (selects, no branch)
+        const int sa = fwd ? si : sj, sb = fwd ? sj : si;
+        const T qa = fwd ? qi : qj, qb = fwd ? qj : qi;
+        const T da = fwd ? dFi : dFj, db = fwd ? dFj : dFi;
+        T e0, e1, e2;
+        const double s = double(pair_force<T>(tb, pc, sa, sb, T(sqrt(r2d)), T(r2d), qa, qb, da,
+                                              db, r2d < g.ream2, a, e0, e1, e2));
+        ep += double(e0);
+        ed += double(e1);
+        ec += double(e2);
+        fx += rxd[0] * s;
+        fy += rxd[1] * s;
+        fz += rxd[2] * s;
+      }
+    }
+    f[3 * i] = fx;
+    f[3 * i + 1] = fy;
+    f[3 * i + 2] = fz;
+    es[0] = 0.5 * ep;
+    es[1] = 0.5 * ed;
+    es[2] = 0.5 * ec;
+  }
+  block_sum<3>(es, lds);
+  if (threadIdx.x == 0) {
+    e_part[3 * blockIdx.x] = es[0];
+    e_part[3 * blockIdx.x + 1] = es[1];
+    e_part[3 * blockIdx.x + 2] = es[2];
+  }
+  if (rdf) {  // block_sum ended with a barrier: the histogram is complete
+    for (int k = threadIdx.x; k < RDF_BINS; k += blockDim.x)
+      if (hist[k]) atomicAdd(&rdf[k], (unsigned long long)hist[k]);
+  }
+}
+
 // MIXED: the queued near pairs (i, j) in FP64, both atoms' shares into facc;
 // their energies in fixed point (2^32 per eV), one atomic per block into
 // eacc[0..2].  A grid-stride loop over min(*nq_n, nq_cap) pairs.
@@ -534,17 +660,21 @@ __global__ void __launch_bounds__(BLOCK) k_force_ctip64(Geom g, const Tables* __
   if (threadIdx.x == 0 && es[0]) fix_add(&eacc[2], es[0]);
 }
 
-// f = facc in eV/A, and facc back to zero for the next pass.  Thread 0 also
+// f = facc in eV/A (add: f += facc, after k_force_full), and facc back to
+// zero for the next pass.  Thread 0 also
 // moves the near-pair energies (eacc, MIXED) into row e_row of the energy
 // partials, resets eacc and the near queue.  The sums are read and cleared
 // with atomic exchanges, the operations that built them: a plain load here
 // has returned a stale zero for one of the eacc words.
 __global__ void k_force_finish(int n, unsigned long long* __restrict__ facc,
-                               double* __restrict__ f, unsigned long long* __restrict__ eacc,
-                               double* __restrict__ e_row, int* __restrict__ nq_n, int nq_cap,
-                               int* __restrict__ err) {
+                               double* __restrict__ f, int add,
+                               unsigned long long* __restrict__ eacc, double* __restrict__ e_row,
+                               int* __restrict__ nq_n, int nq_cap, int* __restrict__ err) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < 3 * n) f[t] = double((long long)atomicExch(&facc[t], 0ull)) * FIX_INV;
+  if (t < 3 * n) {
+    const double v = double((long long)atomicExch(&facc[t], 0ull)) * FIX_INV;
+    f[t] = add ? f[t] + v : v;
+  }
   if (t == 0 && e_row) {
     for (int d = 0; d < 3; ++d) e_row[d] = double((long long)atomicExch(&eacc[d], 0ull)) * EFIX_INV;
     if (atomicExch(nq_n, 0) > nq_cap) atomicCAS(err, 0, SIMOX_ERR_NB_OVERFLOW);

@@ -164,8 +164,8 @@ def test_tile_lists_c2(harness, tmp_path):
 
 def test_tile_several_chunks(harness, tmp_path):
     # 3 x 3 x 3 cells of ~100 atoms: the 28 visits stage ~2,800 atoms, more
-    # than one LDS chunk holds, so each workgroup runs two chunks; three cells
-    # per axis take the FP32 periodic fold
+    # than one LDS chunk holds, so each workgroup runs several chunks; three
+    # cells per axis take the FP32 periodic fold
     rng = np.random.default_rng(4)
     box = np.array([18.1, 18.2, 18.3])
     x = (rng.random((2700, 3)) - 0.5) * box

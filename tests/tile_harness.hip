@@ -92,8 +92,10 @@ int main(int argc, char** argv) {
       for (int v = v0; v < v1; ++v)
         if (v == 0 || v == 14) CHECK(L->vl[v + 1] - L->vl[v] == m);
       for (int t = 0; t < TILE_NT; ++t) tile_stage(A, *L, v0, v1, t, TILE_NT, h);
+      const bool wrap = nc[0] < 4 || nc[1] < 4 || nc[2] < 4;
       for (int t = 0; t < TILE_NT; ++t)
-        for (int a = t; a < m; a += TILE_NT) tile_scan(A, *L, v0, v1, hs, a, h);
+        for (int a = t; a < m; a += TILE_NT)
+          wrap ? tile_scan<true>(A, *L, v0, v1, hs, a, h) : tile_scan<false>(A, *L, v0, v1, hs, a, h);
       v0 = v1;
     }
     for (int t = 0; t < TILE_NT; ++t)
