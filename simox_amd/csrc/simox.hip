@@ -234,6 +234,7 @@ struct simox_ctx {
   double* dscal = nullptr;
   double* hscal = nullptr;
   int* derr = nullptr;
+  unsigned* red_ctr = nullptr;  // finished workgroups of a fused reduction (zero between launches)
   int* herr = nullptr;
   // host staging
   double* hstage = nullptr;
@@ -1721,6 +1722,7 @@ bool fused_tail(const simox_ctx* ctx) { return !ctx->small_path && ctx->prec_on;
 // One inner iteration: H0 p, exact line search, step, new direction.
 int ncg_iter_body(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
+  const int nbr = std::min(nb, RED_GRID);  // the fused-reduction kernels' grid
   const bool f32 = use32(ctx);
   float* p32 = f32 ? ctx->pv32 : nullptr;
   double* ls = ctx->dscal + 40;
@@ -1782,16 +1784,16 @@ int ncg_iter_body(simox_ctx* ctx) {
     // rounds launched: one more than the last iteration's line search ran
     // (8 after one that did not finish, at a solve's start, and in graphs)
     const int rounds = (ctx->ls_adapt && !ctx->use_graph) ? ctx->ls_rounds : 8;
-    for (int k = 0; k < rounds; ++k) {
-      hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                         ctx->qc, ctx->pv, ls, ctx->part);
-      hipLaunchKernelGGL(k_reduce_op<2>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 2,
-                         ctx->dscal + 24, ls);
-    }
+    // each round one launch: its last workgroup sums the rows and updates
+    // the step (RedTail)
+    for (int k = 0; k < rounds; ++k)
+      hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nbr), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                         ctx->qc, ctx->pv, ls, ctx->part,
+                         RedTail{ctx->red_ctr, ctx->dscal + 24, ls, 2});
   }
-  hipLaunchKernelGGL(k_ncg_step_dev, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
-                     ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part);
-  CHK(reduce(ctx, nb, 1, 0, 0));
+  hipLaunchKernelGGL(k_ncg_step_dev, dim3(nbr), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+                     ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part,
+                     RedTail{ctx->red_ctr, ctx->dscal + 0, nullptr, 0});
   if (fused_tail(ctx)) {
     // centre + scale, the preconditioner's grid pass, z with its six sums,
     // projected beta, direction (ncg_issue_iter reads sum g^2 from ls[10])
@@ -1800,28 +1802,28 @@ int ncg_iter_body(simox_ctx* ctx) {
     hipLaunchKernelGGL(k_center_scale, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                        ctx->qc, ctx->dscal + 0, ctx->gv, ctx->cv, ctx->sv);
     CHK(pme_eval(ctx, ctx->sv, nullptr, 0.0, f32, true, &pc));
+    // z, its six sums and (last workgroup) the projected beta in one launch
+    const RedTail rt{ctx->red_ctr, ctx->dscal + 56, ls, 4};
     if (f32 && pc.order == 2 && pc.spread32)
-      hipLaunchKernelGGL(k_prec_finish6_rec, dim3(nb), dim3(BLOCK), 0, ctx->st, gp, ctx->dtab,
+      hipLaunchKernelGGL(k_prec_finish6_rec, dim3(nbr), dim3(BLOCK), 0, ctx->st, gp, ctx->dtab,
                          pc.brec_atom, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv, pc.greal32,
-                         ctx->zv, ctx->part);
+                         ctx->zv, ctx->part, rt);
     else if (f32 && pc.order == 2)
-      hipLaunchKernelGGL((k_prec_finish6<2, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+      hipLaunchKernelGGL((k_prec_finish6<2, float>), dim3(nbr), dim3(BLOCK), 0, ctx->st, gp,
                          ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
-                         pc.greal32, ctx->zv, ctx->part);
+                         pc.greal32, ctx->zv, ctx->part, rt);
     else if (pc.order == 2)
-      hipLaunchKernelGGL((k_prec_finish6<2, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+      hipLaunchKernelGGL((k_prec_finish6<2, double>), dim3(nbr), dim3(BLOCK), 0, ctx->st, gp,
                          ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
-                         pc.greal64, ctx->zv, ctx->part);
+                         pc.greal64, ctx->zv, ctx->part, rt);
     else if (f32)
-      hipLaunchKernelGGL((k_prec_finish6<4, float>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+      hipLaunchKernelGGL((k_prec_finish6<4, float>), dim3(nbr), dim3(BLOCK), 0, ctx->st, gp,
                          ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
-                         pc.greal32, ctx->zv, ctx->part);
+                         pc.greal32, ctx->zv, ctx->part, rt);
     else
-      hipLaunchKernelGGL((k_prec_finish6<4, double>), dim3(nb), dim3(BLOCK), 0, ctx->st, gp,
+      hipLaunchKernelGGL((k_prec_finish6<4, double>), dim3(nbr), dim3(BLOCK), 0, ctx->st, gp,
                          ctx->dtab, ctx->x, ctx->id, ctx->qc, ctx->gv, ctx->rv, ctx->cv,
-                         pc.greal64, ctx->zv, ctx->part);
-    hipLaunchKernelGGL(k_reduce_op<4>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 6,
-                       ctx->dscal + 56, ls);
+                         pc.greal64, ctx->zv, ctx->part, rt);
     hipLaunchKernelGGL(k_pcg_dir_proj, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ls, ctx->zv, ctx->pv,
                        p32);
     HIPCHK(hipGetLastError());
@@ -2482,6 +2484,8 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   HIPCHK(hipHostMalloc((void**)&ctx->hscal, 64 * sizeof(double), hipHostMallocDefault));
   CHK(dalloc(ctx, &ctx->derr, 1));
   HIPCHK(hipMemset(ctx->derr, 0, sizeof(int)));
+  CHK(dalloc(ctx, &ctx->red_ctr, 1));
+  HIPCHK(hipMemset(ctx->red_ctr, 0, sizeof(unsigned)));
   HIPCHK(hipHostMalloc((void**)&ctx->herr, sizeof(int), hipHostMallocDefault));
   {
     // partial-sum buffer sized once for every launch that uses it
@@ -2539,7 +2543,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->dF32, ctx->nb_fwd, ctx->nq, ctx->nq_n, ctx->facc, ctx->eacc, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
                   ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
-                  ctx->part, ctx->dscal, ctx->derr, ctx->dstage, ctx->part_eg};
+                  ctx->part, ctx->dscal, ctx->derr, ctx->red_ctr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   free_grid(ctx->pm);

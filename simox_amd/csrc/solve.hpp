@@ -20,6 +20,84 @@ namespace sx {
 __device__ inline void ls_init(const double* red, double* ls);
 __device__ inline void ls_update(const double* red, double* ls);
 __device__ inline void beta_update(const double* red, double* ls);
+template <int OP>
+__device__ inline void reduce_scalar(const double* out, double* ls);
+
+// Final reduction done by the producer's last workgroup instead of a
+// k_reduce_op launch: ctr (zero between launches) counts the finished
+// workgroups, out receives the row sums, op the scalar step of k_reduce_op
+// (0 none, 2 ls_update, 4 projected beta); ctr == null: rows only.
+constexpr int RED_GRID = 256;
+struct RedTail {
+  unsigned* ctr;
+  double* out;
+  double* ls;
+  int op;
+};
+
+// The workgroup's row of NV partial sums (valid in thread 0 after
+// block_sum) to part[NV*block ..].  With t.ctr, each workgroup then counts
+// itself done; the one that counts last sums every row in workgroup order, as
+// k_reduce_op does (fixed rows per thread, fixed shuffle tree, fixed wave
+// order: deterministic), writes t.out, runs the scalar step and clears the
+// counter for the next launch.  The rows are stored and read with agent-scope
+// atomic operations, which are coherent across the XCDs' L2 caches, and each
+// workgroup waits for its stores to be acknowledged before it counts itself
+// done.  An acquire-release count would do the same with an L2 writeback and
+// an L2 invalidate in every workgroup: measured ten times slower (6.9 -> 70 us
+// per line-search round at C3).
+// The producers run on RED_GRID workgroups at most (grid-stride over the
+// atoms): at C3, 4,096 workgroups counting on one address took 40 us per
+// launch, 512 took 11 us and 256 took 10 us per line-search round (the
+// preconditioner's finish, whose gathers want more workgroups: 29 us at 256,
+// 33 us at 1,024).
+// lds: the block_sum scratch (NV * blockDim / WAVE doubles).
+template <int NV>
+__device__ inline void partials_out(const double (&acc)[NV], double* part, const RedTail& t,
+                                    double* lds) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&part[size_t(NV) * blockIdx.x + k], acc[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    last = 0;
+    if (t.ctr) {
+      __builtin_amdgcn_s_waitcnt(0);  // the row's stores acknowledged
+      last = __hip_atomic_fetch_add(t.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             gridDim.x - 1;
+    }
+  }
+  if (!t.ctr) return;  // uniform over the launch
+  __syncthreads();
+  if (!last) return;   // uniform over the workgroup
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  for (int k = threadIdx.x; k < int(gridDim.x); k += blockDim.x)
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      s[v] += __hip_atomic_load(&part[size_t(NV) * k + v], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double x = wave_sum(s[v]);
+    if (lane == 0) lds[v * nw + w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double x = lds[v * nw];
+      for (int q = 1; q < nw; ++q) x += lds[v * nw + q];
+      t.out[v] = x;
+    }
+    if (t.op == 2) reduce_scalar<2>(t.out, t.ls);
+    if (t.op == 4) reduce_scalar<4>(t.out, t.ls);
+    __hip_atomic_store(t.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 template <class Tv, class Tp>
 __device__ inline Tv ell_dot(int c, size_t n, int i, const int* __restrict__ col,
@@ -1205,25 +1283,23 @@ __global__ void __launch_bounds__(BLOCK) k_prec_finish6(Geom g, const Tables* __
                                                         const double* __restrict__ c,
                                                         const Rg* __restrict__ grid,
                                                         double* __restrict__ z,
-                                                        double* __restrict__ part) {
+                                                        double* __restrict__ part, RedTail rt) {
   __shared__ double lds[6 * BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < g.n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
     const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
     const double gi = gr[i], go = g_old[i];
     const double zi = (gi / sd - c[i] * gather_ord<ORD>(g, x + 3 * i, grid)) / sd;
     z[i] = zi;
-    acc[0] = zi;
-    acc[1] = zi * gi;
-    acc[2] = zi * go;
-    acc[3] = gi;
-    acc[4] = go;
-    acc[5] = gi * gi;
+    acc[0] += zi;
+    acc[1] += zi * gi;
+    acc[2] += zi * go;
+    acc[3] += gi;
+    acc[4] += go;
+    acc[5] += gi * gi;
   }
   block_sum<6>(acc, lds);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
+  partials_out<6>(acc, part, rt, lds);
 }
 
 // k_prec_finish6<2, float> with the preconditioner grid's atom-order stencil
@@ -1238,25 +1314,23 @@ __global__ void __launch_bounds__(BLOCK) k_prec_finish6_rec(Geom g, const Tables
                                                             const double* __restrict__ c,
                                                             const float* __restrict__ grid,
                                                             double* __restrict__ z,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, RedTail rt) {
   __shared__ double lds[6 * BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < g.n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
     const double sd = sqrt(diag_m(tb->sc[id[i] - 1], q[i]));
     const double gi = gr[i], go = g_old[i];
     const double zi = (gi / sd - c[i] * gather_cic_rec(g, rec_atom[i], grid)) / sd;
     z[i] = zi;
-    acc[0] = zi;
-    acc[1] = zi * gi;
-    acc[2] = zi * go;
-    acc[3] = gi;
-    acc[4] = go;
-    acc[5] = gi * gi;
+    acc[0] += zi;
+    acc[1] += zi * gi;
+    acc[2] += zi * go;
+    acc[3] += gi;
+    acc[4] += go;
+    acc[5] += gi * gi;
   }
   block_sum<6>(acc, lds);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = acc[k];
+  partials_out<6>(acc, part, rt, lds);
 }
 
 // p = (z - mean z) + beta p (mean z in ls[11], beta in ls[7])
@@ -1349,23 +1423,19 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
                                                       const double* __restrict__ q,
                                                       const double* __restrict__ p,
                                                       const double* __restrict__ ls,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, RedTail rt) {
   __shared__ double lds[2 * BLOCK / WAVE];
-  if (ls[5] != 0.0) return;
+  if (ls[5] != 0.0) return;  // uniform: the search is done
   const double a = ls[2];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[2] = {0.0, 0.0};
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double pi = p[i], qi = q[i], qa = qi + a * pi;
-    acc[0] = pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
-    acc[1] = outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+    acc[0] += pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
+    acc[1] += outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
   }
   block_sum<2>(acc, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = acc[0];
-    part[2 * blockIdx.x + 1] = acc[1];
-  }
+  partials_out<2>(acc, part, rt, lds);
 }
 
 // The whole line search in one workgroup (systems of up to LS_SMALL_MAX atoms;
@@ -1603,17 +1673,20 @@ __global__ void k_reduce_op(const double* __restrict__ part, int nparts, int nv,
     out[v] = t;
   }
   __syncthreads();  // out[] complete before the scalar step below reads it
-  if (threadIdx.x == 0 && ls) {
-    if (OP == 1) ls_init(out, ls);
-    if (OP == 2) ls_update(out, ls);
-    if (OP == 3) beta_update(out, ls);
-    if (OP == 4) {  // out = the six sums of k_prec_finish6 (n = nparts' atom count in ls[12])
-      const double zbar = out[0] / ls[12];
-      const double red[2] = {out[1] - zbar * out[3], out[2] - zbar * out[4]};
-      beta_update(red, ls);
-      ls[10] = out[5];
-      ls[11] = zbar;
-    }
+  if (threadIdx.x == 0 && ls) reduce_scalar<OP>(out, ls);
+}
+
+template <int OP>
+__device__ inline void reduce_scalar(const double* out, double* ls) {
+  if (OP == 1) ls_init(out, ls);
+  if (OP == 2) ls_update(out, ls);
+  if (OP == 3) beta_update(out, ls);
+  if (OP == 4) {  // out = the six sums of k_prec_finish6 (n = nparts' atom count in ls[12])
+    const double zbar = out[0] / ls[12];
+    const double red[2] = {out[1] - zbar * out[3], out[2] - zbar * out[4]};
+    beta_update(red, ls);
+    ls[10] = out[5];
+    ls[11] = zbar;
   }
 }
 
@@ -1646,12 +1719,11 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __r
                                                         double* __restrict__ q,
                                                         double* __restrict__ g,
                                                         double* __restrict__ g_old,
-                                                        double* __restrict__ part) {
+                                                        double* __restrict__ part, RedTail rt) {
   __shared__ double lds[BLOCK / WAVE];
   const double a = ls[8] != 0.0 ? 0.0 : ls[2];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double qi = q[i], qn = qi + a * p[i];
     const double gi = g[i];
@@ -1659,10 +1731,10 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __r
     const double gn = gi - a * v[i] - (pen_d1(sc, qn) - pen_d1(sc, qi));
     q[i] = qn;
     g[i] = gn;
-    acc[0] = gn;
+    acc[0] += gn;
   }
   block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+  partials_out<1>(acc, part, rt, lds);
 }
 
 __global__ void k_pcg_dir_dev(int n, const double* __restrict__ ls, const double* __restrict__ z,
