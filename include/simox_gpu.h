@@ -169,6 +169,13 @@ int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]);
    chain)}.  Every choice gives the same results bit for bit.  Inspection
    only (no counterpart in the reference). */
 int simox_inner_shapes(const simox_ctx* ctx, int shapes[6]);
+/* Cell-tile shape of the neighbour-list build {atoms staged per LDS chunk,
+   threads per workgroup}, which the context times on its first list build
+   (SIMOX_AUTOTUNE=0 keeps the default {1024, 128}; SIMOX_TILE_SHAPE = 0-5
+   fixes one of {1024,128}, {512,128}, {2048,128}, {1024,64}, {1024,256},
+   {512,256}).  Every shape gives the same list.  Inspection only (no
+   counterpart in the reference). */
+int simox_tile_shape(const simox_ctx* ctx, int shape[2]);
 int simox_get_geometry(const simox_ctx* ctx, int ncell[3], double lcell[3], double* alpha);
 int simox_set_mode(simox_ctx* ctx, int mode);
 /* Tolerance (RMS projected gradient, eV/e) and iteration cap of the converged

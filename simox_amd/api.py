@@ -22,7 +22,7 @@ EXPORTS = [
     "simox_pme_self", "simox_pme", "simox_phi", "simox_ctip_solve", "simox_dudq",
     "simox_get_neighbors", "simox_vv1", "simox_vv2", "simox_step", "simox_kernel_timing",
     "simox_timing_enable", "simox_timing_select", "simox_set_charge_precond", "simox_set_inner_grid", "simox_inner_entries", "simox_charge_stats", "simox_set_params",
-    "simox_autotune", "simox_fft_shapes", "simox_inner_shapes", "simox_set_eam_cutoff", "simox_set_eam_splines",
+    "simox_autotune", "simox_fft_shapes", "simox_inner_shapes", "simox_tile_shape", "simox_set_eam_cutoff", "simox_set_eam_splines",
     "simox_eam_spline_grad", "simox_param_grad_all", "simox_eam_spline_force_vjp",
     "simox_param_force_grad",
     "simox_set_thermostat", "simox_set_rdf", "simox_get_rdf", "simox_set_reflect",
@@ -103,6 +103,7 @@ def load_library(path=LIB_PATH):
     lib.simox_get_rdf.argtypes = [vp, P(C.c_longlong), i]
     lib.simox_fft_shapes.argtypes = [vp, P(i)]
     lib.simox_inner_shapes.argtypes = [vp, P(i)]
+    lib.simox_tile_shape.argtypes = [vp, P(i)]
     lib.simox_set_state.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.simox_get_state.argtypes = [vp, vp, vp, vp, vp]
     lib.simox_cell_sort.argtypes = [vp]
@@ -270,6 +271,13 @@ class Simox:
         self._check(self.lib.simox_inner_shapes(self.h, a))
         return {"matvec": a[0], "gather": a[1], "inner_grid": (a[2], a[3]),
                 "prec_grid": (a[4], a[5])}
+
+    def tile_shape(self):
+        """Cell tile of the neighbour-list build (tuned at the first list
+        build): (atoms staged per LDS chunk, threads per workgroup)."""
+        a = (C.c_int * 2)()
+        self._check(self.lib.simox_tile_shape(self.h, a))
+        return (a[0], a[1])
 
     def set_params(self, params):
         """Replace the potential parameters (a Params from potential_setup())."""

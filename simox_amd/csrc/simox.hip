@@ -138,6 +138,8 @@ struct simox_ctx {
   int* nq_n = nullptr;
   int nq_cap = 0;
   int tile_xcd = 1;  // k_nlist_tile: neighbouring cells on one XCD
+  int tshape = 0;    // k_nlist_tile's cell-tile shape (TILE_SHAPES), autotuned
+  bool tile_tuned = false, tile_fixed = false;
   bool ctip64_on = false;  // MIXED: species present form a pc.ctip64 pair
   bool force_half = false;  // K4 on the forward entries with fixed-point sums (SIMOX_FORCE_HALF=1)
   unsigned spec_present = 0u;  // bit s: species s among the atoms
@@ -563,14 +565,30 @@ int do_cell_sort(simox_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // K2-K4: FORCE
 // ---------------------------------------------------------------------------
+// k_nlist_tile in cell-tile shape sh (TILE_SHAPES).
+template <int CAP, int NT>
+void launch_tile_t(simox_ctx* ctx, const TileArgs& ta) {
+  hipLaunchKernelGGL((k_nlist_tile<CAP, NT>), dim3(ctx->g.ncall), dim3(NT), 0, ctx->st, ta);
+}
+void launch_tile(simox_ctx* ctx, int sh) {
+  TileArgs ta{ctx->g, ctx->x, ctx->start, ctx->cap, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col,
+              ctx->derr};
+  ta.g.xcd = ctx->tile_xcd;
+  switch (sh) {
+    case 1: launch_tile_t<TILE_SHAPES[1][0], TILE_SHAPES[1][1]>(ctx, ta); break;
+    case 2: launch_tile_t<TILE_SHAPES[2][0], TILE_SHAPES[2][1]>(ctx, ta); break;
+    case 3: launch_tile_t<TILE_SHAPES[3][0], TILE_SHAPES[3][1]>(ctx, ta); break;
+    case 4: launch_tile_t<TILE_SHAPES[4][0], TILE_SHAPES[4][1]>(ctx, ta); break;
+    case 5: launch_tile_t<TILE_SHAPES[5][0], TILE_SHAPES[5][1]>(ctx, ta); break;
+    default: launch_tile_t<TILE_SHAPES[0][0], TILE_SHAPES[0][1]>(ctx, ta); break;
+  }
+}
+
 // Neighbour list (k_nlist_tile, one workgroup per cell) + EAM density and
 // embedding (k_density), once per set of positions.
 void launch_nlist(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
-  TileArgs ta{ctx->g, ctx->x, ctx->start, ctx->cap, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col,
-              ctx->derr};
-  ta.g.xcd = ctx->tile_xcd;
-  hipLaunchKernelGGL(k_nlist_tile, dim3(ctx->g.ncall), dim3(TILE_NT), 0, ctx->st, ta);
+  launch_tile(ctx, ctx->tshape);
   if (use32(ctx))
     hipLaunchKernelGGL(k_density<float>, dim3(nb), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
                        ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_fwd, ctx->nb_col, ctx->rho,
@@ -581,8 +599,28 @@ void launch_nlist(simox_ctx* ctx) {
                        ctx->dF64, ctx->part_eg, ctx->derr);
 }
 
+bool autotune_on();
+template <class F>
+int time_launch(simox_ctx* ctx, F launch, float* ms);
+
+// Autotuner of the cell tile: every shape of TILE_SHAPES timed on the first
+// list build of the context, the fastest kept (the lists are the same).
+int tune_tile(simox_ctx* ctx) {
+  ctx->tile_tuned = true;
+  if (ctx->tile_fixed || !autotune_on()) return 0;
+  float best = 1e30f, ms = 0.f;
+  int pick = ctx->tshape;
+  for (int sh = 0; sh < NTILE_SHAPES; ++sh) {
+    CHK(time_launch(ctx, [&] { launch_tile(ctx, sh); }, &ms));
+    if (ms < best) { best = ms; pick = sh; }
+  }
+  ctx->tshape = pick;
+  return 0;
+}
+
 int ensure_nlist(simox_ctx* ctx) {
   if (ctx->nlist_valid) return 0;
+  if (!ctx->tile_tuned) CHK(tune_tile(ctx));
   hipEvent_t ev{};
   tstart(ctx, 8, &ev);
   launch_nlist(ctx);
@@ -2414,6 +2452,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   CHK(dalloc(ctx, &ctx->nq, size_t(ctx->nq_cap))); CHK(dalloc(ctx, &ctx->nq_n, 1));
   HIPCHK(hipMemset(ctx->nq_n, 0, sizeof(int)));
   if (const char* e = std::getenv("SIMOX_TILE_XCD")) ctx->tile_xcd = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_TILE_SHAPE")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v < NTILE_SHAPES) { ctx->tshape = v; ctx->tile_fixed = true; }
+  }
   // the half-shell force pass with fixed-point sums: measured slower at C3
   // (15.8 ms against 4.0 ms for the owner-computes pass), so off by default
   if (const char* e = std::getenv("SIMOX_FORCE_HALF")) ctx->force_half = std::atoi(e) != 0;
@@ -2517,6 +2559,13 @@ int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]) {
     shapes[3 * k + 1] = c[k]->my;
     shapes[3 * k + 2] = c[k]->mz;
   }
+  return SIMOX_OK;
+}
+
+int simox_tile_shape(const simox_ctx* ctx, int shape[2]) {
+  if (!ctx || !shape) return SIMOX_ERR_ARG;
+  shape[0] = TILE_SHAPES[ctx->tshape][0];
+  shape[1] = TILE_SHAPES[ctx->tshape][1];
   return SIMOX_OK;
 }
 

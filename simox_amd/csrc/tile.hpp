@@ -62,11 +62,18 @@ constexpr double EFIX_MAX = 1048576.0;
 // is 2^-13 Rcut^2 = 1.2e-4 Rcut^2.
 constexpr float TILE_BAND = 1.0f / 8192.0f;
 
-constexpr int TILE_CAP = 1024;  // staged atoms per chunk (16 KB of LDS: 4+ waves per SIMD)
-constexpr int TILE_NT = 128;    // threads per workgroup (two waves)
+// Cell-tile shapes (staged atoms per LDS chunk, threads per workgroup), chosen
+// per context by timing at the first list build (all give the same list: the
+// chunks split the visits, whose order is fixed).  The first is the default.
+constexpr int NTILE_SHAPES = 6;
+constexpr int TILE_SHAPES[NTILE_SHAPES][2] = {{1024, 128}, {512, 128}, {2048, 128},
+                                              {1024, 64},  {1024, 256}, {512, 256}};
+constexpr int TILE_CAP = 1024;  // the default shape: 16 KB of LDS, 4+ waves per SIMD
+constexpr int TILE_NT = 128;    // two waves
 
+template <int CAP>
 struct TileLds {
-  float4 p[TILE_CAP];             // x, y, z relative to the home centre (w unused)
+  float4 p[CAP];                  // x, y, z relative to the home centre (w unused)
   int vcell[NVISIT];        // cell of each visit
   int vl[NVISIT + 1];       // LDS start of each visit of the chunk
   int kcnt[NPT_MAX];        // entries found so far, per home atom
@@ -102,14 +109,16 @@ __host__ __device__ inline float rel_coord(double x, double h, double box) {
 }
 
 // First visit of the chunk after [v0, ...) and the LDS layout of the chunk:
-// visits are taken in order while their atoms fit in TILE_CAP.  A visit never
-// exceeds NPT_MAX <= TILE_CAP atoms, so every chunk holds at least one.
-__host__ __device__ inline int tile_chunk_end(const TileArgs& A, const TileLds& L, int v0) {
+// visits are taken in order while their atoms fit in CAP.  A visit never
+// exceeds NPT_MAX <= CAP atoms, so every chunk holds at least one.
+template <int CAP>
+__host__ __device__ inline int tile_chunk_end(const TileArgs& A, const TileLds<CAP>& L, int v0) {
+  static_assert(NPT_MAX <= CAP, "a cell fits in one chunk");
   int tot = 0, v = v0;
   for (; v < NVISIT; ++v) {
     const int c = L.vcell[v];
     const int m = A.start[c + 1] - A.start[c];
-    if (v > v0 && tot + m > TILE_CAP) break;
+    if (v > v0 && tot + m > CAP) break;
     tot += m;
   }
   return v;
@@ -117,7 +126,8 @@ __host__ __device__ inline int tile_chunk_end(const TileArgs& A, const TileLds& 
 
 // Stage the atoms of visits [v0, v1) (thread t of nt); vl is filled by every
 // thread identically before the barrier that follows.
-__host__ __device__ inline void tile_stage(const TileArgs& A, TileLds& L, int v0, int v1, int t,
+template <int CAP>
+__host__ __device__ inline void tile_stage(const TileArgs& A, TileLds<CAP>& L, int v0, int v1, int t,
                                            int nt, const double h[3]) {
   int lo = 0;
   for (int v = v0; v < v1; ++v) {
@@ -133,7 +143,8 @@ __host__ __device__ inline void tile_stage(const TileArgs& A, TileLds& L, int v0
   }
 }
 
-__host__ __device__ inline void tile_layout(const TileArgs& A, TileLds& L, int v0, int v1) {
+template <int CAP>
+__host__ __device__ inline void tile_layout(const TileArgs& A, TileLds<CAP>& L, int v0, int v1) {
   int lo = 0;
   for (int v = v0; v < v1; ++v) {
     L.vl[v] = lo;
@@ -147,8 +158,8 @@ __host__ __device__ inline void tile_layout(const TileArgs& A, TileLds& L, int v
 // Four candidates per round: their LDS reads and distances are independent,
 // so their latencies overlap.  WRAP: the box has fewer than four cells along
 // some axis, where the staged image need not be the pair's nearest.
-template <bool WRAP>
-__host__ __device__ inline void tile_scan(const TileArgs& A, TileLds& L, int v0, int v1, int hs,
+template <bool WRAP, int CAP>
+__host__ __device__ inline void tile_scan(const TileArgs& A, TileLds<CAP>& L, int v0, int v1, int hs,
                                           int a, const double h[3]) {
   const Geom& g = A.g;
   const int i = hs + a;
@@ -226,7 +237,8 @@ __host__ __device__ inline void tile_home(const TileArgs& A, int b, int& cx, int
 
 // Counts of home atom a; true when its list overflowed (reported like
 // FORCE's list overflow, force.f90:79).
-__host__ __device__ inline bool tile_finish(const TileArgs& A, const TileLds& L, int hs, int a) {
+template <int CAP>
+__host__ __device__ inline bool tile_finish(const TileArgs& A, const TileLds<CAP>& L, int hs, int a) {
   const int i = hs + a, k = L.kcnt[a], kf = L.kfwd[a];
   A.cnt[i] = k < A.cap ? k : A.cap;
   A.cnt_fwd[i] = kf < A.cap ? kf : A.cap;
@@ -235,7 +247,8 @@ __host__ __device__ inline bool tile_finish(const TileArgs& A, const TileLds& L,
 
 // Every visited cell holds at most NPT_MAX atoms (a fuller cell is an error
 // of the cell sort, which reports it; the workgroup then stages nothing).
-__host__ __device__ inline bool tile_cells_ok(const TileArgs& A, const TileLds& L) {
+template <int CAP>
+__host__ __device__ inline bool tile_cells_ok(const TileArgs& A, const TileLds<CAP>& L) {
   for (int v = 0; v < NVISIT; ++v) {
     const int c = L.vcell[v];
     if (A.start[c + 1] - A.start[c] > NPT_MAX) return false;
@@ -243,10 +256,11 @@ __host__ __device__ inline bool tile_cells_ok(const TileArgs& A, const TileLds& 
   return true;
 }
 
-// One workgroup of TILE_NT threads per cell (grid = ncall; with g.xcd the
+// One workgroup of NT threads per cell (grid = ncall; with g.xcd the
 // cells go to XCDs in runs of neighbours, which share staged cells in L2).
-__global__ void __launch_bounds__(TILE_NT) k_nlist_tile(TileArgs A) {
-  __shared__ TileLds L;
+template <int CAP, int NT>
+__global__ void __launch_bounds__(NT) k_nlist_tile(TileArgs A) {
+  __shared__ TileLds<CAP> L;
   const int b = A.g.xcd ? xcd_tile(blockIdx.x, gridDim.x) : int(blockIdx.x);
   int cx, cy, cz, hs, m;
   double h[3];
@@ -254,7 +268,7 @@ __global__ void __launch_bounds__(TILE_NT) k_nlist_tile(TileArgs A) {
   if (m == 0 || m > NPT_MAX) return;  // uniform over the workgroup
   const int t = threadIdx.x;
   if (t < NVISIT) L.vcell[t] = visit_cell(A.g, cx, cy, cz, t);
-  for (int a = t; a < m; a += TILE_NT) { L.kcnt[a] = 0; L.kfwd[a] = 0; }
+  for (int a = t; a < m; a += NT) { L.kcnt[a] = 0; L.kfwd[a] = 0; }
   __syncthreads();
   if (!tile_cells_ok(A, L)) return;  // uniform: every thread reads the same counts
   const bool wrap = A.g.nc[0] < 4 || A.g.nc[1] < 4 || A.g.nc[2] < 4;
@@ -262,15 +276,15 @@ __global__ void __launch_bounds__(TILE_NT) k_nlist_tile(TileArgs A) {
     const int v1 = tile_chunk_end(A, L, v0);
     if (v0 > 0) __syncthreads();  // the previous chunk is read by everyone
     if (t == 0) tile_layout(A, L, v0, v1);
-    tile_stage(A, L, v0, v1, t, TILE_NT, h);
+    tile_stage(A, L, v0, v1, t, NT, h);
     __syncthreads();
     if (wrap)
-      for (int a = t; a < m; a += TILE_NT) tile_scan<true>(A, L, v0, v1, hs, a, h);
+      for (int a = t; a < m; a += NT) tile_scan<true>(A, L, v0, v1, hs, a, h);
     else
-      for (int a = t; a < m; a += TILE_NT) tile_scan<false>(A, L, v0, v1, hs, a, h);
+      for (int a = t; a < m; a += NT) tile_scan<false>(A, L, v0, v1, hs, a, h);
     v0 = v1;
   }
-  for (int a = t; a < m; a += TILE_NT)
+  for (int a = t; a < m; a += NT)
     if (tile_finish(A, L, hs, a)) atomicCAS(A.err, 0, SIMOX_ERR_NB_OVERFLOW);
 }
 

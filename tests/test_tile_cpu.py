@@ -26,14 +26,26 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
-@pytest.fixture(scope="module")
-def harness(tmp_path_factory):
+def build_harness(tmp_path_factory, cap=None, nt=None):
     out = str(tmp_path_factory.mktemp("tile") / "tile_harness")
     cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-g", "-Xarch_host",
            "-fsanitize=address", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            os.path.join(ROOT, "tests", "tile_harness.hip"), "-o", out]
+    if cap:
+        cmd[1:1] = ["-DHARNESS_CAP=%d" % cap, "-DHARNESS_NT=%d" % nt]
     subprocess.run(cmd, check=True, capture_output=True)
     return out
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    return build_harness(tmp_path_factory)
+
+
+@pytest.fixture(scope="module")
+def harness_small(tmp_path_factory):
+    """The 512-atom, 256-thread cell tile (one of the autotuner's shapes)."""
+    return build_harness(tmp_path_factory, 512, 256)
 
 
 def geometry(box, rcut):
@@ -181,3 +193,12 @@ def test_tile_overflow_reported(harness, tmp_path):
     r = run(harness, tmp_path, x, box, 6.5, cap=300)
     assert r["err"] == 2  # SIMOX_ERR_NB_OVERFLOW
     assert r["cnt"].max() == 300
+
+
+@pytest.mark.parametrize("case", ["tiny_s2", "c1_s1", "alloy_s1"])
+def test_tile_shapes_give_the_same_lists(harness, harness_small, tmp_path, case):
+    d = np.load(os.path.join(GOLDEN, case + ".npz"))
+    r0 = run(harness, tmp_path, d["x"], d["box"], float(d["rcut"]))
+    r1 = run(harness_small, tmp_path, d["x"], d["box"], float(d["rcut"]))
+    assert np.array_equal(r0["cnt"], r1["cnt"]) and np.array_equal(r0["fwd"], r1["fwd"])
+    assert np.array_equal(r0["col"], r1["col"])  # same entries in the same order

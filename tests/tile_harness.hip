@@ -17,6 +17,15 @@
 
 using namespace sx;
 
+// the cell-tile shape under test (staged atoms per LDS chunk, threads)
+#ifndef HARNESS_CAP
+#define HARNESS_CAP TILE_CAP
+#endif
+#ifndef HARNESS_NT
+#define HARNESS_NT TILE_NT
+#endif
+constexpr int CAP = HARNESS_CAP, NT = HARNESS_NT;
+
 #define CHECK(c)                                                            \
   do {                                                                      \
     if (!(c)) {                                                             \
@@ -61,7 +70,7 @@ int main(int argc, char** argv) {
   std::vector<int> cnt(n, -1), cnt_fwd(n, -1), col(size_t(cap) * n, -1);
   int err = 0;
   TileArgs A{g, x.data(), start.data(), cap, cnt.data(), cnt_fwd.data(), col.data(), &err};
-  TileLds* L = new TileLds;
+  TileLds<CAP>* L = new TileLds<CAP>;
 
   for (int b = 0; b < ncall; ++b) {
     int cx, cy, cz, hs, m;
@@ -69,13 +78,13 @@ int main(int argc, char** argv) {
     tile_home(A, b, cx, cy, cz, h, hs, m);
     CHECK(b >= 0 && b + 1 <= ncall);
     if (m == 0 || m > NPT_MAX) continue;
-    for (int t = 0; t < TILE_NT; ++t)
+    for (int t = 0; t < NT; ++t)
       if (t < NVISIT) {
         L->vcell[t] = visit_cell(g, cx, cy, cz, t);
         CHECK(L->vcell[t] >= 0 && L->vcell[t] < ncall);
       }
-    for (int t = 0; t < TILE_NT; ++t)
-      for (int a = t; a < m; a += TILE_NT) {
+    for (int t = 0; t < NT; ++t)
+      for (int a = t; a < m; a += NT) {
         CHECK(a < NPT_MAX);
         L->kcnt[a] = 0;
         L->kfwd[a] = 0;
@@ -86,20 +95,21 @@ int main(int argc, char** argv) {
       CHECK(v1 > v0 && v1 <= NVISIT);
       tile_layout(A, *L, v0, v1);
       // every LDS slot the chunk uses lies in the position arrays
-      CHECK(L->vl[v1] <= TILE_CAP);
+      CHECK(L->vl[v1] <= CAP);
       for (int v = v0; v < v1; ++v) CHECK(L->vl[v] >= 0 && L->vl[v] <= L->vl[v + 1]);
       // the home cell's visits stage it whole, so slot s0 + a is home atom a
       for (int v = v0; v < v1; ++v)
         if (v == 0 || v == 14) CHECK(L->vl[v + 1] - L->vl[v] == m);
-      for (int t = 0; t < TILE_NT; ++t) tile_stage(A, *L, v0, v1, t, TILE_NT, h);
+      for (int t = 0; t < NT; ++t) tile_stage(A, *L, v0, v1, t, NT, h);
       const bool wrap = nc[0] < 4 || nc[1] < 4 || nc[2] < 4;
-      for (int t = 0; t < TILE_NT; ++t)
-        for (int a = t; a < m; a += TILE_NT)
-          wrap ? tile_scan<true>(A, *L, v0, v1, hs, a, h) : tile_scan<false>(A, *L, v0, v1, hs, a, h);
+      for (int t = 0; t < NT; ++t)
+        for (int a = t; a < m; a += NT)
+          wrap ? tile_scan<true, CAP>(A, *L, v0, v1, hs, a, h)
+               : tile_scan<false, CAP>(A, *L, v0, v1, hs, a, h);
       v0 = v1;
     }
-    for (int t = 0; t < TILE_NT; ++t)
-      for (int a = t; a < m; a += TILE_NT) {
+    for (int t = 0; t < NT; ++t)
+      for (int a = t; a < m; a += NT) {
         CHECK(hs + a < n);
         if (tile_finish(A, *L, hs, a) && err == 0) err = SIMOX_ERR_NB_OVERFLOW;
       }
