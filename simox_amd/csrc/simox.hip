@@ -154,7 +154,7 @@ struct simox_ctx {
   PmeGrid pi;
   bool inner_coarse = false;
   // launch shapes of the inner solve that the autotuner sets (tune_inner)
-  int mv_variant = 1;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads)
+  int mv_variant = 1;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads); 4-7 the same with XCD-ordered row blocks
   bool inner_tuned = false;
   bool mv_fixed = false, gather_fixed = false, xynt_fixed = false, nl_fixed = false;
   // line-search rounds per inner iteration (multi-block path): adapted to the
@@ -433,6 +433,8 @@ void set_ctip64(simox_ctx* ctx) {
           ctx->htab.pc[pidx(a, b)].ctip64)
         ctx->ctip64_on = true;
 }
+
+constexpr int NMV_VARIANTS = 8;  // k_h0matvec_sell4 shapes the autotuner times
 
 int ensure_part(simox_ctx* ctx, size_t count) {
   if (count <= ctx->part_cap) return 0;
@@ -1733,6 +1735,10 @@ void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
     const auto kv = v == 0   ? &k_h0matvec_sell4<4, false>
                     : v == 2 ? &k_h0matvec_sell4<8, false>
                     : v == 3 ? &k_h0matvec_sell4<8, true>
+                    : v == 4 ? &k_h0matvec_sell4<4, false, true>
+                    : v == 5 ? &k_h0matvec_sell4<4, true, true>
+                    : v == 6 ? &k_h0matvec_sell4<8, false, true>
+                    : v == 7 ? &k_h0matvec_sell4<8, true, true>
                              : &k_h0matvec_sell4<4, true>;
     hipExtLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
                        ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi, ctx->pk_slen,
@@ -1996,7 +2002,7 @@ int tune_inner(simox_ctx* ctx) {
   float best = 1e30f, ms = 0.f;
   if (!ctx->mv_fixed && ctx->val16_valid && ctx->pk_valid) {
     int pick = ctx->mv_variant;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NMV_VARIANTS; ++v) {
       ctx->mv_variant = v;
       CHK(time_launch(ctx, [&] { launch_matvec(ctx, true); }, &ms));
       if (ms < best) { best = ms; pick = v; }
@@ -2473,7 +2479,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_MV_VARIANT")) {
     const int v = std::atoi(e);
-    if (v >= 0 && v < 4) { ctx->mv_variant = v; ctx->mv_fixed = true; }
+    if (v >= 0 && v < NMV_VARIANTS) { ctx->mv_variant = v; ctx->mv_fixed = true; }
   }
   ctx->gather_fixed = std::getenv("SIMOX_GATHER_SHAPE") != nullptr;
   ctx->xynt_fixed = std::getenv("SIMOX_FFT_XYNT") != nullptr;

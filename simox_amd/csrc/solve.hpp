@@ -902,8 +902,11 @@ __device__ inline u32x4 pk_group(const uint4* __restrict__ a) {
 }
 
 // k_h0matvec16 over the SELL-4 packed matrix: each lane streams its row as
-// 16-byte groups of 4 entries, U groups (4 U entries) in flight.
-template <int U, bool NT>
+// 16-byte groups of 4 entries, U groups (4 U entries) in flight.  XCD: the
+// row blocks go to the XCDs in contiguous runs (xcd_tile), so that each
+// XCD's L2 holds the part of the gathered vector its rows read; the block
+// partials stay indexed by row block, so every variant sums in the same order.
+template <int U, bool NT, bool XCD = false>
 __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* __restrict__ tb,
                                                           const int* __restrict__ id,
                                                           const double* __restrict__ q,
@@ -917,7 +920,8 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
                                                           double* __restrict__ y,
                                                           double* __restrict__ part) {
   __shared__ double lds[3 * BLOCK / WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rb = XCD ? xcd_tile(blockIdx.x, gridDim.x) : int(blockIdx.x);
+  const int i = rb * blockDim.x + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < g.n) {
     const int s = i / SLICE, lane = i % SLICE;
@@ -954,9 +958,9 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
   }
   block_sum<3>(acc, lds);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = acc[0];
-    part[3 * blockIdx.x + 1] = acc[1];
-    part[3 * blockIdx.x + 2] = acc[2];
+    part[3 * rb] = acc[0];
+    part[3 * rb + 1] = acc[1];
+    part[3 * rb + 2] = acc[2];
   }
 }
 
