@@ -154,7 +154,7 @@ struct simox_ctx {
   PmeGrid pi;
   bool inner_coarse = false;
   // launch shapes of the inner solve that the autotuner sets (tune_inner)
-  int mv_variant = 1;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads); 4-7 the same with XCD-ordered row blocks
+  int mv_variant = 8;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads); 4-7 the same with XCD-ordered row blocks; 8/9 4/8 groups streaming with the LDS halo
   bool inner_tuned = false;
   bool mv_fixed = false, gather_fixed = false, xynt_fixed = false, nl_fixed = false;
   // line-search rounds per inner iteration (multi-block path): adapted to the
@@ -200,6 +200,12 @@ struct simox_ctx {
   // their SELL-4 slices of 64 rows: groups of 4 entries per row, offsets
   int *pk_slen = nullptr, *pk_soff = nullptr, *pk_bsum = nullptr, *pk_total = nullptr;
   bool inner_pack = true, pk_valid = false;
+  // the LDS halo of each 256-row block (k_pk_halo): entries with LDS slots,
+  // runs of staged atoms, staged-atom and run counts per block
+  unsigned* nb_pkh = nullptr;
+  int2* pk_runs = nullptr;
+  int* pk_hcount = nullptr;
+  bool pkh_valid = false;
   double inner_drop = 1e-3;  // entries with |H0_ij| below this (eV/e^2) left out
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
@@ -434,7 +440,7 @@ void set_ctip64(simox_ctx* ctx) {
         ctx->ctip64_on = true;
 }
 
-constexpr int NMV_VARIANTS = 8;  // k_h0matvec_sell4 shapes the autotuner times
+constexpr int NMV_VARIANTS = 10;  // k_h0matvec_sell4 shapes the autotuner times
 
 int ensure_part(simox_ctx* ctx, size_t count) {
   if (count <= ctx->part_cap) return 0;
@@ -1337,6 +1343,20 @@ int build_matrix(simox_ctx* ctx) {
                          ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->pk_slen,
                          ctx->pk_soff, ctx->nb_pk);
       ctx->pk_valid = true;
+      // the halo copy where the halo variants may run: before the autotuner
+      // has timed them, or when it picked one
+      ctx->pkh_valid = false;
+      if (!ctx->inner_tuned || ctx->mv_variant >= 8) {
+        if (!ctx->nb_pkh) {
+          CHK(dalloc(ctx, &ctx->nb_pkh, size_t(ctx->cap + 4) * (size_t(nsl) * SLICE)));
+          CHK(dalloc(ctx, &ctx->pk_runs, size_t(nb) * HALO_RUNS));
+          CHK(dalloc(ctx, &ctx->pk_hcount, size_t(nb)));
+        }
+        hipLaunchKernelGGL(k_pk_halo, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->pk_slen,
+                           ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk),
+                           reinterpret_cast<uint4*>(ctx->nb_pkh), ctx->pk_runs, ctx->pk_hcount);
+        ctx->pkh_valid = true;
+      }
     } else {
       if (!ctx->nb_val16) {
         CHK(dalloc(ctx, &ctx->nb_val16, size_t(ctx->cap) * ctx->n));
@@ -1727,11 +1747,24 @@ void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
   const bool f32 = use32(ctx);
   if (coarse && ctx->val16_valid && ctx->pk_valid) {
     // groups in flight per lane and streaming loads (mv_variant: 1 = 4 groups
-    // non-temporal, the default; 0 = 4 groups, 2 = 8, 3 = 8 non-temporal; the
-    // autotuner picks one, SIMOX_MV_VARIANT fixes it).  At the 1M-atom slab
-    // the non-temporal stream is 4 % faster and the whole step 4 % (the other
-    // kernels keep their L2 lines).  All four sum in the same order.
+    // non-temporal; 0 = 4 groups, 2 = 8, 3 = 8 non-temporal; 4-7 the same
+    // with XCD-ordered row blocks; 8 and 9 = 4 and 8 groups non-temporal with
+    // the gathers from the LDS halo, 8 the default; the autotuner picks one,
+    // SIMOX_MV_VARIANT fixes it).  At the 1M-atom slab the non-temporal
+    // stream is 4 % faster than 0 and the whole step 4 % (the other kernels
+    // keep their L2 lines), the halo 1.3x faster than 1.  All sum in the same
+    // order.
     const int v = ctx->mv_variant;
+    if (v >= 8 && ctx->pkh_valid) {
+      // the LDS-halo variants: gathers from the staged halo, the same sums
+      const auto kh = v == 8 ? &k_h0matvec_sell4h<4, true> : &k_h0matvec_sell4h<8, true>;
+      hipExtLaunchKernelGGL(kh, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
+                            ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
+                            ctx->pk_slen, ctx->pk_soff,
+                            reinterpret_cast<const uint4*>(ctx->nb_pkh), ctx->pk_runs,
+                            ctx->pk_hcount, ctx->apv, ctx->part);
+      return;
+    }
     const auto kv = v == 0   ? &k_h0matvec_sell4<4, false>
                     : v == 2 ? &k_h0matvec_sell4<8, false>
                     : v == 3 ? &k_h0matvec_sell4<8, true>
@@ -2596,7 +2629,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->nb_fwd, ctx->nq, ctx->nq_n, ctx->facc, ctx->eacc, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->nb_pkh, ctx->pk_runs, ctx->pk_hcount, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->red_ctr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

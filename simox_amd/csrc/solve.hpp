@@ -964,6 +964,254 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
   }
 }
 
+// Halo of a row block of the packed matrix: the direction entries its 256
+// rows read, staged in LDS by k_h0matvec_sell4h so that the matvec's gathers
+// (about 60 per row at the 1M-atom slab) read LDS instead of global memory.
+// At that slab every block has one, 1,638 atoms in 80 runs on average, and
+// the matvec takes 57.8 us instead of 76.5 (4,096 slots of 16 KB: a 32 KB
+// halo halves the workgroups per CU and is slower; gaps up to 32 atoms give
+// 2,691 atoms in 10 runs, 59.4 us).
+// Each workgroup of k_pk_halo marks the columns of its rows in an LDS bitmap
+// over HALO_SPAN atoms from the smallest, fills every gap of fewer than
+// HALO_GAP unmarked atoms between two marked ones (so that the staged atoms
+// form few runs), ranks the marked atoms (their LDS slots), and writes
+//   runs[b][r]  = (first atom, first slot) of run r, in slot order;
+//   hcount[b]   = (staged atoms << 10) | runs, or 0 when the block's columns
+//                 span HALO_SPAN or more (the blocks at the periodic z
+//                 boundary) or its halo exceeds HALO_MAX atoms or HALO_RUNS runs;
+//   pkh         = the block's entries with the LDS slot in the column field
+//                 (the global column where hcount is 0).
+constexpr int HALO_SPAN = 1 << 17;  // bitmap of 16 KB
+constexpr int HALO_MAX = 4096;      // staged atoms (16 KB of LDS)
+constexpr int HALO_RUNS = 256;
+constexpr int HALO_GAP = 8;         // hi - lo <= HALO_GAP for marked lo < gap < hi
+
+// exclusive block scan of two counters (BLOCK threads), with the totals
+__device__ inline void block_scan2(int a, int b, int* red, int& ea, int& eb, int& ta, int& tb) {
+  const int t = threadIdx.x, l = t & (WAVE - 1);
+  int ia = a, ib = b;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int va = __shfl_up(ia, o, WAVE), vb = __shfl_up(ib, o, WAVE);
+    if (l >= o) { ia += va; ib += vb; }
+  }
+  if (l == WAVE - 1) { red[2 * (t / WAVE)] = ia; red[2 * (t / WAVE) + 1] = ib; }
+  __syncthreads();
+  int ba = 0, bb = 0;
+  ta = 0; tb = 0;
+  for (int w = 0; w < BLOCK / WAVE; ++w) {
+    if (w < t / WAVE) { ba += red[2 * w]; bb += red[2 * w + 1]; }
+    ta += red[2 * w]; tb += red[2 * w + 1];
+  }
+  ea = ba + ia - a;
+  eb = bb + ib - b;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict__ slen,
+                                                   const int* __restrict__ soff,
+                                                   const uint4* __restrict__ pk,
+                                                   uint4* __restrict__ pkh, int2* __restrict__ runs,
+                                                   int* __restrict__ hcount) {
+  constexpr int NW = HALO_SPAN / 32;  // bitmap words
+  constexpr int WPT = NW / BLOCK;     // consecutive words per thread
+  __shared__ unsigned bits[NW];
+  __shared__ int woff[NW];            // slot of the word's first marked atom
+  __shared__ int red[2 * (BLOCK / WAVE)];
+  __shared__ int cmin_s, cmax_s;
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int i = b * BLOCK + t;
+  const int sl = i / SLICE, lane = i % SLICE;
+  const int L = i < n ? slen[sl] : 0;
+  const size_t r0 = (i < n ? size_t(soff[sl]) : 0) + lane;
+  // the block's column range (padding entries hold the row's own index)
+  int mn = 0x7fffffff, mx = -1;
+  for (int k = 0; k < L; ++k) {
+    const uint4 w = pk[r0 + size_t(k) * SLICE];
+    mn = min(mn, min(min(pk_col(w.x), pk_col(w.y)), min(pk_col(w.z), pk_col(w.w))));
+    mx = max(mx, max(max(pk_col(w.x), pk_col(w.y)), max(pk_col(w.z), pk_col(w.w))));
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_xor(mn, o, WAVE));
+    mx = max(mx, __shfl_xor(mx, o, WAVE));
+  }
+  if (t == 0) { cmin_s = 0x7fffffff; cmax_s = -1; }
+#pragma unroll
+  for (int k = 0; k < WPT; ++k) bits[t * WPT + k] = 0u;
+  __syncthreads();
+  if ((t & (WAVE - 1)) == 0) { atomicMin(&cmin_s, mn); atomicMax(&cmax_s, mx); }
+  __syncthreads();
+  const int c0 = cmin_s;
+  const bool span_ok = cmax_s >= 0 && cmax_s - c0 < HALO_SPAN;
+  if (span_ok)
+    for (int k = 0; k < L; ++k) {
+      const uint4 w = pk[r0 + size_t(k) * SLICE];
+      const int c[4] = {pk_col(w.x) - c0, pk_col(w.y) - c0, pk_col(w.z) - c0, pk_col(w.w) - c0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) atomicOr(&bits[c[u] >> 5], 1u << (c[u] & 31));
+    }
+  __syncthreads();
+  // gap filling from the marked bitmap: an unmarked atom a is filled when the
+  // nearest marked atoms lo < a < hi lie within HALO_GAP of each other (both
+  // then lie in the words either side of a's)
+  unsigned f[WPT];
+#pragma unroll
+  for (int k = 0; k < WPT; ++k) {
+    const int w = t * WPT + k;
+    const unsigned m = bits[w];
+    f[k] = m;
+    if (!span_ok || m == 0u || m == ~0u) continue;  // an empty word is a gap of 32 or more
+    const unsigned long long lw = (static_cast<unsigned long long>(m) << 32) | (w > 0 ? bits[w - 1] : 0u);
+    const unsigned long long hw =
+        (static_cast<unsigned long long>(w + 1 < NW ? bits[w + 1] : 0u) << 32) | m;
+    unsigned holes = ~m;
+    while (holes) {
+      const int bit = __ffs(holes) - 1;
+      holes &= holes - 1u;
+      const unsigned below = unsigned(lw >> bit);       // bit j: atom a - 32 + j
+      const unsigned above = unsigned(hw >> (bit + 1));  // bit j: atom a + 1 + j
+      if (below && above && (__clz(below) + 1) + __ffs(above) <= HALO_GAP) f[k] |= 1u << bit;
+    }
+  }
+  __syncthreads();
+  int cs = 0, cr = 0;
+#pragma unroll
+  for (int k = 0; k < WPT; ++k) {
+    bits[t * WPT + k] = f[k];
+    cs += __popc(f[k]);
+  }
+  __syncthreads();
+  // run starts: a marked atom whose predecessor is unmarked
+  unsigned st[WPT];
+#pragma unroll
+  for (int k = 0; k < WPT; ++k) {
+    const int w = t * WPT + k;
+    const unsigned prev = w > 0 ? bits[w - 1] >> 31 : 0u;
+    st[k] = f[k] & ~((f[k] << 1) | prev);
+    cr += __popc(st[k]);
+  }
+  int es, er, ts, tr;
+  block_scan2(cs, cr, red, es, er, ts, tr);
+  const bool ok = span_ok && ts <= HALO_MAX && tr <= HALO_RUNS;
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+      const int w = t * WPT + k;
+      woff[w] = es;
+      unsigned s = st[k];
+      while (s) {
+        const int bit = __ffs(s) - 1;
+        s &= s - 1u;
+        runs[size_t(b) * HALO_RUNS + er] =
+            make_int2(c0 + w * 32 + bit, es + __popc(f[k] & ((1u << bit) - 1u)));
+        ++er;
+      }
+      es += __popc(f[k]);
+    }
+  }
+  if (t == 0) hcount[b] = ok ? (ts << 10) | tr : 0;
+  __syncthreads();
+  for (int k = 0; k < L; ++k) {
+    uint4 w = pk[r0 + size_t(k) * SLICE];
+    if (ok) {
+      unsigned* e = reinterpret_cast<unsigned*>(&w);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int d = pk_col(e[u]) - c0;
+        const int slot = woff[d >> 5] + __popc(bits[d >> 5] & ((1u << (d & 31)) - 1u));
+        e[u] = (e[u] & ~((1u << PK_COL_BITS) - 1u)) | unsigned(slot);
+      }
+    }
+    pkh[r0 + size_t(k) * SLICE] = w;
+  }
+}
+
+// k_h0matvec_sell4 over the halo-slot copy of k_pk_halo: a block with a halo
+// stages its runs of pt in LDS and gathers from there; the others gather from
+// global memory as k_h0matvec_sell4 does.  Same products in the same order.
+template <int U, bool NT>
+__global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables* __restrict__ tb,
+                                                           const int* __restrict__ id,
+                                                           const double* __restrict__ q,
+                                                           const double* __restrict__ p,
+                                                           const float* __restrict__ pt,
+                                                           const double* __restrict__ gr,
+                                                           const double* __restrict__ phi_in,
+                                                           const int* __restrict__ slen,
+                                                           const int* __restrict__ soff,
+                                                           const uint4* __restrict__ pk,
+                                                           const int2* __restrict__ runs,
+                                                           const int* __restrict__ hcount,
+                                                           double* __restrict__ y,
+                                                           double* __restrict__ part) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  __shared__ float hv[HALO_MAX];
+  __shared__ int2 rs[HALO_RUNS];
+  const int b = blockIdx.x;
+  const int hc = hcount[b];
+  const int nh = hc >> 10, nr = hc & 1023;
+  if (nh > 0) {
+    // runs in slot order; slot s lies in the last run starting at or before it
+    if (threadIdx.x < nr) rs[threadIdx.x] = runs[size_t(b) * HALO_RUNS + threadIdx.x];
+    __syncthreads();
+    for (int s = threadIdx.x; s < nh; s += BLOCK) {
+      int lo = 0;
+#pragma unroll
+      for (int step = HALO_RUNS / 2; step > 0; step >>= 1)
+        if (lo + step < nr && rs[lo + step].y <= s) lo += step;
+      hv[s] = pt[rs[lo].x + (s - rs[lo].y)];
+    }
+    __syncthreads();
+  }
+  const int i = b * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < g.n) {
+    const int s = i / SLICE, lane = i % SLICE;
+    const int L = slen[s];
+    const uint4* row = pk + size_t(soff[s]) + lane;
+    float sum = 0.f;
+    auto rowsum = [&](auto gat) {
+      int k = 0;
+      for (; k + U <= L; k += U) {
+        u32x4 w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = pk_group<NT>(row + size_t(k + u) * SLICE);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          sum += pk_value(w[u].x) * gat(pk_col(w[u].x));
+          sum += pk_value(w[u].y) * gat(pk_col(w[u].y));
+          sum += pk_value(w[u].z) * gat(pk_col(w[u].z));
+          sum += pk_value(w[u].w) * gat(pk_col(w[u].w));
+        }
+      }
+      for (; k < L; ++k) {
+        const u32x4 w = pk_group<NT>(row + size_t(k) * SLICE);
+        sum += pk_value(w.x) * gat(pk_col(w.x));
+        sum += pk_value(w.y) * gat(pk_col(w.y));
+        sum += pk_value(w.z) * gat(pk_col(w.z));
+        sum += pk_value(w.w) * gat(pk_col(w.w));
+      }
+    };
+    if (nh > 0)
+      rowsum([&](int c) { return hv[c]; });
+    else
+      rowsum([&](int c) { return pt[c]; });
+    const double pi = p[i];
+    const SpecCoef& sc = tb->sc[id[i] - 1];
+    const double yi = sc.Jp * pi + phi_in[i] + double(sum);
+    y[i] = yi;
+    acc[0] = pi * yi;
+    acc[1] = pi * gr[i];
+    acc[2] = outside(sc, q[i]) ? pi * pi * 2.0 * WPEN : 0.0;
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[3 * b] = acc[0];
+    part[3 * b + 1] = acc[1];
+    part[3 * b + 2] = acc[2];
+  }
+}
+
 // charge predictor: out = c0 q + c1 h0 + c2 h1 + c3 h2 with the first ord+1
 // terms (polynomial extrapolation in time; the coefficients sum to 1, which
 // preserves the total charge)
