@@ -82,8 +82,20 @@ def kernel_name(w, mode, s=None):
     if w == 10 and s is not None and coarse_inner(s, mode):
         return "k_gather_tile_rec"
     if w == 11 and s is not None and coarse_inner(s, mode):
-        return "k_spread_tile_i32 (inner grid)"
+        return ("k_spread_own + k_spread_fold (inner grid)" if owner_spread(s)
+                else "k_spread_tile_i32 (inner grid)")
     return KERNELS[w]
+
+
+def owner_spread(s):
+    """The coarse inner grid's 32-bit spread runs owner-computes with its fold
+    where its tile (32 x 32 x 16; 16 x 16 x 8 with more atoms than grid
+    points) divides the grid (SIMOX_SPREAD_OWN=0 off)."""
+    if os.environ.get("SIMOX_SPREAD_OWN", "1") == "0" or os.environ.get("SIMOX_SPREAD_BIG") == "0":
+        return False
+    k = [f // 2 for f in s.fft]
+    t = (16, 16, 8) if s.n > k[0] * k[1] * k[2] else (32, 32, 16)
+    return all(a % b == 0 for a, b in zip(k, t))
 
 
 def measured_traffic(kernel, cfg, mode):

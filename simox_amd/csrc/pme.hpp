@@ -1616,6 +1616,256 @@ __global__ void __launch_bounds__(NT) k_spread_tile_i32(Geom g, const double* __
   }
 }
 
+// Owner-computes 32-bit spread of the MIXED coarse grids (tiles that divide
+// the grid).  A workgroup takes only the atoms whose base point lies in its
+// tile -- its 4^3 bricks, in rows of TXg/4 bricks along x that are contiguous
+// in brick order -- and adds all ORD^3 stencil points of each into an LDS
+// region extended by the stencil's reach (PL points below the tile, PH
+// above), so each atom is processed once (the tiled spread above processes
+// every atom whose stencil meets a tile, 2.3 times each for 32 x 32 x 16
+// tiles) and no point needs a range test or a periodic wrap.  Sums are 32-bit
+// fixed point at the tile's own scale, from the bound of k_spread_tile_i32
+// over the owned bricks.  Points that no other tile reaches (the interior)
+// are final and go to the grid; the others (within PH of the tile's lower
+// faces or PL of its upper ones, and the shell outside the tile) go to the
+// tile's slot of ext, and k_spread_fold sums them over the tiles that reach
+// each point in a fixed order, so the grid is deterministic.
+template <int ORD, int TXg, int TYg, int TZg>
+struct SpreadOwn {
+  static constexpr int PL = ORD == 4 ? 1 : 0, PH = ORD == 4 ? 2 : 1;
+  static constexpr int EX = TXg + PL + PH, EY = TYg + PL + PH, EZ = TZg + PL + PH;
+  static constexpr int EXT = EX * EY * EZ;
+  static constexpr int NBX = TXg / BRK, NBY = TYg / BRK, ROWS = NBY * (TZg / BRK);
+  static constexpr int QCAP = 8192;  // owned atoms whose FP32 charges stay in LDS
+  static_assert(TXg % BRK == 0 && TYg % BRK == 0 && TZg % BRK == 0, "whole bricks");
+  static bool fits(const Geom& g) {
+    return g.K[0] % TXg == 0 && g.K[1] % TYg == 0 && g.K[2] % TZg == 0;
+  }
+  static int tiles(const Geom& g) { return (g.K[0] / TXg) * (g.K[1] / TYg) * (g.K[2] / TZg); }
+  // point r (per axis, relative to the tile's first) receives no other tile's atoms
+  __host__ __device__ static bool inner(int r, int T) { return r >= PH && r < T - PL; }
+};
+
+template <int ORD, int TXg, int TYg, int TZg, int NT>
+__global__ void __launch_bounds__(NT) k_spread_own(Geom g, const double* __restrict__ q,
+                                                   const int* __restrict__ bstart,
+                                                   const int* __restrict__ border,
+                                                   const float4* __restrict__ rec,
+                                                   float* __restrict__ grid, int* __restrict__ ext,
+                                                   float* __restrict__ tinv) {
+  using P = SpreadOwn<ORD, TXg, TYg, TZg>;
+  constexpr int EX = P::EX, EY = P::EY, EXT = P::EXT, ROWS = P::ROWS, QCAP = P::QCAP;
+  static_assert(ROWS <= NT && ROWS <= WAVE, "one lane per row of bricks");
+  __shared__ int acc[EXT];
+  __shared__ float qf[QCAP];
+  __shared__ int rofs[ROWS + 1];
+  __shared__ int rbeg[ROWS];
+  __shared__ double wmax[NT / WAVE];
+  __shared__ int rcnt;
+  const int ntx = g.K[0] / TXg, nty = g.K[1] / TYg;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int lo[3] = {tx * TXg, ty * TYg, tz * TZg};
+  for (int k = threadIdx.x; k < EXT; k += NT) acc[k] = 0;
+  // the owned rows of bricks (the first wave): offsets and the fullest brick
+  if (threadIdx.x < WAVE) {
+    const int r = threadIdx.x;
+    int cnt = 0, cmax = 0;
+    if (r < ROWS) {
+      const int nb0 = g.K[0] / BRK, nb1 = g.K[1] / BRK;
+      const int by = lo[1] / BRK + r % P::NBY, bz = lo[2] / BRK + r / P::NBY;
+      const int id0 = lo[0] / BRK + nb0 * (by + nb1 * bz);
+      const int b0 = bstart[id0];
+      int prev = b0;
+      for (int j = 1; j <= P::NBX; ++j) {
+        const int bj = bstart[id0 + j];
+        cmax = max(cmax, bj - prev);
+        prev = bj;
+      }
+      rbeg[r] = b0;
+      cnt = prev - b0;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int v = __shfl_up(incl, o, WAVE);
+      if (r >= o) incl += v;
+      cmax = max(cmax, __shfl_xor(cmax, o, WAVE));
+    }
+    if (r < ROWS) rofs[r + 1] = incl;
+    if (r == 0) { rofs[0] = 0; rcnt = cmax; }
+  }
+  __syncthreads();
+  const int total = rofs[ROWS];
+  const bool stage = total <= QCAP;
+  auto slot_of = [&](int f) {
+    int l = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1)
+      if (s < ROWS && l + s < ROWS && rofs[l + s] <= f) l += s;
+    return rbeg[l] + (f - rofs[l]);
+  };
+  // the tile's scale from the largest |q| it owns and its fullest brick
+  double m = 0.0;
+  for (int f0 = threadIdx.x; f0 < total; f0 += 4 * NT) {
+    // four atoms' gathers in flight
+    int at[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * NT;
+      at[u] = f < total ? border[slot_of(f)] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * NT;
+      if (at[u] >= 0) {
+        const double v = q[at[u]];
+        m = fmax(m, fabs(v));
+        if (stage) qf[f] = float(v);
+      }
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & (WAVE - 1)) == 0) wmax[threadIdx.x / WAVE] = m;
+  __syncthreads();
+  m = wmax[0];
+  for (int w = 1; w < NT / WAVE; ++w) m = fmax(m, wmax[w]);
+  constexpr double WMAX = ORD == 4 ? 0.3 : 1.0;  // (2/3)^3 = 0.296, rounded up
+  const double bound = 8.0 * double(rcnt) * WMAX * m;
+  int e = 0;
+  (void)frexp(bound, &e);                  // bound < 2^e
+  const int S = bound > 0.0 ? 30 - e : 0;  // bound 2^S < 2^30
+  const double scale = ldexp(1.0, S);
+  const float scalef = float(scale);
+  // lanes of a wave take atoms 16 apart (NT / WAVE), in different bricks, so
+  // that their atomics spread over the LDS banks
+  const int tp = int((threadIdx.x % WAVE) * (NT / WAVE) + threadIdx.x / WAVE);
+  for (int f = tp; f < total; f += NT) {
+    const int sl = slot_of(f);
+    const float qs = stage ? qf[f] * scalef : float(q[border[sl]] * scale);
+    float AX[3][ORD];
+    int bm[3];
+    rec_weights<ORD>(rec[sl], AX, bm);
+    const int e0 = bm[0] - lo[0], e1 = bm[1] - lo[1], e2 = bm[2] - lo[2];
+    // the atom's base lies in the tile (its brick's); a stale record is skipped
+    if (unsigned(e0) >= unsigned(TXg) || unsigned(e1) >= unsigned(TYg) ||
+        unsigned(e2) >= unsigned(TZg))
+      continue;
+    int* base = acc + e0 + EX * (e1 + EY * e2);
+#pragma unroll
+    for (int c = 0; c < ORD; ++c)
+#pragma unroll
+      for (int bb = 0; bb < ORD; ++bb) {
+        const float w12 = qs * AX[1][bb] * AX[2][c];
+        int* row = base + EX * (bb + EY * c);
+#pragma unroll
+        for (int a = 0; a < ORD; ++a) atomicAdd(row + a, __float2int_rn(w12 * AX[0][a]));
+      }
+  }
+  __syncthreads();
+  const float inv = float(ldexp(1.0, -S));
+  if (threadIdx.x == 0) tinv[t] = inv;
+  int* mine = ext + size_t(t) * EXT;
+  for (int k = threadIdx.x; k < EXT; k += NT) {
+    const int rx = k % EX - P::PL, ry = (k / EX) % EY - P::PL, rz = k / (EX * EY) - P::PL;
+    if (P::inner(rx, TXg) && P::inner(ry, TYg) && P::inner(rz, TZg))
+      grid[size_t(lo[0] + rx) +
+           size_t(g.K[0]) * (size_t(lo[1] + ry) + size_t(g.K[1]) * size_t(lo[2] + rz))] =
+          float(acc[k]) * inv;
+    else
+      mine[k] = acc[k];
+  }
+}
+
+// The grid points k_spread_own leaves to the fold: each sums the values of
+// the tiles whose extended regions hold it (its own tile, and per axis the
+// tile below when it lies within PH of its tile's lower face, the tile above
+// within PL of the upper one; periodic), each at its tile's scale, in a fixed
+// order (z, y, x options nested, own tile first).  Workgroups take the
+// boundary points of one tile, enumerated without gaps: the planes with a
+// boundary z, then the rows with a boundary y in the inner planes, then the
+// boundary x of the inner rows.
+template <int ORD, int TXg, int TYg, int TZg>
+struct SpreadFold {
+  using P = SpreadOwn<ORD, TXg, TYg, TZg>;
+  static constexpr int NR = P::PL + P::PH;  // boundary positions per axis
+  static constexpr int NIY = TYg - NR, NIZ = TZg - NR;
+  static constexpr int NA = NR * TYg * TXg, NB = NIZ * NR * TXg, NC = NIZ * NIY * NR;
+  static constexpr int NPT = NA + NB + NC;  // boundary points of a tile
+  static constexpr int BPT = (NPT + BLOCK - 1) / BLOCK;  // workgroups per tile
+  static_assert(TXg > NR && TYg > NR && TZg > NR, "one neighbour per axis at most");
+  __device__ static int rmap(int k, int T) { return k < P::PH ? k : T - NR + k; }
+};
+
+template <int ORD, int TXg, int TYg, int TZg>
+__global__ void __launch_bounds__(BLOCK) k_spread_fold(Geom g, const int* __restrict__ ext,
+                                                       const float* __restrict__ tinv,
+                                                       float* __restrict__ grid) {
+  using P = SpreadOwn<ORD, TXg, TYg, TZg>;
+  using F = SpreadFold<ORD, TXg, TYg, TZg>;
+  const int t = blockIdx.x / F::BPT;
+  const int j = (blockIdx.x % F::BPT) * BLOCK + threadIdx.x;
+  if (j >= F::NPT) return;
+  int r[3];
+  if (j < F::NA) {
+    r[0] = j % TXg;
+    r[1] = (j / TXg) % TYg;
+    r[2] = F::rmap(j / (TXg * TYg), TZg);
+  } else if (j < F::NA + F::NB) {
+    const int k = j - F::NA;
+    r[0] = k % TXg;
+    r[1] = F::rmap((k / TXg) % F::NR, TYg);
+    r[2] = P::PH + k / (TXg * F::NR);
+  } else {
+    const int k = j - F::NA - F::NB;
+    r[0] = F::rmap(k % F::NR, TXg);
+    r[1] = P::PH + (k / F::NR) % F::NIY;
+    r[2] = P::PH + k / (F::NR * F::NIY);
+  }
+  const int T[3] = {TXg, TYg, TZg};
+  const int nt[3] = {g.K[0] / TXg, g.K[1] / TYg, g.K[2] / TZg};
+  const int tc[3] = {t % nt[0], (t / nt[0]) % nt[1], t / (nt[0] * nt[1])};
+  // per axis the own tile and at most one neighbour (the tiles hold more
+  // than PL + PH points per axis, so no point is within reach of both)
+  int ot[3][2], oe[3][2];
+  bool two[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ot[d][0] = ot[d][1] = tc[d];
+    oe[d][0] = oe[d][1] = r[d] + P::PL;
+    two[d] = false;
+    if (r[d] < P::PH) {
+      ot[d][1] = tc[d] == 0 ? nt[d] - 1 : tc[d] - 1;
+      oe[d][1] = r[d] + T[d] + P::PL;
+      two[d] = true;
+    } else if (r[d] >= T[d] - P::PL) {
+      ot[d][1] = tc[d] + 1 == nt[d] ? 0 : tc[d] + 1;
+      oe[d][1] = r[d] - T[d] + P::PL;
+      two[d] = true;
+    }
+  }
+  // all loads first (one memory latency, not one per tile), then the sum in
+  // the fixed order
+  float c[8];
+#pragma unroll
+  for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+    for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+      for (int ix = 0; ix < 2; ++ix) {
+        const bool ok = (ix == 0 || two[0]) && (iy == 0 || two[1]) && (iz == 0 || two[2]);
+        const int tt = ot[0][ix] + nt[0] * (ot[1][iy] + nt[1] * ot[2][iz]);
+        const int ee = oe[0][ix] + P::EX * (oe[1][iy] + P::EY * oe[2][iz]);
+        c[ix + 2 * (iy + 2 * iz)] = ok ? float(ext[size_t(tt) * P::EXT + ee]) * tinv[tt] : 0.f;
+      }
+  float v = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v += c[k];
+  grid[size_t(tc[0] * TXg + r[0]) +
+       size_t(g.K[0]) * (size_t(tc[1] * TYg + r[1]) + size_t(g.K[1]) * size_t(tc[2] * TZg + r[2]))] =
+      v;
+}
+
 // K9 (tiled): out_i = scale * sum w Qfin over the order-4 stencil of atom i
 // (the gather of PHI_MESH, ctip.f90:381-393).  One workgroup per tile of
 // GX x GY x GZ grid points: the tile and its stencil halo (one point below,

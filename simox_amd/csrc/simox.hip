@@ -107,6 +107,12 @@ struct PmeGrid {
   bool spread32 = false;  // FP32 spreads in 32-bit fixed point (the MIXED inner grid)
   float4* brec = nullptr;  // its stencil records in brick order (k_brick_records)
   float4* brec_atom = nullptr;  // the same records in atom order
+  // owner-computes spread (k_spread_own): each tile's extended-region values
+  // that the fold sums, and each tile's 2^-S
+  int* sp_ext = nullptr;
+  float* sp_tinv = nullptr;
+  size_t sp_ext_n = 0;
+  int sp_tiles = 0;
   int gshape = 0;  // tile shape of the record gather (GATHER_SHAPES), autotuned
 };
 constexpr int NGATHER_SHAPES = 6;
@@ -164,6 +170,7 @@ struct simox_ctx {
   bool inner_f16 = true;
   bool prec_spread32 = true;   // the preconditioner grid's FP32 spread in 32-bit fixed point
   bool spread_big = true;      // 32 x 32 x 16 tiles for the 32-bit spreads
+  bool spread_own = true;      // owner-computes 32-bit spreads with a fold (k_spread_own)
   int prec_div = 2;            // the preconditioner grid: the context's grid / prec_div per axis
   int inner_div = 2;           // the coarse inner operator's grid: the context's grid / inner_div
   bool inner_spread32 = true;  // the coarse grid's spread in 32-bit fixed point  // with the coarse grid: FP16 matrix values in the inner matvec
@@ -1049,13 +1056,50 @@ int ensure_bricks(simox_ctx* ctx, PmeGrid& pm) {
   return 0;
 }
 
+// Owner-computes spread of tile TXg x TYg x TZg, then the fold of the points
+// several tiles reach.
+template <int ORD, int TXg, int TYg, int TZg, int NT>
+int spread_own(simox_ctx* ctx, const Geom& g, PmeGrid& pm, const double* src) {
+  using P = SpreadOwn<ORD, TXg, TYg, TZg>;
+  const int nt = P::tiles(g);
+  const size_t need = size_t(nt) * P::EXT;
+  if (pm.sp_ext_n < need || pm.sp_tiles < nt) {
+    if (pm.sp_ext) (void)hipFree(pm.sp_ext);
+    if (pm.sp_tinv) (void)hipFree(pm.sp_tinv);
+    pm.sp_ext = nullptr;
+    pm.sp_tinv = nullptr;
+    pm.sp_ext_n = 0;
+    pm.sp_tiles = 0;
+    CHK(dalloc(ctx, &pm.sp_ext, need));
+    CHK(dalloc(ctx, &pm.sp_tinv, size_t(nt)));
+    pm.sp_ext_n = need;
+    pm.sp_tiles = nt;
+  }
+  hipLaunchKernelGGL((k_spread_own<ORD, TXg, TYg, TZg, NT>), dim3(nt), dim3(NT), 0, ctx->st, g,
+                     src, pm.bstart, pm.border, pm.brec, pm.greal32, pm.sp_ext, pm.sp_tinv);
+  hipLaunchKernelGGL((k_spread_fold<ORD, TXg, TYg, TZg>),
+                     dim3(nt * SpreadFold<ORD, TXg, TYg, TZg>::BPT), dim3(BLOCK), 0, ctx->st, g,
+                     pm.sp_ext, pm.sp_tinv, pm.greal32);
+  return 0;
+}
+
 // The 32-bit spread of a coarse FP32 grid: 32 x 32 x 16 tiles with 1024
 // threads where the grid holds them, 16 x 16 x 8 tiles with 1024 threads on
 // grids with more atoms than points (SIMOX_SPREAD_BIG=0: 32 x 16 x 8 tiles
-// with 256 threads).
+// with 256 threads); owner-computes with the fold where the tile divides the
+// grid (SIMOX_SPREAD_OWN=0: the tiled spread over the candidates of each tile).
 template <int ORD>
 void spread_i32(simox_ctx* ctx, const Geom& g, PmeGrid& pm, const double* src) {
   const double ng = double(g.K[0]) * g.K[1] * g.K[2];
+  if (ctx->spread_own && ctx->spread_big) {
+    if (double(ctx->n) > ng && SpreadOwn<ORD, 16, 16, 8>::fits(g)) {
+      if (spread_own<ORD, 16, 16, 8, 1024>(ctx, g, pm, src)) ctx->spread_own = false;
+      else return;
+    } else if (double(ctx->n) <= ng && SpreadOwn<ORD, 32, 32, 16>::fits(g)) {
+      if (spread_own<ORD, 32, 32, 16, 1024>(ctx, g, pm, src)) ctx->spread_own = false;
+      else return;
+    }
+  }
   if (ctx->spread_big && double(ctx->n) > ng && g.K[0] >= 16 && g.K[1] >= 16 && g.K[2] >= 8) {
     using P = SpreadI32<ORD, 16, 16, 8, 1024>;
     hipLaunchKernelGGL((k_spread_tile_i32<ORD, 16, 16, 8, 1024>), dim3(P::tiles(g)), dim3(1024),
@@ -1245,7 +1289,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
 void free_grid(PmeGrid& pm) {
   void* bufs[] = {pm.brick, pm.bunord, pm.border, pm.bcount, pm.bfill, pm.bstart, pm.bsum,
                   pm.btotal, pm.greal64, pm.greal32, pm.spec64, pm.spec32, pm.G64, pm.G32,
-                  pm.F64, pm.F32, pm.brec, pm.brec_atom, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
+                  pm.F64, pm.F32, pm.brec, pm.brec_atom, pm.sp_ext, pm.sp_tinv, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
                   pm.tw32[2]};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -2521,6 +2565,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   if (const char* e = std::getenv("SIMOX_INNER_SPREAD32")) ctx->inner_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_PREC_SPREAD32")) ctx->prec_spread32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("SIMOX_SPREAD_BIG")) ctx->spread_big = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_SPREAD_OWN")) ctx->spread_own = std::atoi(e) != 0;
   // the preconditioner's grid: a quarter of the resolution from 2^23 points
   // (C3, 256 x 256 x 512: same iteration count as half, 143 -> 135 ms per
   // step), half below (C2, 96 x 96 x 192: a quarter needs more iterations)

@@ -339,6 +339,44 @@ def test_mixed_coarse_inner_grid_matches_fp64(monkeypatch, pack):
             g.close()
 
 
+def test_owner_spread_matches_tiled_spread(monkeypatch):
+    """The owner-computes 32-bit spread with its fold (k_spread_own,
+    k_spread_fold), taken on both coarse grids when their tiles divide them
+    (here 64 x 64 x 32 each: inner operator and preconditioner), against the
+    tiled spread over each tile's candidates (SIMOX_SPREAD_OWN=0): both solves
+    converge to FP64's charges within 1e-9 and forces within 1e-5 of the RMS
+    force, and the owner path gives the same bits when repeated (integer sums
+    per tile, the fold in a fixed order)."""
+    from simox_amd.systems import al_o_slab
+    s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (128, 128, 64), 10.0, vacuum=20.0,
+                  o_layer=(2.0, 10.0), o_dmin=2.0, name="own")
+    print("owner spread, n", s.n, flush=True)
+
+    def run(mode, own):
+        monkeypatch.setenv("SIMOX_SPREAD_OWN", "1" if own else "0")
+        g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
+        if mode == MODE_MIXED:
+            g.set_inner_grid(True)
+        g.set_charge_tol(1e-10, 5000)
+        g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
+        r = g.ctip_solve()
+        g.force()
+        g.pme()
+        _, _, f, q = g.get_state()
+        g.close()
+        return q, f, r["iters"]
+
+    q64, f64, _ = run(MODE_FP64, True)
+    q1, f1, it1 = run(MODE_MIXED, True)
+    q0, f0, it0 = run(MODE_MIXED, False)
+    q2, f2, _ = run(MODE_MIXED, True)
+    print("iterations owner", it1, "tiled", it0, flush=True)
+    for q, f in ((q1, f1), (q0, f0)):
+        assert np.abs(q - q64).max() <= 1e-9
+        assert e_F(f, f64) <= 1e-5
+    assert np.array_equal(q1, q2) and np.array_equal(f1, f2)
+
+
 def test_inner_shapes_change_no_result(monkeypatch):
     """The inner solve's tuned launch shapes (simox_inner_shapes: packed
     matvec variant, record gather tile, plane FFT threads and z lines of both
