@@ -1844,6 +1844,7 @@ bool fused_tail(const simox_ctx* ctx) { return !ctx->small_path && ctx->prec_on;
 int ncg_iter_body(simox_ctx* ctx) {
   const int n = ctx->n, nb = grid_blocks(n);
   const int nbr = std::min(nb, RED_GRID);  // the fused-reduction kernels' grid
+  const int nbw = std::min(grid_blocks(n, RED_NT), RED_GRID);  // the same, RED_NT threads each (the step)
   const bool f32 = use32(ctx);
   float* p32 = f32 ? ctx->pv32 : nullptr;
   double* ls = ctx->dscal + 40;
@@ -1859,6 +1860,9 @@ int ncg_iter_body(simox_ctx* ctx) {
     tkernel(ctx, 6, &ka, &kb);
     launch_matvec(ctx, ig != nullptr, ka, kb);
   }
+  // (the matvec's 4,096 workgroups summing in their last one, as the
+  // grid-stride kernels below do, took 88 us instead of 60 at the 1M-atom
+  // slab: they serialise on the completion counter)
   hipLaunchKernelGGL(k_reduce_op<1>, dim3(1), dim3(1024), 0, ctx->st, ctx->part, nb, 3,
                      ctx->dscal + 22, ls);
   // exact line search, Newton with bracketing, on the device: one workgroup
@@ -1912,7 +1916,7 @@ int ncg_iter_body(simox_ctx* ctx) {
                          ctx->qc, ctx->pv, ls, ctx->part,
                          RedTail{ctx->red_ctr, ctx->dscal + 24, ls, 2});
   }
-  hipLaunchKernelGGL(k_ncg_step_dev, dim3(nbr), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
+  hipLaunchKernelGGL(k_ncg_step_dev, dim3(nbw), dim3(RED_NT), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part,
                      RedTail{ctx->red_ctr, ctx->dscal + 0, nullptr, 0});
   if (fused_tail(ctx)) {

@@ -28,6 +28,11 @@ __device__ inline void reduce_scalar(const double* out, double* ls);
 // workgroups, out receives the row sums, op the scalar step of k_reduce_op
 // (0 none, 2 ls_update, 4 projected beta); ctr == null: rows only.
 constexpr int RED_GRID = 256;
+// threads per workgroup of k_ncg_step_dev (four times BLOCK: its 256
+// workgroups then keep 16 waves per CU in flight, 14.4 -> 13.7 us at the
+// 1M-atom slab; the line search and the preconditioner's finish are faster
+// with BLOCK)
+constexpr int RED_NT = 1024;
 struct RedTail {
   unsigned* ctr;
   double* out;
@@ -1680,12 +1685,28 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
   if (ls[5] != 0.0) return;  // uniform: the search is done
   const double a = ls[2];
   double acc[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const SpecCoef& sc = tb->sc[id[i] - 1];
-    const double pi = p[i], qi = q[i], qa = qi + a * pi;
+  auto term = [&](int idi, double qi, double pi) {
+    const SpecCoef& sc = tb->sc[idi - 1];
+    const double qa = qi + a * pi;
     acc[0] += pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
     acc[1] += outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+  };
+  // grid-stride, four atoms' loads in flight per trip, summed in atom order
+  const int S = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n; i += 4 * S) {
+    int iv[4];
+    double qv[4], pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      iv[u] = id[i + u * S];
+      qv[u] = q[i + u * S];
+      pv[u] = p[i + u * S];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) term(iv[u], qv[u], pv[u]);
   }
+  for (; i < n; i += S) term(id[i], q[i], p[i]);
   block_sum<2>(acc, lds);
   partials_out<2>(acc, part, rt, lds);
 }
@@ -1963,7 +1984,7 @@ __global__ void k_reduce_if(const double* __restrict__ part, int nparts, int nv,
 }
 
 // q += a p; g_old = g; g -= a v + dpen' with a from device memory; part = sum g
-__global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __restrict__ tb,
+__global__ void __launch_bounds__(RED_NT) k_ncg_step_dev(int n, const Tables* __restrict__ tb,
                                                         const int* __restrict__ id,
                                                         const double* __restrict__ ls,
                                                         const double* __restrict__ p,
@@ -1972,19 +1993,36 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_step_dev(int n, const Tables* __r
                                                         double* __restrict__ g,
                                                         double* __restrict__ g_old,
                                                         double* __restrict__ part, RedTail rt) {
-  __shared__ double lds[BLOCK / WAVE];
+  __shared__ double lds[RED_NT / WAVE];
   const double a = ls[8] != 0.0 ? 0.0 : ls[2];
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const SpecCoef& sc = tb->sc[id[i] - 1];
-    const double qi = q[i], qn = qi + a * p[i];
-    const double gi = g[i];
+  auto step = [&](int i, int idi, double qi, double pi, double gi, double vi) {
+    const SpecCoef& sc = tb->sc[idi - 1];
+    const double qn = qi + a * pi;
     g_old[i] = gi;
-    const double gn = gi - a * v[i] - (pen_d1(sc, qn) - pen_d1(sc, qi));
+    const double gn = gi - a * vi - (pen_d1(sc, qn) - pen_d1(sc, qi));
     q[i] = qn;
     g[i] = gn;
     acc[0] += gn;
+  };
+  // grid-stride, four atoms' loads in flight per trip, summed in atom order
+  const int S = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n; i += 4 * S) {
+    int iv[4];
+    double qv[4], pv[4], gv[4], vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      iv[u] = id[i + u * S];
+      qv[u] = q[i + u * S];
+      pv[u] = p[i + u * S];
+      gv[u] = g[i + u * S];
+      vv[u] = v[i + u * S];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(i + u * S, iv[u], qv[u], pv[u], gv[u], vv[u]);
   }
+  for (; i < n; i += S) step(i, id[i], q[i], p[i], g[i], v[i]);
   block_sum<1>(acc, lds);
   partials_out<1>(acc, part, rt, lds);
 }
