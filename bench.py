@@ -70,13 +70,15 @@ PLANE_KERNELS = {12: "k_fft_xy_r2c (inner grid)", 13: "-", 14: "k_fft_zp (inner 
                  15: "k_fft_xy_c2r (inner grid)"}
 
 
-def kernel_name(w, mode, s=None):
+def kernel_name(w, mode, s=None, shapes=None):
+    """shapes: simox_inner_shapes of the run (the matvec variant: 8 and 9
+    gather from the LDS halo, k_h0matvec_sell4h)."""
     if w == 6 and mode == "faithful":
         return "k_dudq"
     if w in PLANE_KERNELS and plane_inner(s, mode):
         return PLANE_KERNELS[w]
     if w == 6 and s is not None and packed_inner(s, mode):
-        return "k_h0matvec_sell4"
+        return "k_h0matvec_sell4h" if shapes and shapes["matvec"] >= 8 else "k_h0matvec_sell4"
     if w == 6 and s is not None and coarse_inner(s, mode):
         return "k_h0matvec16"
     if w == 10 and s is not None and coarse_inner(s, mode):
@@ -98,17 +100,23 @@ def owner_spread(s):
     return all(a % b == 0 for a, b in zip(k, t))
 
 
-def measured_traffic(kernel, cfg, mode):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 counter summary (FETCH_SIZE x fetch_scale + WRITE_SIZE x
-    write_scale, KiB, scaled by a plain copy of known size), or None."""
+def profile_entry(kernel, cfg, mode):
+    """The committed rocprofv3 figures of `kernel` on this workload
+    (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
             tab = json.load(fh)
     except (OSError, ValueError):
         return None
-    e = tab.get("%s|%s|%s" % (kernel, cfg, mode))
+    return tab.get("%s|%s|%s" % (kernel, cfg, mode))
+
+
+def measured_traffic(kernel, cfg, mode):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE x fetch_scale + WRITE_SIZE x
+    write_scale, KiB, scaled by a plain copy of known size), or None."""
+    e = profile_entry(kernel, cfg, mode)
     if not e:
         return None
     return (e["fetch_kib"] * e["fetch_scale"] + e["write_kib"] * e["write_scale"]) * 1024.0
@@ -638,7 +646,14 @@ def main():
                     "region's last solve buffers, context stream")
     achieved = bytes_launch / per_launch_s / 1e9
     valu = pair_valu(g, s, args.mode, nb_entries, breakdown)
-    traffic = measured_traffic(kernel_name(dom, args.mode, s), args.config, args.mode)
+    shapes = g.inner_shapes() if args.mode == "mixed" else None
+    dom_name = kernel_name(dom, args.mode, s, shapes)
+    traffic = measured_traffic(dom_name, args.config, args.mode)
+    # the same kernel's average duration in the committed kernel trace of the
+    # bench command (rocprofv3 --kernel-trace --stats), and the fraction of
+    # peak it gives
+    pe = profile_entry(dom_name, args.config, args.mode)
+    trace_ms = pe.get("trace_avg_us", 0.0) / 1e3 if pe else 0.0
     value = s.n * args.steps / elapsed
     out = {
         "metric": "atom-steps/s", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
@@ -655,18 +670,22 @@ def main():
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
                    "inner_operator": inner_operator(s, args.mode),
                    "fft_tiles": fft_tiles,
-                   "inner_shapes": g.inner_shapes() if args.mode == "mixed" else None},
-        "roofline": {"bound": "hbm", "kernel": kernel_name(dom, args.mode, s), "achieved": achieved,
+                   "inner_shapes": shapes},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": per_launch_s * 1e3, "timed_as": timed_as,
                      "avg_launch_ms_in_steps": per_launch_events_s * 1e3,
-                     "launches_in_steps": launches},
+                     "launches_in_steps": launches,
+                     "avg_launch_ms_trace": trace_ms or None,
+                     "frac_trace": (bytes_launch / (trace_ms / 1e3) / 1e9 / HBM_PEAK_GBS
+                                    if trace_ms else None),
+                     "trace_source": pe.get("source") if trace_ms else None},
         "valu": valu,
-        "kernels_ms": {kernel_name(w, args.mode, s): round(breakdown[w][0], 3) for w in KERNELS
-                       if kernel_name(w, args.mode, s) != "-"},
-        "kernels_launches": {kernel_name(w, args.mode, s): breakdown[w][1] for w in KERNELS
-                             if kernel_name(w, args.mode, s) != "-"},
+        "kernels_ms": {kernel_name(w, args.mode, s, shapes): round(breakdown[w][0], 3)
+                       for w in KERNELS if kernel_name(w, args.mode, s) != "-"},
+        "kernels_launches": {kernel_name(w, args.mode, s, shapes): breakdown[w][1]
+                             for w in KERNELS if kernel_name(w, args.mode, s) != "-"},
         "kernels_note": "kernels_ms / kernels_launches: one untimed step after the warm-up "
                         "with every category timed; the timed steps time the roofline kernel only",
         "per_step": tm["per_step"],
