@@ -435,6 +435,43 @@ def nve_drift(args):
     return out
 
 
+def charge_trend(args):
+    """Per step of the workload (untimed apart from the step itself): the
+    charge solve's iterations and FP64 restarts, and the charge extremes by
+    species, over args.trend steps from the bench's start (forces at x(0)
+    with q = 0, then one full step after another).  Shows what drives the
+    iteration count as the run goes on."""
+    from simox_amd import MODE_FP64, MODE_MIXED, Simox
+    from simox_amd.systems import config
+    s = config(args.config)
+    g = Simox.from_system(s, mode=MODE_MIXED if args.mode == "mixed" else MODE_FP64)
+    g.set_charge_tol(args.tol, 5000)
+    if args.mode != "mixed":
+        g.set_charge_precond(1)
+    g.force()
+    g.pme()
+    species = sorted(set(int(v) for v in np.unique(s.ids)))
+    rows = []
+    for k in range(args.trend):
+        t0 = time.perf_counter()
+        st = g.step(1, s.dt, 1, 1, stats=True)
+        ms = (time.perf_counter() - t0) * 1e3
+        it, rs = g.charge_stats()
+        q = g.get_state()[3]
+        row = {"step": k + 1, "ms": round(ms, 2), "charge_iters": it, "charge_restarts": rs,
+               "E_total": st["Etotal"]}
+        for sp in species:
+            qs = q[s.ids == sp]
+            row["q_min_%d" % sp] = round(float(qs.min()), 4)
+            row["q_max_%d" % sp] = round(float(qs.max()), 4)
+        rows.append(row)
+        print("bench: trend step %d, %d iterations" % (k + 1, it), file=sys.stderr, flush=True)
+    g.close()
+    return {"metric": "charge-solve trend", "config": {"workload": WORKLOADS[args.config],
+            "mode": args.mode, "steps": args.trend, "charge_tol": args.tol,
+            "species": "1 = Al, 2 = Ni, 3 = Fe, 4 = O, 5 = H"}, "per_step": rows}
+
+
 def bench_gradients(args):
     """Parameter gradients of the fitting loss L = E + w sum |F|^2 over the 64
     C5 snapshots (FP64): per snapshot a converged charge solve (from q = 0),
@@ -578,6 +615,9 @@ def main():
     ap.add_argument("--drift", type=int, default=0,
                     help="instead of the throughput line: NVE drift of E_total over this many "
                          "C1 steps, MIXED and FP64")
+    ap.add_argument("--trend", type=int, default=0,
+                    help="instead of the throughput line: per-step charge-solve iterations, "
+                         "restarts and charge extremes over this many steps of the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU hot path "
@@ -586,7 +626,7 @@ def main():
 
     # the CPU legs first, in processes started before this one touches the GPU
     cpu = None
-    if not args.no_cpu and args.drift <= 0 and args.config != "c5":
+    if not args.no_cpu and args.drift <= 0 and args.trend <= 0 and args.config != "c5":
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_seconds)
     if args.copy_calibration:
@@ -601,6 +641,9 @@ def main():
         torch.cuda.empty_cache()
     if args.drift > 0:
         print(json.dumps(nve_drift(args)))
+        return 0
+    if args.trend > 0:
+        print(json.dumps(charge_trend(args)))
         return 0
     if args.config == "c5":
         print(json.dumps(bench_gradients(args)))
