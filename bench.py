@@ -71,13 +71,15 @@ PLANE_KERNELS = {12: "k_fft_xy_r2c (inner grid)", 13: "-", 14: "k_fft_zp (inner 
 
 
 def kernel_name(w, mode, s=None, shapes=None):
-    """shapes: simox_inner_shapes of the run (the matvec variant: 8 and 9
-    gather from the LDS halo, k_h0matvec_sell4h)."""
+    """shapes: simox_inner_shapes of the run (the matvec variant: 8 to 10
+    gather from the LDS halo, k_h0matvec_sell4h; 10 with 24-bit entries)."""
     if w == 6 and mode == "faithful":
         return "k_dudq"
     if w in PLANE_KERNELS and plane_inner(s, mode):
         return PLANE_KERNELS[w]
     if w == 6 and s is not None and packed_inner(s, mode):
+        if shapes and shapes["matvec"] == 10:
+            return "k_h0matvec_sell4h (24-bit entries)"
         return "k_h0matvec_sell4h" if shapes and shapes["matvec"] >= 8 else "k_h0matvec_sell4"
     if w == 6 and s is not None and coarse_inner(s, mode):
         return "k_h0matvec16"
@@ -132,9 +134,10 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None):
+def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None, shapes=None):
     """Unique HBM bytes one launch of kernel `which` must move (each array read or
-    written once).  r = bytes of the grid/spectrum real type."""
+    written once).  r = bytes of the grid/spectrum real type.  shapes: the run's
+    simox_inner_shapes (matvec variant 10 streams 3-byte entries)."""
     n = s.n
     Kx, Ky, Kz = s.fft
     if which >= 11 or (which == 10 and coarse_inner(s, mode)):
@@ -163,7 +166,8 @@ def algorithmic_bytes(which, s, mode, nb_entries, inner_entries=None):
         # H0 p: id q p pt gr phi cnt in, y out, ELL (col + value) rows; FP16
         # values in the coarse-grid inner operator
         if packed_inner(s, mode):  # compacted matrix, packed 32-bit entries in SELL-4 slices
-            return n * (4 + 8 + 8 + r + 8 + 8 + 8) + inner_entries * 4
+            per = 3 if shapes and shapes["matvec"] == 10 else 4  # 24-bit halo entries
+            return n * (4 + 8 + 8 + r + 8 + 8 + 8) + inner_entries * per
         if coarse_inner(s, mode):  # compacted FP16 matrix of the inner operator
             return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + inner_entries * (4 + 2)
         return n * (4 + 8 + 8 + r + 8 + 8 + 4 + 8) + nb_entries * (4 + r)
@@ -667,7 +671,8 @@ def main():
     ms, launches = times[dom]
     per_launch_events_s = ms / 1e3 / max(launches, 1)
     inner_entries = g.inner_entries() if args.mode != "faithful" else nb_entries
-    bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries, inner_entries)
+    shapes = g.inner_shapes() if args.mode == "mixed" else None
+    bytes_launch = algorithmic_bytes(dom, s, args.mode, nb_entries, inner_entries, shapes)
     # inside the timed steps the matvec's events are the ones the runtime
     # records at the kernel's own start and end (hipExtLaunchKernel), so they
     # time the kernel alone, as a kernel trace does; the other kernels' events
@@ -689,7 +694,6 @@ def main():
                     "region's last solve buffers, context stream")
     achieved = bytes_launch / per_launch_s / 1e9
     valu = pair_valu(g, s, args.mode, nb_entries, breakdown)
-    shapes = g.inner_shapes() if args.mode == "mixed" else None
     dom_name = kernel_name(dom, args.mode, s, shapes)
     traffic = measured_traffic(dom_name, args.config, args.mode)
     # the same kernel's average duration in the committed kernel trace of the

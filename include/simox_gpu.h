@@ -161,11 +161,11 @@ int simox_fft_shapes(const simox_ctx* ctx, int shapes[6]);
 /* Launch shapes of the MIXED inner solve with the coarse inner operator,
    which the context times on its own buffers at its first solve (environment
    SIMOX_AUTOTUNE=0 keeps the defaults; SIMOX_MV_VARIANT, SIMOX_GATHER_SHAPE,
-   SIMOX_FFT_XYNT and SIMOX_FFT_NL fix one each): {packed matvec variant (0-9:
+   SIMOX_FFT_XYNT and SIMOX_FFT_NL fix one each): {packed matvec variant (0-10:
    4 or 8 groups of 4 entries in flight per lane, odd: streaming loads,
    4-7: the same with the row blocks in contiguous runs per XCD, 8 and 9:
    4 and 8 groups, streaming, gathering from each row block's neighbourhood
-   of the direction staged in LDS),
+   of the direction staged in LDS, 10: as 8 with 3-byte entries),
    record gather tile (0-5: 64x8x8, 32x8x8, 32x16x8, 64x16x8, 32x16x16,
    64x16x16), inner grid's x-y pass threads per workgroup and z-pass lines
    per workgroup, the same two of the preconditioner's grid (0: no plane

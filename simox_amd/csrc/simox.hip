@@ -160,7 +160,7 @@ struct simox_ctx {
   PmeGrid pi;
   bool inner_coarse = false;
   // launch shapes of the inner solve that the autotuner sets (tune_inner)
-  int mv_variant = 8;       // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads); 4-7 the same with XCD-ordered row blocks; 8/9 4/8 groups streaming with the LDS halo
+  int mv_variant = 10;      // packed matvec: 0/1 4 groups, 2/3 8 (odd: streaming loads); 4-7 the same with XCD-ordered row blocks; 8/9 4/8 groups streaming with the LDS halo; 10 as 8 with 24-bit entries
   bool inner_tuned = false;
   bool mv_fixed = false, gather_fixed = false, xynt_fixed = false, nl_fixed = false;
   // line-search rounds per inner iteration (multi-block path): adapted to the
@@ -210,6 +210,7 @@ struct simox_ctx {
   // the LDS halo of each 256-row block (k_pk_halo): entries with LDS slots,
   // runs of staged atoms, staged-atom and run counts per block
   unsigned* nb_pkh = nullptr;
+  unsigned* nb_pk24 = nullptr;  // the same entries in 24 bits (3 words per group of 4)
   int2* pk_runs = nullptr;
   int* pk_hcount = nullptr;
   bool pkh_valid = false;
@@ -447,7 +448,7 @@ void set_ctip64(simox_ctx* ctx) {
         ctx->ctip64_on = true;
 }
 
-constexpr int NMV_VARIANTS = 10;  // k_h0matvec_sell4 shapes the autotuner times
+constexpr int NMV_VARIANTS = 11;  // k_h0matvec_sell4 shapes the autotuner times
 
 int ensure_part(simox_ctx* ctx, size_t count) {
   if (count <= ctx->part_cap) return 0;
@@ -1393,12 +1394,14 @@ int build_matrix(simox_ctx* ctx) {
       if (!ctx->inner_tuned || ctx->mv_variant >= 8) {
         if (!ctx->nb_pkh) {
           CHK(dalloc(ctx, &ctx->nb_pkh, size_t(ctx->cap + 4) * (size_t(nsl) * SLICE)));
+          CHK(dalloc(ctx, &ctx->nb_pk24, 3 * (size_t(ctx->cap + 4) / 4) * (size_t(nsl) * SLICE)));
           CHK(dalloc(ctx, &ctx->pk_runs, size_t(nb) * HALO_RUNS));
           CHK(dalloc(ctx, &ctx->pk_hcount, size_t(nb)));
         }
         hipLaunchKernelGGL(k_pk_halo, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->pk_slen,
                            ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk),
-                           reinterpret_cast<uint4*>(ctx->nb_pkh), ctx->pk_runs, ctx->pk_hcount);
+                           reinterpret_cast<uint4*>(ctx->nb_pkh), ctx->nb_pk24, ctx->pk_runs,
+                           ctx->pk_hcount);
         ctx->pkh_valid = true;
       }
     } else {
@@ -1793,7 +1796,8 @@ void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
     // groups in flight per lane and streaming loads (mv_variant: 1 = 4 groups
     // non-temporal; 0 = 4 groups, 2 = 8, 3 = 8 non-temporal; 4-7 the same
     // with XCD-ordered row blocks; 8 and 9 = 4 and 8 groups non-temporal with
-    // the gathers from the LDS halo, 8 the default; the autotuner picks one,
+    // the gathers from the LDS halo; 10 = 8 with 24-bit entries, the default
+    // (C3: 52.7 us against 57.6 for 8); the autotuner picks one,
     // SIMOX_MV_VARIANT fixes it).  At the 1M-atom slab the non-temporal
     // stream is 4 % faster than 0 and the whole step 4 % (the other kernels
     // keep their L2 lines), the halo 1.3x faster than 1.  All sum in the same
@@ -1801,11 +1805,13 @@ void launch_matvec(simox_ctx* ctx, bool coarse, hipEvent_t ea = nullptr,
     const int v = ctx->mv_variant;
     if (v >= 8 && ctx->pkh_valid) {
       // the LDS-halo variants: gathers from the staged halo, the same sums
-      const auto kh = v == 8 ? &k_h0matvec_sell4h<4, true> : &k_h0matvec_sell4h<8, true>;
+      const auto kh = v == 8   ? &k_h0matvec_sell4h<4, true>
+                      : v == 9 ? &k_h0matvec_sell4h<8, true>
+                               : &k_h0matvec_sell4h<4, true, true>;
       hipExtLaunchKernelGGL(kh, dim3(nb), dim3(BLOCK), 0, ctx->st, ea, eb, 0, ctx->g, ctx->dtab,
                             ctx->id, ctx->qc, ctx->pv, ctx->pv32, ctx->gv, ctx->phi,
                             ctx->pk_slen, ctx->pk_soff,
-                            reinterpret_cast<const uint4*>(ctx->nb_pkh), ctx->pk_runs,
+                            reinterpret_cast<const uint4*>(ctx->nb_pkh), ctx->nb_pk24, ctx->pk_runs,
                             ctx->pk_hcount, ctx->apv, ctx->part);
       return;
     }
@@ -2678,7 +2684,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->nb_fwd, ctx->nq, ctx->nq_n, ctx->facc, ctx->eacc, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->nb_pkh, ctx->pk_runs, ctx->pk_hcount, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->nb_pkh, ctx->nb_pk24, ctx->pk_runs, ctx->pk_hcount, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->red_ctr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

@@ -985,7 +985,10 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4(Geom g, const Tables* 
 //                 span HALO_SPAN or more (the blocks at the periodic z
 //                 boundary) or its halo exceeds HALO_MAX atoms or HALO_RUNS runs;
 //   pkh         = the block's entries with the LDS slot in the column field
-//                 (the global column where hcount is 0).
+//                 (the global column where hcount is 0);
+//   pk24        = (blocks with a halo) the same entries in 24 bits each: the
+//                 12-bit value code over the 12-bit slot, a group of 4 in
+//                 12 bytes (pk24_group), 3 bytes per entry instead of 4.
 constexpr int HALO_SPAN = 1 << 17;  // bitmap of 16 KB
 constexpr int HALO_MAX = 4096;      // staged atoms (16 KB of LDS)
 constexpr int HALO_RUNS = 256;
@@ -1015,8 +1018,10 @@ __device__ inline void block_scan2(int a, int b, int* red, int& ea, int& eb, int
 __global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict__ slen,
                                                    const int* __restrict__ soff,
                                                    const uint4* __restrict__ pk,
-                                                   uint4* __restrict__ pkh, int2* __restrict__ runs,
+                                                   uint4* __restrict__ pkh, unsigned* __restrict__ pk24,
+                                                   int2* __restrict__ runs,
                                                    int* __restrict__ hcount) {
+  static_assert(HALO_MAX <= 4096, "slots in 12 bits");
   constexpr int NW = HALO_SPAN / 32;  // bitmap words
   constexpr int WPT = NW / BLOCK;     // consecutive words per thread
   __shared__ unsigned bits[NW];
@@ -1126,6 +1131,14 @@ __global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict_
         const int slot = woff[d >> 5] + __popc(bits[d >> 5] & ((1u << (d & 31)) - 1u));
         e[u] = (e[u] & ~((1u << PK_COL_BITS) - 1u)) | unsigned(slot);
       }
+      // 24-bit entries: value code (bits 31..20 of the packed word) << 12 | slot
+      unsigned c24[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c24[u] = ((e[u] >> PK_COL_BITS) << 12) | (e[u] & 0xFFFu);
+      unsigned* g24 = pk24 + 3 * (r0 + size_t(k) * SLICE);
+      g24[0] = c24[0] | (c24[1] << 24);
+      g24[1] = (c24[1] >> 8) | (c24[2] << 16);
+      g24[2] = (c24[2] >> 16) | (c24[3] << 8);
     }
     pkh[r0 + size_t(k) * SLICE] = w;
   }
@@ -1134,7 +1147,33 @@ __global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict_
 // k_h0matvec_sell4 over the halo-slot copy of k_pk_halo: a block with a halo
 // stages its runs of pt in LDS and gathers from there; the others gather from
 // global memory as k_h0matvec_sell4 does.  Same products in the same order.
-template <int U, bool NT>
+// A group of 4 24-bit entries (12 bytes at 3 g words of pk24) as four
+// 32-bit packed words whose column field holds the slot (pk_value, pk_col).
+template <bool NT>
+__device__ inline u32x4 pk24_group(const unsigned* __restrict__ pk24, size_t g) {
+  const unsigned* a = pk24 + 3 * g;
+  unsigned w0, w1, w2;
+  if constexpr (NT) {
+    w0 = __builtin_nontemporal_load(a);
+    w1 = __builtin_nontemporal_load(a + 1);
+    w2 = __builtin_nontemporal_load(a + 2);
+  } else {
+    w0 = a[0];
+    w1 = a[1];
+    w2 = a[2];
+  }
+  const unsigned e0 = w0 & 0xFFFFFFu, e1 = (w0 >> 24) | ((w1 & 0xFFFFu) << 8);
+  const unsigned e2 = (w1 >> 16) | ((w2 & 0xFFu) << 16), e3 = w2 >> 8;
+  auto word = [](unsigned e) { return ((e >> 12) << PK_COL_BITS) | (e & 0xFFFu); };
+  u32x4 w;
+  w.x = word(e0);
+  w.y = word(e1);
+  w.z = word(e2);
+  w.w = word(e3);
+  return w;
+}
+
+template <int U, bool NT, bool P24 = false>
 __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables* __restrict__ tb,
                                                            const int* __restrict__ id,
                                                            const double* __restrict__ q,
@@ -1145,6 +1184,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables*
                                                            const int* __restrict__ slen,
                                                            const int* __restrict__ soff,
                                                            const uint4* __restrict__ pk,
+                                                           const unsigned* __restrict__ pk24,
                                                            const int2* __restrict__ runs,
                                                            const int* __restrict__ hcount,
                                                            double* __restrict__ y,
@@ -1173,14 +1213,23 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables*
   if (i < g.n) {
     const int s = i / SLICE, lane = i % SLICE;
     const int L = slen[s];
-    const uint4* row = pk + size_t(soff[s]) + lane;
+    const size_t g0 = size_t(soff[s]) + lane;
+    const uint4* row = pk + g0;
     float sum = 0.f;
-    auto rowsum = [&](auto gat) {
+    // the group of entry group k of this row: 24-bit entries of the halo copy
+    // (P24, blocks with a halo), else the 32-bit words
+    auto grp = [&](int kk, bool h24) {
+      if constexpr (P24) {
+        if (h24) return pk24_group<NT>(pk24, g0 + size_t(kk) * SLICE);
+      }
+      return pk_group<NT>(row + size_t(kk) * SLICE);
+    };
+    auto rowsum = [&](auto gat, bool h24) {
       int k = 0;
       for (; k + U <= L; k += U) {
         u32x4 w[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) w[u] = pk_group<NT>(row + size_t(k + u) * SLICE);
+        for (int u = 0; u < U; ++u) w[u] = grp(k + u, h24);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           sum += pk_value(w[u].x) * gat(pk_col(w[u].x));
@@ -1190,7 +1239,7 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables*
         }
       }
       for (; k < L; ++k) {
-        const u32x4 w = pk_group<NT>(row + size_t(k) * SLICE);
+        const u32x4 w = grp(k, h24);
         sum += pk_value(w.x) * gat(pk_col(w.x));
         sum += pk_value(w.y) * gat(pk_col(w.y));
         sum += pk_value(w.z) * gat(pk_col(w.z));
@@ -1198,9 +1247,9 @@ __global__ void __launch_bounds__(BLOCK) k_h0matvec_sell4h(Geom g, const Tables*
       }
     };
     if (nh > 0)
-      rowsum([&](int c) { return hv[c]; });
+      rowsum([&](int c) { return hv[c]; }, true);
     else
-      rowsum([&](int c) { return pt[c]; });
+      rowsum([&](int c) { return pt[c]; }, false);
     const double pi = p[i];
     const SpecCoef& sc = tb->sc[id[i] - 1];
     const double yi = sc.Jp * pi + phi_in[i] + double(sum);

@@ -407,9 +407,9 @@ def test_inner_shapes_change_no_result(monkeypatch):
 
     q0, f0, sh = run({})
     print("tuned inner shapes", sh, flush=True)
-    assert sh["matvec"] in range(10) and sh["gather"] in range(6)
+    assert sh["matvec"] in range(11) and sh["gather"] in range(6)
     assert sh["inner_grid"][0] in (256, 512, 1024) and sh["inner_grid"][1] in (4, 8, 11, 16)
-    cases = [{"SIMOX_MV_VARIANT": str(v)} for v in range(10)]
+    cases = [{"SIMOX_MV_VARIANT": str(v)} for v in range(11)]
     cases += [{"SIMOX_GATHER_SHAPE": str(v)} for v in range(6)]
     cases += [{"SIMOX_FFT_XYNT": t} for t in ("256", "512", "1024")]
     cases += [{"SIMOX_FFT_NL": m} for m in ("4", "8", "16")]
