@@ -1635,6 +1635,11 @@ struct SpreadOwn {
   static constexpr int PL = ORD == 4 ? 1 : 0, PH = ORD == 4 ? 2 : 1;
   static constexpr int EX = TXg + PL + PH, EY = TYg + PL + PH, EZ = TZg + PL + PH;
   static constexpr int EXT = EX * EY * EZ;
+  // the x columns of the extended region that the fold of the tile's x faces
+  // reads (the NR lowest and the NR from TXg on, NR = PL + PH), also kept
+  // transposed per (y, z) row: XCOL ints per tile
+  static constexpr int NR = PL + PH, NXC = 2 * NR, XCOL = EY * EZ * NXC;
+  __host__ __device__ static int xcol_of(int ex) { return ex < NR ? ex : NR + (ex - TXg); }
   static constexpr int NBX = TXg / BRK, NBY = TYg / BRK, ROWS = NBY * (TZg / BRK);
   static constexpr int QCAP = 8192;  // owned atoms whose FP32 charges stay in LDS
   static_assert(TXg % BRK == 0 && TYg % BRK == 0 && TZg % BRK == 0, "whole bricks");
@@ -1652,7 +1657,7 @@ __global__ void __launch_bounds__(NT) k_spread_own(Geom g, const double* __restr
                                                    const int* __restrict__ border,
                                                    const float4* __restrict__ rec,
                                                    float* __restrict__ grid, int* __restrict__ ext,
-                                                   float* __restrict__ tinv) {
+                                                   int* __restrict__ xcol, float* __restrict__ tinv) {
   using P = SpreadOwn<ORD, TXg, TYg, TZg>;
   constexpr int EX = P::EX, EY = P::EY, EXT = P::EXT, ROWS = P::ROWS, QCAP = P::QCAP;
   static_assert(ROWS <= NT && ROWS <= WAVE, "one lane per row of bricks");
@@ -1766,14 +1771,18 @@ __global__ void __launch_bounds__(NT) k_spread_own(Geom g, const double* __restr
   const float inv = float(ldexp(1.0, -S));
   if (threadIdx.x == 0) tinv[t] = inv;
   int* mine = ext + size_t(t) * EXT;
+  int* mcol = xcol + size_t(t) * P::XCOL;
   for (int k = threadIdx.x; k < EXT; k += NT) {
-    const int rx = k % EX - P::PL, ry = (k / EX) % EY - P::PL, rz = k / (EX * EY) - P::PL;
-    if (P::inner(rx, TXg) && P::inner(ry, TYg) && P::inner(rz, TZg))
+    const int ex = k % EX, eyz = k / EX;
+    const int rx = ex - P::PL, ry = eyz % EY - P::PL, rz = eyz / EY - P::PL;
+    if (P::inner(rx, TXg) && P::inner(ry, TYg) && P::inner(rz, TZg)) {
       grid[size_t(lo[0] + rx) +
            size_t(g.K[0]) * (size_t(lo[1] + ry) + size_t(g.K[1]) * size_t(lo[2] + rz))] =
           float(acc[k]) * inv;
-    else
+    } else {
       mine[k] = acc[k];
+      if (ex < P::NR || ex >= TXg) mcol[eyz * P::NXC + P::xcol_of(ex)] = acc[k];
+    }
   }
 }
 
@@ -1799,6 +1808,7 @@ struct SpreadFold {
 
 template <int ORD, int TXg, int TYg, int TZg>
 __global__ void __launch_bounds__(BLOCK) k_spread_fold(Geom g, const int* __restrict__ ext,
+                                                       const int* __restrict__ xcol,
                                                        const float* __restrict__ tinv,
                                                        float* __restrict__ grid) {
   using P = SpreadOwn<ORD, TXg, TYg, TZg>;
@@ -1825,6 +1835,27 @@ __global__ void __launch_bounds__(BLOCK) k_spread_fold(Geom g, const int* __rest
   const int T[3] = {TXg, TYg, TZg};
   const int nt[3] = {g.K[0] / TXg, g.K[1] / TYg, g.K[2] / TZg};
   const int tc[3] = {t % nt[0], (t / nt[0]) % nt[1], t / (nt[0] * nt[1])};
+  const size_t gidx = size_t(tc[0] * TXg + r[0]) +
+                      size_t(g.K[0]) * (size_t(tc[1] * TYg + r[1]) +
+                                        size_t(g.K[1]) * size_t(tc[2] * TZg + r[2]));
+  if (j >= F::NA + F::NB) {
+    // an x face only: the own tile and the x neighbour, both from the
+    // transposed x columns (the sum as below: own, then neighbour)
+    const bool low = r[0] < P::PH;
+    const int tn = low ? (tc[0] == 0 ? nt[0] - 1 : tc[0] - 1) : (tc[0] + 1 == nt[0] ? 0 : tc[0] + 1);
+    const int en = low ? r[0] + TXg + P::PL : r[0] - TXg + P::PL;
+    const int row = (r[2] + P::PL) * P::EY + (r[1] + P::PL);
+    const int ttn = tn + nt[0] * (tc[1] + nt[1] * tc[2]);
+    const float c0 = float(xcol[size_t(t) * P::XCOL + row * P::NXC + P::xcol_of(r[0] + P::PL)]) *
+                     tinv[t];
+    const float c1 = float(xcol[size_t(ttn) * P::XCOL + row * P::NXC + P::xcol_of(en)]) *
+                     tinv[ttn];
+    float v = 0.f;
+    v += c0;
+    v += c1;
+    grid[gidx] = v;
+    return;
+  }
   // per axis the own tile and at most one neighbour (the tiles hold more
   // than PL + PH points per axis, so no point is within reach of both)
   int ot[3][2], oe[3][2];
@@ -1861,9 +1892,7 @@ __global__ void __launch_bounds__(BLOCK) k_spread_fold(Geom g, const int* __rest
   float v = 0.f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) v += c[k];
-  grid[size_t(tc[0] * TXg + r[0]) +
-       size_t(g.K[0]) * (size_t(tc[1] * TYg + r[1]) + size_t(g.K[1]) * size_t(tc[2] * TZg + r[2]))] =
-      v;
+  grid[gidx] = v;
 }
 
 // K9 (tiled): out_i = scale * sum w Qfin over the order-4 stencil of atom i

@@ -110,6 +110,8 @@ struct PmeGrid {
   // owner-computes spread (k_spread_own): each tile's extended-region values
   // that the fold sums, and each tile's 2^-S
   int* sp_ext = nullptr;
+  int* sp_xcol = nullptr;  // the x-face columns of each tile's region, transposed
+  size_t sp_xcol_n = 0;
   float* sp_tinv = nullptr;
   size_t sp_ext_n = 0;
   int sp_tiles = 0;
@@ -1063,24 +1065,29 @@ template <int ORD, int TXg, int TYg, int TZg, int NT>
 int spread_own(simox_ctx* ctx, const Geom& g, PmeGrid& pm, const double* src) {
   using P = SpreadOwn<ORD, TXg, TYg, TZg>;
   const int nt = P::tiles(g);
-  const size_t need = size_t(nt) * P::EXT;
-  if (pm.sp_ext_n < need || pm.sp_tiles < nt) {
+  const size_t need = size_t(nt) * P::EXT, need_x = size_t(nt) * P::XCOL;
+  if (pm.sp_ext_n < need || pm.sp_xcol_n < need_x || pm.sp_tiles < nt) {
     if (pm.sp_ext) (void)hipFree(pm.sp_ext);
+    if (pm.sp_xcol) (void)hipFree(pm.sp_xcol);
     if (pm.sp_tinv) (void)hipFree(pm.sp_tinv);
     pm.sp_ext = nullptr;
+    pm.sp_xcol = nullptr;
     pm.sp_tinv = nullptr;
-    pm.sp_ext_n = 0;
+    pm.sp_ext_n = pm.sp_xcol_n = 0;
     pm.sp_tiles = 0;
     CHK(dalloc(ctx, &pm.sp_ext, need));
+    CHK(dalloc(ctx, &pm.sp_xcol, need_x));
     CHK(dalloc(ctx, &pm.sp_tinv, size_t(nt)));
     pm.sp_ext_n = need;
+    pm.sp_xcol_n = need_x;
     pm.sp_tiles = nt;
   }
   hipLaunchKernelGGL((k_spread_own<ORD, TXg, TYg, TZg, NT>), dim3(nt), dim3(NT), 0, ctx->st, g,
-                     src, pm.bstart, pm.border, pm.brec, pm.greal32, pm.sp_ext, pm.sp_tinv);
+                     src, pm.bstart, pm.border, pm.brec, pm.greal32, pm.sp_ext, pm.sp_xcol,
+                     pm.sp_tinv);
   hipLaunchKernelGGL((k_spread_fold<ORD, TXg, TYg, TZg>),
                      dim3(nt * SpreadFold<ORD, TXg, TYg, TZg>::BPT), dim3(BLOCK), 0, ctx->st, g,
-                     pm.sp_ext, pm.sp_tinv, pm.greal32);
+                     pm.sp_ext, pm.sp_xcol, pm.sp_tinv, pm.greal32);
   return 0;
 }
 
@@ -1290,7 +1297,7 @@ int init_grid(simox_ctx* ctx, PmeGrid& pm, const int K[3], bool with_G, int orde
 void free_grid(PmeGrid& pm) {
   void* bufs[] = {pm.brick, pm.bunord, pm.border, pm.bcount, pm.bfill, pm.bstart, pm.bsum,
                   pm.btotal, pm.greal64, pm.greal32, pm.spec64, pm.spec32, pm.G64, pm.G32,
-                  pm.F64, pm.F32, pm.brec, pm.brec_atom, pm.sp_ext, pm.sp_tinv, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
+                  pm.F64, pm.F32, pm.brec, pm.brec_atom, pm.sp_ext, pm.sp_xcol, pm.sp_tinv, pm.tw64[0], pm.tw64[1], pm.tw64[2], pm.tw32[0], pm.tw32[1],
                   pm.tw32[2]};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
