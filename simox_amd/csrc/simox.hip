@@ -216,6 +216,7 @@ struct simox_ctx {
   int2* pk_runs = nullptr;
   int* pk_hcount = nullptr;
   bool pkh_valid = false;
+  int halo_max = HALO_MAX;  // SIMOX_HALO_MAX (tests: blocks past it gather globally)
   double inner_drop = 1e-3;  // entries with |H0_ij| below this (eV/e^2) left out
   double* bvec = nullptr;
   double *qc = nullptr, *ql = nullptr, *gv = nullptr, *dv = nullptr, *pv = nullptr, *phi = nullptr;
@@ -1405,7 +1406,8 @@ int build_matrix(simox_ctx* ctx) {
           CHK(dalloc(ctx, &ctx->pk_runs, size_t(nb) * HALO_RUNS));
           CHK(dalloc(ctx, &ctx->pk_hcount, size_t(nb)));
         }
-        hipLaunchKernelGGL(k_pk_halo, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->pk_slen,
+        hipLaunchKernelGGL(k_pk_halo, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->halo_max,
+                           ctx->pk_slen,
                            ctx->pk_soff, reinterpret_cast<const uint4*>(ctx->nb_pk),
                            reinterpret_cast<uint4*>(ctx->nb_pkh), ctx->nb_pk24, ctx->pk_runs,
                            ctx->pk_hcount);
@@ -2571,6 +2573,10 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
   ctx->inner_coarse = mode == SIMOX_MODE_MIXED &&
                       double(fft[0]) * double(fft[1]) * double(fft[2]) >= 8388608.0;
   if (const char* e = std::getenv("SIMOX_INNER_COARSE")) ctx->inner_coarse = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SIMOX_HALO_MAX")) {
+    const int v = std::atoi(e);
+    if (v > 0 && v <= HALO_MAX) ctx->halo_max = v;
+  }
   if (const char* e = std::getenv("SIMOX_MV_VARIANT")) {
     const int v = std::atoi(e);
     if (v >= 0 && v < NMV_VARIANTS) { ctx->mv_variant = v; ctx->mv_fixed = true; }

@@ -1015,7 +1015,9 @@ __device__ inline void block_scan2(int a, int b, int* red, int& ea, int& eb, int
   eb = bb + ib - b;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict__ slen,
+// hmax: the staged-atom limit (at most HALO_MAX; lower values send more
+// blocks to the global gathers, which tests use to run both kinds of block).
+__global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, int hmax, const int* __restrict__ slen,
                                                    const int* __restrict__ soff,
                                                    const uint4* __restrict__ pk,
                                                    uint4* __restrict__ pkh, unsigned* __restrict__ pk24,
@@ -1102,7 +1104,7 @@ __global__ void __launch_bounds__(BLOCK) k_pk_halo(int n, const int* __restrict_
   }
   int es, er, ts, tr;
   block_scan2(cs, cr, red, es, er, ts, tr);
-  const bool ok = span_ok && ts <= HALO_MAX && tr <= HALO_RUNS;
+  const bool ok = span_ok && ts <= hmax && tr <= HALO_RUNS;
   if (ok) {
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {

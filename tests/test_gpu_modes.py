@@ -407,6 +407,11 @@ def test_inner_shapes_change_no_result(monkeypatch):
 
     q0, f0, sh = run({})
     print("tuned inner shapes", sh, flush=True)
+    # row blocks past a lowered halo limit gather p from global memory: the
+    # halo variants then run both kinds of block, with the same results
+    for v in ("8", "10"):
+        q, f, _ = run({"SIMOX_AUTOTUNE": "0", "SIMOX_MV_VARIANT": v, "SIMOX_HALO_MAX": "1200"})
+        assert np.array_equal(q, q0) and np.array_equal(f, f0), v
     assert sh["matvec"] in range(11) and sh["gather"] in range(6)
     assert sh["inner_grid"][0] in (256, 512, 1024) and sh["inner_grid"][1] in (4, 8, 11, 16)
     cases = [{"SIMOX_MV_VARIANT": str(v)} for v in range(11)]
