@@ -339,24 +339,31 @@ def test_mixed_coarse_inner_grid_matches_fp64(monkeypatch, pack):
             g.close()
 
 
-def test_owner_spread_matches_tiled_spread(monkeypatch):
+@pytest.mark.parametrize("fft,coarse", [((128, 128, 64), True), ((32, 32, 32), False)])
+def test_owner_spread_matches_tiled_spread(monkeypatch, fft, coarse):
     """The owner-computes 32-bit spread with its fold (k_spread_own,
-    k_spread_fold), taken on both coarse grids when their tiles divide them
-    (here 64 x 64 x 32 each: inner operator and preconditioner), against the
-    tiled spread over each tile's candidates (SIMOX_SPREAD_OWN=0): both solves
-    converge to FP64's charges within 1e-9 and forces within 1e-5 of the RMS
-    force, and the owner path gives the same bits when repeated (integer sums
-    per tile, the fold in a fixed order)."""
+    k_spread_fold), taken on the coarse grids when their tiles divide them
+    (grid 128 x 128 x 64: the inner operator's and the preconditioner's
+    64 x 64 x 32 grids, 32 x 32 x 16 tiles; grid 32^3 with the inner operator
+    on the reference grid, whose 16^3 half-resolution grid is too coarse for
+    the outer iteration to converge: the preconditioner's 16^3 grid, 16 x 16 x 8
+    tiles owning about 9,700 atoms, more than the 8,192 whose charges a tile
+    keeps in LDS, so the spread reads them again; one tile along x and y, so
+    the fold adds a tile's own overhang across the periodic boundary),
+    against the tiled spread over each tile's candidates (SIMOX_SPREAD_OWN=0):
+    both solves converge to FP64's charges within 1e-9 and forces within 1e-5
+    of the RMS force, and the owner path gives the same bits when repeated
+    (integer sums per tile, the fold in a fixed order)."""
     from simox_amd.systems import al_o_slab
-    s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (128, 128, 64), 10.0, vacuum=20.0,
+    s = al_o_slab(24, 24, 8, 1024, 5, 600.0, fft, 10.0, vacuum=20.0,
                   o_layer=(2.0, 10.0), o_dmin=2.0, name="own")
-    print("owner spread, n", s.n, flush=True)
+    print("owner spread, n", s.n, "fft", fft, flush=True)
 
     def run(mode, own):
         monkeypatch.setenv("SIMOX_SPREAD_OWN", "1" if own else "0")
         g = Simox(s.n, s.box, s.fft, s.rcut, mode=mode)
         if mode == MODE_MIXED:
-            g.set_inner_grid(True)
+            g.set_inner_grid(coarse)
         g.set_charge_tol(1e-10, 5000)
         g.set_state(s.ids, s.x, s.v, None, np.zeros(s.n))
         r = g.ctip_solve()
