@@ -477,7 +477,7 @@ __global__ void k_sum_rows(const double* __restrict__ part, int nparts, int nv,
 // the list's entry k.  Only sym is stored; the asym part enters the solve only
 // through b_i = sum_j Z_j asym_ij, which is constant for a whole solve (the
 // reference recomputes it per gradient, ctip.f90:327).
-template <class T>
+template <class T, int U = 1>
 __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __restrict__ tb,
                                                        const double* __restrict__ x,
                                                        const int* __restrict__ id,
@@ -493,17 +493,33 @@ __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __r
   const int c = cnt[i];
   double bi = 0.0;
   if (si >= 1 && si <= NSPEC) {
-    for (int k = 0; k < c; ++k) {
-      const int j = col[size_t(k) * n + i];
-      double rx[3], r2;
-      pair_sep(g, x, i, j, rx, r2);
-      const int sj = id[j];
-      if (sj < 1 || sj > NSPEC) continue;
-      const PairCoef& pc = tb->pc[pidx(si, sj)];
-      T sym, asym;
-      ctip_matrix_terms<T>(pc, T(sqrt(r2)), alpha, sym, asym);
-      bi += double(T(pc.Z_j) * asym);
-      val[size_t(k) * n + i] = sym;
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    // U entries' neighbour data loaded ahead, then taken in list order
+    for (int k0 = 0; k0 < c; k0 += U) {
+      int sv[U];
+      double xv[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = k0 + u < c ? col[size_t(k0 + u) * n + i] : i;
+        sv[u] = id[j];
+        xv[u][0] = x[3 * j];
+        xv[u][1] = x[3 * j + 1];
+        xv[u][2] = x[3 * j + 2];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u;
+        if (k >= c) break;
+        double rx[3], r2;
+        min_image(g, xi, yi, zi, xv[u][0], xv[u][1], xv[u][2], rx, r2);
+        const int sj = sv[u];
+        if (sj < 1 || sj > NSPEC) continue;
+        const PairCoef& pc = tb->pc[pidx(si, sj)];
+        T sym, asym;
+        ctip_matrix_terms<T>(pc, T(sqrt(r2)), alpha, sym, asym);
+        bi += double(T(pc.Z_j) * asym);
+        val[size_t(k) * n + i] = sym;
+      }
     }
   }
   b[i] = bi;

@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(NT) k_nlist_tile(TileArgs A) {
 // closer than R_NEAR; F(rho), F'(rho) in FP64 in every mode (force.f90:256-348).
 // egroup_part[blockIdx.x] = sum of F over the block.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int U = 1>
 __global__ void __launch_bounds__(BLOCK) k_density(Geom g, const Tables* __restrict__ tb,
                                                    const double* __restrict__ x,
                                                    const int* __restrict__ id,
@@ -317,32 +317,47 @@ __global__ void __launch_bounds__(BLOCK) k_density(Geom g, const Tables* __restr
     const int c = cnt[i], cf = cnt_fwd[i];
     double rho = 0.0;
     bool bad = si < 1 || si > NSPEC;
-    for (int k = 0; k < c && !bad; ++k) {
-      const int j = col[size_t(k) * n + i];
-      const int sj = id[j];
-      if (sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid ||
-          (tb->sp.on && !(spl_species_ok(si) && spl_species_ok(sj)))) {
-        bad = true;
-        break;
+    // U entries' neighbour data loaded ahead, then taken in list order
+    for (int k0 = 0; k0 < c && !bad; k0 += U) {
+      int sv[U];
+      double xv[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = k0 + u < c ? col[size_t(k0 + u) * n + i] : i;
+        sv[u] = id[j];
+        xv[u][0] = x[3 * j];
+        xv[u][1] = x[3 * j + 1];
+        xv[u][2] = x[3 * j + 2];
       }
-      double rx[3], r2;
-      min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rx, r2);
-      if (!(r2 < g.ream2)) continue;  // split cutoff: EAM only inside r_eam
-      const bool fwd = k < cf;
-      if (tb->sp.on) {
-        T f, dfr;
-        spl_eval<T>(tb->sp, spl_dens(si, sj), T(sqrt(r2)), f, dfr);
-        rho += double(f);
-      } else {
-        const DensCoef& dc = fwd ? tb->pc[pidx(si, sj)].dj : tb->pc[pidx(sj, si)].di;
-        if (near_pair<T>(r2)) {
-          double f, dfr;
-          dens_eval<double>(dc, sqrt(r2), f, dfr);
-          rho += f;
-        } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u;
+        if (k >= c || bad) break;
+        const int sj = sv[u];
+        if (sj < 1 || sj > NSPEC || !tb->pc[pidx(si, sj)].valid ||
+            (tb->sp.on && !(spl_species_ok(si) && spl_species_ok(sj)))) {
+          bad = true;
+          break;
+        }
+        double rx[3], r2;
+        min_image(g, xi, yi, zi, xv[u][0], xv[u][1], xv[u][2], rx, r2);
+        if (!(r2 < g.ream2)) continue;  // split cutoff: EAM only inside r_eam
+        const bool fwd = k < cf;
+        if (tb->sp.on) {
           T f, dfr;
-          dens_eval<T>(dc, T(sqrt(r2)), f, dfr);
+          spl_eval<T>(tb->sp, spl_dens(si, sj), T(sqrt(r2)), f, dfr);
           rho += double(f);
+        } else {
+          const DensCoef& dc = fwd ? tb->pc[pidx(si, sj)].dj : tb->pc[pidx(sj, si)].di;
+          if (near_pair<T>(r2)) {
+            double f, dfr;
+            dens_eval<double>(dc, sqrt(r2), f, dfr);
+            rho += f;
+          } else {
+            T f, dfr;
+            dens_eval<T>(dc, T(sqrt(r2)), f, dfr);
+            rho += double(f);
+          }
         }
       }
     }
@@ -482,8 +497,8 @@ __global__ void __launch_bounds__(BLOCK) k_force_half(Geom g, const Tables* __re
 // forward entries go to the near queue, which adds both atoms' shares in fixed
 // point (facc), as k_force_ctip64 adds the FP64 Slater terms of pc.ctip64
 // pairs; k_force_finish then adds facc to f.  rdf: as k_force_half, one count
-// per entry.
-template <class T>
+// per entry.  U: list entries whose neighbour data is loaded ahead.
+template <class T, int U = 1>
 __global__ void __launch_bounds__(BLOCK) k_force_full(Geom g, const Tables* __restrict__ tb,
                                                       const double* __restrict__ x,
                                                       const double* __restrict__ q,
@@ -516,41 +531,63 @@ __global__ void __launch_bounds__(BLOCK) k_force_full(Geom g, const Tables* __re
     if (si >= 1 && si <= NSPEC) {
       const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
       const int c = cnt[i], cf = cnt_fwd[i];
-      for (int k = 0; k < c; ++k) {
-        const int j = col[size_t(k) * n + i];
-        const int sj = id[j];
-        if (sj < 1 || sj > NSPEC) continue;
-        const bool fwd = k < cf;
-        const PairCoef& pc = fwd ? tb->pc[pidx(si, sj)] : tb->pc[pidx(sj, si)];
-        if (!pc.valid) continue;
-        double rxd[3], r2d;
-        min_image(g, xi, yi, zi, x[3 * j], x[3 * j + 1], x[3 * j + 2], rxd, r2d);
-        if (rdf) {
-          const int bin = int(sqrt(r2d) / dr);  // INT(r/pair%dr)
-          if (bin >= 1 && bin <= NRDF + 1)
-            atomicAdd(&hist[(bin - 1) * 25 + (sj - 1) * 5 + (si - 1)], 1u);
+      // U entries' neighbour data loaded ahead of their use, then the
+      // entries taken in list order (the same sums as one at a time)
+      for (int k0 = 0; k0 < c; k0 += U) {
+        int jv[U], sv[U];
+        double xv[U][3];
+        T qv[U], dv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jv[u] = k0 + u < c ? col[size_t(k0 + u) * n + i] : i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = jv[u];
+          sv[u] = id[j];
+          xv[u][0] = x[3 * j];
+          xv[u][1] = x[3 * j + 1];
+          xv[u][2] = x[3 * j + 2];
+          qv[u] = T(q[j]);
+          dv[u] = dF[j];
         }
-        if (near_pair<T>(r2d)) {
-          if (fwd) {
-            const int p = atomicAdd(nq_n, 1);
-            if (p < nq_cap) nq[p] = make_int2(i, j);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u;
+          if (k >= c) break;
+          const int j = jv[u];
+          const int sj = sv[u];
+          if (sj < 1 || sj > NSPEC) continue;
+          const bool fwd = k < cf;
+          const PairCoef& pc = fwd ? tb->pc[pidx(si, sj)] : tb->pc[pidx(sj, si)];
+          if (!pc.valid) continue;
+          double rxd[3], r2d;
+          min_image(g, xi, yi, zi, xv[u][0], xv[u][1], xv[u][2], rxd, r2d);
+          if (rdf) {
+            const int bin = int(sqrt(r2d) / dr);  // INT(r/pair%dr)
+            if (bin >= 1 && bin <= NRDF + 1)
+              atomicAdd(&hist[(bin - 1) * 25 + (sj - 1) * 5 + (si - 1)], 1u);
           }
-          continue;
+          if (near_pair<T>(r2d)) {
+            if (fwd) {
+              const int p = atomicAdd(nq_n, 1);
+              if (p < nq_cap) nq[p] = make_int2(i, j);
+            }
+            continue;
+          }
+          const T qj = qv[u], dFj = dv[u];
+          // one evaluation with the pair's own "i" first (selects, no branch)
+          const int sa = fwd ? si : sj, sb = fwd ? sj : si;
+          const T qa = fwd ? qi : qj, qb = fwd ? qj : qi;
+          const T da = fwd ? dFi : dFj, db = fwd ? dFj : dFi;
+          T e0, e1, e2;
+          const double s = double(pair_force<T>(tb, pc, sa, sb, T(sqrt(r2d)), T(r2d), qa, qb,
+                                                da, db, r2d < g.ream2, a, e0, e1, e2));
+          ep += double(e0);
+          ed += double(e1);
+          ec += double(e2);
+          fx += rxd[0] * s;
+          fy += rxd[1] * s;
+          fz += rxd[2] * s;
         }
-        const T qj = T(q[j]), dFj = dF[j];
-        // one evaluation with the pair's own "i" first (selects, no branch)
-        const int sa = fwd ? si : sj, sb = fwd ? sj : si;
-        const T qa = fwd ? qi : qj, qb = fwd ? qj : qi;
-        const T da = fwd ? dFi : dFj, db = fwd ? dFj : dFi;
-        T e0, e1, e2;
-        const double s = double(pair_force<T>(tb, pc, sa, sb, T(sqrt(r2d)), T(r2d), qa, qb, da,
-                                              db, r2d < g.ream2, a, e0, e1, e2));
-        ep += double(e0);
-        ed += double(e1);
-        ec += double(e2);
-        fx += rxd[0] * s;
-        fy += rxd[1] * s;
-        fz += rxd[2] * s;
       }
     }
     f[3 * i] = fx;
