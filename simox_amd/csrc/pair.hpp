@@ -477,14 +477,29 @@ __global__ void k_sum_rows(const double* __restrict__ part, int nparts, int nv,
 // the list's entry k.  Only sym is stored; the asym part enters the solve only
 // through b_i = sum_j Z_j asym_ij, which is constant for a whole solve (the
 // reference recomputes it per gradient, ctip.f90:327).
-template <class T, int U = 1>
+// The packed inner matrix's word of an entry (the SELL-4 slices' encoding):
+// the value as FP16 with its mantissa rounded to 6 bits in bits 31..20, the
+// column in bits 19..0; PK_DROP for an entry left out (|v| < tau), which no
+// finite value's word equals.
+constexpr unsigned PK_DROP = 0xFFFFFFFFu;
+__device__ inline unsigned pk_word(double v, int j) {
+  const unsigned hb = __builtin_bit_cast(unsigned short, _Float16(float(v)));
+  const unsigned code = (hb & 0x8000u) | (((hb & 0x7FFFu) + 8u) & 0x7FF0u);
+  return (code << 16) | unsigned(j);
+}
+
+// PACK: each entry's packed word (or PK_DROP) into pkw (the list's layout)
+// and the row's kept-entry count into kept, for k_ell_to_sell4.
+template <class T, int U = 1, bool PACK = false>
 __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __restrict__ tb,
                                                        const double* __restrict__ x,
                                                        const int* __restrict__ id,
                                                        const int* __restrict__ cnt,
                                                        const int* __restrict__ col,
                                                        T* __restrict__ val,
-                                                       double* __restrict__ b) {
+                                                       double* __restrict__ b, double tau = 0.0,
+                                                       unsigned* __restrict__ pkw = nullptr,
+                                                       int* __restrict__ kept = nullptr) {
   const int i = pair_block(g) * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   const int si = id[i];
@@ -492,15 +507,17 @@ __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __r
   const size_t n = size_t(g.n);
   const int c = cnt[i];
   double bi = 0.0;
+  int nk = 0;
   if (si >= 1 && si <= NSPEC) {
     const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
     // U entries' neighbour data loaded ahead, then taken in list order
     for (int k0 = 0; k0 < c; k0 += U) {
-      int sv[U];
+      int sv[U], jv[U];
       double xv[U][3];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = k0 + u < c ? col[size_t(k0 + u) * n + i] : i;
+        jv[u] = j;
         sv[u] = id[j];
         xv[u][0] = x[3 * j];
         xv[u][1] = x[3 * j + 1];
@@ -513,14 +530,27 @@ __global__ void __launch_bounds__(BLOCK) k_ctip_matrix(Geom g, const Tables* __r
         double rx[3], r2;
         min_image(g, xi, yi, zi, xv[u][0], xv[u][1], xv[u][2], rx, r2);
         const int sj = sv[u];
-        if (sj < 1 || sj > NSPEC) continue;
+        if (sj < 1 || sj > NSPEC) {
+          if constexpr (PACK) pkw[size_t(k) * n + i] = PK_DROP;
+          continue;
+        }
         const PairCoef& pc = tb->pc[pidx(si, sj)];
         T sym, asym;
         ctip_matrix_terms<T>(pc, T(sqrt(r2)), alpha, sym, asym);
         bi += double(T(pc.Z_j) * asym);
         val[size_t(k) * n + i] = sym;
+        if constexpr (PACK) {
+          const bool keep = fabs(double(sym)) >= tau;
+          pkw[size_t(k) * n + i] = keep ? pk_word(double(sym), jv[u]) : PK_DROP;
+          nk += keep;
+        }
       }
     }
+  }
+  if constexpr (PACK) {
+    if (!(si >= 1 && si <= NSPEC))
+      for (int k = 0; k < c; ++k) pkw[size_t(k) * n + i] = PK_DROP;
+    kept[i] = nk;
   }
   b[i] = bi;
 }

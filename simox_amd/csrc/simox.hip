@@ -209,6 +209,7 @@ struct simox_ctx {
   _Float16* nb_val16 = nullptr;
   int *nb_cnt16 = nullptr, *nb_col16 = nullptr;
   unsigned* nb_pk = nullptr;  // packed 32-bit entries (column + 12-bit value) when n <= 2^20
+  unsigned* pk_words = nullptr;  // their words in the list's layout (k_ctip_matrix, PACK)
   // their SELL-4 slices of 64 rows: groups of 4 entries per row, offsets
   int *pk_slen = nullptr, *pk_soff = nullptr, *pk_bsum = nullptr, *pk_total = nullptr;
   bool inner_pack = true, pk_valid = false;
@@ -1361,18 +1362,31 @@ int ensure_val32(simox_ctx* ctx) {
 }
 
 void launch_ctip_matrix(simox_ctx* ctx) {
-  // list entries whose neighbour data each lane loads ahead (SIMOX_CTIP_BATCH)
-  const auto km = ctx->ctip_batch >= 4   ? &k_ctip_matrix<double, 4>
-                  : ctx->ctip_batch == 2 ? &k_ctip_matrix<double, 2>
-                                         : &k_ctip_matrix<double, 1>;
+  // list entries whose neighbour data each lane loads ahead (SIMOX_CTIP_BATCH);
+  // with the packed inner matrix (pk_words set by build_matrix) also each
+  // entry's packed word and each row's kept count
+  const bool pack = ctx->pk_words != nullptr;
+  const auto km = ctx->ctip_batch >= 4
+                      ? (pack ? &k_ctip_matrix<double, 4, true> : &k_ctip_matrix<double, 4>)
+                  : ctx->ctip_batch == 2
+                      ? (pack ? &k_ctip_matrix<double, 2, true> : &k_ctip_matrix<double, 2>)
+                      : (pack ? &k_ctip_matrix<double, 1, true> : &k_ctip_matrix<double, 1>);
   hipLaunchKernelGGL(km, dim3(grid_blocks(ctx->n)), dim3(BLOCK), 0, ctx->st, ctx->g, ctx->dtab,
-                     ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->bvec);
+                     ctx->x, ctx->id, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->bvec,
+                     ctx->inner_drop, ctx->pk_words, ctx->nb_cnt16);
 }
 
 int build_matrix(simox_ctx* ctx) {
   const int nb = grid_blocks(ctx->n);
   CHK(ensure_nlist(ctx));
   CHK(check_err(ctx, 1));
+  const bool pack = use32(ctx) && ctx->inner_coarse && ctx->inner_f16 && ctx->inner_pack &&
+                    ctx->n <= (1 << PK_COL_BITS);
+  if (pack) {
+    // the CTIP matrix pass also writes the packed words and the kept counts
+    if (!ctx->nb_cnt16) CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
+    if (!ctx->pk_words) CHK(dalloc(ctx, &ctx->pk_words, size_t(ctx->cap) * ctx->n));
+  }
   hipEvent_t ev{};
   tstart(ctx, 9, &ev);
   launch_ctip_matrix(ctx);
@@ -1381,7 +1395,6 @@ int build_matrix(simox_ctx* ctx) {
   ctx->val16_valid = false;
   ctx->pk_valid = false;
   if (use32(ctx) && ctx->inner_coarse && ctx->inner_f16) {
-    const bool pack = ctx->inner_pack && ctx->n <= (1 << PK_COL_BITS);
     if (!ctx->nb_cnt16) CHK(dalloc(ctx, &ctx->nb_cnt16, size_t(ctx->n)));
     if (pack) {
       const int n = ctx->n, nsl = (n + SLICE - 1) / SLICE, nsb = (nsl + 1023) / 1024;
@@ -1393,8 +1406,6 @@ int build_matrix(simox_ctx* ctx) {
         CHK(dalloc(ctx, &ctx->pk_bsum, size_t(nsb) + 1));
         CHK(dalloc(ctx, &ctx->pk_total, 1));
       }
-      hipLaunchKernelGGL(k_pk_count, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->cap, ctx->nb_cnt,
-                         ctx->nb_val64, ctx->inner_drop, ctx->nb_cnt16);
       hipLaunchKernelGGL(k_pk_slices, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->nb_cnt16,
                          ctx->pk_slen);
       hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(1024), 0, ctx->st, nsl, ctx->pk_slen,
@@ -1405,9 +1416,9 @@ int build_matrix(simox_ctx* ctx) {
                          ctx->pk_bsum, ctx->pk_total);
       hipLaunchKernelGGL(k_pk_scale, dim3(grid_blocks(nsl + 1)), dim3(BLOCK), 0, ctx->st, nsl,
                          ctx->pk_soff);
-      hipLaunchKernelGGL(k_ell_to_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->cap,
-                         ctx->nb_cnt, ctx->nb_col, ctx->nb_val64, ctx->inner_drop, ctx->pk_slen,
-                         ctx->pk_soff, ctx->nb_pk);
+      hipLaunchKernelGGL(k_ell_to_sell4, dim3(nb), dim3(BLOCK), 0, ctx->st, n, ctx->nb_cnt,
+                         ctx->pk_words, ctx->pk_slen, ctx->pk_soff,
+                         reinterpret_cast<uint4*>(ctx->nb_pk));
       ctx->pk_valid = true;
       // the halo copy where the halo variants may run: before the autotuner
       // has timed them, or when it picked one
@@ -2713,7 +2724,7 @@ int simox_destroy(simox_ctx* ctx) {
                   ctx->x2, ctx->v2, ctx->f2, ctx->q2, ctx->id2, ctx->perm2, ctx->cell2,
                   ctx->count, ctx->start, ctx->fill, ctx->slot, ctx->order, ctx->rho, ctx->dF64,
                   ctx->dF32, ctx->nb_fwd, ctx->nq, ctx->nq_n, ctx->facc, ctx->eacc, ctx->frec, ctx->sv, ctx->cv, ctx->spl_grad, ctx->vjp_u, ctx->vjp_t, ctx->vjp_up, ctx->dip_acc, ctx->nb_cnt, ctx->nb_col, ctx->nb_val64,
-                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->nb_pkh, ctx->nb_pk24, ctx->pk_runs, ctx->pk_hcount, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
+                  ctx->nb_val32, ctx->nb_val16, ctx->nb_col16, ctx->nb_cnt16, ctx->nb_pk, ctx->pk_words, ctx->nb_pkh, ctx->nb_pk24, ctx->pk_runs, ctx->pk_hcount, ctx->pk_slen, ctx->pk_soff, ctx->pk_bsum, ctx->pk_total, ctx->bvec, ctx->qc, ctx->ql, ctx->gv, ctx->dv, ctx->pv, ctx->phi,
                   ctx->rv, ctx->zv, ctx->apv, ctx->mv, ctx->qhist[0], ctx->qhist[1], ctx->qhist[2], ctx->qtmp, ctx->pv32, ctx->rdf, ctx->adj,
                   ctx->part, ctx->dscal, ctx->derr, ctx->red_ctr, ctx->dstage, ctx->part_eg};
   for (void* b : bufs)

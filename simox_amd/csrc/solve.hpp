@@ -1289,15 +1289,8 @@ __global__ void k_extrapolate(int n, int ord, double c0, double c1, double c2, d
 // of its rows / 4), group g of lane l at uint4 index soff[s] + 64 g + l, so a
 // wave reads 1 KiB per 16-byte load and each lane 4 entries.  Rows shorter
 // than the slice are padded with zero values on their own column.
-__global__ void k_pk_count(int n, int cap, const int* __restrict__ cnt,
-                           const double* __restrict__ v64, double tau, int* __restrict__ cnt16) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = cnt[i] < cap ? cnt[i] : cap;
-  int m = 0;
-  for (int k = 0; k < c; ++k) m += fabs(v64[size_t(k) * n + i]) >= tau;
-  cnt16[i] = m;
-}
+// The kept-entry count of each row (cnt16) comes from k_ctip_matrix's PACK
+// pass, which also writes each entry's packed word.
 
 // slen[s] = ceil(max over the slice's rows of cnt16 / 4)
 __global__ void k_pk_slices(int n, const int* __restrict__ cnt16, int* __restrict__ slen) {
@@ -1316,25 +1309,39 @@ __global__ void k_pk_scale(int nsl, int* __restrict__ soff) {
   if (s <= nsl) soff[s] *= SLICE;
 }
 
-__global__ void k_ell_to_sell4(int n, int cap, const int* __restrict__ cnt,
-                               const int* __restrict__ col, const double* __restrict__ v64,
-                               double tau, const int* __restrict__ slen,
-                               const int* __restrict__ soff, unsigned* __restrict__ pk) {
+// The SELL-4 slices from the packed words k_ctip_matrix<..., PACK> wrote in
+// the list's layout (4 bytes per entry read, not the FP64 value and the
+// column): each row's kept words in list order, each lane's groups of 4
+// stored whole (16 bytes), then padding groups (zero values on the row's own
+// column) up to the slice's length.
+__global__ void k_ell_to_sell4(int n, const int* __restrict__ cnt,
+                               const unsigned* __restrict__ pkw, const int* __restrict__ slen,
+                               const int* __restrict__ soff, uint4* __restrict__ pk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int s = i / SLICE, lane = i % SLICE;
-  unsigned* base = pk + 4 * (size_t(soff[s]) + lane);
-  const int c = cnt[i] < cap ? cnt[i] : cap;
+  uint4* base = pk + size_t(soff[s]) + lane;
+  const int c = cnt[i];
+  const unsigned pad = unsigned(i);
+  unsigned a0 = pad, a1 = pad, a2 = pad, a3 = pad;
   int m = 0;
   for (int k = 0; k < c; ++k) {
-    const double v = v64[size_t(k) * n + i];
-    if (fabs(v) < tau) continue;
-    const unsigned hb = __builtin_bit_cast(unsigned short, _Float16(float(v)));
-    const unsigned code = (hb & 0x8000u) | (((hb & 0x7FFFu) + 8u) & 0x7FF0u);
-    base[size_t(m / 4) * 4 * SLICE + (m % 4)] = (code << 16) | unsigned(col[size_t(k) * n + i]);
+    const unsigned w = pkw[size_t(k) * n + i];
+    if (w == PK_DROP) continue;
+    const int r = m & 3;
+    a0 = r == 0 ? w : a0;
+    a1 = r == 1 ? w : a1;
+    a2 = r == 2 ? w : a2;
+    a3 = r == 3 ? w : a3;
     ++m;
+    if (r == 3) {
+      base[size_t(m / 4 - 1) * SLICE] = make_uint4(a0, a1, a2, a3);
+      a0 = a1 = a2 = a3 = pad;
+    }
   }
-  for (; m < 4 * slen[s]; ++m) base[size_t(m / 4) * 4 * SLICE + (m % 4)] = unsigned(i);
+  int gdone = m / 4;
+  if (m & 3) base[size_t(gdone++) * SLICE] = make_uint4(a0, a1, a2, a3);
+  for (; gdone < slen[s]; ++gdone) base[size_t(gdone) * SLICE] = make_uint4(pad, pad, pad, pad);
 }
 
 // Line-search terms at step a: part[2k] = sum p (pen'(q + a p) - pen'(q)),
