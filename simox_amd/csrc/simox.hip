@@ -1953,7 +1953,7 @@ int ncg_iter_body(simox_ctx* ctx) {
     for (int k = 0; k < rounds; ++k)
       hipLaunchKernelGGL(k_ncg_ls_dev, dim3(nbr), dim3(BLOCK), 0, ctx->st, n, ctx->dtab, ctx->id,
                          ctx->qc, ctx->pv, ls, ctx->part,
-                         RedTail{ctx->red_ctr, ctx->dscal + 24, ls, 2});
+                         RedTail{ctx->red_ctr, ctx->dscal + 64, ls, 2});
   }
   hipLaunchKernelGGL(k_ncg_step_dev, dim3(nbw), dim3(RED_NT), 0, ctx->st, n, ctx->dtab, ctx->id,
                      ls, ctx->pv, ctx->apv, ctx->qc, ctx->gv, ctx->rv, ctx->part,
@@ -2655,7 +2655,7 @@ int simox_create(int n, const double box[3], const int fft[3], double rcut, int 
     CHK((gather_attr<PGX, PGY, PGZ>(ctx)));
     CHK(gather_rec_attrs(ctx));
   }
-  CHK(dalloc(ctx, &ctx->dscal, 64));
+  CHK(dalloc(ctx, &ctx->dscal, 72));
   CHK(dalloc(ctx, &ctx->part_eg, grid_blocks(n)));
   HIPCHK(hipHostMalloc((void**)&ctx->hscal, 64 * sizeof(double), hipHostMallocDefault));
   CHK(dalloc(ctx, &ctx->derr, 1));

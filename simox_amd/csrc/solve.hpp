@@ -18,7 +18,7 @@ namespace sx {
 // many gathers in flight.
 // scalar steps of the device-side CG (defined with the line-search kernels)
 __device__ inline void ls_init(const double* red, double* ls);
-__device__ inline void ls_update(const double* red, double* ls);
+__device__ inline void ls_update(const double* red, double* ls, bool bounds = false);
 __device__ inline void beta_update(const double* red, double* ls);
 template <int OP>
 __device__ inline void reduce_scalar(const double* out, double* ls);
@@ -100,6 +100,66 @@ __device__ inline void partials_out(const double (&acc)[NV], double* part, const
     }
     if (t.op == 2) reduce_scalar<2>(t.out, t.ls);
     if (t.op == 4) reduce_scalar<4>(t.out, t.ls);
+    __hip_atomic_store(t.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// partials_out for the line-search rounds: rows of {D1, S2, up, dn}; the last
+// workgroup sums the first two in workgroup order, takes min / max of the
+// others, writes t.out[0..3] and runs ls_update with the crossings.
+__device__ inline void partials_out_ls(const double (&row)[4], double* part, const RedTail& t,
+                                       double* lds) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      __hip_atomic_store(&part[size_t(4) * blockIdx.x + k], row[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the row's stores acknowledged
+    last = __hip_atomic_fetch_add(t.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;  // uniform over the workgroup
+  double s0 = 0.0, s1 = 0.0, up = 1e300, dn = -1e300;
+  for (int k = threadIdx.x; k < int(gridDim.x); k += blockDim.x) {
+    s0 += __hip_atomic_load(&part[size_t(4) * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s1 += __hip_atomic_load(&part[size_t(4) * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    up = fmin(up, __hip_atomic_load(&part[size_t(4) * k + 2], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT));
+    dn = fmax(dn, __hip_atomic_load(&part[size_t(4) * k + 3], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT));
+  }
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  up = -wave_max(-up);
+  dn = wave_max(dn);
+  if (lane == 0) {
+    lds[w] = s0;
+    lds[nw + w] = s1;
+  }
+  __shared__ double mm[2 * 16];
+  if (lane == 0) {
+    mm[2 * w] = up;
+    mm[2 * w + 1] = dn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x0 = lds[0], x1 = lds[nw];
+    for (int q = 1; q < nw; ++q) {
+      x0 += lds[q];
+      x1 += lds[nw + q];
+      up = fmin(up, mm[2 * q]);
+      dn = fmax(dn, mm[2 * q + 1]);
+    }
+    up = fmin(up, mm[0]);
+    dn = fmax(dn, mm[1]);
+    t.out[0] = x0;
+    t.out[1] = x1;
+    t.out[2] = up;
+    t.out[3] = dn;
+    ls_update(t.out, t.ls, true);
     __hip_atomic_store(t.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -1706,7 +1766,13 @@ __device__ inline void ls_init(const double* red, double* ls) {
 __global__ void k_ls_init(const double* __restrict__ red, double* __restrict__ ls) { ls_init(red, ls); }
 
 // Newton step on phi'(a) = -gp + a pHp + D1(a), bracketed; red = {D1(a), S2(a)}
-__device__ inline void ls_update(const double* red, double* ls) {
+// bounds: red[2], red[3] = the nearest steps above and below a at which an
+// atom's charge crosses a bound (k_ncg_ls_dev, each pulled towards a by its
+// rounding margin).  When a and the Newton step an both lie strictly between
+// them, no atom changes sides on [a, an], D1 is linear there with slope S2(a),
+// and phi'(an) = phi'(a) + (an - a)(pHp + S2(a)) = 0: the search ends at an
+// without the round that would evaluate it.
+__device__ inline void ls_update(const double* red, double* ls, bool bounds) {
   if (ls[5] != 0.0) return;
   ls[9] += 1.0;
   const double gp = ls[0], pHp = ls[1], a = ls[2];
@@ -1715,8 +1781,10 @@ __device__ inline void ls_update(const double* red, double* ls) {
   double lo = ls[3], hi = ls[4];
   if (fprime < 0.0) lo = a; else hi = a;
   double an = a - fprime / (pHp + red[1]);
-  if (an <= lo || (hi >= 0.0 && an >= hi)) an = hi >= 0.0 ? 0.5 * (lo + hi) : 2.0 * a;
+  const bool newton = !(an <= lo || (hi >= 0.0 && an >= hi));
+  if (!newton) an = hi >= 0.0 ? 0.5 * (lo + hi) : 2.0 * a;
   if (fabs(an - a) <= 1e-15 * fabs(a)) ls[5] = 1.0;
+  if (bounds && newton && red[3] < a && a < red[2] && red[3] < an && an < red[2]) ls[5] = 1.0;
   ls[2] = an;
   ls[3] = lo;
   ls[4] = hi;
@@ -1743,11 +1811,21 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
   if (ls[5] != 0.0) return;  // uniform: the search is done
   const double a = ls[2];
   double acc[2] = {0.0, 0.0};
+  double up = 1e300, dn = -1e300;  // nearest crossings above and below a
   auto term = [&](int idi, double qi, double pi) {
     const SpecCoef& sc = tb->sc[idi - 1];
     const double qa = qi + a * pi;
     acc[0] += pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
     acc[1] += outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+    if (pi != 0.0) {
+      // q + t p crosses qmin and qmax at these t; each pulled towards a by
+      // far more than the rounding of q + t p (|t p| stays below the bounds'
+      // scale on the interval)
+      const double m = 1e-14 * (fabs(qi) + fabs(sc.qmin) + fabs(sc.qmax)) / fabs(pi);
+      const double b1 = (sc.qmin - qi) / pi, b2 = (sc.qmax - qi) / pi;
+      if (b1 > a) up = fmin(up, b1 - m); else dn = fmax(dn, b1 + m);
+      if (b2 > a) up = fmin(up, b2 - m); else dn = fmax(dn, b2 + m);
+    }
   };
   // grid-stride, four atoms' loads in flight per trip, summed in atom order
   const int S = gridDim.x * blockDim.x;
@@ -1766,7 +1844,22 @@ __global__ void __launch_bounds__(BLOCK) k_ncg_ls_dev(int n, const Tables* __res
   }
   for (; i < n; i += S) term(id[i], q[i], p[i]);
   block_sum<2>(acc, lds);
-  partials_out<2>(acc, part, rt, lds);
+  // the block's nearest crossings (min, max: order-independent)
+  __shared__ double bnd[2 * BLOCK / WAVE];
+  up = -wave_max(-up);
+  dn = wave_max(dn);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    bnd[2 * (threadIdx.x / WAVE)] = up;
+    bnd[2 * (threadIdx.x / WAVE) + 1] = dn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < BLOCK / WAVE; ++w) {
+      up = fmin(up, bnd[2 * w]);
+      dn = fmax(dn, bnd[2 * w + 1]);
+    }
+  const double row[4] = {acc[0], acc[1], up, dn};
+  partials_out_ls(row, part, rt, lds);
 }
 
 // The whole line search in one workgroup (systems of up to LS_SMALL_MAX atoms;
