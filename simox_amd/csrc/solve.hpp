@@ -1874,9 +1874,9 @@ __global__ void __launch_bounds__(1024) k_ncg_ls_small(int n, const Tables* __re
                                                        const double* __restrict__ p,
                                                        double* __restrict__ ls,
                                                        double* __restrict__ red) {
-  __shared__ double lds[2 * 1024 / WAVE];
+  __shared__ double lds[4 * 1024 / WAVE];
   __shared__ double st[10];  // ls[0..9]
-  __shared__ double sums[2];
+  __shared__ double sums[4];
   if (threadIdx.x < 10) st[threadIdx.x] = ls[threadIdx.x];
   if (threadIdx.x < 2) sums[threadIdx.x] = red[threadIdx.x];
   __syncthreads();
@@ -1884,25 +1884,47 @@ __global__ void __launch_bounds__(1024) k_ncg_ls_small(int n, const Tables* __re
     if (st[5] != 0.0) break;  // uniform: read after the barrier
     const double a = st[2];
     double acc[2] = {0.0, 0.0};
+    double up = 1e300, dn = -1e300;  // nearest bound crossings, as k_ncg_ls_dev
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const SpecCoef& sc = tb->sc[id[i] - 1];
       const double pi = p[i], qi = q[i], qa = qi + a * pi;
       acc[0] += pi * (pen_d1(sc, qa) - pen_d1(sc, qi));
       acc[1] += outside(sc, qa) ? pi * pi * 2.0 * WPEN : 0.0;
+      if (pi != 0.0) {
+        const double m = 1e-14 * (fabs(qi) + fabs(sc.qmin) + fabs(sc.qmax)) / fabs(pi);
+        const double b1 = (sc.qmin - qi) / pi, b2 = (sc.qmax - qi) / pi;
+        if (b1 > a) up = fmin(up, b1 - m); else dn = fmax(dn, b1 + m);
+        if (b2 > a) up = fmin(up, b2 - m); else dn = fmax(dn, b2 + m);
+      }
     }
+    constexpr int NW = 1024 / WAVE;
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
       const double w = wave_sum(acc[v]);
-      if ((threadIdx.x & (WAVE - 1)) == 0) lds[v * (1024 / WAVE) + threadIdx.x / WAVE] = w;
+      if ((threadIdx.x & (WAVE - 1)) == 0) lds[v * NW + threadIdx.x / WAVE] = w;
+    }
+    up = -wave_max(-up);
+    dn = wave_max(dn);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      lds[2 * NW + threadIdx.x / WAVE] = up;
+      lds[3 * NW + threadIdx.x / WAVE] = dn;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+      const int nw = int(blockDim.x / WAVE);
       for (int v = 0; v < 2; ++v) {
-        double t = lds[v * (1024 / WAVE)];
-        for (int k = 1; k < int(blockDim.x / WAVE); ++k) t += lds[v * (1024 / WAVE) + k];
+        double t = lds[v * NW];
+        for (int k = 1; k < nw; ++k) t += lds[v * NW + k];
         sums[v] = t;
       }
-      ls_update(sums, st);
+      double u = lds[2 * NW], d = lds[3 * NW];
+      for (int k = 1; k < nw; ++k) {
+        u = fmin(u, lds[2 * NW + k]);
+        d = fmax(d, lds[3 * NW + k]);
+      }
+      sums[2] = u;
+      sums[3] = d;
+      ls_update(sums, st, true);
     }
     __syncthreads();
   }
