@@ -389,14 +389,15 @@ def test_inner_shapes_change_no_result(monkeypatch):
     matvec variant, record gather tile, plane FFT threads and z lines of both
     coarse grids) give the same charges and forces bit for bit whichever is
     chosen: each fixed in turn (SIMOX_AUTOTUNE=0 and one switch) against the
-    tuned run."""
+    tuned run; so do the pair kernels' batch depths (list entries loaded
+    ahead, sums in the same order)."""
     from simox_amd.systems import al_o_slab
     s = al_o_slab(24, 24, 8, 1024, 5, 600.0, (96, 96, 64), 10.0, vacuum=20.0,
                   o_layer=(2.0, 10.0), o_dmin=2.0, name="inner")
 
     def run(env):
         for k in ("SIMOX_AUTOTUNE", "SIMOX_MV_VARIANT", "SIMOX_GATHER_SHAPE", "SIMOX_FFT_XYNT",
-                  "SIMOX_FFT_NL"):
+                  "SIMOX_FFT_NL", "SIMOX_FORCE_BATCH", "SIMOX_DENS_BATCH", "SIMOX_CTIP_BATCH"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -425,6 +426,10 @@ def test_inner_shapes_change_no_result(monkeypatch):
     cases += [{"SIMOX_GATHER_SHAPE": str(v)} for v in range(6)]
     cases += [{"SIMOX_FFT_XYNT": t} for t in ("256", "512", "1024")]
     cases += [{"SIMOX_FFT_NL": m} for m in ("4", "8", "16")]
+    # list entries whose neighbour data each lane loads ahead: the same sums
+    # (each lane walks its atom's whole list, so this system covers them)
+    cases += [{k: b} for k in ("SIMOX_FORCE_BATCH", "SIMOX_DENS_BATCH", "SIMOX_CTIP_BATCH")
+              for b in ("1", "2")]
     for env in cases:
         env = dict(env, SIMOX_AUTOTUNE="0")
         q, f, sh = run(env)
