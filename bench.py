@@ -530,15 +530,18 @@ def bench_gradients(args):
     }
 
 
-def inner_operator(s, mode):
-    """The MIXED inner solves' operator (simox_set_inner_grid's default rule)."""
+def inner_operator(s, mode, shapes=None):
+    """The MIXED inner solves' operator (simox_set_inner_grid's default rule;
+    shapes: simox_inner_shapes, whose matvec variant sets the entry format)."""
     if mode != "mixed":
         return "the reference's operator, FP64"
     if packed_inner(s, mode):
-        return ("FP32 with 12-bit matrix values packed with their 20-bit columns, entries "
-                "below 1e-3 eV/e^2 left out; reciprocal part on the %dx%dx%d grid (half "
-                "resolution, order 4, plane-resident FFT passes); restarts FP64 with the full "
-                "matrix on the reference grid" % tuple(k // 2 for k in s.fft))
+        ent = ("24-bit entries (12-bit value, 12-bit slot in the row block's LDS halo of p)"
+               if shapes and shapes["matvec"] == 10 else
+               "12-bit matrix values packed with their 20-bit columns")
+        return ("FP32 with %s, entries below 1e-3 eV/e^2 left out; reciprocal part on the "
+                "%dx%dx%d grid (half resolution, order 4, plane-resident FFT passes); restarts "
+                "FP64 with the full matrix on the reference grid" % ((ent,) + tuple(k // 2 for k in s.fft)))
     if coarse_inner(s, mode):
         return ("FP32 with FP16 matrix values, entries below 1e-3 eV/e^2 left out; reciprocal "
                 "part on the %dx%dx%d grid (half resolution, order 4); restarts FP64 with the "
@@ -715,7 +718,7 @@ def main():
                    "charge_tol": args.tol if args.mode != "faithful" else "reference (1e-3 on max|alpha p|)",
                    "charge_iters_last_step": cg_iters, "charge_restarts_last_step": cg_restarts,
                    "charge_precond": {"mixed": "long-range", "fp64": "jacobi"}.get(args.mode, "reference nonlinear CG + Brent"),
-                   "inner_operator": inner_operator(s, args.mode),
+                   "inner_operator": inner_operator(s, args.mode, shapes),
                    "fft_tiles": fft_tiles,
                    "inner_shapes": shapes},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
