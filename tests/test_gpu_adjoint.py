@@ -32,7 +32,10 @@ def test_param_grad_matches_central_differences(system):
         tp[k] += h
         tm[k] -= h
         fd = (solve_energy(system, tp)[0] - solve_energy(system, tm)[0]) / (2 * h)
-        assert grad[k] == pytest.approx(fd, rel=1e-4, abs=1e-6), (k, grad[k], fd)
+        # relative 1e-6 above the rounding floor of the difference quotient
+        floor = 1e-12 * abs(E0) / h
+        print("dE/dtheta", k, grad[k], fd, abs(fd - grad[k]) / max(abs(grad[k]), 1e-300), flush=True)
+        assert abs(fd - grad[k]) <= 1e-6 * abs(grad[k]) + floor, (k, grad[k], fd)
     # species absent from the system: no dependence
     assert np.all(grad[[1, 2, 4, 6, 7, 9]] == 0.0)
 
@@ -109,7 +112,8 @@ def test_force_loss_gradient_matches_central_differences(system):
         tm[k] -= h
         fd = (solve_loss(system, tp, w)[0] - solve_loss(system, tm, w)[0]) / (2 * h)
         floor = 1e-12 * abs(L0) / h
-        assert abs(fd - grad[k]) <= 1e-5 * abs(grad[k]) + floor, (k, grad[k], fd)
+        print("dL/dtheta", k, grad[k], fd, abs(fd - grad[k]) / max(abs(grad[k]), 1e-300), flush=True)
+        assert abs(fd - grad[k]) <= 1e-6 * abs(grad[k]) + floor, (k, grad[k], fd)
     # vector-Jacobian product with a random u: d(sum u.F)/dtheta
     u = np.random.default_rng(3).normal(size=(system.n, 3))
 
@@ -133,4 +137,5 @@ def test_force_loss_gradient_matches_central_differences(system):
         gm.close()
         fd = (sp - sm) / (2 * h)
         floor = 1e-12 * np.abs(u).sum() * 10.0 / h
-        assert abs(fd - vjp[k]) <= 1e-5 * abs(vjp[k]) + floor, (k, vjp[k], fd)
+        print("d(u.F)/dtheta", k, vjp[k], fd, abs(fd - vjp[k]) / max(abs(vjp[k]), 1e-300), flush=True)
+        assert abs(fd - vjp[k]) <= 1e-6 * abs(vjp[k]) + floor, (k, vjp[k], fd)
